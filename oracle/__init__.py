@@ -1,0 +1,226 @@
+"""CPU oracle for the index build + adaptamer merge path.
+
+TEST INFRASTRUCTURE ONLY.  `tests/`, `__graft_entry__.smoke()` and the
+CPU-baseline leg of `bench.py` use it as the checker; the product
+(`distributedpanga_amd`) never imports it.
+
+Two shared libraries are bound here:
+  * ``oracle/liboracle.so`` -- the C restatement in ``oracle/pga_oracle.c``
+    (built by ``__graft_entry__.build()``);
+  * ``oracle/_ref/libpgaref.so`` -- the reference's own extraction, sort, LCP
+    and GIX-adapter sources compiled by ``oracle/ref_build.sh`` (present only
+    where the reference sources were available at build time).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libpgaref.so")
+NPRE = 1 << 24
+
+_u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
+
+
+class ExtractStats(C.Structure):
+    _fields_ = [("n_sel", C.c_int64), ("n_right", C.c_int64),
+                ("n_left", C.c_int64), ("n_tie", C.c_int64)]
+
+
+class OracleResult(C.Structure):
+    _fields_ = [("P", C.c_int), ("C", C.c_int), ("rsize", C.c_int), ("e", C.c_int), ("s", C.c_int),
+                ("nrec", C.c_int64), ("records", _u8p), ("entries", _u8p), ("cum", _i64p),
+                ("maxpre", C.c_int64), ("nseeds", _u8p), ("nlen", C.c_int64),
+                ("cseeds", _u8p), ("clen", C.c_int64), ("nbuck", _i64p), ("cbuck", _i64p),
+                ("ncontig", C.c_int), ("xs", ExtractStats),
+                ("t_extract", C.c_double), ("t_sort", C.c_double),
+                ("t_adapter", C.c_double), ("t_merge", C.c_double)]
+
+
+class RefResult(C.Structure):
+    _fields_ = [("P", C.c_int), ("C", C.c_int), ("rsize", C.c_int), ("e", C.c_int),
+                ("nrec", C.c_int64), ("extracted", _u8p), ("sorted", _u8p), ("entries", _u8p),
+                ("cum", _i64p), ("maxpre", C.c_int64),
+                ("t_extract", C.c_double), ("t_sort", C.c_double), ("t_adapter", C.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run __graft_entry__.build())")
+        L = C.CDLL(ORACLE_SO)
+        L.oracle_record_sizing.argtypes = [C.c_int64, C.c_int64, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.oracle_pack_forward.argtypes = [_u8p, _u8p]
+        L.oracle_pack_rc.argtypes = [_u8p, _u8p]
+        L.oracle_comp_table.argtypes = [C.POINTER(C.c_int)]
+        L.oracle_tmap.argtypes = [C.c_int]
+        L.oracle_mz.argtypes = [_u8p, C.c_int64]
+        L.oracle_extract.argtypes = [_u8p, _i64p, _i64p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(_u8p), _i64p, C.POINTER(ExtractStats)]
+        L.oracle_sort_records.argtypes = [_u8p, C.c_int64, C.c_int]
+        L.oracle_sort_records_qsort.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, C.c_int]
+        L.oracle_lcp_bases.argtypes = [_u8p, _u8p]
+        L.oracle_recalc_lcps.argtypes = [_u8p, C.c_int64, C.c_int]
+        L.oracle_build_adapter.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, _u8p, _i64p, _i64p]
+        L.oracle_merge.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(_u8p), _i64p, C.POINTER(_u8p), _i64p, _i64p, _i64p]
+        L.oracle_pipeline.argtypes = [_u8p, _i64p, _i64p, C.c_int, C.c_int, C.POINTER(OracleResult)]
+        L.oracle_result_free.argtypes = [C.POINTER(OracleResult)]
+        L.oracle_free.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference build missing: {REF_SO} (run oracle/ref_build.sh)")
+        L = C.CDLL(REF_SO)
+        L.ref_run.argtypes = [_u8p, C.c_int64, _i64p, _i64p, C.c_int, C.c_int, C.POINTER(RefResult)]
+        L.ref_result_free.argtypes = [C.POINTER(RefResult)]
+        L.ref_lcp_bases.argtypes = [_u8p, _u8p]
+        _ref = L
+    return _ref
+
+
+def _u8(a):
+    return a.ctypes.data_as(_u8p)
+
+
+def _i64(a):
+    return a.ctypes.data_as(_i64p)
+
+
+def _take(ptr, n, dtype=np.uint8):
+    if n <= 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def sizing(maxctg, ncontig):
+    P, Cb, rs = C.c_int(), C.c_int(), C.c_int()
+    lib().oracle_record_sizing(maxctg, ncontig, C.byref(P), C.byref(Cb), C.byref(rs))
+    return P.value, Cb.value, rs.value
+
+
+def extract(bps, boff, clen, mode=1):
+    """Records (nrec x rsize uint8) in extraction order, plus the selection counters."""
+    bps = np.ascontiguousarray(bps, dtype=np.uint8)
+    boff = np.ascontiguousarray(boff, dtype=np.int64)
+    clen = np.ascontiguousarray(clen, dtype=np.int64)
+    n = len(clen)
+    P, Cb, rs = sizing(int(clen.max()) if n else 0, n)
+    out = _u8p()
+    nrec = C.c_int64()
+    st = ExtractStats()
+    rc = lib().oracle_extract(_u8(bps), _i64(boff), _i64(clen), n, mode, P, Cb,
+                              C.byref(out), C.byref(nrec), C.byref(st))
+    if rc:
+        raise MemoryError("oracle_extract failed")
+    recs = _take(out, nrec.value * rs).reshape(-1, rs)
+    lib().oracle_free(C.cast(out, C.c_void_p))
+    return recs, dict(sel=st.n_sel, right=st.n_right, left=st.n_left, tie=st.n_tie), (P, Cb)
+
+
+def sort_records(recs):
+    recs = np.ascontiguousarray(recs, dtype=np.uint8).copy()
+    lib().oracle_sort_records(_u8(recs), recs.shape[0], recs.shape[1])
+    return recs
+
+
+def sort_records_qsort(recs, P, Cb):
+    recs = np.ascontiguousarray(recs, dtype=np.uint8).copy()
+    lib().oracle_sort_records_qsort(_u8(recs), recs.shape[0], recs.shape[1], P, Cb)
+    return recs
+
+
+def recalc_lcps(recs):
+    recs = np.ascontiguousarray(recs, dtype=np.uint8).copy()
+    lib().oracle_recalc_lcps(_u8(recs), recs.shape[0], recs.shape[1])
+    return recs
+
+
+def build_adapter(recs, P, Cb):
+    recs = np.ascontiguousarray(recs, dtype=np.uint8)
+    n = recs.shape[0]
+    e = 9 + P + Cb
+    ent = np.zeros(max(n * e, 1), dtype=np.uint8)
+    cum = np.zeros(NPRE, dtype=np.int64)
+    mp = C.c_int64()
+    lib().oracle_build_adapter(_u8(recs), n, P, Cb, _u8(ent), _i64(cum), C.byref(mp))
+    return ent[: n * e].reshape(n, e), cum, mp.value
+
+
+def merge(entries, P, Cb, ncontig, freq=10, mode=0):
+    entries = np.ascontiguousarray(entries, dtype=np.uint8)
+    n = entries.shape[0]
+    nb = np.zeros(max(ncontig, 1), dtype=np.int64)
+    cb = np.zeros(max(ncontig, 1), dtype=np.int64)
+    ns, cs = _u8p(), _u8p()
+    nl, cl = C.c_int64(), C.c_int64()
+    rc = lib().oracle_merge(_u8(entries), n, P, Cb, ncontig, freq, mode,
+                            C.byref(ns), C.byref(nl), C.byref(cs), C.byref(cl), _i64(nb), _i64(cb))
+    if rc:
+        raise MemoryError("oracle_merge failed")
+    N = _take(ns, nl.value)
+    Cs = _take(cs, cl.value)
+    lib().oracle_free(C.cast(ns, C.c_void_p))
+    lib().oracle_free(C.cast(cs, C.c_void_p))
+    return N, Cs, nb[:ncontig], cb[:ncontig]
+
+
+def pipeline(bps, boff, clen, freq=10):
+    """The whole path on the CPU.  Returns a dict of numpy arrays and timings."""
+    bps = np.ascontiguousarray(bps, dtype=np.uint8)
+    boff = np.ascontiguousarray(boff, dtype=np.int64)
+    clen = np.ascontiguousarray(clen, dtype=np.int64)
+    r = OracleResult()
+    rc = lib().oracle_pipeline(_u8(bps), _i64(boff), _i64(clen), len(clen), freq, C.byref(r))
+    if rc:
+        lib().oracle_result_free(C.byref(r))
+        raise MemoryError("oracle_pipeline failed")
+    out = dict(P=r.P, C=r.C, rsize=r.rsize, e=r.e, s=r.s, nrec=r.nrec,
+               records=_take(r.records, r.nrec * r.rsize).reshape(-1, r.rsize),
+               entries=_take(r.entries, r.nrec * r.e).reshape(-1, r.e),
+               cum=_take(r.cum, NPRE, np.int64), maxpre=r.maxpre,
+               nseeds=_take(r.nseeds, r.nlen), cseeds=_take(r.cseeds, r.clen),
+               nbuck=_take(r.nbuck, r.ncontig, np.int64), cbuck=_take(r.cbuck, r.ncontig, np.int64),
+               stats=dict(sel=r.xs.n_sel),
+               t_extract=r.t_extract, t_sort=r.t_sort, t_adapter=r.t_adapter, t_merge=r.t_merge)
+    lib().oracle_result_free(C.byref(r))
+    return out
+
+
+def ref_run(bps, boff, clen, do_index=True):
+    """The reference's own extraction (and sort + LCP + adapter) on the same GDB."""
+    bps = np.ascontiguousarray(bps, dtype=np.uint8)
+    boff = np.ascontiguousarray(boff, dtype=np.int64)
+    clen = np.ascontiguousarray(clen, dtype=np.int64)
+    r = RefResult()
+    rc = ref().ref_run(_u8(bps), len(bps), _i64(boff), _i64(clen), len(clen), int(do_index), C.byref(r))
+    if rc:
+        raise MemoryError("ref_run failed")
+    rs = r.rsize
+    out = dict(P=r.P, C=r.C, rsize=rs, e=r.e, nrec=r.nrec,
+               extracted=_take(r.extracted, r.nrec * rs).reshape(-1, rs),
+               t_extract=r.t_extract, t_sort=r.t_sort, t_adapter=r.t_adapter)
+    if do_index:
+        out.update(sorted=_take(r.sorted, r.nrec * rs).reshape(-1, rs),
+                   entries=_take(r.entries, r.nrec * r.e).reshape(-1, r.e),
+                   cum=_take(r.cum, NPRE, np.int64), maxpre=r.maxpre)
+    ref().ref_result_free(C.byref(r))
+    return out
