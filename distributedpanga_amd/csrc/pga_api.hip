@@ -1,0 +1,803 @@
+// pga_api.hip -- the C-ABI of include/libpga_hip.h: batch bookkeeping, device
+// buffer pool, kernel launch sequence, exports.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+#include "libpga_hip.h"
+#include "pga_device.h"
+
+#include "pga_kernels.hip"
+
+using namespace pga;
+
+namespace {
+
+// The 4-mer scramble of the syncmer hash (src/tables.c:5-22): data of the
+// selection rule's definition.
+const uint8_t TMAP[256] = {
+    0xff,0xd4,0xf5,0xfd,0xe4,0xad,0x21,0xa5,0xed,0x64,0xbf,0xa9,0xf3,0x70,0xd6,0xf0,
+    0xca,0x89,0xcb,0xc9,0x82,0x9d,0x13,0x79,0x0a,0x0f,0x25,0x19,0x3e,0x47,0xa3,0xa8,
+    0xf9,0x5e,0xe8,0xa1,0xb0,0x71,0x1d,0x8c,0xde,0x69,0xe7,0x7c,0x56,0x3f,0x90,0xa4,
+    0xeb,0x45,0x59,0xf1,0x97,0x4c,0x08,0xa0,0xb8,0x4a,0x86,0xc8,0xcd,0x98,0x7d,0xfc,
+    0xef,0x4d,0x83,0x7e,0xdc,0x66,0x2b,0x8e,0xe0,0xa7,0xd0,0xa2,0x88,0x5f,0x7f,0xd9,
+    0x9b,0x78,0xd1,0x8b,0xc3,0x8f,0x2d,0xe6,0x18,0x27,0x2c,0x24,0x94,0xb7,0xce,0xbd,
+    0x0d,0x04,0x1c,0x09,0x16,0x23,0x00,0x1e,0x1a,0x29,0x2e,0x15,0x01,0x10,0x2a,0x20,
+    0xbe,0x31,0x43,0x58,0xc2,0xaa,0x1f,0xe5,0xc5,0x9e,0xcf,0xc6,0x68,0xb2,0x80,0xf4,
+    0xf8,0x53,0xb6,0x93,0x76,0x37,0x11,0x40,0xda,0x51,0xba,0x46,0x42,0x30,0x60,0x6d,
+    0x5c,0x39,0x9f,0x48,0x6c,0x62,0x28,0x67,0x06,0x12,0x26,0x0e,0x33,0x50,0xa6,0x63,
+    0xdd,0x3b,0xab,0x4b,0x72,0x5b,0x22,0x6f,0xb4,0x61,0x92,0x99,0x36,0x38,0x65,0xac,
+    0x4f,0x2f,0x32,0x44,0x54,0x3c,0x03,0x5d,0x73,0x3a,0x77,0x84,0x8d,0x4e,0x49,0xd2,
+    0xfb,0x91,0x6a,0xcc,0x8a,0x35,0x02,0x55,0x7a,0x34,0x96,0x3d,0xd3,0x41,0x85,0xf2,
+    0xb1,0x75,0xc4,0xb5,0xbb,0xb3,0x1b,0xd5,0x07,0x05,0x17,0x0b,0x7b,0xd7,0xdf,0xea,
+    0xe3,0x57,0xc0,0x95,0x9c,0x6e,0x14,0xae,0xb9,0x6b,0xc1,0x81,0x87,0x74,0xd8,0xe2,
+    0xec,0x52,0xbc,0xe9,0xe1,0xdb,0x0c,0xf7,0xaf,0x5a,0x9a,0xc7,0xfa,0xf6,0xee,0xfe
+};
+
+constexpr int XBLOCK = 256 * 4;           // positions per extraction block (XT * XU)
+constexpr int NPRE = 1 << 24;
+constexpr int MAX_LEVELS = 16;
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct HostPair {
+    int ncontig = 0;
+    int64_t seqtot = 0, maxctg = 0;
+    int64_t blk0 = 0, nblk = 0;
+    int ctg0 = 0;
+};
+
+int bytes_for(int64_t v)          // smallest b with 256^b >= v+1 (src/record.c:6-16)
+{
+    int b = 0;
+    int64_t cap = 1;
+    while (cap < v + 1) { cap *= 256; b++; }
+    return b;
+}
+
+int bits_for(int64_t v)           // bits to hold every value in [0, v]
+{
+    int b = 0;
+    while (b < 63 && (int64_t(1) << b) <= v) b++;
+    return b;
+}
+
+} // namespace
+
+struct pga_ctx {
+    pga_opts o{};
+    hipStream_t st = nullptr;
+    char err[512] = {0};
+
+    // loaded batch
+    int npairs = 0;
+    std::vector<HostPair> hp;
+    int nctg = 0;
+    int64_t F = 0;
+    int64_t nblk = 0;
+    int jbits = 0;
+    bool loaded = false, built = false, merged = false;
+
+    // host mirrors
+    std::vector<PairDev> pairs;
+    int64_t R = 0;
+    int ntiles = 0;
+    std::vector<int64_t> nseeds, cseeds;
+
+    // device buffers
+    DBuf bps, ctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
+        counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
+        mblk, pofs, seedN, seedC, nbuck, cbuck;
+
+    // timing
+    hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> sev;       // scatter launch start/stop pairs
+    int nscatter = 0;
+    int64_t scatter_bytes = 0;
+    pga_timings tm{};
+};
+
+namespace {
+
+int fail(pga_ctx *c, int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), PGA_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                   \
+    } while (0)
+
+#define LAUNCHCHK(ctx)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = hipGetLastError();                                                     \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), PGA_EHIP, "kernel launch failed: %s (%s:%d)",                  \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                            \
+    } while (0)
+
+#define TRY(x)                                                                                 \
+    do {                                                                                       \
+        int r_ = (x);                                                                          \
+        if (r_) return r_;                                                                     \
+    } while (0)
+
+int ensure(pga_ctx *c, DBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return PGA_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return fail(c, PGA_EHIP, "hipFree: %s", hipGetErrorString(e));
+    }
+    size_t want = bytes + bytes / 8;                 // headroom so small growth does not reallocate
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        (void) hipGetLastError();
+        if (hipMalloc(&b.p, bytes) != hipSuccess) {
+            (void) hipGetLastError();
+            b.p = nullptr;
+            return fail(c, PGA_ENOMEM, "device allocation of %zu bytes failed", bytes);
+        }
+        want = bytes;
+    }
+    b.cap = want;
+    return PGA_OK;
+}
+
+template <typename T> T *P(DBuf &b) { return reinterpret_cast<T *>(b.p); }
+
+int fill_u64(pga_ctx *c, void *p, int64_t n, unsigned long long v)
+{
+    if (n <= 0) return PGA_OK;
+    int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) blocks), dim3(256), 0, c->st,
+                       (unsigned long long *) p, n, v);
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+// Exclusive scan of a uint32 array into int64 out[0..n], out[n] = total.
+int scan_u32(pga_ctx *c, const uint32_t *in, int64_t n, int64_t *out)
+{
+    const int64_t span = 256 * 16;
+    int64_t nb = (n + span - 1) / span;
+    if (nb == 0) nb = 1;
+    TRY(ensure(c, c->bsum, (size_t) (nb + 1) * sizeof(int64_t)));
+    hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3((unsigned) nb), dim3(256), 0, c->st, in, n,
+                       P<int64_t>(c->bsum));
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->st, P<int64_t>(c->bsum), nb);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_scan_down<uint32_t>, dim3((unsigned) nb), dim3(256), 0, c->st, in, n,
+                       (const int64_t *) P<int64_t>(c->bsum), out);
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+int cap_of(const pga_ctx *c) { return c->o.tile_records; }
+
+int chunk_per(const pga_ctx *c) { return c->o.chunk_records / 256; }
+
+int launch_seg_hist(pga_ctx *c, int per, unsigned grid, const Rec *src, const Seg *segs, int nseg,
+                    const int64_t *chk0, unsigned long long *hist)
+{
+    switch (per) {
+    case 8:  hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, c->st, src, segs, nseg, chk0, hist); break;
+    case 32: hipLaunchKernelGGL(k_seg_hist<32>, dim3(grid), dim3(256), 0, c->st, src, segs, nseg, chk0, hist); break;
+    default: hipLaunchKernelGGL(k_seg_hist<16>, dim3(grid), dim3(256), 0, c->st, src, segs, nseg, chk0, hist); break;
+    }
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+int launch_seg_scatter(pga_ctx *c, int per, unsigned grid, const Rec *src, Rec *dst, const Seg *segs,
+                       int nseg, const int64_t *chk0, unsigned long long *cursor)
+{
+    switch (per) {
+    case 8:  hipLaunchKernelGGL(k_seg_scatter<8>, dim3(grid), dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor); break;
+    case 32: hipLaunchKernelGGL(k_seg_scatter<32>, dim3(grid), dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor); break;
+    default: hipLaunchKernelGGL(k_seg_scatter<16>, dim3(grid), dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor); break;
+    }
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+int launch_tile_sort(pga_ctx *c, unsigned grid)
+{
+    const int T = c->o.tile_records / 8;
+    const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + 16;
+    Rec *const *bufs = (Rec *const *) c->bufs.p;
+    const Tile *tiles = P<Tile>(c->tiles);
+    const PairDev *pairs = P<PairDev>(c->dpairs);
+    switch (T) {
+    case 256:
+        hipLaunchKernelGGL(k_tile_sort<256>, dim3(grid), dim3(256), lds, c->st, bufs, tiles, pairs, c->jbits,
+                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), P<uint32_t>(c->pre));
+        break;
+    case 1024:
+        hipLaunchKernelGGL(k_tile_sort<1024>, dim3(grid), dim3(1024), lds, c->st, bufs, tiles, pairs, c->jbits,
+                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), P<uint32_t>(c->pre));
+        break;
+    default:
+        hipLaunchKernelGGL(k_tile_sort<512>, dim3(grid), dim3(512), lds, c->st, bufs, tiles, pairs, c->jbits,
+                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), P<uint32_t>(c->pre));
+        break;
+    }
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void) hipGetLastError();
+        return -1.f;
+    }
+    return ms;
+}
+
+int record(pga_ctx *c, int i)
+{
+    HIPCHK(c, hipEventRecord(c->ev[i], c->st));
+    return PGA_OK;
+}
+
+int scatter_event(pga_ctx *c, hipEvent_t *out)
+{
+    if ((size_t) c->nscatter * 2 + 2 > c->sev.size()) {
+        hipEvent_t a, b;
+        HIPCHK(c, hipEventCreate(&a));
+        HIPCHK(c, hipEventCreate(&b));
+        c->sev.push_back(a);
+        c->sev.push_back(b);
+    }
+    out[0] = c->sev[2 * c->nscatter];
+    out[1] = c->sev[2 * c->nscatter + 1];
+    return PGA_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void pga_default_opts(pga_opts *o)
+{
+    o->device = 0;
+    o->freq = 10;
+    o->tile_records = 4096;
+    o->chunk_records = 4096;
+    o->merge_tile = 256;
+    o->build_cum = 1;
+}
+
+const char *pga_last_error(const pga_ctx *c) { return c ? c->err : "no context"; }
+
+int pga_create(const pga_opts *o, pga_ctx **out)
+{
+    *out = nullptr;
+    pga_ctx *c = new (std::nothrow) pga_ctx();
+    if (!c) return PGA_ENOMEM;
+    if (o) c->o = *o; else pga_default_opts(&c->o);
+    const pga_opts &q = c->o;
+    if (q.freq < 1 || q.freq > MAX_FREQ ||
+        (q.tile_records != 2048 && q.tile_records != 4096 && q.tile_records != 8192) ||
+        (q.chunk_records != 2048 && q.chunk_records != 4096 && q.chunk_records != 8192) ||
+        (q.merge_tile != 256 && q.merge_tile != 512 && q.merge_tile != 1024)) {
+        delete c;
+        return PGA_EINVAL;
+    }
+    if (hipSetDevice(q.device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        (void) hipGetLastError();
+        delete c;
+        return PGA_EHIP;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return PGA_EHIP; }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tmap), TMAP, sizeof(TMAP)) != hipSuccess) { delete c; return PGA_EHIP; }
+    const size_t big = (size_t) 8192 * sizeof(Rec) + 16;
+    (void) hipFuncSetAttribute((const void *) k_tile_sort<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
+    (void) hipFuncSetAttribute((const void *) k_tile_sort<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
+    (void) hipFuncSetAttribute((const void *) k_tile_sort<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
+    (void) hipGetLastError();
+    *out = c;
+    return PGA_OK;
+}
+
+void pga_destroy(pga_ctx *c)
+{
+    if (!c) return;
+    (void) hipSetDevice(c->o.device);
+    if (c->st) (void) hipStreamSynchronize(c->st);
+    DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
+                   &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
+                   &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs,
+                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck};
+    for (DBuf *b : all)
+        if (b->p) (void) hipFree(b->p);
+    for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
+    for (auto &e : c->sev) (void) hipEventDestroy(e);
+    if (c->st) (void) hipStreamDestroy(c->st);
+    delete c;
+}
+
+int pga_sync(pga_ctx *c)
+{
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int64_t *bps_bytes,
+                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
+{
+    if (!c || npairs <= 0) return c ? fail(c, PGA_EINVAL, "npairs must be > 0") : PGA_EINVAL;
+    HIPCHK(c, hipSetDevice(c->o.device));
+    c->loaded = c->built = c->merged = false;
+    c->npairs = npairs;
+    c->hp.assign(npairs, HostPair());
+    int64_t total_bytes = 0;
+    std::vector<int64_t> byte0(npairs);
+    int nctg = 0;
+    int64_t maxctg = 0, maxn = 1;
+    for (int p = 0; p < npairs; p++) {
+        if (ncontig[p] < 0 || bps_bytes[p] < 0) return fail(c, PGA_EINVAL, "pair %d: bad sizes", p);
+        byte0[p] = total_bytes;
+        total_bytes += (bps_bytes[p] + 7) & ~(int64_t) 7;
+        HostPair &h = c->hp[p];
+        h.ncontig = ncontig[p];
+        h.ctg0 = nctg;
+        nctg += ncontig[p];
+        for (int i = 0; i < ncontig[p]; i++) {
+            if (clen[p][i] < 0 || clen[p][i] > INT32_MAX)
+                return fail(c, PGA_EOVERFLOW, "pair %d contig %d: length %lld", p, i, (long long) clen[p][i]);
+            if (boff[p][i] < 0 || boff[p][i] + (clen[p][i] + 3) / 4 > bps_bytes[p])
+                return fail(c, PGA_EINVAL, "pair %d contig %d: sequence outside the .bps image", p, i);
+            h.seqtot += clen[p][i];
+            h.maxctg = std::max<int64_t>(h.maxctg, clen[p][i]);
+        }
+        maxctg = std::max(maxctg, h.maxctg);
+        maxn = std::max<int64_t>(maxn, ncontig[p]);
+        int Pb = bytes_for(h.maxctg), Cb = bytes_for(2 * (int64_t) h.ncontig - 1);
+        if (Pb + Cb > 7) return fail(c, PGA_EOVERFLOW, "pair %d: payload of %d bytes exceeds 7", p, Pb + Cb);
+    }
+    c->jbits = bits_for(maxctg);
+    int cbits = bits_for(maxn - 1);
+    if (c->jbits + cbits + 1 > 48)
+        return fail(c, PGA_EOVERFLOW, "offset (%d bits) + contig (%d bits) + strand exceed 48 bits", c->jbits, cbits);
+
+    // contig table over the flat candidate-position space (pairs block-aligned)
+    std::vector<ContigDev> tab(std::max(nctg, 1));
+    int64_t pos = 0;
+    for (int p = 0; p < npairs; p++) {
+        HostPair &h = c->hp[p];
+        pos = (pos + XBLOCK - 1) / XBLOCK * XBLOCK;
+        h.blk0 = pos / XBLOCK;
+        int64_t start = pos;
+        for (int i = 0; i < h.ncontig; i++) {
+            ContigDev &d = tab[h.ctg0 + i];
+            d.bitoff = 8 * (byte0[p] + boff[p][i]);
+            d.pos0 = pos;
+            d.ncand = clen[p][i] >= 12 ? clen[p][i] - 11 : 0;
+            d.clen = (int32_t) clen[p][i];
+            d.local = i;
+            d.pair = p;
+            d.pad = 0;
+            pos += d.ncand;
+        }
+        h.nblk = (pos - start + XBLOCK - 1) / XBLOCK;
+    }
+    c->nctg = nctg;
+    c->F = pos;
+    c->nblk = (pos + XBLOCK - 1) / XBLOCK;
+
+    TRY(ensure(c, c->bps, (size_t) total_bytes + 64));
+    HIPCHK(c, hipMemsetAsync(c->bps.p, 0, (size_t) total_bytes + 64, c->st));
+    for (int p = 0; p < npairs; p++)
+        if (bps_bytes[p] > 0)
+            HIPCHK(c, hipMemcpyAsync(P<uint8_t>(c->bps) + byte0[p], bps[p], (size_t) bps_bytes[p],
+                                     hipMemcpyHostToDevice, c->st));
+    TRY(ensure(c, c->ctg, tab.size() * sizeof(ContigDev)));
+    HIPCHK(c, hipMemcpyAsync(c->ctg.p, tab.data(), tab.size() * sizeof(ContigDev), hipMemcpyHostToDevice, c->st));
+    std::vector<int64_t> pblk(npairs + 1);
+    for (int p = 0; p < npairs; p++) pblk[p] = c->hp[p].blk0;
+    pblk[npairs] = c->nblk;
+    TRY(ensure(c, c->pblk, pblk.size() * sizeof(int64_t)));
+    HIPCHK(c, hipMemcpyAsync(c->pblk.p, pblk.data(), pblk.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));     // host arrays above are stack/vector owned
+    c->loaded = true;
+    return PGA_OK;
+}
+
+int pga_load_gdb(pga_ctx *c, const uint8_t *bps, int64_t bps_bytes, const int64_t *boff,
+                 const int64_t *clen, int ncontig)
+{
+    return pga_load_batch(c, 1, &bps, &bps_bytes, &boff, &clen, &ncontig);
+}
+
+int pga_build_index(pga_ctx *c)
+{
+    if (!c || !c->loaded) return c ? fail(c, PGA_EINVAL, "no GDB loaded") : PGA_EINVAL;
+    HIPCHK(c, hipSetDevice(c->o.device));
+    c->built = c->merged = false;
+    c->nscatter = 0;
+    c->scatter_bytes = 0;
+    memset(&c->tm, 0, sizeof(c->tm));
+    const int npairs = c->npairs;
+    const int CAP = cap_of(c);
+    const int per = chunk_per(c);
+    const int CH = per * 256;
+
+    TRY(record(c, 0));
+    // ---- extraction: count, scan, pair record ranges
+    const int64_t nblk = c->nblk;
+    TRY(ensure(c, c->blkcnt, (size_t) std::max<int64_t>(nblk, 1) * sizeof(uint32_t)));
+    TRY(ensure(c, c->blkoff, (size_t) (nblk + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_extract_count, dim3((unsigned) nblk), dim3(256), 0, c->st,
+                           P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, P<uint32_t>(c->blkcnt));
+        LAUNCHCHK(c);
+        TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
+    } else {
+        TRY(fill_u64(c, c->blkoff.p, 1, 0));
+    }
+    hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
+                       (const int64_t *) P<int64_t>(c->blkoff), (const int64_t *) P<int64_t>(c->pblk), npairs,
+                       P<int64_t>(c->pr));
+    LAUNCHCHK(c);
+    std::vector<int64_t> pr(npairs + 1);
+    HIPCHK(c, hipMemcpyAsync(pr.data(), c->pr.p, pr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const int64_t R = pr[npairs];
+    c->R = R;
+
+    // ---- per-pair layout
+    c->pairs.assign(npairs, PairDev());
+    int64_t ent = 0;
+    for (int p = 0; p < npairs; p++) {
+        PairDev &d = c->pairs[p];
+        const HostPair &h = c->hp[p];
+        d.rec0 = pr[p];
+        d.nrec = pr[p + 1] - pr[p];
+        d.P = bytes_for(h.maxctg);
+        d.C = bytes_for(2 * (int64_t) h.ncontig - 1);
+        d.e = 9 + d.P + d.C;
+        d.s = 1 + 2 * (d.P + d.C);
+        d.ncontig = h.ncontig;
+        d.ent_off = ent;
+        ent += d.nrec * d.e;
+        d.cum_off = (int64_t) p * NPRE;
+        d.buck_off = h.ctg0;
+    }
+    const size_t nrec_alloc = (size_t) std::max<int64_t>(R, 1);
+    TRY(ensure(c, c->rec[0], nrec_alloc * sizeof(Rec)));
+    TRY(ensure(c, c->rec[1], nrec_alloc * sizeof(Rec)));
+    TRY(ensure(c, c->bufs, 2 * sizeof(Rec *)));
+    {
+        Rec *bp[2] = {P<Rec>(c->rec[0]), P<Rec>(c->rec[1])};
+        HIPCHK(c, hipMemcpyAsync(c->bufs.p, bp, sizeof(bp), hipMemcpyHostToDevice, c->st));
+    }
+    TRY(ensure(c, c->entries, (size_t) ent + 32));
+    TRY(ensure(c, c->ls, nrec_alloc + 64));
+    TRY(ensure(c, c->pre, (nrec_alloc + 1) * sizeof(uint32_t)));
+    TRY(ensure(c, c->hist1, (size_t) npairs * 256 * sizeof(uint64_t)));
+    TRY(ensure(c, c->dpairs, (size_t) npairs * sizeof(PairDev)));
+    TRY(ensure(c, c->counters, 16 * sizeof(int)));
+    const int64_t seg_cap = R / CAP + npairs + 16;
+    const int64_t tile_cap = 34 * (R / CAP) + npairs + 1024;
+    TRY(ensure(c, c->segs[0], (size_t) seg_cap * sizeof(Seg)));
+    TRY(ensure(c, c->segs[1], (size_t) seg_cap * sizeof(Seg)));
+    TRY(ensure(c, c->seghist, (size_t) seg_cap * 256 * sizeof(uint64_t)));
+    TRY(ensure(c, c->cursor, (size_t) seg_cap * 256 * sizeof(uint64_t)));
+    TRY(ensure(c, c->chk0, (size_t) (seg_cap + 2) * sizeof(int64_t)));
+    TRY(ensure(c, c->tiles, (size_t) tile_cap * sizeof(Tile)));
+    HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev),
+                             hipMemcpyHostToDevice, c->st));
+    TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
+
+    // ---- extraction: emit records in extraction order + level-1 digit counts
+    if (nblk > 0 && R > 0) {
+        hipLaunchKernelGGL(k_extract_emit, dim3((unsigned) nblk), dim3(256), 0, c->st,
+                           P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, c->jbits,
+                           (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
+                           P<unsigned long long>(c->hist1));
+        LAUNCHCHK(c);
+    }
+    TRY(record(c, 1));
+
+    // ---- MSD radix partition
+    std::vector<Seg> seg0;
+    std::vector<Tile> tile0;
+    for (int p = 0; p < npairs; p++) {
+        const PairDev &d = c->pairs[p];
+        if (d.nrec > CAP) seg0.push_back(Seg{d.rec0, d.nrec, p, 1});
+        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 0, 0});
+    }
+    int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles, [2] long fills
+    int hcount[4] = {0, (int) tile0.size(), 0, 0};
+    HIPCHK(c, hipMemcpyAsync(dcount, hcount, sizeof(hcount), hipMemcpyHostToDevice, c->st));
+    if (!tile0.empty())
+        HIPCHK(c, hipMemcpyAsync(c->tiles.p, tile0.data(), tile0.size() * sizeof(Tile), hipMemcpyHostToDevice, c->st));
+    int nseg = (int) seg0.size();
+    if (nseg)
+        HIPCHK(c, hipMemcpyAsync(c->segs[0].p, seg0.data(), seg0.size() * sizeof(Seg), hipMemcpyHostToDevice, c->st));
+    int cur = 0, level = 1;
+    while (nseg > 0) {
+        if (level > MAX_LEVELS) return fail(c, PGA_EINVAL, "radix partition did not converge");
+        hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
+                           nseg, CH, P<int64_t>(c->chk0));
+        LAUNCHCHK(c);
+        int64_t ck[2] = {0, 0};
+        HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        const int64_t nchunks = ck[0];
+        c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * ck[1];   // each record read and written once
+        const Rec *src = P<Rec>(c->rec[(level - 1) & 1]);
+        Rec *dst = P<Rec>(c->rec[level & 1]);
+        if (level > 1) {
+            TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * 256, 0));
+            TRY(launch_seg_hist(c, per, (unsigned) nchunks, src, P<Seg>(c->segs[cur]), nseg,
+                                P<int64_t>(c->chk0), P<unsigned long long>(c->seghist)));
+        }
+        HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(int), c->st));
+        hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
+                           (const unsigned long long *) (level == 1 ? P<unsigned long long>(c->hist1)
+                                                                     : P<unsigned long long>(c->seghist)),
+                           level == 1 ? 1 : 0, CAP, level & 1, P<unsigned long long>(c->cursor),
+                           P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
+                           (int) tile_cap);
+        LAUNCHCHK(c);
+        hipEvent_t se[2];
+        TRY(scatter_event(c, se));
+        HIPCHK(c, hipEventRecord(se[0], c->st));
+        TRY(launch_seg_scatter(c, per, (unsigned) nchunks, src, dst, P<Seg>(c->segs[cur]), nseg,
+                               P<int64_t>(c->chk0), P<unsigned long long>(c->cursor)));
+        HIPCHK(c, hipEventRecord(se[1], c->st));
+        c->nscatter++;
+        HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        nseg = hcount[0];
+        cur ^= 1;
+        level++;
+        if (nseg > seg_cap) return fail(c, PGA_EOVERFLOW, "segment list overflow");
+        if (hcount[1] > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
+    }
+    c->tm.partition_levels = level - 1;
+    c->tm.partition_launches = c->nscatter;
+    HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ntiles = hcount[1];
+    if (c->ntiles > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
+    TRY(record(c, 2));
+
+    // ---- per-tile LDS sort, LCP, GIX entries
+    if (c->ntiles > 0) TRY(launch_tile_sort(c, (unsigned) c->ntiles));
+    TRY(record(c, 3));
+    if (c->ntiles > 0) {
+        hipLaunchKernelGGL(k_tile_fix, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->st,
+                           (const Tile *) P<Tile>(c->tiles), c->ntiles, (const PairDev *) P<PairDev>(c->dpairs),
+                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), (const uint32_t *) P<uint32_t>(c->pre));
+        LAUNCHCHK(c);
+    }
+
+    // ---- 2^24 prefix table and largest prefix group
+    TRY(ensure(c, c->maxpre, (size_t) npairs * sizeof(uint64_t)));
+    TRY(fill_u64(c, c->maxpre.p, npairs, 0));
+    if (c->o.build_cum) {
+        TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
+        TRY(ensure(c, c->longs, (size_t) npairs * 65536 * sizeof(LongFill)));
+        for (int p = 0; p < npairs; p++)
+            if (c->pairs[p].nrec == 0) TRY(fill_u64(c, P<int64_t>(c->cum) + (int64_t) p * NPRE, NPRE, 0));
+        HIPCHK(c, hipMemsetAsync(dcount + 2, 0, sizeof(int), c->st));
+        if (R > 0) {
+            hipLaunchKernelGGL(k_cum_fill, dim3((unsigned) ((R + 255) / 256)), dim3(256), 0, c->st,
+                               (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
+                               P<int64_t>(c->cum), P<LongFill>(c->longs), dcount + 2);
+            LAUNCHCHK(c);
+            int nlong = 0;
+            HIPCHK(c, hipMemcpyAsync(&nlong, dcount + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            if (nlong > 0) {
+                hipLaunchKernelGGL(k_long_fill, dim3(nlong), dim3(256), 0, c->st, (const LongFill *) P<LongFill>(c->longs));
+                LAUNCHCHK(c);
+            }
+            hipLaunchKernelGGL(k_maxpre, dim3((unsigned) ((R + 255) / 256)), dim3(256), 0, c->st,
+                               (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
+                               (const int64_t *) P<int64_t>(c->cum), P<unsigned long long>(c->maxpre));
+            LAUNCHCHK(c);
+        }
+    }
+    TRY(record(c, 4));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+
+    c->tm.extract_ms = ev_ms(c->ev[0], c->ev[1]);
+    c->tm.partition_ms = ev_ms(c->ev[1], c->ev[2]);
+    c->tm.tile_sort_ms = ev_ms(c->ev[2], c->ev[3]);
+    c->tm.table_ms = ev_ms(c->ev[3], c->ev[4]);
+    c->tm.total_ms = ev_ms(c->ev[0], c->ev[4]);
+    c->tm.tiles = c->ntiles;
+    double sc = 0;
+    for (int i = 0; i < c->nscatter; i++) sc += ev_ms(c->sev[2 * i], c->sev[2 * i + 1]);
+    c->tm.scatter_ms = sc;
+    c->tm.scatter_launches = c->nscatter;
+    c->tm.scatter_bytes = c->scatter_bytes;
+    c->built = true;
+    return PGA_OK;
+}
+
+int pga_index_info(pga_ctx *c, int pair, pga_index_meta *info)
+{
+    if (!c || !c->built || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no index for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    uint64_t mp = 0;
+    HIPCHK(c, hipMemcpyAsync(&mp, P<uint64_t>(c->maxpre) + pair, sizeof(mp), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    info->nrec = d.nrec;
+    info->P = d.P;
+    info->C = d.C;
+    info->e = d.e;
+    info->s = d.s;
+    info->maxpre = c->o.build_cum ? (int64_t) mp : -1;
+    info->ncontig = d.ncontig;
+    info->seqtot = c->hp[pair].seqtot;
+    return PGA_OK;
+}
+
+int pga_index_export(pga_ctx *c, int pair, uint8_t *entries, int64_t *cum)
+{
+    if (!c || !c->built || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no index for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    if (entries && d.nrec > 0)
+        HIPCHK(c, hipMemcpyAsync(entries, P<uint8_t>(c->entries) + d.ent_off, (size_t) (d.nrec * d.e),
+                                 hipMemcpyDeviceToHost, c->st));
+    if (cum) {
+        if (!c->o.build_cum) return fail(c, PGA_EINVAL, "prefix table not built (opts.build_cum = 0)");
+        HIPCHK(c, hipMemcpyAsync(cum, P<int64_t>(c->cum) + d.cum_off, (size_t) NPRE * sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+int pga_merge(pga_ctx *c)
+{
+    if (!c || !c->built) return c ? fail(c, PGA_EINVAL, "pga_merge before pga_build_index") : PGA_EINVAL;
+    HIPCHK(c, hipSetDevice(c->o.device));
+    const int npairs = c->npairs;
+    const int MT = c->o.merge_tile;
+    std::vector<int64_t> mblk(npairs + 1);
+    int64_t nmb = 0;
+    for (int p = 0; p < npairs; p++) {
+        c->pairs[p].mblk0 = nmb;
+        mblk[p] = nmb;
+        nmb += (c->pairs[p].nrec + MT - 1) / MT;
+    }
+    mblk[npairs] = nmb;
+    TRY(record(c, 5));
+    TRY(ensure(c, c->mblk, mblk.size() * sizeof(int64_t)));
+    TRY(ensure(c, c->mcntN, (size_t) std::max<int64_t>(nmb, 1) * sizeof(uint32_t)));
+    TRY(ensure(c, c->mcntC, (size_t) std::max<int64_t>(nmb, 1) * sizeof(uint32_t)));
+    TRY(ensure(c, c->mofsN, (size_t) (nmb + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->mofsC, (size_t) (nmb + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->pofs, (size_t) 2 * (npairs + 1) * sizeof(int64_t)));
+    HIPCHK(c, hipMemcpyAsync(c->mblk.p, mblk.data(), mblk.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
+    std::vector<int64_t> po(2 * (npairs + 1), 0);
+    if (nmb > 0) {
+        switch (MT) {
+        case 512: hipLaunchKernelGGL(k_merge_count<512>, dim3((unsigned) nmb), dim3(512), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC)); break;
+        case 1024: hipLaunchKernelGGL(k_merge_count<1024>, dim3((unsigned) nmb), dim3(1024), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC)); break;
+        default: hipLaunchKernelGGL(k_merge_count<256>, dim3((unsigned) nmb), dim3(256), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC)); break;
+        }
+        LAUNCHCHK(c);
+        TRY(scan_u32(c, P<uint32_t>(c->mcntN), nmb, P<int64_t>(c->mofsN)));
+        TRY(scan_u32(c, P<uint32_t>(c->mcntC), nmb, P<int64_t>(c->mofsC)));
+        hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
+                           (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mblk), npairs,
+                           P<int64_t>(c->pofs));
+        hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
+                           (const int64_t *) P<int64_t>(c->mofsC), (const int64_t *) P<int64_t>(c->mblk), npairs,
+                           P<int64_t>(c->pofs) + (npairs + 1));
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(po.data(), c->pofs.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    c->nseeds.assign(npairs, 0);
+    c->cseeds.assign(npairs, 0);
+    int64_t nbytes = 0, cbytes = 0;
+    for (int p = 0; p < npairs; p++) {
+        PairDev &d = c->pairs[p];
+        c->nseeds[p] = po[p + 1] - po[p];
+        c->cseeds[p] = po[npairs + 1 + p + 1] - po[npairs + 1 + p];
+        d.nseed_off = nbytes;
+        d.cseed_off = cbytes;
+        nbytes += c->nseeds[p] * d.s;
+        cbytes += c->cseeds[p] * d.s;
+    }
+    TRY(ensure(c, c->seedN, (size_t) nbytes + 16));
+    TRY(ensure(c, c->seedC, (size_t) cbytes + 16));
+    TRY(ensure(c, c->nbuck, (size_t) std::max(c->nctg, 1) * sizeof(uint64_t)));
+    TRY(ensure(c, c->cbuck, (size_t) std::max(c->nctg, 1) * sizeof(uint64_t)));
+    TRY(fill_u64(c, c->nbuck.p, c->nctg, 0));
+    TRY(fill_u64(c, c->cbuck.p, c->nctg, 0));
+    HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
+    if (nmb > 0) {
+#define PGA_EMIT(MTV)                                                                                   \
+    hipLaunchKernelGGL(k_merge_emit<MTV>, dim3((unsigned) nmb), dim3(MTV), 0, c->st,                    \
+                       (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
+                       (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
+                       (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
+                       P<uint8_t>(c->seedN), P<uint8_t>(c->seedC), P<unsigned long long>(c->nbuck),     \
+                       P<unsigned long long>(c->cbuck))
+        switch (MT) {
+        case 512: PGA_EMIT(512); break;
+        case 1024: PGA_EMIT(1024); break;
+        default: PGA_EMIT(256); break;
+        }
+#undef PGA_EMIT
+        LAUNCHCHK(c);
+    }
+    TRY(record(c, 6));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->tm.merge_ms = ev_ms(c->ev[5], c->ev[6]);
+    c->merged = true;
+    return PGA_OK;
+}
+
+int pga_seeds_info(pga_ctx *c, int pair, pga_seeds_meta *info)
+{
+    if (!c || !c->merged || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no seeds for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    info->nlen = c->nseeds[pair] * d.s;
+    info->clen = c->cseeds[pair] * d.s;
+    info->s = d.s;
+    info->ncontig = d.ncontig;
+    return PGA_OK;
+}
+
+int pga_seeds_export(pga_ctx *c, int pair, uint8_t *nseeds, uint8_t *cseeds, int64_t *nbuck, int64_t *cbuck)
+{
+    if (!c || !c->merged || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no seeds for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    if (nseeds && c->nseeds[pair] > 0)
+        HIPCHK(c, hipMemcpyAsync(nseeds, P<uint8_t>(c->seedN) + d.nseed_off, (size_t) (c->nseeds[pair] * d.s),
+                                 hipMemcpyDeviceToHost, c->st));
+    if (cseeds && c->cseeds[pair] > 0)
+        HIPCHK(c, hipMemcpyAsync(cseeds, P<uint8_t>(c->seedC) + d.cseed_off, (size_t) (c->cseeds[pair] * d.s),
+                                 hipMemcpyDeviceToHost, c->st));
+    if (nbuck && d.ncontig > 0)
+        HIPCHK(c, hipMemcpyAsync(nbuck, P<uint64_t>(c->nbuck) + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, c->st));
+    if (cbuck && d.ncontig > 0)
+        HIPCHK(c, hipMemcpyAsync(cbuck, P<uint64_t>(c->cbuck) + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+int pga_timings_get(pga_ctx *c, pga_timings *t)
+{
+    if (!c) return PGA_EINVAL;
+    *t = c->tm;
+    return PGA_OK;
+}
+
+} // extern "C"

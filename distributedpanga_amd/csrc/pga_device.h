@@ -1,0 +1,49 @@
+// pga_device.h -- types shared by the kernels and the C-ABI host code.
+#pragma once
+#include <stdint.h>
+
+namespace pga {
+
+constexpr int MAX_FREQ = 64;     // largest frequency cut the merge window supports
+
+struct Rec {                     // 16-byte device record (see pga_kernels.hip)
+    uint64_t w0, w1;
+};
+
+struct ContigDev {
+    int64_t bitoff;              // bit offset of base 0 in the device .bps image
+    int64_t pos0;                // first flat candidate position of the contig
+    int64_t ncand;               // candidate offsets: max(0, clen - 11)
+    int32_t clen;
+    int32_t local;               // contig id inside its GDB
+    int32_t pair;
+    int32_t pad;
+};
+
+struct PairDev {
+    int64_t rec0, nrec;          // the pair's records in the global record space
+    int64_t ent_off;             // byte offset of its GIX entries
+    int64_t cum_off;             // element offset of its prefix table
+    int64_t buck_off;            // offset of its contigs in the buck vectors
+    int64_t nseed_off, cseed_off;// byte offsets of its seed streams
+    int64_t mblk0;               // first merge block
+    int32_t P, C, e, s;
+    int32_t ncontig, pad;
+};
+
+struct Seg {                     // a key range still to be partitioned
+    int64_t start, count;
+    int32_t pair, level;         // level = digit (1..16) to split on next
+};
+
+struct Tile {                    // a bucket run small enough for one workgroup sort
+    int64_t start;
+    int32_t count, pair, buf, pad;
+};
+
+struct LongFill {
+    int64_t *cum;
+    int64_t from, to, v;
+};
+
+} // namespace pga

@@ -1,0 +1,927 @@
+// pga_kernels.hip -- CDNA4 (gfx950) kernels of the index build and adaptamer merge.
+//
+// Device record (16 B, one per emitted k-mer occurrence):
+//   w0 = k-mer bases 0..31, MSB-first (base 0 in bits 63..62)
+//   w1 = bases 32..39 in bits 63..48 | payload in bits 47..0,
+//        payload = contig << (jbits+1) | j << 1 | strand
+// where j is the syncmer offset (the stored position is j + 12*strand,
+// src/extract.c:200-228).  Comparing (w0, w1) as one 128-bit number orders by
+// k-mer, then contig, offset and forward-before-reverse: the extraction order,
+// which is the canonical tie order for equal k-mers.  Keys are unique, so any
+// correct sort gives the same array; no stage needs to be stable.
+//
+// Pipeline: syncmer scan (count, scan, emit) -> MSD radix partition passes of
+// 8 bits until every bucket fits one workgroup -> per-bucket LDS merge sort
+// that also writes LCPs and GIX entries -> prefix table -> merge.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pga_device.h"
+
+namespace pga {
+
+__constant__ uint8_t c_tmap[256];
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint64_t field_rev64(uint64_t x)
+{
+    uint64_t y = __builtin_bitreverse64(x);
+    return ((y >> 1) & 0x5555555555555555ull) | ((y & 0x5555555555555555ull) << 1);
+}
+
+__device__ __forceinline__ uint32_t field_rev16(uint32_t x)
+{
+    uint32_t y = __builtin_bitreverse32(x) >> 16;
+    return ((y >> 1) & 0x5555u) | ((y & 0x5555u) << 1);
+}
+
+// 80 bits (40 bases, LSB-first as in .bps) starting at bit offset `bit`.
+__device__ __forceinline__ void window80(const uint64_t *__restrict__ b, int64_t bit,
+                                         uint64_t &lo, uint32_t &hi)
+{
+    int64_t w = bit >> 6;
+    int s = (int) (bit & 63);
+    uint64_t x = b[w], y = b[w + 1];
+    if (s) {
+        uint64_t z = b[w + 2];
+        lo = (x >> s) | (y << (64 - s));
+        hi = (uint32_t) (((y >> s) | (z << (64 - s))) & 0xFFFFu);
+    } else {
+        lo = x;
+        hi = (uint32_t) (y & 0xFFFFu);
+    }
+}
+
+// Closed (12,8)-syncmer test on the 12 bases in bits 0..23 of W (LSB-first).
+// tl[x] = TMap[field-reversed x], tm[x] = TMap[x]; the reverse-complement
+// 4-mer of an LSB-first byte x, packed MSB-first, is ~x.
+__device__ __forceinline__ bool syncmer(uint64_t W, const uint32_t *tl, const uint32_t *tm)
+{
+    uint32_t h[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        uint32_t lo = (uint32_t) (W >> (2 * t)) & 0xFFu;
+        uint32_t hi = (uint32_t) (W >> (2 * t + 8)) & 0xFFu;
+        uint32_t fw = (tl[lo] << 8) | tl[hi];
+        uint32_t rv = (tm[hi ^ 0xFFu] << 8) | tm[lo ^ 0xFFu];
+        h[t] = fw < rv ? fw : rv;
+    }
+    uint32_t m = min(min(min(h[0], h[1]), min(h[2], h[3])), h[4]);
+    return h[0] == m || h[4] == m;
+}
+
+__device__ __forceinline__ int locate_contig(const ContigDev *__restrict__ ctg, int nctg, int64_t f)
+{
+    int lo = 0, hi = nctg;                      // ctg[nctg].pos0 is the end sentinel
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (ctg[mid].pos0 <= f) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void load_tables(uint32_t *tl, uint32_t *tm)
+{
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        uint32_t r = ((t & 3) << 6) | (((t >> 2) & 3) << 4) | (((t >> 4) & 3) << 2) | ((t >> 6) & 3);
+        tl[t] = c_tmap[r];
+        tm[t] = c_tmap[t];
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t *red)
+{
+    // wave64 reduction through LDS
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    uint64_t s = 0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NT / 64; k++) s += red[k];
+        red[NT / 64] = s;
+    }
+    __syncthreads();
+    s = red[NT / 64];
+    __syncthreads();
+    return s;
+}
+
+// Exclusive scan of one value per thread over the block (NT threads); returns
+// the thread's exclusive prefix, *total gets the block total.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *sh, T *total)
+{
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T run = 0;
+        for (int k = 0; k < NT / 64; k++) { T t = sh[k]; sh[k] = run; run += t; }
+        sh[NT / 64] = run;
+    }
+    __syncthreads();
+    T r = x - v + sh[wv];
+    *total = sh[NT / 64];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------- extraction
+
+// Positions per thread and threads per block of the syncmer scan.  Flat
+// position f enumerates every offset j in [0, clen-12] of every contig, pair
+// after pair; each pair's range starts on a block boundary.
+constexpr int XT = 256;
+constexpr int XU = 4;
+
+struct XEmit {
+    uint64_t f0, f1, r0, r1;     // forward and reverse-complement record keys
+    bool fwd, rev;
+};
+
+// Syncmer test at offset j of contig c; with keys, also the records' keys.
+__device__ __forceinline__ int scan_position(const uint64_t *__restrict__ bps64,
+                                             const ContigDev &c, int64_t j, int jbits,
+                                             const uint32_t *tl, const uint32_t *tm,
+                                             bool keys, XEmit &out)
+{
+    uint64_t W0;
+    uint32_t W1;
+    out.fwd = out.rev = false;
+    window80(bps64, c.bitoff + 2 * j, W0, W1);
+    if (!syncmer(W0, tl, tm)) return 0;
+    out.fwd = j <= (int64_t) c.clen - 40;          // forward 40-mer fits
+    out.rev = j >= 28;                             // reverse complement of [j-28, j+12) fits
+    if (keys) {
+        const uint64_t pay = ((uint64_t) c.local << (jbits + 1)) | ((uint64_t) j << 1);
+        // forward: MSB-first packing of [j, j+40) = field-reversed LSB-first window
+        out.f0 = field_rev64(W0);
+        out.f1 = ((uint64_t) field_rev16(W1) << 48) | pay;
+        if (out.rev) {
+            // reverse complement: the MSB-first RC 40-mer is the bitwise NOT of
+            // the LSB-first window of [j-28, j+12)
+            uint64_t V0;
+            uint32_t V1;
+            window80(bps64, c.bitoff + 2 * (j - 28), V0, V1);
+            out.r0 = ~(((uint64_t) V1 << 48) | (V0 >> 16));
+            out.r1 = ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull;
+        }
+    }
+    return (int) out.fwd + (int) out.rev;
+}
+
+__global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
+                                                      const ContigDev *__restrict__ ctg, int nctg,
+                                                      int64_t F, uint32_t *__restrict__ blkcnt)
+{
+    __shared__ uint32_t tl[256], tm[256];
+    __shared__ uint64_t red[XT / 64 + 1];
+    load_tables(tl, tm);
+    __syncthreads();
+    int64_t f0 = ((int64_t) blockIdx.x * XT + threadIdx.x) * XU;
+    uint32_t cnt = 0;
+    if (f0 < F) {
+        int c = locate_contig(ctg, nctg, f0);
+        XEmit dummy;
+#pragma unroll
+        for (int u = 0; u < XU; u++) {
+            int64_t f = f0 + u;
+            if (f >= F) break;
+            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+            const ContigDev cc = ctg[c];
+            if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
+            cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, dummy);
+        }
+    }
+    uint64_t tot = block_sum_u64<XT>(cnt, red);
+    if (threadIdx.x == 0) blkcnt[blockIdx.x] = (uint32_t) tot;
+}
+
+// Emit records in extraction order at blkoff[block] and count the first
+// (level-1) radix digit per pair.  Every block lies inside one pair.
+__global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict__ bps64,
+                                                     const ContigDev *__restrict__ ctg, int nctg,
+                                                     int64_t F, int jbits,
+                                                     const int64_t *__restrict__ blkoff,
+                                                     Rec *__restrict__ out,
+                                                     unsigned long long *__restrict__ hist1)
+{
+    __shared__ uint32_t tl[256], tm[256];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t scan_sh[XT / 64 + 1];
+    load_tables(tl, tm);
+    for (int t = threadIdx.x; t < 256; t += XT) hist[t] = 0;
+    __syncthreads();
+    const int64_t fb = (int64_t) blockIdx.x * XT * XU;
+    const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
+    uint64_t k0[2 * XU], k1[2 * XU];
+    uint32_t valid = 0;
+    int pair0 = fb < F ? ctg[locate_contig(ctg, nctg, fb)].pair : -1;
+    if (f0 < F) {
+        int c = locate_contig(ctg, nctg, f0);
+#pragma unroll
+        for (int u = 0; u < XU; u++) {
+            int64_t f = f0 + u;
+            if (f >= F) break;
+            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+            const ContigDev cc = ctg[c];
+            if (f >= cc.pos0 + cc.ncand) continue;
+            XEmit e;
+            scan_position(bps64, cc, f - cc.pos0, jbits, tl, tm, true, e);
+            if (e.fwd) {
+                k0[2 * u] = e.f0; k1[2 * u] = e.f1; valid |= 1u << (2 * u);
+                uint32_t d = (uint32_t) (e.f0 >> 56);
+                if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
+                else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
+            }
+            if (e.rev) {
+                k0[2 * u + 1] = e.r0; k1[2 * u + 1] = e.r1; valid |= 1u << (2 * u + 1);
+                uint32_t d = (uint32_t) (e.r0 >> 56);
+                if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
+                else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
+            }
+        }
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan<XT, uint32_t>((uint32_t) __builtin_popcount(valid), scan_sh, &total);
+    Rec *dst = out + blkoff[blockIdx.x] + off;
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < 2 * XU; s++) {
+        if ((valid >> s) & 1u) {
+            Rec r;
+            r.w0 = k0[s];
+            r.w1 = k1[s];
+            dst[k++] = r;
+        }
+    }
+    __syncthreads();
+    if (pair0 >= 0)
+        for (int t = threadIdx.x; t < 256; t += XT)
+            if (hist[t]) atomicAdd(&hist1[(size_t) pair0 * 256 + t], (unsigned long long) hist[t]);
+}
+
+// ---------------------------------------------------------------- scans
+
+constexpr int SCAN_T = 256;
+constexpr int SCAN_PER = 16;
+constexpr int SCAN_SPAN = SCAN_T * SCAN_PER;
+
+template <typename TIn>
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const TIn *__restrict__ in, int64_t n,
+                                                        int64_t *__restrict__ bsum)
+{
+    __shared__ uint64_t red[SCAN_T / 64 + 1];
+    int64_t base = (int64_t) blockIdx.x * SCAN_SPAN;
+    uint64_t s = 0;
+    for (int k = 0; k < SCAN_PER; k++) {
+        int64_t i = base + (int64_t) k * SCAN_T + threadIdx.x;
+        if (i < n) s += (uint64_t) in[i];
+    }
+    s = block_sum_u64<SCAN_T>(s, red);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (int64_t) s;
+}
+
+// In-place exclusive scan of bsum[0..nb), total into bsum[nb].  One block.
+__global__ __launch_bounds__(1024) void k_scan_top(int64_t *__restrict__ bsum, int64_t nb)
+{
+    __shared__ int64_t sh[1024 / 64 + 1];
+    int64_t per = (nb + 1023) / 1024;
+    int64_t b0 = threadIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
+    int64_t s = 0;
+    for (int64_t i = b0; i < b1; i++) s += bsum[i];
+    int64_t tot;
+    int64_t run = block_excl_scan<1024, int64_t>(s, sh, &tot);
+    for (int64_t i = b0; i < b1; i++) { int64_t t = bsum[i]; bsum[i] = run; run += t; }
+    if (threadIdx.x == 0) bsum[nb] = tot;
+}
+
+template <typename TIn>
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(const TIn *__restrict__ in, int64_t n,
+                                                      const int64_t *__restrict__ bsum,
+                                                      int64_t *__restrict__ out)
+{
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    int64_t base = (int64_t) blockIdx.x * SCAN_SPAN + (int64_t) threadIdx.x * SCAN_PER;
+    int64_t v[SCAN_PER];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        int64_t i = base + k;
+        v[k] = i < n ? (int64_t) in[i] : 0;
+        s += v[k];
+    }
+    int64_t tot;
+    int64_t run = block_excl_scan<SCAN_T, int64_t>(s, sh, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        int64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = bsum[gridDim.x];
+}
+
+template __global__ void k_scan_reduce<uint32_t>(const uint32_t *, int64_t, int64_t *);
+template __global__ void k_scan_down<uint32_t>(const uint32_t *, int64_t, const int64_t *, int64_t *);
+
+// ---------------------------------------------------------------- small utilities
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n, unsigned long long v)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Record ranges of the pairs from the extraction block offsets.
+__global__ void k_pair_ranges(const int64_t *__restrict__ blkoff, const int64_t *__restrict__ pblk,
+                              int npairs, int64_t *__restrict__ out)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p <= npairs) out[p] = blkoff[pblk[p]];
+}
+
+// ---------------------------------------------------------------- radix partition
+
+__device__ __forceinline__ uint32_t digit_at(uint64_t w0, uint64_t w1, int level)
+{
+    return level <= 8 ? (uint32_t) (w0 >> (64 - 8 * level)) & 0xFFu
+                      : (uint32_t) (w1 >> (128 - 8 * level)) & 0xFFu;
+}
+
+__device__ __forceinline__ int seg_of_chunk(const int64_t *__restrict__ chk0, int nseg, int64_t b)
+{
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (chk0[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Exclusive prefix of per-segment chunk counts (one block); chk0[nseg] = total
+// chunks, chk0[nseg+1] = total records of the segments.
+__global__ __launch_bounds__(1024) void k_chunk_prefix(const Seg *__restrict__ segs, int nseg, int chunk,
+                                                       int64_t *__restrict__ chk0)
+{
+    __shared__ int64_t sh[1024 / 64 + 1];
+    int per = (nseg + 1023) / 1024;
+    int b0 = threadIdx.x * per, b1 = b0 + per < nseg ? b0 + per : nseg;
+    int64_t s = 0, r = 0;
+    for (int i = b0; i < b1; i++) { s += (segs[i].count + chunk - 1) / chunk; r += segs[i].count; }
+    int64_t tot, rtot;
+    int64_t run = block_excl_scan<1024, int64_t>(s, sh, &tot);
+    (void) block_excl_scan<1024, int64_t>(r, sh, &rtot);
+    for (int i = b0; i < b1; i++) { chk0[i] = run; run += (segs[i].count + chunk - 1) / chunk; }
+    if (threadIdx.x == 0) { chk0[nseg] = tot; chk0[nseg + 1] = rtot; }
+}
+
+constexpr int PT = 256;   // threads of the partition kernels
+
+template <int PER>
+__global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ src, const Seg *__restrict__ segs,
+                                                 int nseg, const int64_t *__restrict__ chk0,
+                                                 unsigned long long *__restrict__ seghist)
+{
+    __shared__ uint32_t hist[256];
+    const int CH = PT * PER;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    int s = seg_of_chunk(chk0, nseg, blockIdx.x);
+    const Seg sg = segs[s];
+    int64_t off = (blockIdx.x - chk0[s]) * (int64_t) CH;
+    int n = (int) min((int64_t) CH, sg.count - off);
+    const Rec *p = src + sg.start + off;
+#pragma unroll 4
+    for (int k = 0; k < PER; k++) {
+        int i = k * PT + threadIdx.x;
+        if (i < n) {
+            Rec r = p[i];
+            atomicAdd(&hist[digit_at(r.w0, r.w1, sg.level)], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t h = hist[threadIdx.x];
+    if (h) atomicAdd(&seghist[(size_t) s * 256 + threadIdx.x], (unsigned long long) h);
+}
+
+// Per segment: digit offsets (cursors) and the children.  Children of at most
+// `cap` records are grouped into sort tiles (runs of consecutive non-empty
+// buckets); larger ones become segments of the next level.
+__global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
+                                                  const unsigned long long *__restrict__ seghist,
+                                                  int hist_by_pair, int cap, int dstbuf,
+                                                  unsigned long long *__restrict__ cursor,
+                                                  Seg *__restrict__ next, int *__restrict__ nnext, int next_cap,
+                                                  Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap)
+{
+    __shared__ int64_t sh[256 / 64 + 1];
+    __shared__ int64_t cnt[256], off[256];
+    const Seg sg = segs[blockIdx.x];
+    size_t row = hist_by_pair ? (size_t) sg.pair : (size_t) blockIdx.x;
+    int64_t c = (int64_t) seghist[row * 256 + threadIdx.x];
+    int64_t tot;
+    int64_t o = block_excl_scan<256, int64_t>(c, sh, &tot);
+    cnt[threadIdx.x] = c;
+    off[threadIdx.x] = o;
+    cursor[(size_t) blockIdx.x * 256 + threadIdx.x] = (unsigned long long) (sg.start + o);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t ts = 0, tc = 0;
+        for (int d = 0; d < 256; d++) {
+            int64_t k = cnt[d];
+            if (k == 0) continue;
+            if (k > cap) {
+                if (tc) {
+                    int t = atomicAdd(ntiles, 1);
+                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf};
+                    tc = 0;
+                }
+                int q = atomicAdd(nnext, 1);
+                if (q < next_cap) next[q] = Seg{sg.start + off[d], k, sg.pair, sg.level + 1};
+            } else {
+                if (tc + k > cap) {
+                    int t = atomicAdd(ntiles, 1);
+                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf};
+                    tc = 0;
+                }
+                if (tc == 0) ts = sg.start + off[d];
+                tc += k;
+            }
+        }
+        if (tc) {
+            int t = atomicAdd(ntiles, 1);
+            if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf};
+        }
+    }
+}
+
+template <int PER>
+__global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src, Rec *__restrict__ dst,
+                                                    const Seg *__restrict__ segs, int nseg,
+                                                    const int64_t *__restrict__ chk0,
+                                                    unsigned long long *__restrict__ cursor)
+{
+    constexpr int CH = PT * PER;
+    __shared__ Rec stage[CH];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t loff[256];
+    __shared__ int64_t gbase[256];
+    __shared__ uint32_t sh[PT / 64 + 1];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    int s = seg_of_chunk(chk0, nseg, blockIdx.x);
+    const Seg sg = segs[s];
+    int64_t off = (blockIdx.x - chk0[s]) * (int64_t) CH;
+    int n = (int) min((int64_t) CH, sg.count - off);
+    const Rec *p = src + sg.start + off;
+    Rec r[PER];
+    uint32_t rk[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        int i = k * PT + threadIdx.x;
+        if (i < n) {
+            r[k] = p[i];
+            rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, sg.level)], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t h = hist[threadIdx.x];
+    uint32_t tot;
+    uint32_t lo = block_excl_scan<PT, uint32_t>(h, sh, &tot);
+    loff[threadIdx.x] = lo;
+    if (h) gbase[threadIdx.x] = (int64_t) atomicAdd(&cursor[(size_t) s * 256 + threadIdx.x],
+                                                    (unsigned long long) h);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        int i = k * PT + threadIdx.x;
+        if (i < n) stage[loff[digit_at(r[k].w0, r[k].w1, sg.level)] + rk[k]] = r[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        int i = k * PT + threadIdx.x;
+        if (i < n) {
+            Rec x = stage[i];
+            uint32_t d = digit_at(x.w0, x.w1, sg.level);
+            dst[gbase[d] + (i - (int64_t) loff[d])] = x;
+        }
+    }
+}
+
+template __global__ void k_seg_hist<16>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+template __global__ void k_seg_hist<8>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+template __global__ void k_seg_hist<32>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+template __global__ void k_seg_scatter<16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+template __global__ void k_seg_scatter<8>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+template __global__ void k_seg_scatter<32>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+
+// ---------------------------------------------------------------- tile sort
+
+__device__ __forceinline__ bool key_le(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1)
+{
+    return a0 < b0 || (a0 == b0 && a1 <= b1);
+}
+
+__device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1)
+{
+    uint64_t x = a0 ^ b0;
+    if (x) return __builtin_clzll(x) >> 1;
+    uint32_t y = (uint32_t) ((a1 ^ b1) >> 48);
+    if (y) return 32 + ((__builtin_clz(y) - 16) >> 1);
+    return 40;
+}
+
+#define PGA_CE(i, j)                                                             \
+    do {                                                                          \
+        bool sw_ = (a0[j] < a0[i]) || (a0[j] == a0[i] && a1[j] < a1[i]);          \
+        uint64_t x0_ = sw_ ? a0[j] : a0[i], x1_ = sw_ ? a1[j] : a1[i];            \
+        uint64_t y0_ = sw_ ? a0[i] : a0[j], y1_ = sw_ ? a1[i] : a1[j];            \
+        a0[i] = x0_; a1[i] = x1_; a0[j] = y0_; a1[j] = y1_;                       \
+    } while (0)
+
+// One workgroup sorts one tile (<= T*8 records) with an in-register 8-sort and
+// merge-path merges through LDS, then writes, per record: the GIX entry
+// (src/kmer_adapter.c:55-62 layout), the LCP/strand byte and the 24-bit prefix.
+// The LCP of a tile's first record is written by k_tile_fix.
+template <int T>
+__global__ __launch_bounds__(T) void k_tile_sort(Rec *const *__restrict__ bufs,
+                                                 const Tile *__restrict__ tiles,
+                                                 const PairDev *__restrict__ pairs, int jbits,
+                                                 uint8_t *__restrict__ entries,
+                                                 uint8_t *__restrict__ ls, uint32_t *__restrict__ pre)
+{
+    constexpr int E = 8;
+    constexpr int CAP = T * E;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Rec *sk = reinterpret_cast<Rec *>(smem_raw);
+    const Tile tl = tiles[blockIdx.x];
+    const PairDev pd = pairs[tl.pair];
+    const Rec *src = bufs[tl.buf] + tl.start;
+    const int n = tl.count;
+    const int tid = threadIdx.x;
+
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        int i = k * T + tid;
+        Rec r;
+        if (i < n) r = src[i];
+        else { r.w0 = ~0ull; r.w1 = ~0ull; }
+        sk[i] = r;
+    }
+    __syncthreads();
+    uint64_t a0[E], a1[E];
+#pragma unroll
+    for (int k = 0; k < E; k++) { Rec r = sk[tid * E + k]; a0[k] = r.w0; a1[k] = r.w1; }
+    // optimal 19-comparator network for 8 inputs
+    PGA_CE(0, 2); PGA_CE(1, 3); PGA_CE(4, 6); PGA_CE(5, 7);
+    PGA_CE(0, 4); PGA_CE(1, 5); PGA_CE(2, 6); PGA_CE(3, 7);
+    PGA_CE(0, 1); PGA_CE(2, 3); PGA_CE(4, 5); PGA_CE(6, 7);
+    PGA_CE(2, 4); PGA_CE(3, 5);
+    PGA_CE(1, 4); PGA_CE(3, 6);
+    PGA_CE(1, 2); PGA_CE(3, 4); PGA_CE(5, 6);
+
+    for (int w = E; w < CAP; w <<= 1) {
+        // runs of w are sorted; skip levels where the whole tile is one run already
+        if (n <= w) break;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < E; k++) { Rec r; r.w0 = a0[k]; r.w1 = a1[k]; sk[tid * E + k] = r; }
+        __syncthreads();
+        const int o = tid * E;
+        const int base = o & ~(2 * w - 1);
+        const int diag = o - base;
+        const Rec *A = sk + base;
+        const Rec *B = sk + base + w;
+        int lo = diag - w > 0 ? diag - w : 0;
+        int hi = diag < w ? diag : w;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            Rec x = A[mid], y = B[diag - 1 - mid];
+            if (key_le(x.w0, x.w1, y.w0, y.w1)) lo = mid + 1; else hi = mid;
+        }
+        int i = lo, j = diag - lo;
+        Rec ca = A[i < w ? i : w - 1], cb = B[j < w ? j : w - 1];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            bool takeA = (j >= w) || (i < w && key_le(ca.w0, ca.w1, cb.w0, cb.w1));
+            a0[k] = takeA ? ca.w0 : cb.w0;
+            a1[k] = takeA ? ca.w1 : cb.w1;
+            if (takeA) { i++; ca = A[i < w ? i : w - 1]; }
+            else       { j++; cb = B[j < w ? j : w - 1]; }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; k++) { Rec r; r.w0 = a0[k]; r.w1 = a1[k]; sk[tid * E + k] = r; }
+    __syncthreads();
+    uint64_t p0 = 0, p1 = 0;
+    if (tid > 0) { Rec r = sk[tid * E - 1]; p0 = r.w0; p1 = r.w1; }
+    __syncthreads();                                // LDS is reused for entry bytes below
+
+    const int e = pd.e, P = pd.P, C = pd.C;
+    const int64_t ebase = pd.ent_off + (tl.start - pd.rec0) * (int64_t) e;
+    const int shift = (int) (ebase & 15);
+    uint8_t *sb = smem_raw;
+    const uint64_t jmask = (1ull << jbits) - 1ull;
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const int idx = tid * E + k;
+        if (idx < n) {
+            const uint64_t w0 = a0[k], w1 = a1[k];
+            int lcp = (idx == 0) ? 0 : lcp_bases(p0, p1, w0, w1);
+            p0 = w0; p1 = w1;
+            const uint32_t strand = (uint32_t) (w1 & 1ull);
+            const uint64_t j = (w1 >> 1) & jmask;
+            const uint64_t contig = (w1 & 0xFFFFFFFFFFFFull) >> (jbits + 1);
+            const uint64_t pos = j + 12ull * strand;
+            const uint64_t cs = contig | ((uint64_t) strand << (8 * C - 1));
+            uint8_t *d = sb + shift + idx * e;
+            d[0] = (uint8_t) (w0 >> 32); d[1] = (uint8_t) (w0 >> 24); d[2] = (uint8_t) (w0 >> 16);
+            d[3] = (uint8_t) (w0 >> 8);  d[4] = (uint8_t) w0;
+            d[5] = (uint8_t) (w1 >> 56); d[6] = (uint8_t) (w1 >> 48);
+            d[7] = 0;
+            d[8] = (uint8_t) lcp;
+            for (int b = 0; b < P; b++) d[9 + b] = (uint8_t) (pos >> (8 * b));
+            for (int b = 0; b < C; b++) d[9 + P + b] = (uint8_t) (cs >> (8 * b));
+            ls[tl.start + idx] = (uint8_t) (lcp | (strand << 7));
+            pre[tl.start + idx] = (uint32_t) (w0 >> 40);
+        }
+    }
+    __syncthreads();
+    // coalesced copy of the tile's entry bytes: 16-byte aligned chunks, byte
+    // stores only for the partial first and last chunk
+    const int nb = n * e;
+    const int nchunks = (shift + nb + 15) >> 4;
+    uint8_t *g = entries + (ebase - shift);
+    for (int c = tid; c < nchunks; c += T) {
+        int b0 = c << 4;
+        if (b0 >= shift && b0 + 16 <= shift + nb) {
+            *reinterpret_cast<uint4 *>(g + b0) = *reinterpret_cast<const uint4 *>(sb + b0);
+        } else {
+            for (int b = b0; b < b0 + 16; b++)
+                if (b >= shift && b < shift + nb) g[b] = sb[b];
+        }
+    }
+}
+
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *);
+
+// Pair of a global record index.
+__device__ __forceinline__ int pair_of(const PairDev *__restrict__ pairs, int npairs, int64_t g)
+{
+    int lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (pairs[mid].rec0 <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// LCP of each tile's first record with its predecessor (0 at a pair's start),
+// from the 24-bit prefixes and the 7 suffix bytes of the two entries.
+__global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const PairDev *__restrict__ pairs,
+                           uint8_t *__restrict__ entries, uint8_t *__restrict__ ls,
+                           const uint32_t *__restrict__ pre)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const Tile tl = tiles[t];
+    const PairDev pd = pairs[tl.pair];
+    int lcp = 0;
+    const int64_t li = tl.start - pd.rec0;
+    uint8_t *cur = entries + pd.ent_off + li * (int64_t) pd.e;
+    if (li > 0) {
+        const uint8_t *prv = cur - pd.e;
+        uint64_t x0 = ((uint64_t) pre[tl.start - 1] << 40), y0 = ((uint64_t) pre[tl.start] << 40);
+        for (int b = 0; b < 5; b++) {
+            x0 |= (uint64_t) prv[b] << (32 - 8 * b);
+            y0 |= (uint64_t) cur[b] << (32 - 8 * b);
+        }
+        uint64_t x1 = ((uint64_t) prv[5] << 56) | ((uint64_t) prv[6] << 48);
+        uint64_t y1 = ((uint64_t) cur[5] << 56) | ((uint64_t) cur[6] << 48);
+        lcp = lcp_bases(x0, x1, y0, y1);
+    }
+    cur[8] = (uint8_t) lcp;
+    ls[tl.start] = (uint8_t) ((ls[tl.start] & 0x80) | lcp);
+}
+
+// ---------------------------------------------------------------- prefix table
+
+constexpr int LONG_FILL = 256;
+
+__device__ __forceinline__ void fill_range(int64_t *__restrict__ cum, int64_t from, int64_t to, int64_t v,
+                                           LongFill *__restrict__ longs, int *__restrict__ nlong)
+{
+    int64_t end = to - from > LONG_FILL ? from + LONG_FILL : to;
+    for (int64_t q = from; q < end; q++) cum[q] = v;
+    if (end < to) {
+        int k = atomicAdd(nlong, 1);
+        longs[k] = LongFill{cum, end, to, v};
+    }
+}
+
+// cum[p] = number of entries of the pair with prefix <= p: the last record of
+// each prefix group writes its (index+1) over [its prefix, next prefix).
+__global__ void k_cum_fill(const PairDev *__restrict__ pairs, int npairs, int64_t R,
+                           const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
+                           LongFill *__restrict__ longs, int *__restrict__ nlong)
+{
+    int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= R) return;
+    const int p = pair_of(pairs, npairs, g);
+    const PairDev pd = pairs[p];
+    const int64_t li = g - pd.rec0;
+    int64_t *cp = cum + pd.cum_off;
+    const int64_t cur = pre[g];
+    const int64_t nxt = (li + 1 < pd.nrec) ? (int64_t) pre[g + 1] : (int64_t) (1 << 24);
+    if (li == 0 && cur > 0) fill_range(cp, 0, cur, 0, longs, nlong);
+    if (cur != nxt) fill_range(cp, cur, nxt, li + 1, longs, nlong);
+}
+
+__global__ void k_long_fill(const LongFill *__restrict__ longs)
+{
+    const LongFill lf = longs[blockIdx.x];
+    for (int64_t q = lf.from + threadIdx.x; q < lf.to; q += blockDim.x) lf.cum[q] = lf.v;
+}
+
+__global__ void k_maxpre(const PairDev *__restrict__ pairs, int npairs, int64_t R,
+                         const uint32_t *__restrict__ pre, const int64_t *__restrict__ cum,
+                         unsigned long long *__restrict__ maxpre)
+{
+    int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= R) return;
+    const int p = pair_of(pairs, npairs, g);
+    const PairDev pd = pairs[p];
+    const int64_t li = g - pd.rec0;
+    const int64_t cur = pre[g];
+    const bool last = (li + 1 == pd.nrec) || pre[g + 1] != (uint32_t) cur;
+    if (!last) return;
+    const int64_t *cp = cum + pd.cum_off;
+    int64_t size = cp[cur] - (cur > 0 ? cp[cur - 1] : 0);
+    atomicMax(&maxpre[p], (unsigned long long) size);
+}
+
+// ---------------------------------------------------------------- adaptamer merge
+
+// The interval of anchor g (global index) in a window of LCP/strand bytes.
+// Returns the partner counts (same strand, other strand), or -1 if the anchor
+// emits nothing.  L(k) < 12 marks a prefix-group start; outside the pair the
+// window holds 0.  Follows src/fastga_pipeline.c:315-353: plen = max(LCP[g],
+// LCP[g+1]) with group edges < 12, [a,b) the maximal run sharing >= plen
+// bases, skipped when b - a >= freq.
+__device__ __forceinline__ bool anchor_interval(const uint8_t *__restrict__ win, int gi, int freq,
+                                                int &plen, int &a, int &b)
+{
+    int Li = win[gi] & 63, Ln = win[gi + 1] & 63;
+    int li = Li >= 12 ? Li : 0, ln = Ln >= 12 ? Ln : 0;
+    plen = li > ln ? li : ln;
+    if (plen < 12) return false;
+    a = gi;
+    int steps = 0;
+    while (steps < freq && (win[a] & 63) >= plen) { a--; steps++; }
+    if ((win[a] & 63) >= plen) return false;      // the run reaches past FREQ entries back
+    b = gi + 1;
+    while (b - a < freq && (win[b] & 63) >= plen) b++;
+    return b - a < freq;
+}
+
+__device__ __forceinline__ int merge_block_pair(const PairDev *__restrict__ pairs, int npairs, int64_t b)
+{
+    int lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (pairs[mid].mblk0 <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int MT>
+__global__ __launch_bounds__(MT) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,
+                                                    const uint8_t *__restrict__ ls, int freq,
+                                                    uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC)
+{
+    __shared__ uint8_t win[MT + 2 * MAX_FREQ + 4];
+    __shared__ uint64_t red[MT / 64 + 1];
+    const int p = merge_block_pair(pairs, npairs, blockIdx.x);
+    const PairDev pd = pairs[p];
+    const int64_t lo = pd.rec0 + (blockIdx.x - pd.mblk0) * (int64_t) MT;
+    const int64_t end = pd.rec0 + pd.nrec;
+    const int H = freq + 1;
+    for (int k = threadIdx.x; k < MT + 2 * H; k += MT) {
+        int64_t g = lo - H + k;
+        win[k] = (g >= pd.rec0 && g < end) ? ls[g] : 0;
+    }
+    __syncthreads();
+    const int64_t g = lo + threadIdx.x;
+    uint32_t nN = 0, nC = 0;
+    int plen, a, b;
+    if (g < end && anchor_interval(win, threadIdx.x + H, freq, plen, a, b)) {
+        const int gi = threadIdx.x + H;
+        const uint8_t si = win[gi] & 0x80;
+        for (int q = a; q < b; q++) {
+            if (q == gi) continue;
+            if ((win[q] & 0x80) == si) nN++; else nC++;
+        }
+    }
+    uint64_t sn = block_sum_u64<MT>(nN, red);
+    uint64_t sc = block_sum_u64<MT>(nC, red);
+    if (threadIdx.x == 0) { cntN[blockIdx.x] = (uint32_t) sn; cntC[blockIdx.x] = (uint32_t) sc; }
+}
+
+template <int MT>
+__global__ __launch_bounds__(MT) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
+                                                   const uint8_t *__restrict__ ls,
+                                                   const uint8_t *__restrict__ entries, int freq,
+                                                   const int64_t *__restrict__ ofsN,
+                                                   const int64_t *__restrict__ ofsC,
+                                                   uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
+                                                   unsigned long long *__restrict__ nbuck,
+                                                   unsigned long long *__restrict__ cbuck)
+{
+    constexpr int W = MT + 2 * MAX_FREQ + 4;
+    __shared__ uint8_t win[W];
+    __shared__ uint8_t pay[W * 8];
+    __shared__ uint32_t sh[MT / 64 + 1];
+    const int p = merge_block_pair(pairs, npairs, blockIdx.x);
+    const PairDev pd = pairs[p];
+    const int64_t lo = pd.rec0 + (blockIdx.x - pd.mblk0) * (int64_t) MT;
+    const int64_t end = pd.rec0 + pd.nrec;
+    const int H = freq + 1;
+    const int pc = pd.P + pd.C, e = pd.e;
+    for (int k = threadIdx.x; k < MT + 2 * H; k += MT) {
+        int64_t g = lo - H + k;
+        bool in = g >= pd.rec0 && g < end;
+        win[k] = in ? ls[g] : 0;
+        if (in) {
+            const uint8_t *src = entries + pd.ent_off + (g - pd.rec0) * (int64_t) e + 9;
+            for (int b = 0; b < pc; b++) pay[k * 8 + b] = src[b];
+        }
+    }
+    __syncthreads();
+    const int64_t g = lo + threadIdx.x;
+    const int gi = threadIdx.x + H;
+    uint32_t nN = 0, nC = 0;
+    int plen = 0, a = 0, b = 0;
+    bool act = g < end && anchor_interval(win, gi, freq, plen, a, b);
+    if (act) {
+        const uint8_t si = win[gi] & 0x80;
+        for (int q = a; q < b; q++) {
+            if (q == gi) continue;
+            if ((win[q] & 0x80) == si) nN++; else nC++;
+        }
+    }
+    uint32_t totN, totC;
+    uint32_t oN = block_excl_scan<MT, uint32_t>(nN, sh, &totN);
+    uint32_t oC = block_excl_scan<MT, uint32_t>(nC, sh, &totC);
+    if (!act || (nN + nC) == 0) return;
+    const int s = pd.s;
+    uint8_t *dN = seedN + pd.nseed_off + (ofsN[blockIdx.x] - ofsN[pd.mblk0] + oN) * (int64_t) s;
+    uint8_t *dC = seedC + pd.cseed_off + (ofsC[blockIdx.x] - ofsC[pd.mblk0] + oC) * (int64_t) s;
+    const uint8_t *pi = pay + gi * 8;
+    const uint8_t si = win[gi] & 0x80;
+    for (int q = a; q < b; q++) {
+        if (q == gi) continue;
+        uint8_t *d;
+        if ((win[q] & 0x80) == si) { d = dN; dN += s; } else { d = dC; dC += s; }
+        d[0] = (uint8_t) plen;
+        for (int k = 0; k < pc; k++) d[1 + k] = pi[k];
+        d[pc] &= 0x7F;                              // anchor's strand bit cleared
+        const uint8_t *pq = pay + q * 8;
+        for (int k = 0; k < pc; k++) d[1 + pc + k] = pq[k];
+    }
+    // per-anchor-contig counts (src/fastga_pipeline.c:403)
+    uint64_t icont = 0;
+    for (int k = 0; k < pd.C; k++) icont |= (uint64_t) pi[pd.P + k] << (8 * k);
+    icont &= ~(1ull << (8 * pd.C - 1));
+    if (nN) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nN);
+    if (nC) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nC);
+}
+
+template __global__ void k_merge_count<256>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
+template __global__ void k_merge_emit<256>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
+                                           const int64_t *, const int64_t *, uint8_t *, uint8_t *,
+                                           unsigned long long *, unsigned long long *);
+template __global__ void k_merge_count<512>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
+template __global__ void k_merge_emit<512>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
+                                           const int64_t *, const int64_t *, uint8_t *, uint8_t *,
+                                           unsigned long long *, unsigned long long *);
+template __global__ void k_merge_count<1024>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
+template __global__ void k_merge_emit<1024>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
+                                            const int64_t *, const int64_t *, uint8_t *, uint8_t *,
+                                            unsigned long long *, unsigned long long *);
+
+} // namespace pga

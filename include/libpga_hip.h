@@ -1,0 +1,133 @@
+/* libpga_hip.h -- C-ABI of the MI355X index build + adaptamer seed merge.
+ *
+ * Plain C, plain pointers and sizes; every call returns an int status
+ * (PGA_OK or a negative PGA_E* code) and never exits the process.
+ *
+ * One context = one HIP stream on one device plus a grow-only pool of device
+ * buffers.  A context holds ONE batch at a time: a batch is one or more
+ * independent GDBs ("pairs": genome A's contigs then genome B's), each indexed
+ * and self-merged on its own, in the same kernel launches.  The plain path is a
+ * batch of one.  Calls on one context must not overlap; contexts on different
+ * host threads are independent.
+ *
+ * What each entry point replaces in the reference (SakshamDiwan/DistributedPanGA):
+ *   pga_load_gdb / pga_load_batch   the GDB the driver reads, lib/GDB.c:1171 (Read_GDB)
+ *                                   and :1831-1930 (Get_Contig_Piece) -- `.bps` bytes in HBM
+ *   pga_build_index                 extract_to_records   src/extract.h:40
+ *                                   run_stage2           src/dsort.h:82 (sort + LCP, 1 rank)
+ *                                   build_kmer_adapter   src/kmer_adapter.h:46
+ *   pga_index_info / _export        KmerStreamAdapter + PayloadLayout fields,
+ *                                   src/kmer_adapter.h:16-44 (entries, prefix counts, maxpre)
+ *   pga_merge                       self_adaptamer_merge src/fastga_pipeline.h:18
+ *   pga_seeds_info / _export        Recv_N_Data/Recv_C_Data + N/C_Units[0].buck,
+ *                                   src/pga-mpi-merge.c:523-547 (1-rank shortcut)
+ */
+#ifndef LIBPGA_HIP_H
+#define LIBPGA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGA_OK          0
+#define PGA_ENOMEM     -1   /* device or host allocation failed          */
+#define PGA_EHIP       -2   /* a HIP runtime call failed                 */
+#define PGA_EINVAL     -3   /* bad argument or call order                */
+#define PGA_EOVERFLOW  -4   /* a field does not fit its byte width       */
+
+#define PGA_NUM_PREFIXES (1 << 24)   /* 24-bit k-mer prefix table length */
+
+typedef struct pga_ctx pga_ctx;
+
+typedef struct {
+    int device;          /* HIP device ordinal                                  */
+    int freq;            /* frequency cut: skip groups of >= freq entries (10)  */
+    int tile_records;    /* records per sort workgroup: 2048, 4096 or 8192      */
+    int chunk_records;   /* records per radix-scatter workgroup: 2048..8192     */
+    int merge_tile;      /* entries per merge workgroup: 256, 512 or 1024       */
+    int build_cum;       /* 1: build the 2^24 prefix table (GIX layout)         */
+} pga_opts;
+
+typedef struct {
+    int64_t nrec;          /* entries (== records) of the index                    */
+    int     P, C;          /* position / contig+strand bytes (src/record.c:18-36)   */
+    int     e;             /* entry bytes = 9+P+C (src/kmer_adapter.c:16-20)        */
+    int     s;             /* seed bytes  = 1+2(P+C) (src/pga-mpi-merge.c:68-76)    */
+    int64_t maxpre;        /* largest 24-bit prefix group                           */
+    int     ncontig;
+    int64_t seqtot;
+} pga_index_meta;
+
+typedef struct {
+    int64_t nlen, clen;    /* bytes in the same-strand (N) / cross-strand (C) stream */
+    int     s;             /* seed bytes                                             */
+    int     ncontig;       /* length of each buck vector                             */
+} pga_seeds_meta;
+
+/* Milliseconds of device time per stage of the last build / merge
+ * (HIP events on the context's stream). */
+typedef struct {
+    double extract_ms;     /* syncmer scan: count + emit                      */
+    double partition_ms;   /* MSD radix partition passes                      */
+    double tile_sort_ms;   /* LDS sort of final buckets + LCP + GIX entries   */
+    double table_ms;       /* boundary LCP fix + 2^24 prefix table + maxpre   */
+    double merge_ms;       /* adaptamer merge: count + scan + emit            */
+    double total_ms;       /* first event to last event, wall on the device   */
+    int    partition_levels;
+    int    partition_launches;
+    int64_t tiles;
+    /* the radix partition scatter (dominant HBM kernel): summed kernel time
+       and bytes moved (records read + written) */
+    double scatter_ms;
+    int64_t scatter_bytes;
+    int    scatter_launches;
+} pga_timings;
+
+void        pga_default_opts(pga_opts *o);
+int         pga_create(const pga_opts *o, pga_ctx **out);
+void        pga_destroy(pga_ctx *ctx);
+const char *pga_last_error(const pga_ctx *ctx);
+
+/* One GDB from host memory: `.bps` bytes (LSB-first 2-bit, contigs byte-aligned
+ * at boff[i]) and contig lengths.  Copied to the device. */
+int pga_load_gdb(pga_ctx *ctx, const uint8_t *bps, int64_t bps_bytes,
+                 const int64_t *boff, const int64_t *clen, int ncontig);
+
+/* A batch of npairs GDBs (arrays of per-pair pointers). */
+int pga_load_batch(pga_ctx *ctx, int npairs, const uint8_t *const *bps, const int64_t *bps_bytes,
+                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig);
+
+/* Index every GDB of the loaded batch on the device. */
+int pga_build_index(pga_ctx *ctx);
+
+int pga_index_info(pga_ctx *ctx, int pair, pga_index_meta *info);
+
+/* Copy pair's GIX entries (nrec*e bytes) and, if cum != NULL, the cumulative
+ * prefix counts (PGA_NUM_PREFIXES int64: entries with prefix <= p) to the host.
+ * Byte-identical to build_kmer_adapter's entries / cumulative_count_by_prefix
+ * (src/kmer_adapter.c:55-80) under the canonical tie order (equal k-mers in
+ * (contig, offset, forward-before-reverse) order). */
+int pga_index_export(pga_ctx *ctx, int pair, uint8_t *entries, int64_t *cum);
+
+/* Self-merge every indexed GDB of the batch (frequency cut = opts.freq). */
+int pga_merge(pga_ctx *ctx);
+
+int pga_seeds_info(pga_ctx *ctx, int pair, pga_seeds_meta *info);
+
+/* Copy pair's seed streams and per-anchor-contig counts to the host.  N/C are
+ * the streams new_self_merge_thread writes (src/fastga_pipeline.c:357-404):
+ * anchors in index order, partners ascending, 1+2(P+C) bytes per seed. */
+int pga_seeds_export(pga_ctx *ctx, int pair, uint8_t *nseeds, uint8_t *cseeds,
+                     int64_t *nbuck, int64_t *cbuck);
+
+int pga_timings_get(pga_ctx *ctx, pga_timings *t);
+
+/* Wait for all work queued on the context's stream. */
+int pga_sync(pga_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle, byte for byte.
+
+Outputs compared: GIX entries (k-mer suffix, mask, LCP, payload), the 2^24
+prefix table, the largest prefix group, the N/C seed streams in emission order
+and the per-anchor-contig seed counts.  Integer work: the bar is bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from distributedpanga_amd import pga, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(g, freq=10, **opts):
+    print(f"[parity] ncontig={g.ncontig} bases={g.seqtot} opts={opts} freq={freq}", flush=True)
+    got = pga.run_pair(g, freq=freq, **opts)
+    want = oracle.pipeline(g.bps, g.boff[:-1], g.clen, freq=freq)
+    assert got["info"]["nrec"] == want["nrec"]
+    assert (got["info"]["P"], got["info"]["C"]) == (want["P"], want["C"])
+    assert np.array_equal(got["entries"], want["entries"])
+    if got["cum"] is not None:
+        assert np.array_equal(got["cum"], want["cum"])
+        assert got["info"]["maxpre"] == want["maxpre"]
+    assert np.array_equal(got["nseeds"], want["nseeds"])
+    assert np.array_equal(got["cseeds"], want["cseeds"])
+    assert np.array_equal(got["nbuck"], want["nbuck"])
+    assert np.array_equal(got["cbuck"], want["cbuck"])
+    return got, want
+
+
+def xorshift_seq(n, seed):
+    x = seed or 1
+    out = np.zeros(n, np.uint8)
+    for i in range(n):
+        x ^= (x << 13) & 0xFFFFFFFF
+        x ^= x >> 17
+        x ^= (x << 5) & 0xFFFFFFFF
+        out[i] = x & 3
+    return out
+
+
+def test_e06_contig():
+    g = synth.from_codes([xorshift_seq(4000, 20260910)])
+    got, want = _compare(g)
+    assert got["info"]["nrec"] == 3106
+
+
+def test_workload1_small():
+    _compare(synth.workload(1, 0.05))
+
+
+@pytest.mark.parametrize("tile", [2048, 4096, 8192])
+def test_tile_sizes(tile):
+    _compare(synth.workload(1, 0.2), tile_records=tile)
+
+
+@pytest.mark.parametrize("chunk,mt", [(2048, 512), (8192, 1024)])
+def test_chunk_and_merge_tiles(chunk, mt):
+    _compare(synth.workload(1, 0.2), chunk_records=chunk, merge_tile=mt)
+
+
+@pytest.mark.parametrize("freq", [1, 2, 4, 25, 64])
+def test_frequency_cuts(freq):
+    _compare(synth.workload(1, 0.1), freq=freq)
+
+
+def test_edge_lengths():
+    rng = np.random.default_rng(17)
+    contigs = [rng.integers(0, 4, n).astype(np.uint8) for n in
+               [1, 5, 11, 12, 13, 39, 40, 41, 52, 67, 68, 100, 255, 256, 257, 1023, 4099]]
+    _compare(synth.from_codes(contigs))
+
+
+def test_no_records():
+    _compare(synth.from_codes([np.zeros(5, np.uint8), np.zeros(11, np.uint8)]))
+
+
+def test_low_complexity_deep_partition():
+    """Identical k-mers beyond one tile force every partition level down to the payload bits."""
+    contigs = [np.zeros(30000, np.uint8), np.tile(np.array([0, 1], np.uint8), 12000),
+               np.tile(np.array([2, 2, 3], np.uint8), 5000)]
+    rng = np.random.default_rng(3)
+    contigs += [rng.integers(0, 4, 5000).astype(np.uint8) for _ in range(3)]
+    _compare(synth.from_codes(contigs), tile_records=2048)
+
+
+def test_many_contigs_two_byte_ids():
+    rng = np.random.default_rng(8)
+    contigs = [rng.integers(0, 4, int(n)).astype(np.uint8) for n in rng.integers(12, 400, 300)]
+    got, _ = _compare(synth.from_codes(contigs))
+    assert got["info"]["C"] == 2
+
+
+def test_workload1_full():
+    _compare(synth.workload(1, 1.0))
+
+
+def test_workload2_scaled():
+    _compare(synth.workload(2, 0.05))
+
+
+def test_batch_equals_single():
+    gdbs = synth.batch(6, scale=0.02)
+    with pga.Context() as ctx:
+        ctx.load_batch(gdbs)
+        ctx.build_index()
+        ctx.merge()
+        for p, g in enumerate(gdbs):
+            print(f"[batch] pair {p}", flush=True)
+            ent, cum, info = ctx.index_export(p)
+            N, Cs, nb, cb = ctx.seeds_export(p)
+            want = oracle.pipeline(g.bps, g.boff[:-1], g.clen)
+            assert np.array_equal(ent, want["entries"])
+            assert np.array_equal(cum, want["cum"]) and info["maxpre"] == want["maxpre"]
+            assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"])
+            assert np.array_equal(nb, want["nbuck"]) and np.array_equal(cb, want["cbuck"])
+
+
+def test_context_reuse_smaller_second_call():
+    a = synth.workload(1, 0.3)
+    b = synth.workload(1, 0.05)
+    with pga.Context() as ctx:
+        for g in (a, b, a):
+            ctx.load_gdb(g)
+            ctx.build_index()
+            ctx.merge()
+            ent, cum, _ = ctx.index_export(0)
+            N, Cs, nb, cb = ctx.seeds_export(0)
+            want = oracle.pipeline(g.bps, g.boff[:-1], g.clen)
+            assert np.array_equal(ent, want["entries"]) and np.array_equal(cum, want["cum"])
+            assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"])
