@@ -1,0 +1,229 @@
+"""Benchmark: index build + adaptamer seed merge on one MI355X.
+
+One step = one pass of the hot path over the HBM-resident input: syncmer
+extraction -> radix partition -> tile sort with LCP and GIX entries -> 2^24
+prefix table -> self-merge into N/C seed streams.  The `.bps` bytes are
+copied to the device once before timing (the PCIe-inclusive rate is reported
+separately).
+
+Printed (one JSON line): the plain single-pair rate on the 100 Mbp pair
+workload (`value`), the batched 64-pair workload in one batch and pair by pair,
+the dominant kernel's roofline, and the CPU baseline (the C oracle on a
+bounded sample of the same workload, one core).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from distributedpanga_amd import autotune, pga, synth  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "index+seed Gbases/s and % HBM peak, batched vs plain GPU vs CPU, 1 MI355X"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(ctx, steps, warmup):
+    for _ in range(warmup):
+        ctx.build_index()
+        ctx.merge()
+    ctx.sync()
+    stage = {k: 0.0 for k in ("extract_ms", "partition_ms", "tile_sort_ms", "table_ms", "merge_ms",
+                              "scatter_ms")}
+    scat_bytes = 0
+    scat_launches = 0
+    tiles = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.build_index()
+        ctx.merge()
+        t = ctx.timings()
+        for k in stage:
+            stage[k] += t[k]
+        scat_bytes += t["scatter_bytes"]
+        scat_launches += t["scatter_launches"]
+        tiles = t["tiles"]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    per = {k: v / steps for k, v in stage.items()}
+    return dt, per, scat_bytes / steps, scat_launches / steps, tiles
+
+
+def algorithmic_bytes(seqtot, nrec, e, s, nseed):
+    """Fixed byte model of the path: read .bps, write records, a two-pass sort
+    (2 x read+write 16 B), LCP read, GIX entries written and read by the merge,
+    seeds written."""
+    return seqtot / 4 + 16 * nrec + 64 * nrec + 16 * nrec + 2 * e * nrec + s * nseed
+
+
+def subsample(g, target_bases):
+    """The first A contigs and their B counterparts until target_bases."""
+    n_a = g.n_a
+    pick, tot = [], 0
+    for i in range(n_a):
+        if tot >= target_bases:
+            break
+        pick += [i, n_a + i]
+        tot += int(g.clen[i] + g.clen[n_a + i])
+    codes = [g.contig_codes(c) for c in pick]
+    return synth.from_codes(codes, n_a=len(pick) // 2)
+
+
+def cpu_baseline(g, target_bases):
+    import oracle                                     # the checker, timed as the CPU baseline
+    s = subsample(g, target_bases)
+    t0 = time.perf_counter()
+    o = oracle.pipeline(s.bps, s.boff[:-1], s.clen)
+    dt = time.perf_counter() - t0
+    return dict(value=round(s.seqtot / dt / 1e9, 6), unit="Gbases/s", cores=1, kind="port",
+                sample=f"first {len(s.clen) // 2} A contigs + their B counterparts of the 100 Mbp pair "
+                       f"workload: {s.seqtot} bases, {o['nrec']} records, single-threaded C oracle "
+                       f"(extract {o['t_extract']:.2f}s, sort+LCP {o['t_sort']:.2f}s, "
+                       f"adapter {o['t_adapter']:.2f}s, merge {o['t_merge']:.2f}s)",
+                seconds=round(dt, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0, help="workload scale (1.0 = full size)")
+    ap.add_argument("--tile", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=4096)
+    ap.add_argument("--merge-tile", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-bases", type=int, default=50_000_000)
+    ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="use the fixed --tile/--chunk/--merge-tile")
+    ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU exchange is a design, not a run)", file=sys.stderr)
+        sys.exit(2)
+
+    opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+    t = time.perf_counter()
+    g = synth.workload(2, args.scale)
+    log(f"[bench] workload 2 generated: {g.ncontig} contigs, {g.seqtot} bases ({time.perf_counter() - t:.1f}s)")
+
+    ctx = pga.Context(**opts)
+    t = time.perf_counter()
+    ctx.load_gdb(g)
+    h2d_s = time.perf_counter() - t
+    tuned = None
+    if not args.no_tune:
+        t = time.perf_counter()
+        tuned, _ = autotune.tune(ctx, [g], use_cache=not args.retune, log=log)
+        opts.update(tuned)
+        log(f"[bench] tuned plain: {tuned} ({time.perf_counter() - t:.1f}s)")
+    dt, per, scat_bytes, scat_launches, tiles = time_steps(ctx, args.steps, args.warmup)
+    info = ctx.index_info(0)
+    sinfo = ctx.seeds_info(0)
+    nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
+    ms_step = dt / args.steps * 1e3
+    value = g.seqtot * args.steps / dt / 1e9
+    log(f"[bench] plain: {ms_step:.2f} ms/step, {value:.2f} Gbases/s, {info['nrec']} records, {nseed} seeds, "
+        f"stages {json.dumps({k: round(v, 3) for k, v in per.items()})}")
+
+    # dominant kernel: the LDS tile sort (one launch per step).  Algorithmic bytes:
+    # every record read once (16 B) and its GIX entry (e B), LCP/strand byte (1 B)
+    # and 24-bit prefix (4 B) written once.
+    sort_bytes = info["nrec"] * (16 + info["e"] + 1 + 4)
+    sort_gbs = sort_bytes / (per["tile_sort_ms"] * 1e-3) / 1e9
+    scat_gbs = scat_bytes / (per["scatter_ms"] * 1e-3) / 1e9 if per["scatter_ms"] > 0 else 0.0
+    balg = algorithmic_bytes(g.seqtot, info["nrec"], info["e"], info["s"], nseed)
+    path_gbs = balg / (ms_step * 1e-3) / 1e9
+    roofline = dict(bound="hbm", kernel="k_tile_sort", achieved=round(sort_gbs, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(sort_gbs / HBM_PEAK_GBS, 4), traffic=None,
+                    bytes_per_launch=int(sort_bytes), avg_launch_ms=round(per["tile_sort_ms"], 4),
+                    radix_scatter=dict(achieved=round(scat_gbs, 1), frac=round(scat_gbs / HBM_PEAK_GBS, 4),
+                                       bytes_per_step=int(scat_bytes), launches_per_step=scat_launches,
+                                       ms_per_step=round(per["scatter_ms"], 4)),
+                    path=dict(algorithmic_bytes=int(balg), achieved=round(path_gbs, 1),
+                              frac=round(path_gbs / HBM_PEAK_GBS, 4)))
+    ctx.close()
+
+    out = {"metric": METRIC, "value": round(value, 4), "unit": "Gbases/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic",
+           "config": {"workload": "plain single pair, 100 Mbp A (500 lognormal contigs, GC 0.41, "
+                                  "2000x300 bp repeat at 10%, 50 tandem arrays) + B (5% SNP, 0.5% indel, "
+                                  "10 inversions), one GDB, k=40, FREQ=10",
+                      "bases": g.seqtot, "contigs": g.ncontig, "records": info["nrec"], "seeds": int(nseed),
+                      "scale": args.scale, "tile_records": opts["tile_records"],
+                      "chunk_records": opts["chunk_records"], "merge_tile": opts["merge_tile"],
+                      "autotuned": tuned is not None, "parallelism": "1 GPU"},
+           "stages_ms": {k: round(v, 4) for k, v in per.items()},
+           "h2d_load_s": round(h2d_s, 4),
+           "pcie_inclusive_gbases_s": round(g.seqtot / (ms_step * 1e-3 + h2d_s) / 1e9, 4),
+           "roofline": roofline}
+
+    if not args.no_batch:
+        t = time.perf_counter()
+        gdbs = synth.batch(args.batch_pairs, scale=args.scale)
+        tot = sum(x.seqtot for x in gdbs)
+        log(f"[bench] batch workload generated: {len(gdbs)} pairs, {tot} bases ({time.perf_counter() - t:.1f}s)")
+        bopts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+        bctx = pga.Context(**bopts)
+        bctx.load_batch(gdbs)
+        if not args.no_tune:
+            btuned, _ = autotune.tune(bctx, gdbs, use_cache=not args.retune, log=log)
+            bopts.update(btuned)
+        bsteps = max(2, args.steps // 2)
+        bdt, bper, _, _, _ = time_steps(bctx, bsteps, 1)
+        bctx.close()
+        batched = tot * bsteps / bdt / 1e9
+        # plain form: the same pairs one at a time (input resident, one build + merge per pair),
+        # tuned on the median pair
+        popts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+        pctx = pga.Context(**popts)
+        if not args.no_tune:
+            med = sorted(gdbs, key=lambda x: x.seqtot)[len(gdbs) // 2]
+            pctx.load_gdb(med)
+            ptuned, _ = autotune.tune(pctx, [med], use_cache=not args.retune, log=log)
+            popts.update(ptuned)
+        plain_s = 0.0
+        for x in gdbs:
+            pctx.load_gdb(x)
+            pctx.build_index()
+            pctx.merge()
+            pctx.sync()
+            t0 = time.perf_counter()
+            pctx.build_index()
+            pctx.merge()
+            pctx.sync()
+            plain_s += time.perf_counter() - t0
+        pctx.close()
+        plain = tot / plain_s / 1e9
+        out["batched"] = dict(workload=f"{len(gdbs)} independent pairs, A lognormal 1-12 Mbp, 1-50 contigs, "
+                                       "GC 0.30-0.65, B at 1-15% SNP and 0.1-1% indel",
+                              bases=tot, batched_gbases_s=round(batched, 4), batched_ms=round(bdt / bsteps * 1e3, 3),
+                              plain_gbases_s=round(plain, 4), plain_ms=round(plain_s * 1e3, 3),
+                              speedup=round(batched / plain, 3), batched_opts=bopts, plain_opts=popts,
+                              batched_stages_ms={k: round(v, 3) for k, v in bper.items()})
+        log(f"[bench] batched {batched:.2f} Gbases/s vs plain per pair {plain:.2f} Gbases/s")
+
+    if not args.no_cpu:
+        t = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(g, args.cpu_bases)
+        log(f"[bench] cpu baseline {out['cpu_baseline']['value']} Gbases/s ({time.perf_counter() - t:.1f}s)")
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

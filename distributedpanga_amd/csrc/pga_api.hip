@@ -94,7 +94,7 @@ struct pga_ctx {
     // device buffers
     DBuf bps, ctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
-        mblk, pofs, seedN, seedC, nbuck, cbuck;
+        mblk, pofs, seedN, seedC, nbuck, cbuck, bmax;
 
     // timing
     hipEvent_t ev[8] = {};
@@ -222,7 +222,7 @@ int launch_seg_scatter(pga_ctx *c, int per, unsigned grid, const Rec *src, Rec *
 int launch_tile_sort(pga_ctx *c, unsigned grid)
 {
     const int T = c->o.tile_records / 8;
-    const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + 16;
+    const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
     Rec *const *bufs = (Rec *const *) c->bufs.p;
     const Tile *tiles = P<Tile>(c->tiles);
     const PairDev *pairs = P<PairDev>(c->dpairs);
@@ -290,6 +290,23 @@ void pga_default_opts(pga_opts *o)
 
 const char *pga_last_error(const pga_ctx *c) { return c ? c->err : "no context"; }
 
+static bool opts_valid(const pga_opts &q)
+{
+    return q.freq >= 1 && q.freq <= MAX_FREQ &&
+           (q.tile_records == 2048 || q.tile_records == 4096 || q.tile_records == 8192) &&
+           (q.chunk_records == 2048 || q.chunk_records == 4096 || q.chunk_records == 8192) &&
+           (q.merge_tile == 256 || q.merge_tile == 512 || q.merge_tile == 1024);
+}
+
+int pga_set_opts(pga_ctx *c, const pga_opts *o)
+{
+    if (!c || !o) return PGA_EINVAL;
+    if (!opts_valid(*o) || o->device != c->o.device) return fail(c, PGA_EINVAL, "invalid options");
+    c->o = *o;
+    c->built = c->merged = false;
+    return PGA_OK;
+}
+
 int pga_create(const pga_opts *o, pga_ctx **out)
 {
     *out = nullptr;
@@ -297,10 +314,7 @@ int pga_create(const pga_opts *o, pga_ctx **out)
     if (!c) return PGA_ENOMEM;
     if (o) c->o = *o; else pga_default_opts(&c->o);
     const pga_opts &q = c->o;
-    if (q.freq < 1 || q.freq > MAX_FREQ ||
-        (q.tile_records != 2048 && q.tile_records != 4096 && q.tile_records != 8192) ||
-        (q.chunk_records != 2048 && q.chunk_records != 4096 && q.chunk_records != 8192) ||
-        (q.merge_tile != 256 && q.merge_tile != 512 && q.merge_tile != 1024)) {
+    if (!opts_valid(q)) {
         delete c;
         return PGA_EINVAL;
     }
@@ -312,7 +326,7 @@ int pga_create(const pga_opts *o, pga_ctx **out)
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { delete c; return PGA_EHIP; }
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_tmap), TMAP, sizeof(TMAP)) != hipSuccess) { delete c; return PGA_EHIP; }
-    const size_t big = (size_t) 8192 * sizeof(Rec) + 16;
+    const size_t big = (size_t) 8192 * sizeof(Rec) + 4096 * 4 + 16;
     (void) hipFuncSetAttribute((const void *) k_tile_sort<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
     (void) hipFuncSetAttribute((const void *) k_tile_sort<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
     (void) hipFuncSetAttribute((const void *) k_tile_sort<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
@@ -330,7 +344,7 @@ void pga_destroy(pga_ctx *c)
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs,
-                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck};
+                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
@@ -619,9 +633,16 @@ int pga_build_index(pga_ctx *c)
                 hipLaunchKernelGGL(k_long_fill, dim3(nlong), dim3(256), 0, c->st, (const LongFill *) P<LongFill>(c->longs));
                 LAUNCHCHK(c);
             }
-            hipLaunchKernelGGL(k_maxpre, dim3((unsigned) ((R + 255) / 256)), dim3(256), 0, c->st,
+            const int64_t nmp = (R + MAXPRE_SPAN - 1) / MAXPRE_SPAN;
+            TRY(ensure(c, c->bmax, (size_t) nmp * sizeof(uint64_t)));
+            hipLaunchKernelGGL(k_maxpre, dim3((unsigned) nmp), dim3(256), 0, c->st,
                                (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
-                               (const int64_t *) P<int64_t>(c->cum), P<unsigned long long>(c->maxpre));
+                               (const int64_t *) P<int64_t>(c->cum), P<unsigned long long>(c->bmax),
+                               P<unsigned long long>(c->maxpre));
+            LAUNCHCHK(c);
+            hipLaunchKernelGGL(k_maxpre_reduce, dim3(npairs), dim3(256), 0, c->st,
+                               (const PairDev *) P<PairDev>(c->dpairs), (const unsigned long long *) P<unsigned long long>(c->bmax),
+                               P<unsigned long long>(c->maxpre));
             LAUNCHCHK(c);
         }
     }
@@ -734,19 +755,22 @@ int pga_merge(pga_ctx *c)
     }
     TRY(ensure(c, c->seedN, (size_t) nbytes + 16));
     TRY(ensure(c, c->seedC, (size_t) cbytes + 16));
-    TRY(ensure(c, c->nbuck, (size_t) std::max(c->nctg, 1) * sizeof(uint64_t)));
-    TRY(ensure(c, c->cbuck, (size_t) std::max(c->nctg, 1) * sizeof(uint64_t)));
-    TRY(fill_u64(c, c->nbuck.p, c->nctg, 0));
-    TRY(fill_u64(c, c->cbuck.p, c->nctg, 0));
+    const int64_t nb_elems = std::max(c->nctg, 1);
+    TRY(ensure(c, c->nbuck, (size_t) nb_elems * sizeof(uint64_t)));
+    TRY(ensure(c, c->cbuck, (size_t) nb_elems * sizeof(uint64_t)));
+    TRY(fill_u64(c, c->nbuck.p, nb_elems, 0));
+    TRY(fill_u64(c, c->cbuck.p, nb_elems, 0));
     HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
+    const int64_t egrid = std::min<int64_t>(nmb, 4096);
+    const int64_t eper = egrid > 0 ? (nmb + egrid - 1) / egrid : 0;
     if (nmb > 0) {
 #define PGA_EMIT(MTV)                                                                                   \
-    hipLaunchKernelGGL(k_merge_emit<MTV>, dim3((unsigned) nmb), dim3(MTV), 0, c->st,                    \
+    hipLaunchKernelGGL(k_merge_emit<MTV>, dim3((unsigned) egrid), dim3(MTV), 0, c->st,                  \
                        (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
                        (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
                        (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
                        P<uint8_t>(c->seedN), P<uint8_t>(c->seedC), P<unsigned long long>(c->nbuck),     \
-                       P<unsigned long long>(c->cbuck))
+                       P<unsigned long long>(c->cbuck), nmb, eper)
         switch (MT) {
         case 512: PGA_EMIT(512); break;
         case 1024: PGA_EMIT(1024); break;
@@ -755,6 +779,7 @@ int pga_merge(pga_ctx *c)
 #undef PGA_EMIT
         LAUNCHCHK(c);
     }
+
     TRY(record(c, 6));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->tm.merge_ms = ev_ms(c->ev[5], c->ev[6]);
@@ -783,11 +808,12 @@ int pga_seeds_export(pga_ctx *c, int pair, uint8_t *nseeds, uint8_t *cseeds, int
     if (cseeds && c->cseeds[pair] > 0)
         HIPCHK(c, hipMemcpyAsync(cseeds, P<uint8_t>(c->seedC) + d.cseed_off, (size_t) (c->cseeds[pair] * d.s),
                                  hipMemcpyDeviceToHost, c->st));
+    const int64_t row = 0;
     if (nbuck && d.ncontig > 0)
-        HIPCHK(c, hipMemcpyAsync(nbuck, P<uint64_t>(c->nbuck) + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
+        HIPCHK(c, hipMemcpyAsync(nbuck, P<uint64_t>(c->nbuck) + row + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
                                  hipMemcpyDeviceToHost, c->st));
     if (cbuck && d.ncontig > 0)
-        HIPCHK(c, hipMemcpyAsync(cbuck, P<uint64_t>(c->cbuck) + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
+        HIPCHK(c, hipMemcpyAsync(cbuck, P<uint64_t>(c->cbuck) + row + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
                                  hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return PGA_OK;

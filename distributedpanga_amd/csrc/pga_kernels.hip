@@ -556,7 +556,7 @@ __device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, 
 // (src/kmer_adapter.c:55-62 layout), the LCP/strand byte and the 24-bit prefix.
 // The LCP of a tile's first record is written by k_tile_fix.
 template <int T>
-__global__ __launch_bounds__(T) void k_tile_sort(Rec *const *__restrict__ bufs,
+__global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ bufs,
                                                  const Tile *__restrict__ tiles,
                                                  const PairDev *__restrict__ pairs, int jbits,
                                                  uint8_t *__restrict__ entries,
@@ -572,18 +572,102 @@ __global__ __launch_bounds__(T) void k_tile_sort(Rec *const *__restrict__ bufs,
     const int n = tl.count;
     const int tid = threadIdx.x;
 
+    constexpr int NB = CAP / 2;                     // bins of the binned sort
+    uint32_t *bh = reinterpret_cast<uint32_t *>(smem_raw + (size_t) CAP * sizeof(Rec));
+    __shared__ uint64_t mn0[T / 64], mn1[T / 64], mx0[T / 64], mx1[T / 64];
+    __shared__ uint32_t scan_sh[T / 64 + 1];
+    __shared__ int fallback;
+
+    // Records into registers (striped, coalesced); min and max key of the tile.
+    Rec r[E];
+    uint64_t lo0 = ~0ull, lo1 = ~0ull, hi0 = 0, hi1 = 0;
 #pragma unroll
     for (int k = 0; k < E; k++) {
-        int i = k * T + tid;
-        Rec r;
-        if (i < n) r = src[i];
-        else { r.w0 = ~0ull; r.w1 = ~0ull; }
-        sk[i] = r;
+        const int i = k * T + tid;
+        if (i < n) {
+            r[k] = src[i];
+            if (r[k].w0 < lo0 || (r[k].w0 == lo0 && r[k].w1 < lo1)) { lo0 = r[k].w0; lo1 = r[k].w1; }
+            if (r[k].w0 > hi0 || (r[k].w0 == hi0 && r[k].w1 > hi1)) { hi0 = r[k].w0; hi1 = r[k].w1; }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t x0 = __shfl_xor(lo0, o, 64), x1 = __shfl_xor(lo1, o, 64);
+        if (x0 < lo0 || (x0 == lo0 && x1 < lo1)) { lo0 = x0; lo1 = x1; }
+        uint64_t y0 = __shfl_xor(hi0, o, 64), y1 = __shfl_xor(hi1, o, 64);
+        if (y0 > hi0 || (y0 == hi0 && y1 > hi1)) { hi0 = y0; hi1 = y1; }
+    }
+    if ((tid & 63) == 0) { mn0[tid >> 6] = lo0; mn1[tid >> 6] = lo1; mx0[tid >> 6] = hi0; mx1[tid >> 6] = hi1; }
+    for (int c = tid; c < NB; c += T) bh[c] = 0;
+    if (tid == 0) fallback = 0;
+    __syncthreads();
+    lo0 = mn0[0]; lo1 = mn1[0]; hi0 = mx0[0]; hi1 = mx1[0];
+    for (int w = 1; w < T / 64; w++) {
+        if (mn0[w] < lo0 || (mn0[w] == lo0 && mn1[w] < lo1)) { lo0 = mn0[w]; lo1 = mn1[w]; }
+        if (mx0[w] > hi0 || (mx0[w] == hi0 && mx1[w] > hi1)) { hi0 = mx0[w]; hi1 = mx1[w]; }
+    }
+    // Bin = the log2(NB) key bits right after the tile's common prefix (the
+    // first bit where min and max differ).  Keys are near-uniform below that
+    // prefix, so bins hold ~2 records; a bin is insertion-sorted by one thread.
+    constexpr int BB = __builtin_ctz(NB);
+    const int cp = (lo0 != hi0) ? __builtin_clzll(lo0 ^ hi0)
+                 : (lo1 != hi1 ? 64 + __builtin_clzll(lo1 ^ hi1) : 128);
+    auto bin_of = [&](uint64_t w0, uint64_t w1) -> uint32_t {
+        uint64_t top;
+        if (cp >= 64) top = cp >= 128 ? 0 : (w1 << (cp - 64));
+        else top = cp == 0 ? w0 : ((w0 << cp) | (w1 >> (64 - cp)));
+        return (uint32_t) (top >> (64 - BB));
+    };
+    uint32_t rk[E];
+#pragma unroll
+    for (int k = 0; k < E; k++)
+        if (k * T + tid < n) rk[k] = atomicAdd(&bh[bin_of(r[k].w0, r[k].w1)], 1u);
+    __syncthreads();
+    constexpr int BPT = NB / T;                     // bins per thread
+    uint32_t cnt[BPT], sum = 0, mx = 0;
+#pragma unroll
+    for (int q = 0; q < BPT; q++) { cnt[q] = bh[tid * BPT + q]; sum += cnt[q]; mx = cnt[q] > mx ? cnt[q] : mx; }
+    if (mx > 32) fallback = 1;                      // a crowded bin (repeats): merge sort instead
+    uint32_t tot;
+    uint32_t run = block_excl_scan<T, uint32_t>(sum, scan_sh, &tot);
+#pragma unroll
+    for (int q = 0; q < BPT; q++) { bh[tid * BPT + q] = run; run += cnt[q]; }
+    __syncthreads();
+    const bool binned = !fallback;
+    if (binned) {
+#pragma unroll
+        for (int k = 0; k < E; k++)
+            if (k * T + tid < n) sk[bh[bin_of(r[k].w0, r[k].w1)] + rk[k]] = r[k];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BPT; q++) {
+            const int b0 = (int) bh[tid * BPT + q];
+            const int b1 = b0 + (int) cnt[q];
+            for (int i = b0 + 1; i < b1; i++) {
+                Rec x = sk[i];
+                int j = i - 1;
+                while (j >= b0) {
+                    Rec y = sk[j];
+                    if (key_le(y.w0, y.w1, x.w0, x.w1)) break;
+                    sk[j + 1] = y;
+                    j--;
+                }
+                sk[j + 1] = x;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int i = k * T + tid;
+            if (i >= n) { r[k].w0 = ~0ull; r[k].w1 = ~0ull; }
+            sk[i] = r[k];
+        }
     }
     __syncthreads();
     uint64_t a0[E], a1[E];
 #pragma unroll
-    for (int k = 0; k < E; k++) { Rec r = sk[tid * E + k]; a0[k] = r.w0; a1[k] = r.w1; }
+    for (int k = 0; k < E; k++) { Rec x = sk[tid * E + k]; a0[k] = x.w0; a1[k] = x.w1; }
+    if (!binned) {
     // optimal 19-comparator network for 8 inputs
     PGA_CE(0, 2); PGA_CE(1, 3); PGA_CE(4, 6); PGA_CE(5, 7);
     PGA_CE(0, 4); PGA_CE(1, 5); PGA_CE(2, 6); PGA_CE(3, 7);
@@ -624,8 +708,9 @@ __global__ __launch_bounds__(T) void k_tile_sort(Rec *const *__restrict__ bufs,
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < E; k++) { Rec r; r.w0 = a0[k]; r.w1 = a1[k]; sk[tid * E + k] = r; }
+    for (int k = 0; k < E; k++) { Rec x; x.w0 = a0[k]; x.w1 = a1[k]; sk[tid * E + k] = x; }
     __syncthreads();
+    }
     uint64_t p0 = 0, p1 = 0;
     if (tid > 0) { Rec r = sk[tid * E - 1]; p0 = r.w0; p1 = r.w1; }
     __syncthreads();                                // LDS is reused for entry bytes below
@@ -758,21 +843,79 @@ __global__ void k_long_fill(const LongFill *__restrict__ longs)
     for (int64_t q = lf.from + threadIdx.x; q < lf.to; q += blockDim.x) lf.cum[q] = lf.v;
 }
 
-__global__ void k_maxpre(const PairDev *__restrict__ pairs, int npairs, int64_t R,
-                         const uint32_t *__restrict__ pre, const int64_t *__restrict__ cum,
-                         unsigned long long *__restrict__ maxpre)
+// Largest prefix group per pair, without same-address atomics: each block of
+// MAXPRE_SPAN records writes the largest group ending in it (for the pair of
+// its first record) to bmax[block]; k_maxpre_reduce takes the maximum over each
+// pair's blocks.  Groups of a pair that starts inside a block are rare (one
+// block per pair boundary) and go through an atomic.
+constexpr int MAXPRE_PER = 16;
+constexpr int MAXPRE_SPAN = 256 * MAXPRE_PER;
+
+__global__ __launch_bounds__(256) void k_maxpre(const PairDev *__restrict__ pairs, int npairs, int64_t R,
+                                                const uint32_t *__restrict__ pre,
+                                                const int64_t *__restrict__ cum,
+                                                unsigned long long *__restrict__ bmax,
+                                                unsigned long long *__restrict__ maxpre)
 {
-    int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= R) return;
-    const int p = pair_of(pairs, npairs, g);
-    const PairDev pd = pairs[p];
-    const int64_t li = g - pd.rec0;
-    const int64_t cur = pre[g];
-    const bool last = (li + 1 == pd.nrec) || pre[g + 1] != (uint32_t) cur;
-    if (!last) return;
-    const int64_t *cp = cum + pd.cum_off;
-    int64_t size = cp[cur] - (cur > 0 ? cp[cur - 1] : 0);
-    atomicMax(&maxpre[p], (unsigned long long) size);
+    __shared__ unsigned long long red[4];
+    const int64_t base = (int64_t) blockIdx.x * MAXPRE_SPAN;
+    const int p0 = pair_of(pairs, npairs, base);
+    const PairDev pd0 = pairs[p0];
+    unsigned long long m = 0;
+    for (int k = 0; k < MAXPRE_PER; k++) {
+        const int64_t g = base + (int64_t) k * 256 + threadIdx.x;
+        if (g >= R) break;
+        const bool in0 = g < pd0.rec0 + pd0.nrec;
+        const int p = in0 ? p0 : pair_of(pairs, npairs, g);
+        const int64_t rec0 = in0 ? pd0.rec0 : pairs[p].rec0;
+        const int64_t nrec = in0 ? pd0.nrec : pairs[p].nrec;
+        const int64_t coff = in0 ? pd0.cum_off : pairs[p].cum_off;
+        const uint32_t cur = pre[g];
+        if (!((g + 1 == rec0 + nrec) || pre[g + 1] != cur)) continue;
+        const int64_t *c = cum + coff;
+        const unsigned long long size = (unsigned long long) (c[cur] - (cur > 0 ? c[cur - 1] : 0));
+        if (in0) m = size > m ? size : m;
+        else atomicMax(&maxpre[p], size);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long x = __shfl_xor(m, o, 64);
+        m = x > m ? x : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = red[0];
+        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
+        bmax[blockIdx.x] = x;
+    }
+}
+
+// One block per pair: maximum of bmax over the blocks whose first record is the pair's.
+__global__ __launch_bounds__(256) void k_maxpre_reduce(const PairDev *__restrict__ pairs,
+                                                       const unsigned long long *__restrict__ bmax,
+                                                       unsigned long long *__restrict__ maxpre)
+{
+    __shared__ unsigned long long red[4];
+    const PairDev pd = pairs[blockIdx.x];
+    unsigned long long m = 0;
+    if (pd.nrec > 0) {
+        const int64_t b0 = (pd.rec0 + MAXPRE_SPAN - 1) / MAXPRE_SPAN;
+        const int64_t b1 = (pd.rec0 + pd.nrec - 1) / MAXPRE_SPAN;
+        for (int64_t b = b0 + threadIdx.x; b <= b1; b += 256) m = bmax[b] > m ? bmax[b] : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long x = __shfl_xor(m, o, 64);
+        m = x > m ? x : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = red[0];
+        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
+        if (x) atomicMax(&maxpre[blockIdx.x], x);
+    }
 }
 
 // ---------------------------------------------------------------- adaptamer merge
@@ -842,6 +985,69 @@ __global__ __launch_bounds__(MT) void k_merge_count(const PairDev *__restrict__ 
     if (threadIdx.x == 0) { cntN[blockIdx.x] = (uint32_t) sn; cntC[blockIdx.x] = (uint32_t) sc; }
 }
 
+constexpr int MERGE_STG = 12288;     // bytes of LDS seed staging per round
+constexpr int BUCK_LDS = 2048;       // per-pair contigs counted in LDS by the merge
+
+// Write anchor gi's seeds of one stream (same strand or not) whose indices in
+// the block's output fall in [r0, r1) into the staging buffer.  Payload of
+// window entry k: ent + k*e + 9.
+__device__ __forceinline__ void stage_seeds(uint8_t *__restrict__ stg, int shift, int64_t r0, int64_t r1,
+                                            int64_t o, const uint8_t *__restrict__ win,
+                                            const uint8_t *__restrict__ ent, int e, int gi, int a, int b,
+                                            int plen, bool same, int pc, int s)
+{
+    const uint8_t si = win[gi] & 0x80;
+    const uint8_t *pi = ent + gi * e + 9;
+    int64_t k = o;
+    for (int q = a; q < b && k < r1; q++) {
+        if (q == gi || (((win[q] & 0x80) == si) != same)) continue;
+        if (k >= r0) {
+            uint8_t *d = stg + shift + (k - r0) * s;
+            d[0] = (uint8_t) plen;
+            for (int t = 0; t < pc; t++) d[1 + t] = pi[t];
+            d[pc] &= 0x7F;                          // anchor's strand bit cleared
+            const uint8_t *pq = ent + q * e + 9;
+            for (int t = 0; t < pc; t++) d[1 + pc + t] = pq[t];
+        }
+        k++;
+    }
+}
+
+// Copy nb staged bytes (starting at stg[shift]) to g, 16-byte aligned stores
+// inside, byte stores at the two ends.
+template <int NT>
+__device__ __forceinline__ void copy_out(uint8_t *__restrict__ g, const uint8_t *__restrict__ stg,
+                                         int shift, int nb)
+{
+    const int nchunks = (shift + nb + 15) >> 4;
+    for (int c = threadIdx.x; c < nchunks; c += NT) {
+        const int b0 = c << 4;
+        if (b0 >= shift && b0 + 16 <= shift + nb) {
+            *reinterpret_cast<uint4 *>(g + b0) = *reinterpret_cast<const uint4 *>(stg + b0);
+        } else {
+            for (int b = b0; b < b0 + 16; b++)
+                if (b >= shift && b < shift + nb) g[b] = stg[b];
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void flush_buck(uint32_t *__restrict__ lb, int n, int64_t off,
+                                           unsigned long long *__restrict__ nbuck,
+                                           unsigned long long *__restrict__ cbuck)
+{
+    for (int c = threadIdx.x; c < n; c += NT) {
+        if (lb[c]) { atomicAdd(&nbuck[off + c], (unsigned long long) lb[c]); lb[c] = 0; }
+        if (lb[BUCK_LDS + c]) { atomicAdd(&cbuck[off + c], (unsigned long long) lb[BUCK_LDS + c]); lb[BUCK_LDS + c] = 0; }
+    }
+}
+
+// Seeds of a run of merge tiles (MT anchors each).  Per tile: the LCP/strand
+// bytes and raw GIX entries of the tile plus a FREQ halo on each side go to
+// LDS, each anchor finds its interval, a block scan places its seeds in the
+// tile's slice of the N and C streams (offsets from the count pass), and the
+// seeds are staged in LDS and written out with aligned 16-byte stores.
+// Per-anchor-contig counts accumulate in LDS while the pair stays the same.
 template <int MT>
 __global__ __launch_bounds__(MT) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
                                                    const uint8_t *__restrict__ ls,
@@ -850,78 +1056,126 @@ __global__ __launch_bounds__(MT) void k_merge_emit(const PairDev *__restrict__ p
                                                    const int64_t *__restrict__ ofsC,
                                                    uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
                                                    unsigned long long *__restrict__ nbuck,
-                                                   unsigned long long *__restrict__ cbuck)
+                                                   unsigned long long *__restrict__ cbuck,
+                                                   int64_t nmb, int64_t per_block)
 {
     constexpr int W = MT + 2 * MAX_FREQ + 4;
     __shared__ uint8_t win[W];
-    __shared__ uint8_t pay[W * 8];
+    __shared__ __align__(16) uint8_t ent[W * 16 + 16];
+    __shared__ __align__(16) uint8_t stg[MERGE_STG + 16];
+    __shared__ uint32_t lb[2 * BUCK_LDS];
     __shared__ uint32_t sh[MT / 64 + 1];
-    const int p = merge_block_pair(pairs, npairs, blockIdx.x);
-    const PairDev pd = pairs[p];
-    const int64_t lo = pd.rec0 + (blockIdx.x - pd.mblk0) * (int64_t) MT;
-    const int64_t end = pd.rec0 + pd.nrec;
-    const int H = freq + 1;
-    const int pc = pd.P + pd.C, e = pd.e;
-    for (int k = threadIdx.x; k < MT + 2 * H; k += MT) {
-        int64_t g = lo - H + k;
-        bool in = g >= pd.rec0 && g < end;
-        win[k] = in ? ls[g] : 0;
-        if (in) {
-            const uint8_t *src = entries + pd.ent_off + (g - pd.rec0) * (int64_t) e + 9;
-            for (int b = 0; b < pc; b++) pay[k * 8 + b] = src[b];
+    for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MT) lb[c] = 0;
+    const int64_t t0 = (int64_t) blockIdx.x * per_block;
+    const int64_t t1 = t0 + per_block < nmb ? t0 + per_block : nmb;
+    int cur_pair = -1;
+    for (int64_t tile = t0; tile < t1; tile++) {
+        const int p = merge_block_pair(pairs, npairs, tile);
+        const PairDev pd = pairs[p];
+        const bool lds_buck = pd.ncontig <= BUCK_LDS;
+        if (p != cur_pair) {
+            __syncthreads();
+            if (cur_pair >= 0 && pairs[cur_pair].ncontig <= BUCK_LDS)
+                flush_buck<MT>(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
+            cur_pair = p;
+        }
+        const int64_t lo = pd.rec0 + (tile - pd.mblk0) * (int64_t) MT;
+        const int64_t end = pd.rec0 + pd.nrec;
+        const int H = freq + 1;
+        const int pc = pd.P + pd.C, e = pd.e, s = pd.s;
+        // window [lo-H, lo+MT+H) clipped to the pair; raw entry bytes with 4-byte loads
+        const int64_t wlo = lo - H > pd.rec0 ? lo - H : pd.rec0;
+        const int64_t whi = lo + MT + H < end ? lo + MT + H : end;
+        const int64_t b0 = pd.ent_off + (wlo - pd.rec0) * (int64_t) e;
+        const int64_t b1 = pd.ent_off + (whi - pd.rec0) * (int64_t) e;
+        const int64_t a0 = b0 & ~(int64_t) 3;
+        const int nwords = (int) ((b1 - a0 + 3) >> 2);
+        __syncthreads();
+        for (int k = threadIdx.x; k < MT + 2 * H; k += MT) {
+            const int64_t g = lo - H + k;
+            win[k] = (g >= pd.rec0 && g < end) ? ls[g] : 0;
+        }
+        {
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(entries + a0);
+            uint8_t *dst = ent + (int) (wlo - (lo - H)) * e;
+            for (int k = threadIdx.x; k < nwords; k += MT) {
+                uint32_t v = src[k];
+                dst[4 * k + 0] = (uint8_t) v;
+                dst[4 * k + 1] = (uint8_t) (v >> 8);
+                dst[4 * k + 2] = (uint8_t) (v >> 16);
+                dst[4 * k + 3] = (uint8_t) (v >> 24);
+            }
+        }
+        __syncthreads();
+        // window entry k (global lo-H+k) starts at ew + k*e; entries outside
+        // [wlo, whi) are never read (their LCP byte is 0, a run boundary)
+        const uint8_t *ew = ent + (int) (b0 - a0);
+        const int64_t g = lo + threadIdx.x;
+        const int gi = threadIdx.x + H;
+        uint32_t nN = 0, nC = 0;
+        int plen = 0, a = 0, b = 0;
+        const bool act = g < end && anchor_interval(win, gi, freq, plen, a, b);
+        if (act) {
+            const uint8_t si = win[gi] & 0x80;
+            for (int q = a; q < b; q++) {
+                if (q == gi) continue;
+                if ((win[q] & 0x80) == si) nN++; else nC++;
+            }
+        }
+        uint32_t totN, totC;
+        const uint32_t oN = block_excl_scan<MT, uint32_t>(nN, sh, &totN);
+        const uint32_t oC = block_excl_scan<MT, uint32_t>(nC, sh, &totC);
+        const int per_round = MERGE_STG / s;
+#pragma unroll
+        for (int str = 0; str < 2; str++) {
+            const bool same = str == 0;
+            const uint32_t tot = same ? totN : totC;
+            const uint32_t o = same ? oN : oC;
+            const uint32_t mine = same ? nN : nC;
+            uint8_t *gbase = same ? seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s
+                                  : seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
+            for (int64_t r0 = 0; r0 < tot; r0 += per_round) {
+                const int64_t r1 = r0 + per_round < tot ? r0 + per_round : tot;
+                uint8_t *gr = gbase + r0 * s;
+                const int shift = (int) ((uintptr_t) gr & 15);
+                if (mine && o < r1 && o + mine > r0)
+                    stage_seeds(stg, shift, r0, r1, o, win, ew, e, gi, a, b, plen, same, pc, s);
+                __syncthreads();
+                copy_out<MT>(gr - shift, stg, shift, (int) ((r1 - r0) * s));
+                __syncthreads();
+            }
+        }
+        if (act && (nN | nC)) {
+            // per-anchor-contig counts (src/fastga_pipeline.c:403)
+            const uint8_t *pi = ew + gi * e + 9;
+            uint64_t icont = 0;
+            for (int k = 0; k < pd.C; k++) icont |= (uint64_t) pi[pd.P + k] << (8 * k);
+            icont &= ~(1ull << (8 * pd.C - 1));
+            if (lds_buck) {
+                if (nN) atomicAdd(&lb[icont], nN);
+                if (nC) atomicAdd(&lb[BUCK_LDS + icont], nC);
+            } else {
+                if (nN) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nN);
+                if (nC) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nC);
+            }
         }
     }
     __syncthreads();
-    const int64_t g = lo + threadIdx.x;
-    const int gi = threadIdx.x + H;
-    uint32_t nN = 0, nC = 0;
-    int plen = 0, a = 0, b = 0;
-    bool act = g < end && anchor_interval(win, gi, freq, plen, a, b);
-    if (act) {
-        const uint8_t si = win[gi] & 0x80;
-        for (int q = a; q < b; q++) {
-            if (q == gi) continue;
-            if ((win[q] & 0x80) == si) nN++; else nC++;
-        }
-    }
-    uint32_t totN, totC;
-    uint32_t oN = block_excl_scan<MT, uint32_t>(nN, sh, &totN);
-    uint32_t oC = block_excl_scan<MT, uint32_t>(nC, sh, &totC);
-    if (!act || (nN + nC) == 0) return;
-    const int s = pd.s;
-    uint8_t *dN = seedN + pd.nseed_off + (ofsN[blockIdx.x] - ofsN[pd.mblk0] + oN) * (int64_t) s;
-    uint8_t *dC = seedC + pd.cseed_off + (ofsC[blockIdx.x] - ofsC[pd.mblk0] + oC) * (int64_t) s;
-    const uint8_t *pi = pay + gi * 8;
-    const uint8_t si = win[gi] & 0x80;
-    for (int q = a; q < b; q++) {
-        if (q == gi) continue;
-        uint8_t *d;
-        if ((win[q] & 0x80) == si) { d = dN; dN += s; } else { d = dC; dC += s; }
-        d[0] = (uint8_t) plen;
-        for (int k = 0; k < pc; k++) d[1 + k] = pi[k];
-        d[pc] &= 0x7F;                              // anchor's strand bit cleared
-        const uint8_t *pq = pay + q * 8;
-        for (int k = 0; k < pc; k++) d[1 + pc + k] = pq[k];
-    }
-    // per-anchor-contig counts (src/fastga_pipeline.c:403)
-    uint64_t icont = 0;
-    for (int k = 0; k < pd.C; k++) icont |= (uint64_t) pi[pd.P + k] << (8 * k);
-    icont &= ~(1ull << (8 * pd.C - 1));
-    if (nN) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nN);
-    if (nC) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nC);
+    if (cur_pair >= 0 && pairs[cur_pair].ncontig <= BUCK_LDS)
+        flush_buck<MT>(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
 }
 
 template __global__ void k_merge_count<256>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
 template __global__ void k_merge_emit<256>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
                                            const int64_t *, const int64_t *, uint8_t *, uint8_t *,
-                                           unsigned long long *, unsigned long long *);
+                                           unsigned long long *, unsigned long long *, int64_t, int64_t);
 template __global__ void k_merge_count<512>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
 template __global__ void k_merge_emit<512>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
                                            const int64_t *, const int64_t *, uint8_t *, uint8_t *,
-                                           unsigned long long *, unsigned long long *);
+                                           unsigned long long *, unsigned long long *, int64_t, int64_t);
 template __global__ void k_merge_count<1024>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
 template __global__ void k_merge_emit<1024>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
                                             const int64_t *, const int64_t *, uint8_t *, uint8_t *,
-                                            unsigned long long *, unsigned long long *);
+                                            unsigned long long *, unsigned long long *, int64_t, int64_t);
 
 } // namespace pga

@@ -49,6 +49,7 @@ def _setup(L):
     L.pga_default_opts.argtypes = [C.POINTER(Opts)]
     L.pga_default_opts.restype = None
     L.pga_create.argtypes = [C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    L.pga_set_opts.argtypes = [C.c_void_p, C.POINTER(Opts)]
     L.pga_destroy.argtypes = [C.c_void_p]
     L.pga_destroy.restype = None
     L.pga_last_error.argtypes = [C.c_void_p]
@@ -112,6 +113,15 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_opts(self, **kw):
+        """Change tunables (tile_records, chunk_records, merge_tile, freq, build_cum) in place."""
+        o = Opts()
+        C.memmove(C.byref(o), C.byref(self.opts), C.sizeof(Opts))
+        for k, v in kw.items():
+            setattr(o, k, int(v))
+        self._check(self._L.pga_set_opts(self._h, C.byref(o)), "pga_set_opts")
+        self.opts = o
 
     # -- input
     def load_batch(self, gdbs):

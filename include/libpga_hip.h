@@ -87,6 +87,9 @@ typedef struct {
 
 void        pga_default_opts(pga_opts *o);
 int         pga_create(const pga_opts *o, pga_ctx **out);
+/* Change the tunables of a context (the device stays); buffers are reused.
+ * Takes effect from the next pga_build_index / pga_merge. */
+int         pga_set_opts(pga_ctx *ctx, const pga_opts *o);
 void        pga_destroy(pga_ctx *ctx);
 const char *pga_last_error(const pga_ctx *ctx);
 
