@@ -74,6 +74,7 @@ int bits_for(int64_t v)           // bits to hold every value in [0, v]
 struct pga_ctx {
     pga_opts o{};
     hipStream_t st = nullptr;
+    int ncu = 256;
     char err[512] = {0};
 
     // loaded batch
@@ -193,53 +194,70 @@ int scan_u32(pga_ctx *c, const uint32_t *in, int64_t n, int64_t *out)
 
 int cap_of(const pga_ctx *c) { return c->o.tile_records; }
 
+// Persistent grid: as many blocks as fit on the device at once (or fewer when
+// there is less work); each block loops over logical blocks.
+unsigned grid_for(const pga_ctx *c, const void *fn, int threads, size_t lds, int64_t nlog)
+{
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, lds) != hipSuccess || occ < 1) {
+        (void) hipGetLastError();
+        occ = 1;
+    }
+    const int64_t g = std::min<int64_t>(nlog, (int64_t) c->ncu * occ);
+    return (unsigned) std::max<int64_t>(g, 1);
+}
+
 int chunk_per(const pga_ctx *c) { return c->o.chunk_records / 256; }
 
-int launch_seg_hist(pga_ctx *c, int per, unsigned grid, const Rec *src, const Seg *segs, int nseg,
+int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *src, const Seg *segs, int nseg,
                     const int64_t *chk0, unsigned long long *hist)
 {
+#define PGA_HIST(PER)                                                                                  \
+    hipLaunchKernelGGL(k_seg_hist<PER>, dim3(grid_for(c, (const void *) k_seg_hist<PER>, 256, 0, nlog)), \
+                       dim3(256), 0, c->st, src, segs, nseg, chk0, hist, (int64_t) nlog)
     switch (per) {
-    case 8:  hipLaunchKernelGGL(k_seg_hist<8>, dim3(grid), dim3(256), 0, c->st, src, segs, nseg, chk0, hist); break;
-    case 32: hipLaunchKernelGGL(k_seg_hist<32>, dim3(grid), dim3(256), 0, c->st, src, segs, nseg, chk0, hist); break;
-    default: hipLaunchKernelGGL(k_seg_hist<16>, dim3(grid), dim3(256), 0, c->st, src, segs, nseg, chk0, hist); break;
+    case 8:  PGA_HIST(8); break;
+    case 32: PGA_HIST(32); break;
+    default: PGA_HIST(16); break;
     }
+#undef PGA_HIST
     LAUNCHCHK(c);
     return PGA_OK;
 }
 
-int launch_seg_scatter(pga_ctx *c, int per, unsigned grid, const Rec *src, Rec *dst, const Seg *segs,
+int launch_seg_scatter(pga_ctx *c, int per, unsigned nlog, const Rec *src, Rec *dst, const Seg *segs,
                        int nseg, const int64_t *chk0, unsigned long long *cursor)
 {
+#define PGA_SCAT(PER)                                                                                  \
+    hipLaunchKernelGGL(k_seg_scatter<PER>, dim3(grid_for(c, (const void *) k_seg_scatter<PER>, 256, 0, nlog)), \
+                       dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor, (int64_t) nlog)
     switch (per) {
-    case 8:  hipLaunchKernelGGL(k_seg_scatter<8>, dim3(grid), dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor); break;
-    case 32: hipLaunchKernelGGL(k_seg_scatter<32>, dim3(grid), dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor); break;
-    default: hipLaunchKernelGGL(k_seg_scatter<16>, dim3(grid), dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor); break;
+    case 8:  PGA_SCAT(8); break;
+    case 32: PGA_SCAT(32); break;
+    default: PGA_SCAT(16); break;
     }
+#undef PGA_SCAT
     LAUNCHCHK(c);
     return PGA_OK;
 }
 
-int launch_tile_sort(pga_ctx *c, unsigned grid)
+int launch_tile_sort(pga_ctx *c, unsigned nlog)
 {
     const int T = c->o.tile_records / 8;
     const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
     Rec *const *bufs = (Rec *const *) c->bufs.p;
     const Tile *tiles = P<Tile>(c->tiles);
     const PairDev *pairs = P<PairDev>(c->dpairs);
+#define PGA_TSORT(TV)                                                                                   \
+    hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
+                       dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, P<uint8_t>(c->entries),      \
+                       P<uint8_t>(c->ls), P<uint32_t>(c->pre), (int64_t) nlog)
     switch (T) {
-    case 256:
-        hipLaunchKernelGGL(k_tile_sort<256>, dim3(grid), dim3(256), lds, c->st, bufs, tiles, pairs, c->jbits,
-                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), P<uint32_t>(c->pre));
-        break;
-    case 1024:
-        hipLaunchKernelGGL(k_tile_sort<1024>, dim3(grid), dim3(1024), lds, c->st, bufs, tiles, pairs, c->jbits,
-                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), P<uint32_t>(c->pre));
-        break;
-    default:
-        hipLaunchKernelGGL(k_tile_sort<512>, dim3(grid), dim3(512), lds, c->st, bufs, tiles, pairs, c->jbits,
-                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), P<uint32_t>(c->pre));
-        break;
+    case 256: PGA_TSORT(256); break;
+    case 1024: PGA_TSORT(1024); break;
+    default: PGA_TSORT(512); break;
     }
+#undef PGA_TSORT
     LAUNCHCHK(c);
     return PGA_OK;
 }
@@ -325,6 +343,10 @@ int pga_create(const pga_opts *o, pga_ctx **out)
     }
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { delete c; return PGA_EHIP; }
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, q.device) != hipSuccess || c->ncu < 1) {
+        (void) hipGetLastError();
+        c->ncu = 256;
+    }
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_tmap), TMAP, sizeof(TMAP)) != hipSuccess) { delete c; return PGA_EHIP; }
     const size_t big = (size_t) 8192 * sizeof(Rec) + 4096 * 4 + 16;
     (void) hipFuncSetAttribute((const void *) k_tile_sort<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) big);
@@ -466,8 +488,9 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->blkoff, (size_t) (nblk + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
     if (nblk > 0) {
-        hipLaunchKernelGGL(k_extract_count, dim3((unsigned) nblk), dim3(256), 0, c->st,
-                           P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, P<uint32_t>(c->blkcnt));
+        hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, nblk)), dim3(256),
+                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, P<uint32_t>(c->blkcnt),
+                           nblk);
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
     } else {
@@ -529,10 +552,10 @@ int pga_build_index(pga_ctx *c)
 
     // ---- extraction: emit records in extraction order + level-1 digit counts
     if (nblk > 0 && R > 0) {
-        hipLaunchKernelGGL(k_extract_emit, dim3((unsigned) nblk), dim3(256), 0, c->st,
-                           P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, c->jbits,
+        hipLaunchKernelGGL(k_extract_emit, dim3(grid_for(c, (const void *) k_extract_emit, 256, 0, nblk)), dim3(256),
+                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, c->jbits,
                            (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
-                           P<unsigned long long>(c->hist1));
+                           P<unsigned long long>(c->hist1), nblk);
         LAUNCHCHK(c);
     }
     TRY(record(c, 1));
@@ -543,7 +566,7 @@ int pga_build_index(pga_ctx *c)
     for (int p = 0; p < npairs; p++) {
         const PairDev &d = c->pairs[p];
         if (d.nrec > CAP) seg0.push_back(Seg{d.rec0, d.nrec, p, 1});
-        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 0, 0});
+        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 0, 1});
     }
     int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles, [2] long fills
     int hcount[4] = {0, (int) tile0.size(), 0, 0};
@@ -606,7 +629,8 @@ int pga_build_index(pga_ctx *c)
     if (c->ntiles > 0) TRY(launch_tile_sort(c, (unsigned) c->ntiles));
     TRY(record(c, 3));
     if (c->ntiles > 0) {
-        hipLaunchKernelGGL(k_tile_fix, dim3((c->ntiles + 255) / 256), dim3(256), 0, c->st,
+        hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->ntiles + 255) / 256)),
+                           dim3(256), 0, c->st,
                            (const Tile *) P<Tile>(c->tiles), c->ntiles, (const PairDev *) P<PairDev>(c->dpairs),
                            P<uint8_t>(c->entries), P<uint8_t>(c->ls), (const uint32_t *) P<uint32_t>(c->pre));
         LAUNCHCHK(c);
@@ -622,7 +646,8 @@ int pga_build_index(pga_ctx *c)
             if (c->pairs[p].nrec == 0) TRY(fill_u64(c, P<int64_t>(c->cum) + (int64_t) p * NPRE, NPRE, 0));
         HIPCHK(c, hipMemsetAsync(dcount + 2, 0, sizeof(int), c->st));
         if (R > 0) {
-            hipLaunchKernelGGL(k_cum_fill, dim3((unsigned) ((R + 255) / 256)), dim3(256), 0, c->st,
+            hipLaunchKernelGGL(k_cum_fill, dim3(grid_for(c, (const void *) k_cum_fill, 256, 0, (R + 255) / 256)),
+                               dim3(256), 0, c->st,
                                (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
                                P<int64_t>(c->cum), P<LongFill>(c->longs), dcount + 2);
             LAUNCHCHK(c);
@@ -723,11 +748,17 @@ int pga_merge(pga_ctx *c)
     HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
     std::vector<int64_t> po(2 * (npairs + 1), 0);
     if (nmb > 0) {
+#define PGA_COUNT(A_)                                                                                   \
+    hipLaunchKernelGGL(k_merge_count<A_>, dim3(grid_for(c, (const void *) k_merge_count<A_>, MTH, 0, nmb)), \
+                       dim3(MTH), 0, c->st,                                                             \
+                       (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
+                       c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC), nmb)
         switch (MT) {
-        case 512: hipLaunchKernelGGL(k_merge_count<512>, dim3((unsigned) nmb), dim3(512), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC)); break;
-        case 1024: hipLaunchKernelGGL(k_merge_count<1024>, dim3((unsigned) nmb), dim3(1024), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC)); break;
-        default: hipLaunchKernelGGL(k_merge_count<256>, dim3((unsigned) nmb), dim3(256), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC)); break;
+        case 512: PGA_COUNT(2); break;
+        case 1024: PGA_COUNT(4); break;
+        default: PGA_COUNT(1); break;
         }
+#undef PGA_COUNT
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->mcntN), nmb, P<int64_t>(c->mofsN)));
         TRY(scan_u32(c, P<uint32_t>(c->mcntC), nmb, P<int64_t>(c->mofsC)));
@@ -761,20 +792,30 @@ int pga_merge(pga_ctx *c)
     TRY(fill_u64(c, c->nbuck.p, nb_elems, 0));
     TRY(fill_u64(c, c->cbuck.p, nb_elems, 0));
     HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
-    const int64_t egrid = std::min<int64_t>(nmb, 4096);
+    // a whole number of resident waves of blocks, four rounds deep
+    int occ = 1;
+    {
+        const void *fn = MT == 512 ? (const void *) k_merge_emit<2>
+                       : MT == 1024 ? (const void *) k_merge_emit<4> : (const void *) k_merge_emit<1>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, MTH, 0) != hipSuccess || occ < 1) {
+            (void) hipGetLastError();
+            occ = 1;
+        }
+    }
+    const int64_t egrid = std::min<int64_t>(nmb, (int64_t) c->ncu * occ * 4);
     const int64_t eper = egrid > 0 ? (nmb + egrid - 1) / egrid : 0;
     if (nmb > 0) {
-#define PGA_EMIT(MTV)                                                                                   \
-    hipLaunchKernelGGL(k_merge_emit<MTV>, dim3((unsigned) egrid), dim3(MTV), 0, c->st,                  \
+#define PGA_EMIT(A_)                                                                                    \
+    hipLaunchKernelGGL(k_merge_emit<A_>, dim3((unsigned) egrid), dim3(MTH), 0, c->st,                   \
                        (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
                        (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
                        (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
                        P<uint8_t>(c->seedN), P<uint8_t>(c->seedC), P<unsigned long long>(c->nbuck),     \
                        P<unsigned long long>(c->cbuck), nmb, eper)
         switch (MT) {
-        case 512: PGA_EMIT(512); break;
-        case 1024: PGA_EMIT(1024); break;
-        default: PGA_EMIT(256); break;
+        case 512: PGA_EMIT(2); break;
+        case 1024: PGA_EMIT(4); break;
+        default: PGA_EMIT(1); break;
         }
 #undef PGA_EMIT
         LAUNCHCHK(c);

@@ -38,7 +38,8 @@ struct Seg {                     // a key range still to be partitioned
 
 struct Tile {                    // a bucket run small enough for one workgroup sort
     int64_t start;
-    int32_t count, pair, buf, pad;
+    int32_t count, pair, buf;
+    int32_t level;               // radix digit (1..16) its buckets differ in; all share the digits above
 };
 
 struct LongFill {

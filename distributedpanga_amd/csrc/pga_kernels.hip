@@ -179,81 +179,92 @@ __device__ __forceinline__ int scan_position(const uint64_t *__restrict__ bps64,
     return (int) out.fwd + (int) out.rev;
 }
 
+// Contig of the block's first position (one binary search per block); every
+// thread then steps forward from it.  Contigs hold >= 1 candidate position or
+// none, so a thread of XU positions crosses few contig starts.
+__device__ __forceinline__ int block_contig(const ContigDev *__restrict__ ctg, int nctg, int64_t fb, int *sc)
+{
+    if (threadIdx.x == 0) *sc = locate_contig(ctg, nctg, fb);
+    __syncthreads();
+    return *sc;
+}
+
 __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
                                                       const ContigDev *__restrict__ ctg, int nctg,
-                                                      int64_t F, uint32_t *__restrict__ blkcnt)
+                                                      int64_t F, uint32_t *__restrict__ blkcnt, int64_t nlog)
 {
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint64_t red[XT / 64 + 1];
+    __shared__ int sc;
     load_tables(tl, tm);
-    __syncthreads();
-    int64_t f0 = ((int64_t) blockIdx.x * XT + threadIdx.x) * XU;
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
+    const int64_t fb = LB * XT * XU;
+    int c = block_contig(ctg, nctg, fb, &sc);
+    const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint32_t cnt = 0;
-    if (f0 < F) {
-        int c = locate_contig(ctg, nctg, f0);
-        XEmit dummy;
+    XEmit dummy;
 #pragma unroll
-        for (int u = 0; u < XU; u++) {
-            int64_t f = f0 + u;
-            if (f >= F) break;
-            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
-            const ContigDev cc = ctg[c];
-            if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
-            cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, dummy);
-        }
+    for (int u = 0; u < XU; u++) {
+        const int64_t f = f0 + u;
+        if (f >= F) break;
+        while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+        const ContigDev cc = ctg[c];
+        if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
+        cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, dummy);
     }
     uint64_t tot = block_sum_u64<XT>(cnt, red);
-    if (threadIdx.x == 0) blkcnt[blockIdx.x] = (uint32_t) tot;
+    if (threadIdx.x == 0) blkcnt[LB] = (uint32_t) tot;
+    }
 }
 
-// Emit records in extraction order at blkoff[block] and count the first
-// (level-1) radix digit per pair.  Every block lies inside one pair.
+// Emit records in extraction order at blkoff[block] (staged in LDS, written
+// with coalesced 16-byte stores) and count the first (level-1) radix digit
+// per pair.  Every block lies inside one pair.
 __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict__ bps64,
                                                      const ContigDev *__restrict__ ctg, int nctg,
                                                      int64_t F, int jbits,
                                                      const int64_t *__restrict__ blkoff,
                                                      Rec *__restrict__ out,
-                                                     unsigned long long *__restrict__ hist1)
+                                                     unsigned long long *__restrict__ hist1, int64_t nlog)
 {
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t scan_sh[XT / 64 + 1];
+    __shared__ Rec stage[2 * XT * XU];
+    __shared__ int sc;
     load_tables(tl, tm);
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
     for (int t = threadIdx.x; t < 256; t += XT) hist[t] = 0;
-    __syncthreads();
-    const int64_t fb = (int64_t) blockIdx.x * XT * XU;
+    const int64_t fb = LB * XT * XU;
+    int c = block_contig(ctg, nctg, fb, &sc);
+    const int pair0 = ctg[c].pair;
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint64_t k0[2 * XU], k1[2 * XU];
     uint32_t valid = 0;
-    int pair0 = fb < F ? ctg[locate_contig(ctg, nctg, fb)].pair : -1;
-    if (f0 < F) {
-        int c = locate_contig(ctg, nctg, f0);
 #pragma unroll
-        for (int u = 0; u < XU; u++) {
-            int64_t f = f0 + u;
-            if (f >= F) break;
-            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
-            const ContigDev cc = ctg[c];
-            if (f >= cc.pos0 + cc.ncand) continue;
-            XEmit e;
-            scan_position(bps64, cc, f - cc.pos0, jbits, tl, tm, true, e);
-            if (e.fwd) {
-                k0[2 * u] = e.f0; k1[2 * u] = e.f1; valid |= 1u << (2 * u);
-                uint32_t d = (uint32_t) (e.f0 >> 56);
-                if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
-                else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
-            }
-            if (e.rev) {
-                k0[2 * u + 1] = e.r0; k1[2 * u + 1] = e.r1; valid |= 1u << (2 * u + 1);
-                uint32_t d = (uint32_t) (e.r0 >> 56);
-                if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
-                else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
-            }
+    for (int u = 0; u < XU; u++) {
+        const int64_t f = f0 + u;
+        if (f >= F) break;
+        while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+        const ContigDev cc = ctg[c];
+        if (f >= cc.pos0 + cc.ncand) continue;
+        XEmit e;
+        scan_position(bps64, cc, f - cc.pos0, jbits, tl, tm, true, e);
+        if (e.fwd) {
+            k0[2 * u] = e.f0; k1[2 * u] = e.f1; valid |= 1u << (2 * u);
+            uint32_t d = (uint32_t) (e.f0 >> 56);
+            if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
+            else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
+        }
+        if (e.rev) {
+            k0[2 * u + 1] = e.r0; k1[2 * u + 1] = e.r1; valid |= 1u << (2 * u + 1);
+            uint32_t d = (uint32_t) (e.r0 >> 56);
+            if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
+            else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
         }
     }
     uint32_t total;
     uint32_t off = block_excl_scan<XT, uint32_t>((uint32_t) __builtin_popcount(valid), scan_sh, &total);
-    Rec *dst = out + blkoff[blockIdx.x] + off;
     int k = 0;
 #pragma unroll
     for (int s = 0; s < 2 * XU; s++) {
@@ -261,13 +272,16 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
             Rec r;
             r.w0 = k0[s];
             r.w1 = k1[s];
-            dst[k++] = r;
+            stage[off + k++] = r;
         }
     }
     __syncthreads();
-    if (pair0 >= 0)
-        for (int t = threadIdx.x; t < 256; t += XT)
-            if (hist[t]) atomicAdd(&hist1[(size_t) pair0 * 256 + t], (unsigned long long) hist[t]);
+    Rec *dst = out + blkoff[LB];
+    for (int i = threadIdx.x; i < (int) total; i += XT) dst[i] = stage[i];
+    for (int t = threadIdx.x; t < 256; t += XT)
+        if (hist[t]) atomicAdd(&hist1[(size_t) pair0 * 256 + t], (unsigned long long) hist[t]);
+    __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- scans
@@ -391,28 +405,31 @@ constexpr int PT = 256;   // threads of the partition kernels
 template <int PER>
 __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ src, const Seg *__restrict__ segs,
                                                  int nseg, const int64_t *__restrict__ chk0,
-                                                 unsigned long long *__restrict__ seghist)
+                                                 unsigned long long *__restrict__ seghist, int64_t nlog)
 {
     __shared__ uint32_t hist[256];
     const int CH = PT * PER;
-    hist[threadIdx.x] = 0;
-    __syncthreads();
-    int s = seg_of_chunk(chk0, nseg, blockIdx.x);
-    const Seg sg = segs[s];
-    int64_t off = (blockIdx.x - chk0[s]) * (int64_t) CH;
-    int n = (int) min((int64_t) CH, sg.count - off);
-    const Rec *p = src + sg.start + off;
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        const int s = seg_of_chunk(chk0, nseg, LB);
+        const Seg sg = segs[s];
+        const int64_t off = (LB - chk0[s]) * (int64_t) CH;
+        const int n = (int) min((int64_t) CH, sg.count - off);
+        const Rec *p = src + sg.start + off;
 #pragma unroll 4
-    for (int k = 0; k < PER; k++) {
-        int i = k * PT + threadIdx.x;
-        if (i < n) {
-            Rec r = p[i];
-            atomicAdd(&hist[digit_at(r.w0, r.w1, sg.level)], 1u);
+        for (int k = 0; k < PER; k++) {
+            const int i = k * PT + threadIdx.x;
+            if (i < n) {
+                const Rec r = p[i];
+                atomicAdd(&hist[digit_at(r.w0, r.w1, sg.level)], 1u);
+            }
         }
+        __syncthreads();
+        const uint32_t h = hist[threadIdx.x];
+        if (h) atomicAdd(&seghist[(size_t) s * 256 + threadIdx.x], (unsigned long long) h);
+        __syncthreads();
     }
-    __syncthreads();
-    uint32_t h = hist[threadIdx.x];
-    if (h) atomicAdd(&seghist[(size_t) s * 256 + threadIdx.x], (unsigned long long) h);
 }
 
 // Per segment: digit offsets (cursors) and the children.  Children of at most
@@ -444,7 +461,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
             if (k > cap) {
                 if (tc) {
                     int t = atomicAdd(ntiles, 1);
-                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf};
+                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
                     tc = 0;
                 }
                 int q = atomicAdd(nnext, 1);
@@ -452,7 +469,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
             } else {
                 if (tc + k > cap) {
                     int t = atomicAdd(ntiles, 1);
-                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf};
+                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
                     tc = 0;
                 }
                 if (tc == 0) ts = sg.start + off[d];
@@ -461,7 +478,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
         }
         if (tc) {
             int t = atomicAdd(ntiles, 1);
-            if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf};
+            if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
         }
     }
 }
@@ -470,7 +487,7 @@ template <int PER>
 __global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src, Rec *__restrict__ dst,
                                                     const Seg *__restrict__ segs, int nseg,
                                                     const int64_t *__restrict__ chk0,
-                                                    unsigned long long *__restrict__ cursor)
+                                                    unsigned long long *__restrict__ cursor, int64_t nlog)
 {
     constexpr int CH = PT * PER;
     __shared__ Rec stage[CH];
@@ -478,11 +495,12 @@ __global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src,
     __shared__ uint32_t loff[256];
     __shared__ int64_t gbase[256];
     __shared__ uint32_t sh[PT / 64 + 1];
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
     hist[threadIdx.x] = 0;
     __syncthreads();
-    int s = seg_of_chunk(chk0, nseg, blockIdx.x);
+    int s = seg_of_chunk(chk0, nseg, LB);
     const Seg sg = segs[s];
-    int64_t off = (blockIdx.x - chk0[s]) * (int64_t) CH;
+    int64_t off = (LB - chk0[s]) * (int64_t) CH;
     int n = (int) min((int64_t) CH, sg.count - off);
     const Rec *p = src + sg.start + off;
     Rec r[PER];
@@ -518,14 +536,16 @@ __global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src,
             dst[gbase[d] + (i - (int64_t) loff[d])] = x;
         }
     }
+    __syncthreads();
+    }
 }
 
-template __global__ void k_seg_hist<16>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *);
-template __global__ void k_seg_hist<8>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *);
-template __global__ void k_seg_hist<32>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *);
-template __global__ void k_seg_scatter<16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *);
-template __global__ void k_seg_scatter<8>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *);
-template __global__ void k_seg_scatter<32>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *);
+template __global__ void k_seg_hist<16>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_hist<8>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_hist<32>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<8>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<32>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
 
 // ---------------------------------------------------------------- tile sort
 
@@ -560,13 +580,15 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                                                  const Tile *__restrict__ tiles,
                                                  const PairDev *__restrict__ pairs, int jbits,
                                                  uint8_t *__restrict__ entries,
-                                                 uint8_t *__restrict__ ls, uint32_t *__restrict__ pre)
+                                                 uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
+                                                 int64_t nlog)
 {
     constexpr int E = 8;
     constexpr int CAP = T * E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Rec *sk = reinterpret_cast<Rec *>(smem_raw);
-    const Tile tl = tiles[blockIdx.x];
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
+    const Tile tl = tiles[LB];
     const PairDev pd = pairs[tl.pair];
     const Rec *src = bufs[tl.buf] + tl.start;
     const int n = tl.count;
@@ -606,17 +628,22 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         if (mn0[w] < lo0 || (mn0[w] == lo0 && mn1[w] < lo1)) { lo0 = mn0[w]; lo1 = mn1[w]; }
         if (mx0[w] > hi0 || (mx0[w] == hi0 && mx1[w] > hi1)) { hi0 = mx0[w]; hi1 = mx1[w]; }
     }
-    // Bin = the log2(NB) key bits right after the tile's common prefix (the
-    // first bit where min and max differ).  Keys are near-uniform below that
-    // prefix, so bins hold ~2 records; a bin is insertion-sorted by one thread.
+    // Bin of a record: (its digit at the tile's level - the smallest such
+    // digit, then the key bits right below that digit), with as many low bits
+    // as the digit span leaves.  A tile is a run of buckets of one parent, so
+    // buckets map to disjoint bin ranges and the bits inside a bucket are
+    // near-uniform: bins hold ~2 records unless the k-mers repeat.
     constexpr int BB = __builtin_ctz(NB);
-    const int cp = (lo0 != hi0) ? __builtin_clzll(lo0 ^ hi0)
-                 : (lo1 != hi1 ? 64 + __builtin_clzll(lo1 ^ hi1) : 128);
+    const int L = tl.level;
+    const uint32_t dmin = digit_at(lo0, lo1, L), dmax = digit_at(hi0, hi1, L);
+    const int span = dmax > dmin ? 32 - __builtin_clz(dmax - dmin) : 0;
+    const int lowb = BB - span;                     // >= BB - 8
+    const int cp = 8 * L;                           // first key bit below the digit
     auto bin_of = [&](uint64_t w0, uint64_t w1) -> uint32_t {
         uint64_t top;
         if (cp >= 64) top = cp >= 128 ? 0 : (w1 << (cp - 64));
-        else top = cp == 0 ? w0 : ((w0 << cp) | (w1 >> (64 - cp)));
-        return (uint32_t) (top >> (64 - BB));
+        else top = (w0 << cp) | (w1 >> (64 - cp));
+        return ((digit_at(w0, w1, L) - dmin) << lowb) | (uint32_t) (top >> (64 - lowb));
     };
     uint32_t rk[E];
 #pragma unroll
@@ -664,10 +691,10 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         }
     }
     __syncthreads();
+    if (!binned) {
     uint64_t a0[E], a1[E];
 #pragma unroll
     for (int k = 0; k < E; k++) { Rec x = sk[tid * E + k]; a0[k] = x.w0; a1[k] = x.w1; }
-    if (!binned) {
     // optimal 19-comparator network for 8 inputs
     PGA_CE(0, 2); PGA_CE(1, 3); PGA_CE(4, 6); PGA_CE(5, 7);
     PGA_CE(0, 4); PGA_CE(1, 5); PGA_CE(2, 6); PGA_CE(3, 7);
@@ -711,8 +738,25 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     for (int k = 0; k < E; k++) { Rec x; x.w0 = a0[k]; x.w1 = a1[k]; sk[tid * E + k] = x; }
     __syncthreads();
     }
-    uint64_t p0 = 0, p1 = 0;
-    if (tid > 0) { Rec r = sk[tid * E - 1]; p0 = r.w0; p1 = r.w1; }
+    // Output, striped (record k*T + tid): conflict-free LDS reads, coalesced
+    // LCP/prefix stores.  The LCP of the tile's first record is set by k_tile_fix.
+    uint64_t c0[E], c1[E];
+    uint32_t lc[E];
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const int idx = k * T + tid;
+        c0[k] = c1[k] = 0;
+        lc[k] = 0;
+        if (idx < n) {
+            const Rec x = sk[idx];
+            c0[k] = x.w0;
+            c1[k] = x.w1;
+            if (idx > 0) {
+                const Rec y = sk[idx - 1];
+                lc[k] = (uint32_t) lcp_bases(y.w0, y.w1, x.w0, x.w1);
+            }
+        }
+    }
     __syncthreads();                                // LDS is reused for entry bytes below
 
     const int e = pd.e, P = pd.P, C = pd.C;
@@ -722,11 +766,10 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     const uint64_t jmask = (1ull << jbits) - 1ull;
 #pragma unroll
     for (int k = 0; k < E; k++) {
-        const int idx = tid * E + k;
+        const int idx = k * T + tid;
         if (idx < n) {
-            const uint64_t w0 = a0[k], w1 = a1[k];
-            int lcp = (idx == 0) ? 0 : lcp_bases(p0, p1, w0, w1);
-            p0 = w0; p1 = w1;
+            const uint64_t w0 = c0[k], w1 = c1[k];
+            const uint32_t lcp = lc[k];
             const uint32_t strand = (uint32_t) (w1 & 1ull);
             const uint64_t j = (w1 >> 1) & jmask;
             const uint64_t contig = (w1 & 0xFFFFFFFFFFFFull) >> (jbits + 1);
@@ -759,11 +802,13 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                 if (b >= shift && b < shift + nb) g[b] = sb[b];
         }
     }
+    __syncthreads();
+    }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, int64_t);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, int64_t);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, int64_t);
 
 // Pair of a global record index.
 __device__ __forceinline__ int pair_of(const PairDev *__restrict__ pairs, int npairs, int64_t g)
@@ -782,8 +827,7 @@ __global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const Pai
                            uint8_t *__restrict__ entries, uint8_t *__restrict__ ls,
                            const uint32_t *__restrict__ pre)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
     const Tile tl = tiles[t];
     const PairDev pd = pairs[tl.pair];
     int lcp = 0;
@@ -802,6 +846,7 @@ __global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const Pai
     }
     cur[8] = (uint8_t) lcp;
     ls[tl.start] = (uint8_t) ((ls[tl.start] & 0x80) | lcp);
+    }
 }
 
 // ---------------------------------------------------------------- prefix table
@@ -825,16 +870,17 @@ __global__ void k_cum_fill(const PairDev *__restrict__ pairs, int npairs, int64_
                            const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
                            LongFill *__restrict__ longs, int *__restrict__ nlong)
 {
-    int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= R) return;
-    const int p = pair_of(pairs, npairs, g);
-    const PairDev pd = pairs[p];
-    const int64_t li = g - pd.rec0;
-    int64_t *cp = cum + pd.cum_off;
-    const int64_t cur = pre[g];
-    const int64_t nxt = (li + 1 < pd.nrec) ? (int64_t) pre[g + 1] : (int64_t) (1 << 24);
-    if (li == 0 && cur > 0) fill_range(cp, 0, cur, 0, longs, nlong);
-    if (cur != nxt) fill_range(cp, cur, nxt, li + 1, longs, nlong);
+    for (int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; g < R;
+         g += (int64_t) gridDim.x * blockDim.x) {
+        const int p = pair_of(pairs, npairs, g);
+        const int64_t rec0 = pairs[p].rec0, nrec = pairs[p].nrec;
+        const int64_t li = g - rec0;
+        int64_t *cp = cum + pairs[p].cum_off;
+        const int64_t cur = pre[g];
+        const int64_t nxt = (li + 1 < nrec) ? (int64_t) pre[g + 1] : (int64_t) (1 << 24);
+        if (li == 0 && cur > 0) fill_range(cp, 0, cur, 0, longs, nlong);
+        if (cur != nxt) fill_range(cp, cur, nxt, li + 1, longs, nlong);
+    }
 }
 
 __global__ void k_long_fill(const LongFill *__restrict__ longs)
@@ -952,75 +998,66 @@ __device__ __forceinline__ int merge_block_pair(const PairDev *__restrict__ pair
     return lo;
 }
 
-template <int MT>
-__global__ __launch_bounds__(MT) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,
-                                                    const uint8_t *__restrict__ ls, int freq,
-                                                    uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC)
+constexpr int MTH = 256;             // threads of the merge kernels
+constexpr int BUCK_LDS = 1024;       // per-pair contigs counted in LDS by the merge
+constexpr int STG_N = 16384;         // LDS seed staging per round: same-strand stream
+constexpr int STG_C = 4096;          //                             cross-strand stream
+
+// LCP/strand bytes of window [w0, w0+n) of the pair into win (0 outside it).
+__device__ __forceinline__ void load_win(uint8_t *__restrict__ win, const uint8_t *__restrict__ ls,
+                                         int64_t w0, int n, int64_t rec0, int64_t end)
 {
-    __shared__ uint8_t win[MT + 2 * MAX_FREQ + 4];
-    __shared__ uint64_t red[MT / 64 + 1];
-    const int p = merge_block_pair(pairs, npairs, blockIdx.x);
-    const PairDev pd = pairs[p];
-    const int64_t lo = pd.rec0 + (blockIdx.x - pd.mblk0) * (int64_t) MT;
-    const int64_t end = pd.rec0 + pd.nrec;
-    const int H = freq + 1;
-    for (int k = threadIdx.x; k < MT + 2 * H; k += MT) {
-        int64_t g = lo - H + k;
-        win[k] = (g >= pd.rec0 && g < end) ? ls[g] : 0;
+    for (int k = threadIdx.x; k < n; k += MTH) {
+        const int64_t g = w0 + k;
+        win[k] = (g >= rec0 && g < end) ? ls[g] : 0;
     }
-    __syncthreads();
-    const int64_t g = lo + threadIdx.x;
-    uint32_t nN = 0, nC = 0;
-    int plen, a, b;
-    if (g < end && anchor_interval(win, threadIdx.x + H, freq, plen, a, b)) {
-        const int gi = threadIdx.x + H;
-        const uint8_t si = win[gi] & 0x80;
-        for (int q = a; q < b; q++) {
-            if (q == gi) continue;
-            if ((win[q] & 0x80) == si) nN++; else nC++;
-        }
-    }
-    uint64_t sn = block_sum_u64<MT>(nN, red);
-    uint64_t sc = block_sum_u64<MT>(nC, red);
-    if (threadIdx.x == 0) { cntN[blockIdx.x] = (uint32_t) sn; cntC[blockIdx.x] = (uint32_t) sc; }
 }
 
-constexpr int MERGE_STG = 12288;     // bytes of LDS seed staging per round
-constexpr int BUCK_LDS = 2048;       // per-pair contigs counted in LDS by the merge
-
-// Write anchor gi's seeds of one stream (same strand or not) whose indices in
-// the block's output fall in [r0, r1) into the staging buffer.  Payload of
-// window entry k: ent + k*e + 9.
-__device__ __forceinline__ void stage_seeds(uint8_t *__restrict__ stg, int shift, int64_t r0, int64_t r1,
-                                            int64_t o, const uint8_t *__restrict__ win,
-                                            const uint8_t *__restrict__ ent, int e, int gi, int a, int b,
-                                            int plen, bool same, int pc, int s)
+// Seed counts of the tiles of A*MTH anchors (A consecutive anchors per thread).
+template <int A>
+__global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,
+                                                     const uint8_t *__restrict__ ls, int freq,
+                                                     uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC,
+                                                     int64_t nlog)
 {
-    const uint8_t si = win[gi] & 0x80;
-    const uint8_t *pi = ent + gi * e + 9;
-    int64_t k = o;
-    for (int q = a; q < b && k < r1; q++) {
-        if (q == gi || (((win[q] & 0x80) == si) != same)) continue;
-        if (k >= r0) {
-            uint8_t *d = stg + shift + (k - r0) * s;
-            d[0] = (uint8_t) plen;
-            for (int t = 0; t < pc; t++) d[1 + t] = pi[t];
-            d[pc] &= 0x7F;                          // anchor's strand bit cleared
-            const uint8_t *pq = ent + q * e + 9;
-            for (int t = 0; t < pc; t++) d[1 + pc + t] = pq[t];
+    constexpr int TA = MTH * A;
+    __shared__ uint8_t win[TA + 2 * MAX_FREQ + 4];
+    __shared__ uint64_t red[MTH / 64 + 1];
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
+    const int p = merge_block_pair(pairs, npairs, LB);
+    const PairDev pd = pairs[p];
+    const int64_t lo = pd.rec0 + (LB - pd.mblk0) * (int64_t) TA;
+    const int64_t end = pd.rec0 + pd.nrec;
+    const int H = freq + 1;
+    load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
+    __syncthreads();
+    uint32_t nN = 0, nC = 0;
+#pragma unroll
+    for (int k = 0; k < A; k++) {
+        const int gi = H + threadIdx.x * A + k;
+        int plen, a, b;
+        if (lo + threadIdx.x * A + k < end && anchor_interval(win, gi, freq, plen, a, b)) {
+            const uint8_t si = win[gi] & 0x80;
+            for (int q = a; q < b; q++) {
+                if (q == gi) continue;
+                if ((win[q] & 0x80) == si) nN++; else nC++;
+            }
         }
-        k++;
+    }
+    uint64_t sn = block_sum_u64<MTH>(nN, red);
+    uint64_t sc = block_sum_u64<MTH>(nC, red);
+    if (threadIdx.x == 0) { cntN[LB] = (uint32_t) sn; cntC[LB] = (uint32_t) sc; }
+    __syncthreads();
     }
 }
 
 // Copy nb staged bytes (starting at stg[shift]) to g, 16-byte aligned stores
 // inside, byte stores at the two ends.
-template <int NT>
 __device__ __forceinline__ void copy_out(uint8_t *__restrict__ g, const uint8_t *__restrict__ stg,
                                          int shift, int nb)
 {
     const int nchunks = (shift + nb + 15) >> 4;
-    for (int c = threadIdx.x; c < nchunks; c += NT) {
+    for (int c = threadIdx.x; c < nchunks; c += MTH) {
         const int b0 = c << 4;
         if (b0 >= shift && b0 + 16 <= shift + nb) {
             *reinterpret_cast<uint4 *>(g + b0) = *reinterpret_cast<const uint4 *>(stg + b0);
@@ -1031,75 +1068,85 @@ __device__ __forceinline__ void copy_out(uint8_t *__restrict__ g, const uint8_t 
     }
 }
 
-template <int NT>
 __device__ __forceinline__ void flush_buck(uint32_t *__restrict__ lb, int n, int64_t off,
                                            unsigned long long *__restrict__ nbuck,
                                            unsigned long long *__restrict__ cbuck)
 {
-    for (int c = threadIdx.x; c < n; c += NT) {
+    for (int c = threadIdx.x; c < n; c += MTH) {
         if (lb[c]) { atomicAdd(&nbuck[off + c], (unsigned long long) lb[c]); lb[c] = 0; }
         if (lb[BUCK_LDS + c]) { atomicAdd(&cbuck[off + c], (unsigned long long) lb[BUCK_LDS + c]); lb[BUCK_LDS + c] = 0; }
     }
 }
 
-// Seeds of a run of merge tiles (MT anchors each).  Per tile: the LCP/strand
-// bytes and raw GIX entries of the tile plus a FREQ halo on each side go to
-// LDS, each anchor finds its interval, a block scan places its seeds in the
-// tile's slice of the N and C streams (offsets from the count pass), and the
-// seeds are staged in LDS and written out with aligned 16-byte stores.
-// Per-anchor-contig counts accumulate in LDS while the pair stays the same.
-template <int MT>
-__global__ __launch_bounds__(MT) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
-                                                   const uint8_t *__restrict__ ls,
-                                                   const uint8_t *__restrict__ entries, int freq,
-                                                   const int64_t *__restrict__ ofsN,
-                                                   const int64_t *__restrict__ ofsC,
-                                                   uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
-                                                   unsigned long long *__restrict__ nbuck,
-                                                   unsigned long long *__restrict__ cbuck,
-                                                   int64_t nmb, int64_t per_block)
+// Payload (pc <= 7 bytes, little-endian) of window entry k.
+__device__ __forceinline__ uint64_t payload_at(const uint8_t *__restrict__ ew, int k, int e, int pc)
 {
-    constexpr int W = MT + 2 * MAX_FREQ + 4;
+    const uint8_t *p = ew + k * e + 9;
+    uint64_t v = 0;
+#pragma unroll
+    for (int t = 0; t < 7; t++)
+        if (t < pc) v |= (uint64_t) p[t] << (8 * t);
+    return v;
+}
+
+// Seeds of a run of merge tiles (A*MTH anchors each, A consecutive per
+// thread).  Per tile: the LCP/strand bytes and raw GIX entries of the tile plus
+// a FREQ halo on each side go to LDS, each anchor finds its interval, block
+// scans place the anchors' seeds in the tile's slice of the N and C streams
+// (offsets from the count pass), and the seeds are staged in LDS and written
+// with aligned 16-byte stores.  Per-anchor-contig counts accumulate in LDS
+// while the pair stays the same.
+template <int A>
+__global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
+                                                    const uint8_t *__restrict__ ls,
+                                                    const uint8_t *__restrict__ entries, int freq,
+                                                    const int64_t *__restrict__ ofsN,
+                                                    const int64_t *__restrict__ ofsC,
+                                                    uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
+                                                    unsigned long long *__restrict__ nbuck,
+                                                    unsigned long long *__restrict__ cbuck,
+                                                    int64_t nmb, int64_t per_block)
+{
+    constexpr int TA = MTH * A;
+    constexpr int W = TA + 2 * MAX_FREQ + 4;
     __shared__ uint8_t win[W];
     __shared__ __align__(16) uint8_t ent[W * 16 + 16];
-    __shared__ __align__(16) uint8_t stg[MERGE_STG + 16];
+    __shared__ __align__(16) uint8_t stn[STG_N + 16];
+    __shared__ __align__(16) uint8_t stc[STG_C + 16];
     __shared__ uint32_t lb[2 * BUCK_LDS];
-    __shared__ uint32_t sh[MT / 64 + 1];
-    for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MT) lb[c] = 0;
+    __shared__ uint32_t sh[MTH / 64 + 1];
+    for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MTH) lb[c] = 0;
     const int64_t t0 = (int64_t) blockIdx.x * per_block;
     const int64_t t1 = t0 + per_block < nmb ? t0 + per_block : nmb;
     int cur_pair = -1;
     for (int64_t tile = t0; tile < t1; tile++) {
         const int p = merge_block_pair(pairs, npairs, tile);
         const PairDev pd = pairs[p];
-        const bool lds_buck = pd.ncontig <= BUCK_LDS;
         if (p != cur_pair) {
             __syncthreads();
             if (cur_pair >= 0 && pairs[cur_pair].ncontig <= BUCK_LDS)
-                flush_buck<MT>(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
+                flush_buck(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
             cur_pair = p;
         }
-        const int64_t lo = pd.rec0 + (tile - pd.mblk0) * (int64_t) MT;
+        const bool lds_buck = pd.ncontig <= BUCK_LDS;
+        const int64_t lo = pd.rec0 + (tile - pd.mblk0) * (int64_t) TA;
         const int64_t end = pd.rec0 + pd.nrec;
         const int H = freq + 1;
         const int pc = pd.P + pd.C, e = pd.e, s = pd.s;
-        // window [lo-H, lo+MT+H) clipped to the pair; raw entry bytes with 4-byte loads
+        // window [lo-H, lo+TA+H) clipped to the pair; raw entry bytes with 4-byte loads
         const int64_t wlo = lo - H > pd.rec0 ? lo - H : pd.rec0;
-        const int64_t whi = lo + MT + H < end ? lo + MT + H : end;
+        const int64_t whi = lo + TA + H < end ? lo + TA + H : end;
         const int64_t b0 = pd.ent_off + (wlo - pd.rec0) * (int64_t) e;
         const int64_t b1 = pd.ent_off + (whi - pd.rec0) * (int64_t) e;
         const int64_t a0 = b0 & ~(int64_t) 3;
         const int nwords = (int) ((b1 - a0 + 3) >> 2);
         __syncthreads();
-        for (int k = threadIdx.x; k < MT + 2 * H; k += MT) {
-            const int64_t g = lo - H + k;
-            win[k] = (g >= pd.rec0 && g < end) ? ls[g] : 0;
-        }
+        load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
         {
             const uint32_t *src = reinterpret_cast<const uint32_t *>(entries + a0);
             uint8_t *dst = ent + (int) (wlo - (lo - H)) * e;
-            for (int k = threadIdx.x; k < nwords; k += MT) {
-                uint32_t v = src[k];
+            for (int k = threadIdx.x; k < nwords; k += MTH) {
+                const uint32_t v = src[k];
                 dst[4 * k + 0] = (uint8_t) v;
                 dst[4 * k + 1] = (uint8_t) (v >> 8);
                 dst[4 * k + 2] = (uint8_t) (v >> 16);
@@ -1110,72 +1157,101 @@ __global__ __launch_bounds__(MT) void k_merge_emit(const PairDev *__restrict__ p
         // window entry k (global lo-H+k) starts at ew + k*e; entries outside
         // [wlo, whi) are never read (their LCP byte is 0, a run boundary)
         const uint8_t *ew = ent + (int) (b0 - a0);
-        const int64_t g = lo + threadIdx.x;
-        const int gi = threadIdx.x + H;
-        uint32_t nN = 0, nC = 0;
-        int plen = 0, a = 0, b = 0;
-        const bool act = g < end && anchor_interval(win, gi, freq, plen, a, b);
-        if (act) {
-            const uint8_t si = win[gi] & 0x80;
-            for (int q = a; q < b; q++) {
-                if (q == gi) continue;
-                if ((win[q] & 0x80) == si) nN++; else nC++;
+        int pl[A], aa[A], bb[A];
+        uint32_t nN[A], nC[A];
+        uint32_t sN = 0, sC = 0;
+#pragma unroll
+        for (int k = 0; k < A; k++) {
+            const int gi = H + threadIdx.x * A + k;
+            nN[k] = nC[k] = 0;
+            pl[k] = 0; aa[k] = bb[k] = gi;
+            if (lo + threadIdx.x * A + k < end && anchor_interval(win, gi, freq, pl[k], aa[k], bb[k])) {
+                const uint8_t si = win[gi] & 0x80;
+                for (int q = aa[k]; q < bb[k]; q++) {
+                    if (q == gi) continue;
+                    if ((win[q] & 0x80) == si) nN[k]++; else nC[k]++;
+                }
+            } else {
+                aa[k] = bb[k] = gi;
             }
+            sN += nN[k];
+            sC += nC[k];
         }
         uint32_t totN, totC;
-        const uint32_t oN = block_excl_scan<MT, uint32_t>(nN, sh, &totN);
-        const uint32_t oC = block_excl_scan<MT, uint32_t>(nC, sh, &totC);
-        const int per_round = MERGE_STG / s;
+        const uint32_t oN = block_excl_scan<MTH, uint32_t>(sN, sh, &totN);
+        const uint32_t oC = block_excl_scan<MTH, uint32_t>(sC, sh, &totC);
+        const int capN = STG_N / s, capC = STG_C / s;
+        const int64_t rounds = max(((int64_t) totN + capN - 1) / capN, ((int64_t) totC + capC - 1) / capC);
+        uint8_t *gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
+        uint8_t *gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
+        for (int64_t r = 0; r < rounds; r++) {
+            const int64_t n0 = r * capN, n1 = min(n0 + capN, (int64_t) totN);
+            const int64_t c0 = r * capC, c1 = min(c0 + capC, (int64_t) totC);
+            const int shN = (int) ((uintptr_t) (gN + n0 * s) & 15);
+            const int shC = (int) ((uintptr_t) (gC + c0 * s) & 15);
+            int64_t kn = oN, kc = oC;
 #pragma unroll
-        for (int str = 0; str < 2; str++) {
-            const bool same = str == 0;
-            const uint32_t tot = same ? totN : totC;
-            const uint32_t o = same ? oN : oC;
-            const uint32_t mine = same ? nN : nC;
-            uint8_t *gbase = same ? seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s
-                                  : seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
-            for (int64_t r0 = 0; r0 < tot; r0 += per_round) {
-                const int64_t r1 = r0 + per_round < tot ? r0 + per_round : tot;
-                uint8_t *gr = gbase + r0 * s;
-                const int shift = (int) ((uintptr_t) gr & 15);
-                if (mine && o < r1 && o + mine > r0)
-                    stage_seeds(stg, shift, r0, r1, o, win, ew, e, gi, a, b, plen, same, pc, s);
-                __syncthreads();
-                copy_out<MT>(gr - shift, stg, shift, (int) ((r1 - r0) * s));
-                __syncthreads();
+            for (int k = 0; k < A; k++) {
+                if (nN[k] + nC[k] == 0) continue;
+                const int gi = H + threadIdx.x * A + k;
+                const uint8_t si = win[gi] & 0x80;
+                // anchor payload with its strand bit cleared
+                const uint64_t pi = payload_at(ew, gi, e, pc) & ~(0x80ull << (8 * (pc - 1)));
+                for (int q = aa[k]; q < bb[k]; q++) {
+                    if (q == gi) continue;
+                    const bool same = (win[q] & 0x80) == si;
+                    uint8_t *d = nullptr;
+                    if (same) {
+                        if (kn >= n0 && kn < n1) d = stn + shN + (kn - n0) * s;
+                        kn++;
+                    } else {
+                        if (kc >= c0 && kc < c1) d = stc + shC + (kc - c0) * s;
+                        kc++;
+                    }
+                    if (d) {
+                        const uint64_t pq = payload_at(ew, q, e, pc);
+                        d[0] = (uint8_t) pl[k];
+#pragma unroll
+                        for (int t = 0; t < 7; t++)
+                            if (t < pc) { d[1 + t] = (uint8_t) (pi >> (8 * t)); d[1 + pc + t] = (uint8_t) (pq >> (8 * t)); }
+                    }
+                }
             }
+            __syncthreads();
+            if (n1 > n0) copy_out(gN + n0 * s - shN, stn, shN, (int) ((n1 - n0) * s));
+            if (c1 > c0) copy_out(gC + c0 * s - shC, stc, shC, (int) ((c1 - c0) * s));
+            __syncthreads();
         }
-        if (act && (nN | nC)) {
-            // per-anchor-contig counts (src/fastga_pipeline.c:403)
-            const uint8_t *pi = ew + gi * e + 9;
-            uint64_t icont = 0;
-            for (int k = 0; k < pd.C; k++) icont |= (uint64_t) pi[pd.P + k] << (8 * k);
-            icont &= ~(1ull << (8 * pd.C - 1));
+        // per-anchor-contig counts (src/fastga_pipeline.c:403)
+#pragma unroll
+        for (int k = 0; k < A; k++) {
+            if (nN[k] + nC[k] == 0) continue;
+            const int gi = H + threadIdx.x * A + k;
+            const uint64_t icont = (payload_at(ew, gi, e, pc) >> (8 * pd.P)) & ((1ull << (8 * pd.C - 1)) - 1);
             if (lds_buck) {
-                if (nN) atomicAdd(&lb[icont], nN);
-                if (nC) atomicAdd(&lb[BUCK_LDS + icont], nC);
+                if (nN[k]) atomicAdd(&lb[icont], nN[k]);
+                if (nC[k]) atomicAdd(&lb[BUCK_LDS + icont], nC[k]);
             } else {
-                if (nN) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nN);
-                if (nC) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nC);
+                if (nN[k]) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nN[k]);
+                if (nC[k]) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nC[k]);
             }
         }
     }
     __syncthreads();
     if (cur_pair >= 0 && pairs[cur_pair].ncontig <= BUCK_LDS)
-        flush_buck<MT>(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
+        flush_buck(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
 }
 
-template __global__ void k_merge_count<256>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
-template __global__ void k_merge_emit<256>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
-                                           const int64_t *, const int64_t *, uint8_t *, uint8_t *,
-                                           unsigned long long *, unsigned long long *, int64_t, int64_t);
-template __global__ void k_merge_count<512>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
-template __global__ void k_merge_emit<512>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
-                                           const int64_t *, const int64_t *, uint8_t *, uint8_t *,
-                                           unsigned long long *, unsigned long long *, int64_t, int64_t);
-template __global__ void k_merge_count<1024>(const PairDev *, int, const uint8_t *, int, uint32_t *, uint32_t *);
-template __global__ void k_merge_emit<1024>(const PairDev *, int, const uint8_t *, const uint8_t *, int,
-                                            const int64_t *, const int64_t *, uint8_t *, uint8_t *,
-                                            unsigned long long *, unsigned long long *, int64_t, int64_t);
+#define PGA_MERGE_INST(A_)                                                                              \
+    template __global__ void k_merge_count<A_>(const PairDev *, int, const uint8_t *, int, uint32_t *, \
+                                               uint32_t *, int64_t);                                    \
+    template __global__ void k_merge_emit<A_>(const PairDev *, int, const uint8_t *, const uint8_t *,   \
+                                              int, const int64_t *, const int64_t *, uint8_t *,         \
+                                              uint8_t *, unsigned long long *, unsigned long long *,    \
+                                              int64_t, int64_t);
+PGA_MERGE_INST(1)
+PGA_MERGE_INST(2)
+PGA_MERGE_INST(4)
+#undef PGA_MERGE_INST
 
 } // namespace pga
