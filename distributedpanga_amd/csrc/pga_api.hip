@@ -97,6 +97,10 @@ struct pga_ctx {
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, bmax;
 
+    // sorted device keys kept after the build (serving)
+    bool keep_keys = false;
+    DBuf skeys;
+
     // timing
     hipEvent_t ev[8] = {};
     std::vector<hipEvent_t> sev;       // scatter launch start/stop pairs
@@ -251,7 +255,8 @@ int launch_tile_sort(pga_ctx *c, unsigned nlog)
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
                        dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, P<uint8_t>(c->entries),      \
-                       P<uint8_t>(c->ls), P<uint32_t>(c->pre), (int64_t) nlog)
+                       P<uint8_t>(c->ls), P<uint32_t>(c->pre), c->keep_keys ? P<Rec>(c->skeys) : nullptr,  \
+                       (int64_t) nlog)
     switch (T) {
     case 256: PGA_TSORT(256); break;
     case 1024: PGA_TSORT(1024); break;
@@ -366,7 +371,7 @@ void pga_destroy(pga_ctx *c)
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs,
-                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax};
+                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
@@ -381,8 +386,14 @@ int pga_sync(pga_ctx *c)
     return PGA_OK;
 }
 
-int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int64_t *bps_bytes,
-                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
+} // extern "C"
+
+// Load a batch.  contig_base (may be NULL) offsets each pair's contig ids in
+// the device keys (a query's contigs follow the target's when it is served);
+// jbits_min widens the offset field so keys of different batches compare.
+static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, const int64_t *bps_bytes,
+                           const int64_t *const *boff, const int64_t *const *clen, const int *ncontig,
+                           const int *contig_base, int jbits_min)
 {
     if (!c || npairs <= 0) return c ? fail(c, PGA_EINVAL, "npairs must be > 0") : PGA_EINVAL;
     HIPCHK(c, hipSetDevice(c->o.device));
@@ -410,11 +421,11 @@ int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int6
             h.maxctg = std::max<int64_t>(h.maxctg, clen[p][i]);
         }
         maxctg = std::max(maxctg, h.maxctg);
-        maxn = std::max<int64_t>(maxn, ncontig[p]);
+        maxn = std::max<int64_t>(maxn, (int64_t) ncontig[p] + (contig_base ? contig_base[p] : 0));
         int Pb = bytes_for(h.maxctg), Cb = bytes_for(2 * (int64_t) h.ncontig - 1);
         if (Pb + Cb > 7) return fail(c, PGA_EOVERFLOW, "pair %d: payload of %d bytes exceeds 7", p, Pb + Cb);
     }
-    c->jbits = bits_for(maxctg);
+    c->jbits = std::max(bits_for(maxctg), jbits_min);
     int cbits = bits_for(maxn - 1);
     if (c->jbits + cbits + 1 > 48)
         return fail(c, PGA_EOVERFLOW, "offset (%d bits) + contig (%d bits) + strand exceed 48 bits", c->jbits, cbits);
@@ -433,7 +444,7 @@ int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int6
             d.pos0 = pos;
             d.ncand = clen[p][i] >= 12 ? clen[p][i] - 11 : 0;
             d.clen = (int32_t) clen[p][i];
-            d.local = i;
+            d.local = i + (contig_base ? contig_base[p] : 0);
             d.pair = p;
             d.pad = 0;
             pos += d.ncand;
@@ -460,6 +471,14 @@ int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int6
     HIPCHK(c, hipStreamSynchronize(c->st));     // host arrays above are stack/vector owned
     c->loaded = true;
     return PGA_OK;
+}
+
+extern "C" {
+
+int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int64_t *bps_bytes,
+                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
+{
+    return load_batch_impl(c, npairs, bps, bps_bytes, boff, clen, ncontig, nullptr, 0);
 }
 
 int pga_load_gdb(pga_ctx *c, const uint8_t *bps, int64_t bps_bytes, const int64_t *boff,
@@ -535,6 +554,7 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->entries, (size_t) ent + 32));
     TRY(ensure(c, c->ls, nrec_alloc + 64));
     TRY(ensure(c, c->pre, (nrec_alloc + 1) * sizeof(uint32_t)));
+    if (c->keep_keys) TRY(ensure(c, c->skeys, nrec_alloc * sizeof(Rec)));
     TRY(ensure(c, c->hist1, (size_t) npairs * 256 * sizeof(uint64_t)));
     TRY(ensure(c, c->dpairs, (size_t) npairs * sizeof(PairDev)));
     TRY(ensure(c, c->counters, 16 * sizeof(int)));
@@ -864,6 +884,294 @@ int pga_timings_get(pga_ctx *c, pga_timings *t)
 {
     if (!c) return PGA_EINVAL;
     *t = c->tm;
+    return PGA_OK;
+}
+
+} // extern "C"
+
+// ======================================================================== serving
+
+#include "pga_serve.hip"
+
+struct pga_server {
+    pga_opts o{};
+    pga_ctx *T = nullptr, *Q = nullptr;
+    char err[512] = {0};
+    bool have_target = false, have_result = false;
+    int nT = 0;
+    int64_t maxctgT = 0, seqtotT = 0;
+    int nq = 0;
+    std::vector<QDev> qd;
+    std::vector<int64_t> nseeds, cseeds;
+    DBuf tpos, flag, segidx, segstart, segs, qdev, cntN, cntC, ofsN, ofsC, gath, qrec, seedN, seedC, nbuck, cbuck;
+    hipEvent_t ev[2] = {};
+    double last_ms = 0.0;
+};
+
+namespace {
+
+int sfail(pga_server *s, int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(s->err, sizeof(s->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int from_ctx(pga_server *s, pga_ctx *c, int rc)
+{
+    if (rc) snprintf(s->err, sizeof(s->err), "%s", c ? c->err : "context");
+    return rc;
+}
+
+constexpr int SERVE_JBITS = 32;
+
+} // namespace
+
+extern "C" {
+
+int pga_server_create(const pga_opts *o, pga_server **out)
+{
+    *out = nullptr;
+    pga_server *s = new (std::nothrow) pga_server();
+    if (!s) return PGA_ENOMEM;
+    if (o) s->o = *o; else pga_default_opts(&s->o);
+    if (s->o.freq > MAX_SERVE_FREQ) { delete s; return PGA_EINVAL; }
+    pga_opts to = s->o;
+    to.build_cum = 1;
+    int rc = pga_create(&to, &s->T);
+    if (!rc) rc = pga_create(&s->o, &s->Q);
+    if (rc) { pga_destroy(s->T); pga_destroy(s->Q); delete s; return rc; }
+    s->T->keep_keys = true;
+    s->Q->keep_keys = true;
+    s->Q->o.build_cum = 0;
+    for (auto &e : s->ev)
+        if (hipEventCreate(&e) != hipSuccess) { pga_destroy(s->T); pga_destroy(s->Q); delete s; return PGA_EHIP; }
+    *out = s;
+    return PGA_OK;
+}
+
+void pga_server_destroy(pga_server *s)
+{
+    if (!s) return;
+    pga_ctx *c = s->Q;
+    (void) hipSetDevice(s->o.device);
+    if (c && c->st) (void) hipStreamSynchronize(c->st);
+    DBuf *all[] = {&s->tpos, &s->flag, &s->segidx, &s->segstart, &s->segs, &s->qdev, &s->cntN, &s->cntC,
+                   &s->ofsN, &s->ofsC, &s->gath, &s->qrec, &s->seedN, &s->seedC, &s->nbuck, &s->cbuck};
+    for (DBuf *b : all)
+        if (b->p) (void) hipFree(b->p);
+    for (auto &e : s->ev) if (e) (void) hipEventDestroy(e);
+    pga_destroy(s->T);
+    pga_destroy(s->Q);
+    delete s;
+}
+
+const char *pga_server_error(const pga_server *s) { return s ? s->err : "no server"; }
+
+int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes, const int64_t *boff,
+                          const int64_t *clen, int ncontig)
+{
+    if (!s) return PGA_EINVAL;
+    s->have_target = s->have_result = false;
+    TRY(from_ctx(s, s->T, load_batch_impl(s->T, 1, &bps, &bps_bytes, &boff, &clen, &ncontig, nullptr, SERVE_JBITS)));
+    TRY(from_ctx(s, s->T, pga_build_index(s->T)));
+    s->nT = ncontig;
+    s->maxctgT = s->T->hp[0].maxctg;
+    s->seqtotT = s->T->hp[0].seqtot;
+    s->have_target = true;
+    return PGA_OK;
+}
+
+int pga_server_target_info(pga_server *s, pga_index_meta *info)
+{
+    if (!s || !s->have_target) return s ? sfail(s, PGA_EINVAL, "no target") : PGA_EINVAL;
+    return from_ctx(s, s->T, pga_index_info(s->T, 0, info));
+}
+
+int pga_server_footprint(pga_server *s, int64_t query_bases, int64_t *bytes)
+{
+    (void) s;
+    // records <= 0.8/base in practice (2/base bound); per record: two partition
+    // buffers, kept keys, entry, LCP, prefix, join position, plus seeds
+    *bytes = query_bases * 200 + ((int64_t) 64 << 20);
+    return PGA_OK;
+}
+
+int pga_server_run(pga_server *s, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
+                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
+{
+    if (!s || !s->have_target) return s ? sfail(s, PGA_EINVAL, "no target") : PGA_EINVAL;
+    if (nq <= 0) return sfail(s, PGA_EINVAL, "no queries");
+    pga_ctx *Q = s->Q, *T = s->T;
+    s->have_result = false;
+    std::vector<int> base(nq, s->nT);
+    for (int q = 0; q < nq; q++)
+        if ((int64_t) s->nT + ncontig[q] > 32767) return sfail(s, PGA_EOVERFLOW, "query %d: too many contigs", q);
+    HIPCHK(Q, hipEventRecord(s->ev[0], Q->st));
+    TRY(from_ctx(s, Q, load_batch_impl(Q, nq, bps, bps_bytes, boff, clen, ncontig, base.data(), T->jbits)));
+    if (Q->jbits != T->jbits) return sfail(s, PGA_EOVERFLOW, "query contig longer than the served offset width");
+    TRY(from_ctx(s, Q, pga_build_index(Q)));
+    const int64_t RQ = Q->R;
+    const int W = Q->o.freq + 1;
+    s->nq = nq;
+    s->qd.assign(nq, QDev());
+    int64_t nb = 0;
+    std::vector<int64_t> qrec(nq + 1);
+    for (int q = 0; q < nq; q++) {
+        QDev &d = s->qd[q];
+        d.rec0 = Q->pairs[q].rec0;
+        d.nrec = Q->pairs[q].nrec;
+        d.ncomb = s->nT + ncontig[q];
+        const int64_t mc = std::max(s->maxctgT, Q->hp[q].maxctg);
+        d.P = bytes_for(mc);
+        d.C = bytes_for(2 * (int64_t) d.ncomb - 1);
+        d.s = 1 + 2 * (d.P + d.C);
+        d.buck_off = nb;
+        nb += d.ncomb;
+        qrec[q] = d.rec0;
+    }
+    qrec[nq] = RQ;
+    const size_t rq1 = (size_t) std::max<int64_t>(RQ, 1);
+    TRY(ensure(Q, s->tpos, rq1 * sizeof(int64_t)));
+    TRY(ensure(Q, s->flag, rq1 * sizeof(uint32_t)));
+    TRY(ensure(Q, s->segidx, (rq1 + 1) * sizeof(int64_t)));
+    TRY(ensure(Q, s->segstart, rq1 * sizeof(int64_t)));
+    TRY(ensure(Q, s->qdev, (size_t) nq * sizeof(QDev)));
+    TRY(ensure(Q, s->qrec, (size_t) (nq + 1) * sizeof(int64_t)));
+    TRY(ensure(Q, s->gath, (size_t) 2 * (nq + 1) * sizeof(int64_t)));
+    HIPCHK(Q, hipMemcpyAsync(s->qdev.p, s->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
+    HIPCHK(Q, hipMemcpyAsync(s->qrec.p, qrec.data(), qrec.size() * sizeof(int64_t), hipMemcpyHostToDevice, Q->st));
+    int64_t nseg = 0;
+    std::vector<int64_t> seg0(nq + 1, 0);
+    const int64_t *tcum = P<int64_t>(T->cum);
+    if (RQ > 0) {
+        const unsigned g = grid_for(Q, (const void *) k_join_locate, 256, 0, (RQ + 255) / 256);
+        hipLaunchKernelGGL(k_join_locate, dim3(g), dim3(256), 0, Q->st, (const Rec *) P<Rec>(T->skeys), tcum,
+                           (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre), RQ,
+                           P<int64_t>(s->tpos));
+        LAUNCHCHK(Q);
+        hipLaunchKernelGGL(k_join_flags, dim3(g), dim3(256), 0, Q->st, (const QDev *) P<QDev>(s->qdev), nq,
+                           (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(s->tpos), RQ, W,
+                           P<uint32_t>(s->flag));
+        LAUNCHCHK(Q);
+        TRY(scan_u32(Q, P<uint32_t>(s->flag), RQ, P<int64_t>(s->segidx)));
+        hipLaunchKernelGGL(k_pair_ranges, dim3((nq + 256) / 256), dim3(256), 0, Q->st,
+                           (const int64_t *) P<int64_t>(s->segidx), (const int64_t *) P<int64_t>(s->qrec), nq,
+                           P<int64_t>(s->gath));
+        LAUNCHCHK(Q);
+        HIPCHK(Q, hipMemcpyAsync(seg0.data(), s->gath.p, (size_t) (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
+        HIPCHK(Q, hipStreamSynchronize(Q->st));
+        nseg = seg0[nq];
+    }
+    for (int q = 0; q < nq; q++) s->qd[q].seg0 = seg0[q];
+    std::vector<int64_t> po(2 * (nq + 1), 0);
+    if (nseg > 0) {
+        TRY(ensure(Q, s->segs, (size_t) nseg * sizeof(QSeg)));
+        TRY(ensure(Q, s->cntN, (size_t) nseg * sizeof(uint32_t)));
+        TRY(ensure(Q, s->cntC, (size_t) nseg * sizeof(uint32_t)));
+        TRY(ensure(Q, s->ofsN, (size_t) (nseg + 1) * sizeof(int64_t)));
+        TRY(ensure(Q, s->ofsC, (size_t) (nseg + 1) * sizeof(int64_t)));
+        HIPCHK(Q, hipMemcpyAsync(s->qdev.p, s->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
+        const unsigned g = grid_for(Q, (const void *) k_join_segs, 256, 0, (RQ + 255) / 256);
+        hipLaunchKernelGGL(k_join_segs, dim3(g), dim3(256), 0, Q->st, (const uint32_t *) P<uint32_t>(s->flag),
+                           (const int64_t *) P<int64_t>(s->segidx), RQ, P<int64_t>(s->segstart));
+        LAUNCHCHK(Q);
+        hipLaunchKernelGGL(k_join_segs2, dim3(grid_for(Q, (const void *) k_join_segs2, 256, 0, (nseg + 255) / 256)),
+                           dim3(256), 0, Q->st, (const QDev *) P<QDev>(s->qdev), nq,
+                           (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(s->tpos), tcum, RQ, W,
+                           (const int64_t *) P<int64_t>(s->segstart), P<QSeg>(s->segs), nseg);
+        LAUNCHCHK(Q);
+        const unsigned gj = grid_for(Q, (const void *) k_join_count, JT, 0, nseg);
+        hipLaunchKernelGGL(k_join_count, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(s->segs), nseg,
+                           (const QDev *) P<QDev>(s->qdev), (const Rec *) P<Rec>(T->skeys),
+                           (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre),
+                           (const int64_t *) P<int64_t>(s->tpos), T->jbits, Q->o.freq,
+                           P<uint32_t>(s->cntN), P<uint32_t>(s->cntC));
+        LAUNCHCHK(Q);
+        TRY(scan_u32(Q, P<uint32_t>(s->cntN), nseg, P<int64_t>(s->ofsN)));
+        TRY(scan_u32(Q, P<uint32_t>(s->cntC), nseg, P<int64_t>(s->ofsC)));
+        // per-query seed totals: offsets at each query's first segment
+        std::vector<int64_t> sidx(nq + 1);
+        for (int q = 0; q <= nq; q++) sidx[q] = q < nq ? seg0[q] : nseg;
+        HIPCHK(Q, hipMemcpyAsync(s->qrec.p, sidx.data(), sidx.size() * sizeof(int64_t), hipMemcpyHostToDevice, Q->st));
+        hipLaunchKernelGGL(k_pair_ranges, dim3((nq + 256) / 256), dim3(256), 0, Q->st,
+                           (const int64_t *) P<int64_t>(s->ofsN), (const int64_t *) P<int64_t>(s->qrec), nq,
+                           P<int64_t>(s->gath));
+        hipLaunchKernelGGL(k_pair_ranges, dim3((nq + 256) / 256), dim3(256), 0, Q->st,
+                           (const int64_t *) P<int64_t>(s->ofsC), (const int64_t *) P<int64_t>(s->qrec), nq,
+                           P<int64_t>(s->gath) + (nq + 1));
+        LAUNCHCHK(Q);
+        HIPCHK(Q, hipMemcpyAsync(po.data(), s->gath.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
+        HIPCHK(Q, hipStreamSynchronize(Q->st));
+    }
+    s->nseeds.assign(nq, 0);
+    s->cseeds.assign(nq, 0);
+    int64_t nbytes = 0, cbytes = 0;
+    for (int q = 0; q < nq; q++) {
+        QDev &d = s->qd[q];
+        s->nseeds[q] = po[q + 1] - po[q];
+        s->cseeds[q] = po[nq + 1 + q + 1] - po[nq + 1 + q];
+        d.nseed_off = nbytes;
+        d.cseed_off = cbytes;
+        nbytes += s->nseeds[q] * d.s;
+        cbytes += s->cseeds[q] * d.s;
+    }
+    TRY(ensure(Q, s->seedN, (size_t) nbytes + 16));
+    TRY(ensure(Q, s->seedC, (size_t) cbytes + 16));
+    TRY(ensure(Q, s->nbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+    TRY(ensure(Q, s->cbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+    TRY(fill_u64(Q, s->nbuck.p, nb, 0));
+    TRY(fill_u64(Q, s->cbuck.p, nb, 0));
+    HIPCHK(Q, hipMemcpyAsync(s->qdev.p, s->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
+    if (nseg > 0) {
+        const unsigned gj = grid_for(Q, (const void *) k_join_emit, JT, 0, nseg);
+        hipLaunchKernelGGL(k_join_emit, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(s->segs), nseg,
+                           (const QDev *) P<QDev>(s->qdev), (const Rec *) P<Rec>(T->skeys),
+                           (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre),
+                           (const int64_t *) P<int64_t>(s->tpos), T->jbits, Q->o.freq,
+                           (const int64_t *) P<int64_t>(s->ofsN), (const int64_t *) P<int64_t>(s->ofsC),
+                           P<uint8_t>(s->seedN), P<uint8_t>(s->seedC), P<unsigned long long>(s->nbuck),
+                           P<unsigned long long>(s->cbuck));
+        LAUNCHCHK(Q);
+    }
+    HIPCHK(Q, hipEventRecord(s->ev[1], Q->st));
+    HIPCHK(Q, hipStreamSynchronize(Q->st));
+    s->last_ms = ev_ms(s->ev[0], s->ev[1]);
+    s->have_result = true;
+    return PGA_OK;
+}
+
+int pga_server_seeds_info(pga_server *s, int q, pga_seeds_meta *info)
+{
+    if (!s || !s->have_result || q < 0 || q >= s->nq) return s ? sfail(s, PGA_EINVAL, "no result for query %d", q) : PGA_EINVAL;
+    info->nlen = s->nseeds[q] * s->qd[q].s;
+    info->clen = s->cseeds[q] * s->qd[q].s;
+    info->s = s->qd[q].s;
+    info->ncontig = s->qd[q].ncomb;
+    return PGA_OK;
+}
+
+int pga_server_seeds_export(pga_server *s, int q, uint8_t *nseeds, uint8_t *cseeds, int64_t *nbuck, int64_t *cbuck)
+{
+    if (!s || !s->have_result || q < 0 || q >= s->nq) return s ? sfail(s, PGA_EINVAL, "no result for query %d", q) : PGA_EINVAL;
+    pga_ctx *Q = s->Q;
+    const QDev &d = s->qd[q];
+    if (nseeds && s->nseeds[q] > 0)
+        HIPCHK(Q, hipMemcpyAsync(nseeds, P<uint8_t>(s->seedN) + d.nseed_off, (size_t) (s->nseeds[q] * d.s), hipMemcpyDeviceToHost, Q->st));
+    if (cseeds && s->cseeds[q] > 0)
+        HIPCHK(Q, hipMemcpyAsync(cseeds, P<uint8_t>(s->seedC) + d.cseed_off, (size_t) (s->cseeds[q] * d.s), hipMemcpyDeviceToHost, Q->st));
+    if (nbuck) HIPCHK(Q, hipMemcpyAsync(nbuck, P<uint64_t>(s->nbuck) + d.buck_off, (size_t) d.ncomb * 8, hipMemcpyDeviceToHost, Q->st));
+    if (cbuck) HIPCHK(Q, hipMemcpyAsync(cbuck, P<uint64_t>(s->cbuck) + d.buck_off, (size_t) d.ncomb * 8, hipMemcpyDeviceToHost, Q->st));
+    HIPCHK(Q, hipStreamSynchronize(Q->st));
+    return PGA_OK;
+}
+
+int pga_server_last_ms(pga_server *s, double *ms)
+{
+    if (!s) return PGA_EINVAL;
+    *ms = s->last_ms;
     return PGA_OK;
 }
 

@@ -581,7 +581,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                                                  const PairDev *__restrict__ pairs, int jbits,
                                                  uint8_t *__restrict__ entries,
                                                  uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
-                                                 int64_t nlog)
+                                                 Rec *__restrict__ skeys, int64_t nlog)
 {
     constexpr int E = 8;
     constexpr int CAP = T * E;
@@ -785,6 +785,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             for (int b = 0; b < C; b++) d[9 + P + b] = (uint8_t) (cs >> (8 * b));
             ls[tl.start + idx] = (uint8_t) (lcp | (strand << 7));
             pre[tl.start + idx] = (uint32_t) (w0 >> 40);
+            if (skeys) { Rec x; x.w0 = w0; x.w1 = w1; skeys[tl.start + idx] = x; }
         }
     }
     __syncthreads();
@@ -806,9 +807,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, int64_t);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, int64_t);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, int64_t);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
 
 // Pair of a global record index.
 __device__ __forceinline__ int pair_of(const PairDev *__restrict__ pairs, int npairs, int64_t g)

@@ -417,6 +417,73 @@ int pga_synth_batch_specs(int npairs, uint64_t seed, double scale,
     return 0;
 }
 
+int pga_synth_query(const pga_gdb *target, uint64_t seed, int64_t qbases, double div,
+                    int64_t seg_min, int64_t seg_max, pga_gdb *out)
+{
+    uint64_t s = seed;
+    contig_set Q;
+    int64_t got = 0;
+    int cap = 16;
+    memset(&Q, 0, sizeof(Q));
+    memset(out, 0, sizeof(*out));
+    if (target->ncontig <= 0 || qbases <= 0 || seg_min < 64 || seg_max < seg_min) return -1;
+    Q.seq = (uint8_t **) calloc((size_t) cap, sizeof(uint8_t *));
+    Q.len = (int64_t *) calloc((size_t) cap, sizeof(int64_t));
+    if (!Q.seq || !Q.len) { free_set(&Q); return -1; }
+    int64_t longest = 0;
+    for (int c = 0; c < target->ncontig; c++) if (target->clen[c] > longest) longest = target->clen[c];
+    if (longest < 64) { free_set(&Q); return -1; }
+    uint8_t *src = (uint8_t *) malloc((size_t) longest);
+    if (!src) { free_set(&Q); return -1; }
+    while (got < qbases) {
+        int64_t n = unif_int(&s, seg_min, seg_max);
+        if (n > qbases - got) n = qbases - got;
+        if (n < 64) n = 64;
+        int c;
+        int tries = 0;
+        do { c = (int) unif_int(&s, 0, target->ncontig - 1); } while (target->clen[c] < n && ++tries < 64);
+        if (target->clen[c] < n) n = target->clen[c];
+        int64_t off = unif_int(&s, 0, target->clen[c] - n);
+        pga_gdb_contig_codes(target, c, src);
+        uint8_t *seg = (uint8_t *) malloc((size_t) (n + n / 4 + 64));
+        if (!seg) { free(src); free_set(&Q); return -1; }
+        int64_t o = 0;
+        for (int64_t i = 0; i < n; i++) {
+            double u = unif(&s);
+            if (u < div * 0.05) {                      /* deletion of one base */
+                continue;
+            } else if (u < div * 0.1) {                /* insertion of one base */
+                seg[o++] = draw_base(&s, 0.41);
+            }
+            uint8_t b = src[off + i];
+            seg[o++] = (unif(&s) < div) ? other_base(&s, b) : b;
+        }
+        while (o < 64) seg[o++] = draw_base(&s, 0.41);
+        if (pga_splitmix64(&s) & 1) revcomp_inplace(seg, o);
+        if (Q.n == cap) {
+            cap *= 2;
+            Q.seq = (uint8_t **) realloc(Q.seq, (size_t) cap * sizeof(uint8_t *));
+            Q.len = (int64_t *) realloc(Q.len, (size_t) cap * sizeof(int64_t));
+            if (!Q.seq || !Q.len) { free(seg); free(src); return -1; }
+        }
+        Q.seq[Q.n] = seg;
+        Q.len[Q.n] = o;
+        Q.n++;
+        got += n;
+    }
+    free(src);
+    /* shuffled contig order (Fisher-Yates) */
+    for (int i = Q.n - 1; i > 0; i--) {
+        int j = (int) unif_int(&s, 0, i);
+        uint8_t *ts = Q.seq[i]; Q.seq[i] = Q.seq[j]; Q.seq[j] = ts;
+        int64_t tl = Q.len[i]; Q.len[i] = Q.len[j]; Q.len[j] = tl;
+    }
+    int rc = pack_sets(&Q, NULL, out);
+    free_set(&Q);
+    if (rc) pga_gdb_free(out);
+    return rc;
+}
+
 int pga_gdb_from_codes(const uint8_t *codes, const int64_t *clen, int ncontig,
                        int n_a, pga_gdb *out)
 {

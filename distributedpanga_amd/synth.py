@@ -70,6 +70,8 @@ def _setup(L):
     L.pga_gdb_from_codes.argtypes = [C.POINTER(C.c_uint8), C.POINTER(C.c_int64), C.c_int, C.c_int,
                                      C.POINTER(_Gdb)]
     L.pga_gdb_free.argtypes = [C.POINTER(_Gdb)]
+    L.pga_synth_query.argtypes = [C.POINTER(_Gdb), C.c_uint64, C.c_int64, C.c_double, C.c_int64, C.c_int64,
+                                  C.POINTER(_Gdb)]
     L._synth_ready = True
     return L
 
@@ -121,6 +123,52 @@ def batch_specs(npairs=64, seed=0x5EED0003, scale=1.0):
 def batch(npairs=64, seed=0x5EED0003, scale=1.0):
     specs, seeds = batch_specs(npairs, seed, scale)
     return [pair(s, sd) for s, sd in zip(specs, seeds)]
+
+
+def _as_c(g):
+    """A ctypes view of a Gdb (arrays must stay alive while it is used)."""
+    c = _Gdb()
+    c.ncontig = g.ncontig
+    c.n_a = g.n_a
+    c.clen = g.clen.ctypes.data_as(C.POINTER(C.c_int64))
+    c.boff = g.boff.ctypes.data_as(C.POINTER(C.c_int64))
+    c.bps = g.bps.ctypes.data_as(C.POINTER(C.c_uint8))
+    c.bps_bytes = int(g.bps.size)
+    c.seqtot = g.seqtot
+    c.maxctg = g.maxctg
+    return c
+
+
+def query(target, seed, qbases, div, seg_min=100_000, seg_max=10_000_000):
+    """A query genome from random target segments (see pga_synth_query)."""
+    L = _setup(host_lib())
+    tgt = _Gdb()
+    keep = (np.ascontiguousarray(target.bps), np.ascontiguousarray(target.boff, np.int64),
+            np.ascontiguousarray(target.clen, np.int64))
+    tgt = _as_c(synth_view(target, keep))
+    g = _Gdb()
+    if L.pga_synth_query(C.byref(tgt), C.c_uint64(seed), int(qbases), float(div), int(seg_min), int(seg_max),
+                         C.byref(g)):
+        raise ValueError("pga_synth_query failed")
+    try:
+        out = _adopt(g)
+        out.n_a = out.ncontig
+        return out
+    finally:
+        L.pga_gdb_free(C.byref(g))
+
+
+def synth_view(g, keep):
+    return Gdb(bps=keep[0], boff=keep[1], clen=keep[2], n_a=g.n_a)
+
+
+def concat(a, b):
+    """GDB holding a's contigs then b's (e.g. target then query)."""
+    bps = np.concatenate([a.bps[: int(a.boff[a.ncontig])], b.bps[: int(b.boff[b.ncontig])]])
+    boff = np.concatenate([a.boff[: a.ncontig], b.boff[: b.ncontig] + int(a.boff[a.ncontig]),
+                           [int(a.boff[a.ncontig]) + int(b.boff[b.ncontig])]]).astype(np.int64)
+    clen = np.concatenate([a.clen, b.clen]).astype(np.int64)
+    return Gdb(bps=bps, boff=boff, clen=clen, n_a=a.ncontig)
 
 
 def from_codes(contigs, n_a=None):

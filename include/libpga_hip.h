@@ -130,6 +130,46 @@ int pga_timings_get(pga_ctx *ctx, pga_timings *t);
 /* Wait for all work queued on the context's stream. */
 int pga_sync(pga_ctx *ctx);
 
+/* ------------------------------------------------------------------ serving
+ *
+ * A server keeps one target GDB indexed on the device and answers batches of
+ * query GDBs against it.  A query's answer is exactly the part of the
+ * reference's self-merge of one GDB holding the target's contigs followed by
+ * the query's (src/pga-mpi-merge.c:509, src/fastga_pipeline.c:245-412) in which
+ * the anchor or the partner lies on a query contig: same bytes, same order,
+ * with the query's contig ids numbered after the target's, and P/C widths of
+ * that combined GDB.  The buck vectors count those seeds per anchor contig
+ * (target contigs first).  Each query is answered independently of the others
+ * in its batch.  Limits: freq <= 24; target + query contigs <= 32767; contig
+ * lengths < 2^32.  Calls on one server must not overlap. */
+typedef struct pga_server pga_server;
+
+int         pga_server_create(const pga_opts *o, pga_server **out);
+void        pga_server_destroy(pga_server *s);
+const char *pga_server_error(const pga_server *s);
+
+/* Index the target (same arguments as pga_load_gdb); it stays resident. */
+int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes,
+                          const int64_t *boff, const int64_t *clen, int ncontig);
+int pga_server_target_info(pga_server *s, pga_index_meta *info);
+
+/* Device bytes a query of query_bases bases may take while it is answered
+ * (an upper estimate, for admission control). */
+int pga_server_footprint(pga_server *s, int64_t query_bases, int64_t *bytes);
+
+/* Answer nq queries (arrays of per-query pointers, as pga_load_batch). */
+int pga_server_run(pga_server *s, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
+                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig);
+
+/* Sizes / copies of query q's answer from the last pga_server_run
+ * (ncontig = target + query contigs = length of each buck vector). */
+int pga_server_seeds_info(pga_server *s, int q, pga_seeds_meta *info);
+int pga_server_seeds_export(pga_server *s, int q, uint8_t *nseeds, uint8_t *cseeds,
+                            int64_t *nbuck, int64_t *cbuck);
+
+/* Device milliseconds of the last pga_server_run (load, index, join). */
+int pga_server_last_ms(pga_server *s, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

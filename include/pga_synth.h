@@ -83,6 +83,14 @@ int  pga_synth_pair(const pga_pair_spec *spec, uint64_t seed, pga_gdb *out);
 int  pga_synth_batch_specs(int npairs, uint64_t seed, double scale,
                            pga_pair_spec *specs, uint64_t *seeds);
 
+/* A query genome for the serving workload: qbases taken from random segments
+ * of the target (segments of U[seg_min, seg_max] bases, at random offsets of
+ * random target contigs), each copied with per-base substitution probability
+ * div and indels at div/10, half of them reverse-complemented, in shuffled
+ * order.  Each segment is one contig of the query. */
+int  pga_synth_query(const pga_gdb *target, uint64_t seed, int64_t qbases, double div,
+                     int64_t seg_min, int64_t seg_max, pga_gdb *out);
+
 /* Build a GDB from numeric bases (0..3, one per byte), contig after contig. */
 int  pga_gdb_from_codes(const uint8_t *codes, const int64_t *clen, int ncontig,
                         int n_a, pga_gdb *out);
