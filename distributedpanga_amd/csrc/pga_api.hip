@@ -893,13 +893,13 @@ int pga_timings_get(pga_ctx *c, pga_timings *t)
 
 #include "pga_serve.hip"
 
-struct pga_server {
-    pga_opts o{};
-    pga_ctx *T = nullptr, *Q = nullptr;
+// One slot answers one batch of queries at a time on its own stream; slots of
+// one server share the resident target and may run concurrently (one host
+// thread per slot), so batches overlap on the device.
+struct pga_slot {
+    pga_ctx *Q = nullptr;
     char err[512] = {0};
-    bool have_target = false, have_result = false;
-    int nT = 0;
-    int64_t maxctgT = 0, seqtotT = 0;
+    bool have_result = false;
     int nq = 0;
     std::vector<QDev> qd;
     std::vector<int64_t> nseeds, cseeds;
@@ -908,119 +908,71 @@ struct pga_server {
     double last_ms = 0.0;
 };
 
+struct pga_server {
+    pga_opts o{};
+    pga_ctx *T = nullptr;
+    std::vector<pga_slot *> slots;
+    char err[512] = {0};
+    bool have_target = false;
+    int nT = 0;
+    int64_t maxctgT = 0;
+};
+
 namespace {
 
-int sfail(pga_server *s, int code, const char *fmt, ...)
+constexpr int SERVE_JBITS = 32;    // offset bits of served keys: contigs up to 2^32 bases, 32767 contigs
+constexpr int MAX_SLOTS = 8;
+
+int sfail(char *err, int code, const char *fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(s->err, sizeof(s->err), fmt, ap);
+    vsnprintf(err, 512, fmt, ap);
     va_end(ap);
     return code;
 }
 
-int from_ctx(pga_server *s, pga_ctx *c, int rc)
+// Copy a context's message into the caller-visible one.
+int from_ctx(char *err, pga_ctx *c, int rc)
 {
-    if (rc) snprintf(s->err, sizeof(s->err), "%s", c ? c->err : "context");
+    if (rc) snprintf(err, 512, "%s", c ? c->err : "context");
     return rc;
 }
 
-constexpr int SERVE_JBITS = 32;
-
-} // namespace
-
-extern "C" {
-
-int pga_server_create(const pga_opts *o, pga_server **out)
+void slot_free(pga_slot *q, int device)
 {
-    *out = nullptr;
-    pga_server *s = new (std::nothrow) pga_server();
-    if (!s) return PGA_ENOMEM;
-    if (o) s->o = *o; else pga_default_opts(&s->o);
-    if (s->o.freq > MAX_SERVE_FREQ) { delete s; return PGA_EINVAL; }
-    pga_opts to = s->o;
-    to.build_cum = 1;
-    int rc = pga_create(&to, &s->T);
-    if (!rc) rc = pga_create(&s->o, &s->Q);
-    if (rc) { pga_destroy(s->T); pga_destroy(s->Q); delete s; return rc; }
-    s->T->keep_keys = true;
-    s->Q->keep_keys = true;
-    s->Q->o.build_cum = 0;
-    for (auto &e : s->ev)
-        if (hipEventCreate(&e) != hipSuccess) { pga_destroy(s->T); pga_destroy(s->Q); delete s; return PGA_EHIP; }
-    *out = s;
-    return PGA_OK;
-}
-
-void pga_server_destroy(pga_server *s)
-{
-    if (!s) return;
-    pga_ctx *c = s->Q;
-    (void) hipSetDevice(s->o.device);
-    if (c && c->st) (void) hipStreamSynchronize(c->st);
-    DBuf *all[] = {&s->tpos, &s->flag, &s->segidx, &s->segstart, &s->segs, &s->qdev, &s->cntN, &s->cntC,
-                   &s->ofsN, &s->ofsC, &s->gath, &s->qrec, &s->seedN, &s->seedC, &s->nbuck, &s->cbuck};
+    if (!q) return;
+    (void) hipSetDevice(device);
+    if (q->Q && q->Q->st) (void) hipStreamSynchronize(q->Q->st);
+    DBuf *all[] = {&q->tpos, &q->flag, &q->segidx, &q->segstart, &q->segs, &q->qdev, &q->cntN, &q->cntC,
+                   &q->ofsN, &q->ofsC, &q->gath, &q->qrec, &q->seedN, &q->seedC, &q->nbuck, &q->cbuck};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
-    for (auto &e : s->ev) if (e) (void) hipEventDestroy(e);
-    pga_destroy(s->T);
-    pga_destroy(s->Q);
-    delete s;
+    for (auto &e : q->ev) if (e) (void) hipEventDestroy(e);
+    pga_destroy(q->Q);
+    delete q;
 }
 
-const char *pga_server_error(const pga_server *s) { return s ? s->err : "no server"; }
-
-int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes, const int64_t *boff,
-                          const int64_t *clen, int ncontig)
+int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
+             const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
 {
-    if (!s) return PGA_EINVAL;
-    s->have_target = s->have_result = false;
-    TRY(from_ctx(s, s->T, load_batch_impl(s->T, 1, &bps, &bps_bytes, &boff, &clen, &ncontig, nullptr, SERVE_JBITS)));
-    TRY(from_ctx(s, s->T, pga_build_index(s->T)));
-    s->nT = ncontig;
-    s->maxctgT = s->T->hp[0].maxctg;
-    s->seqtotT = s->T->hp[0].seqtot;
-    s->have_target = true;
-    return PGA_OK;
-}
-
-int pga_server_target_info(pga_server *s, pga_index_meta *info)
-{
-    if (!s || !s->have_target) return s ? sfail(s, PGA_EINVAL, "no target") : PGA_EINVAL;
-    return from_ctx(s, s->T, pga_index_info(s->T, 0, info));
-}
-
-int pga_server_footprint(pga_server *s, int64_t query_bases, int64_t *bytes)
-{
-    (void) s;
-    // records <= 0.8/base in practice (2/base bound); per record: two partition
-    // buffers, kept keys, entry, LCP, prefix, join position, plus seeds
-    *bytes = query_bases * 200 + ((int64_t) 64 << 20);
-    return PGA_OK;
-}
-
-int pga_server_run(pga_server *s, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
-                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
-{
-    if (!s || !s->have_target) return s ? sfail(s, PGA_EINVAL, "no target") : PGA_EINVAL;
-    if (nq <= 0) return sfail(s, PGA_EINVAL, "no queries");
-    pga_ctx *Q = s->Q, *T = s->T;
-    s->have_result = false;
+    pga_ctx *Q = sl->Q, *T = s->T;
     std::vector<int> base(nq, s->nT);
     for (int q = 0; q < nq; q++)
-        if ((int64_t) s->nT + ncontig[q] > 32767) return sfail(s, PGA_EOVERFLOW, "query %d: too many contigs", q);
-    HIPCHK(Q, hipEventRecord(s->ev[0], Q->st));
-    TRY(from_ctx(s, Q, load_batch_impl(Q, nq, bps, bps_bytes, boff, clen, ncontig, base.data(), T->jbits)));
-    if (Q->jbits != T->jbits) return sfail(s, PGA_EOVERFLOW, "query contig longer than the served offset width");
-    TRY(from_ctx(s, Q, pga_build_index(Q)));
+        if ((int64_t) s->nT + ncontig[q] > 32767) return sfail(sl->err, PGA_EOVERFLOW, "query %d: too many contigs", q);
+    HIPCHK(Q, hipSetDevice(s->o.device));
+    HIPCHK(Q, hipEventRecord(sl->ev[0], Q->st));
+    TRY(load_batch_impl(Q, nq, bps, bps_bytes, boff, clen, ncontig, base.data(), T->jbits));
+    if (Q->jbits != T->jbits) return sfail(sl->err, PGA_EOVERFLOW, "a query contig is longer than 2^%d bases", T->jbits);
+    TRY(pga_build_index(Q));
     const int64_t RQ = Q->R;
     const int W = Q->o.freq + 1;
-    s->nq = nq;
-    s->qd.assign(nq, QDev());
+    sl->nq = nq;
+    sl->qd.assign(nq, QDev());
     int64_t nb = 0;
     std::vector<int64_t> qrec(nq + 1);
     for (int q = 0; q < nq; q++) {
-        QDev &d = s->qd[q];
+        QDev &d = sl->qd[q];
         d.rec0 = Q->pairs[q].rec0;
         d.nrec = Q->pairs[q].nrec;
         d.ncomb = s->nT + ncontig[q];
@@ -1034,15 +986,16 @@ int pga_server_run(pga_server *s, int nq, const uint8_t *const *bps, const int64
     }
     qrec[nq] = RQ;
     const size_t rq1 = (size_t) std::max<int64_t>(RQ, 1);
-    TRY(ensure(Q, s->tpos, rq1 * sizeof(int64_t)));
-    TRY(ensure(Q, s->flag, rq1 * sizeof(uint32_t)));
-    TRY(ensure(Q, s->segidx, (rq1 + 1) * sizeof(int64_t)));
-    TRY(ensure(Q, s->segstart, rq1 * sizeof(int64_t)));
-    TRY(ensure(Q, s->qdev, (size_t) nq * sizeof(QDev)));
-    TRY(ensure(Q, s->qrec, (size_t) (nq + 1) * sizeof(int64_t)));
-    TRY(ensure(Q, s->gath, (size_t) 2 * (nq + 1) * sizeof(int64_t)));
-    HIPCHK(Q, hipMemcpyAsync(s->qdev.p, s->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
-    HIPCHK(Q, hipMemcpyAsync(s->qrec.p, qrec.data(), qrec.size() * sizeof(int64_t), hipMemcpyHostToDevice, Q->st));
+    TRY(ensure(Q, sl->tpos, rq1 * sizeof(int64_t)));
+    TRY(ensure(Q, sl->flag, rq1 * sizeof(uint32_t)));
+    TRY(ensure(Q, sl->segidx, (rq1 + 1) * sizeof(int64_t)));
+    TRY(ensure(Q, sl->segstart, rq1 * sizeof(int64_t)));
+    TRY(ensure(Q, sl->qdev, (size_t) nq * sizeof(QDev)));
+    TRY(ensure(Q, sl->qrec, (size_t) (nq + 1) * sizeof(int64_t)));
+    TRY(ensure(Q, sl->gath, (size_t) 2 * (nq + 1) * sizeof(int64_t)));
+    HIPCHK(Q, hipMemcpyAsync(sl->qrec.p, qrec.data(), qrec.size() * sizeof(int64_t), hipMemcpyHostToDevice, Q->st));
+    HIPCHK(Q, hipMemcpyAsync(sl->qdev.p, sl->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
+    // the host vectors stay untouched until the stream has consumed them (synchronised below)
     int64_t nseg = 0;
     std::vector<int64_t> seg0(nq + 1, 0);
     const int64_t *tcum = P<int64_t>(T->cum);
@@ -1050,128 +1003,254 @@ int pga_server_run(pga_server *s, int nq, const uint8_t *const *bps, const int64
         const unsigned g = grid_for(Q, (const void *) k_join_locate, 256, 0, (RQ + 255) / 256);
         hipLaunchKernelGGL(k_join_locate, dim3(g), dim3(256), 0, Q->st, (const Rec *) P<Rec>(T->skeys), tcum,
                            (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre), RQ,
-                           P<int64_t>(s->tpos));
+                           P<int64_t>(sl->tpos));
         LAUNCHCHK(Q);
-        hipLaunchKernelGGL(k_join_flags, dim3(g), dim3(256), 0, Q->st, (const QDev *) P<QDev>(s->qdev), nq,
-                           (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(s->tpos), RQ, W,
-                           P<uint32_t>(s->flag));
+        hipLaunchKernelGGL(k_join_flags, dim3(g), dim3(256), 0, Q->st, (const QDev *) P<QDev>(sl->qdev), nq,
+                           (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(sl->tpos), RQ, W,
+                           P<uint32_t>(sl->flag));
         LAUNCHCHK(Q);
-        TRY(scan_u32(Q, P<uint32_t>(s->flag), RQ, P<int64_t>(s->segidx)));
+        TRY(scan_u32(Q, P<uint32_t>(sl->flag), RQ, P<int64_t>(sl->segidx)));
         hipLaunchKernelGGL(k_pair_ranges, dim3((nq + 256) / 256), dim3(256), 0, Q->st,
-                           (const int64_t *) P<int64_t>(s->segidx), (const int64_t *) P<int64_t>(s->qrec), nq,
-                           P<int64_t>(s->gath));
+                           (const int64_t *) P<int64_t>(sl->segidx), (const int64_t *) P<int64_t>(sl->qrec), nq,
+                           P<int64_t>(sl->gath));
         LAUNCHCHK(Q);
-        HIPCHK(Q, hipMemcpyAsync(seg0.data(), s->gath.p, (size_t) (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
-        HIPCHK(Q, hipStreamSynchronize(Q->st));
-        nseg = seg0[nq];
+        HIPCHK(Q, hipMemcpyAsync(seg0.data(), sl->gath.p, (size_t) (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
     }
-    for (int q = 0; q < nq; q++) s->qd[q].seg0 = seg0[q];
-    std::vector<int64_t> po(2 * (nq + 1), 0);
+    HIPCHK(Q, hipStreamSynchronize(Q->st));
+    nseg = seg0[nq];
+    for (int q = 0; q < nq; q++) sl->qd[q].seg0 = seg0[q];
+    std::vector<int64_t> po(2 * (nq + 1), 0), sidx(nq + 1);
     if (nseg > 0) {
-        TRY(ensure(Q, s->segs, (size_t) nseg * sizeof(QSeg)));
-        TRY(ensure(Q, s->cntN, (size_t) nseg * sizeof(uint32_t)));
-        TRY(ensure(Q, s->cntC, (size_t) nseg * sizeof(uint32_t)));
-        TRY(ensure(Q, s->ofsN, (size_t) (nseg + 1) * sizeof(int64_t)));
-        TRY(ensure(Q, s->ofsC, (size_t) (nseg + 1) * sizeof(int64_t)));
-        HIPCHK(Q, hipMemcpyAsync(s->qdev.p, s->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
+        TRY(ensure(Q, sl->segs, (size_t) nseg * sizeof(QSeg)));
+        TRY(ensure(Q, sl->cntN, (size_t) nseg * sizeof(uint32_t)));
+        TRY(ensure(Q, sl->cntC, (size_t) nseg * sizeof(uint32_t)));
+        TRY(ensure(Q, sl->ofsN, (size_t) (nseg + 1) * sizeof(int64_t)));
+        TRY(ensure(Q, sl->ofsC, (size_t) (nseg + 1) * sizeof(int64_t)));
+        HIPCHK(Q, hipMemcpyAsync(sl->qdev.p, sl->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
         const unsigned g = grid_for(Q, (const void *) k_join_segs, 256, 0, (RQ + 255) / 256);
-        hipLaunchKernelGGL(k_join_segs, dim3(g), dim3(256), 0, Q->st, (const uint32_t *) P<uint32_t>(s->flag),
-                           (const int64_t *) P<int64_t>(s->segidx), RQ, P<int64_t>(s->segstart));
+        hipLaunchKernelGGL(k_join_segs, dim3(g), dim3(256), 0, Q->st, (const uint32_t *) P<uint32_t>(sl->flag),
+                           (const int64_t *) P<int64_t>(sl->segidx), RQ, P<int64_t>(sl->segstart));
         LAUNCHCHK(Q);
         hipLaunchKernelGGL(k_join_segs2, dim3(grid_for(Q, (const void *) k_join_segs2, 256, 0, (nseg + 255) / 256)),
-                           dim3(256), 0, Q->st, (const QDev *) P<QDev>(s->qdev), nq,
-                           (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(s->tpos), tcum, RQ, W,
-                           (const int64_t *) P<int64_t>(s->segstart), P<QSeg>(s->segs), nseg);
+                           dim3(256), 0, Q->st, (const QDev *) P<QDev>(sl->qdev), nq,
+                           (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(sl->tpos), tcum, RQ, W,
+                           (const int64_t *) P<int64_t>(sl->segstart), P<QSeg>(sl->segs), nseg);
         LAUNCHCHK(Q);
         const unsigned gj = grid_for(Q, (const void *) k_join_count, JT, 0, nseg);
-        hipLaunchKernelGGL(k_join_count, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(s->segs), nseg,
-                           (const QDev *) P<QDev>(s->qdev), (const Rec *) P<Rec>(T->skeys),
+        hipLaunchKernelGGL(k_join_count, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(sl->segs), nseg,
+                           (const QDev *) P<QDev>(sl->qdev), (const Rec *) P<Rec>(T->skeys),
                            (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre),
-                           (const int64_t *) P<int64_t>(s->tpos), T->jbits, Q->o.freq,
-                           P<uint32_t>(s->cntN), P<uint32_t>(s->cntC));
+                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq,
+                           P<uint32_t>(sl->cntN), P<uint32_t>(sl->cntC));
         LAUNCHCHK(Q);
-        TRY(scan_u32(Q, P<uint32_t>(s->cntN), nseg, P<int64_t>(s->ofsN)));
-        TRY(scan_u32(Q, P<uint32_t>(s->cntC), nseg, P<int64_t>(s->ofsC)));
-        // per-query seed totals: offsets at each query's first segment
-        std::vector<int64_t> sidx(nq + 1);
-        for (int q = 0; q <= nq; q++) sidx[q] = q < nq ? seg0[q] : nseg;
-        HIPCHK(Q, hipMemcpyAsync(s->qrec.p, sidx.data(), sidx.size() * sizeof(int64_t), hipMemcpyHostToDevice, Q->st));
+        TRY(scan_u32(Q, P<uint32_t>(sl->cntN), nseg, P<int64_t>(sl->ofsN)));
+        TRY(scan_u32(Q, P<uint32_t>(sl->cntC), nseg, P<int64_t>(sl->ofsC)));
+        // per-query seed totals: the scanned offsets at each query's first segment
+        for (int q = 0; q <= nq; q++) sidx[q] = seg0[q];
+        HIPCHK(Q, hipMemcpyAsync(sl->qrec.p, sidx.data(), sidx.size() * sizeof(int64_t), hipMemcpyHostToDevice, Q->st));
         hipLaunchKernelGGL(k_pair_ranges, dim3((nq + 256) / 256), dim3(256), 0, Q->st,
-                           (const int64_t *) P<int64_t>(s->ofsN), (const int64_t *) P<int64_t>(s->qrec), nq,
-                           P<int64_t>(s->gath));
+                           (const int64_t *) P<int64_t>(sl->ofsN), (const int64_t *) P<int64_t>(sl->qrec), nq,
+                           P<int64_t>(sl->gath));
         hipLaunchKernelGGL(k_pair_ranges, dim3((nq + 256) / 256), dim3(256), 0, Q->st,
-                           (const int64_t *) P<int64_t>(s->ofsC), (const int64_t *) P<int64_t>(s->qrec), nq,
-                           P<int64_t>(s->gath) + (nq + 1));
+                           (const int64_t *) P<int64_t>(sl->ofsC), (const int64_t *) P<int64_t>(sl->qrec), nq,
+                           P<int64_t>(sl->gath) + (nq + 1));
         LAUNCHCHK(Q);
-        HIPCHK(Q, hipMemcpyAsync(po.data(), s->gath.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
+        HIPCHK(Q, hipMemcpyAsync(po.data(), sl->gath.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
         HIPCHK(Q, hipStreamSynchronize(Q->st));
     }
-    s->nseeds.assign(nq, 0);
-    s->cseeds.assign(nq, 0);
+    sl->nseeds.assign(nq, 0);
+    sl->cseeds.assign(nq, 0);
     int64_t nbytes = 0, cbytes = 0;
     for (int q = 0; q < nq; q++) {
-        QDev &d = s->qd[q];
-        s->nseeds[q] = po[q + 1] - po[q];
-        s->cseeds[q] = po[nq + 1 + q + 1] - po[nq + 1 + q];
+        QDev &d = sl->qd[q];
+        sl->nseeds[q] = po[q + 1] - po[q];
+        sl->cseeds[q] = po[nq + 1 + q + 1] - po[nq + 1 + q];
         d.nseed_off = nbytes;
         d.cseed_off = cbytes;
-        nbytes += s->nseeds[q] * d.s;
-        cbytes += s->cseeds[q] * d.s;
+        nbytes += sl->nseeds[q] * d.s;
+        cbytes += sl->cseeds[q] * d.s;
     }
-    TRY(ensure(Q, s->seedN, (size_t) nbytes + 16));
-    TRY(ensure(Q, s->seedC, (size_t) cbytes + 16));
-    TRY(ensure(Q, s->nbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
-    TRY(ensure(Q, s->cbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
-    TRY(fill_u64(Q, s->nbuck.p, nb, 0));
-    TRY(fill_u64(Q, s->cbuck.p, nb, 0));
-    HIPCHK(Q, hipMemcpyAsync(s->qdev.p, s->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
+    TRY(ensure(Q, sl->seedN, (size_t) nbytes + 16));
+    TRY(ensure(Q, sl->seedC, (size_t) cbytes + 16));
+    TRY(ensure(Q, sl->nbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+    TRY(ensure(Q, sl->cbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+    TRY(fill_u64(Q, sl->nbuck.p, nb, 0));
+    TRY(fill_u64(Q, sl->cbuck.p, nb, 0));
+    HIPCHK(Q, hipMemcpyAsync(sl->qdev.p, sl->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
     if (nseg > 0) {
         const unsigned gj = grid_for(Q, (const void *) k_join_emit, JT, 0, nseg);
-        hipLaunchKernelGGL(k_join_emit, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(s->segs), nseg,
-                           (const QDev *) P<QDev>(s->qdev), (const Rec *) P<Rec>(T->skeys),
+        hipLaunchKernelGGL(k_join_emit, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(sl->segs), nseg,
+                           (const QDev *) P<QDev>(sl->qdev), (const Rec *) P<Rec>(T->skeys),
                            (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre),
-                           (const int64_t *) P<int64_t>(s->tpos), T->jbits, Q->o.freq,
-                           (const int64_t *) P<int64_t>(s->ofsN), (const int64_t *) P<int64_t>(s->ofsC),
-                           P<uint8_t>(s->seedN), P<uint8_t>(s->seedC), P<unsigned long long>(s->nbuck),
-                           P<unsigned long long>(s->cbuck));
+                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq,
+                           (const int64_t *) P<int64_t>(sl->ofsN), (const int64_t *) P<int64_t>(sl->ofsC),
+                           P<uint8_t>(sl->seedN), P<uint8_t>(sl->seedC), P<unsigned long long>(sl->nbuck),
+                           P<unsigned long long>(sl->cbuck));
         LAUNCHCHK(Q);
     }
-    HIPCHK(Q, hipEventRecord(s->ev[1], Q->st));
+    HIPCHK(Q, hipEventRecord(sl->ev[1], Q->st));
     HIPCHK(Q, hipStreamSynchronize(Q->st));
-    s->last_ms = ev_ms(s->ev[0], s->ev[1]);
-    s->have_result = true;
+    sl->last_ms = ev_ms(sl->ev[0], sl->ev[1]);
+    sl->have_result = true;
     return PGA_OK;
 }
 
-int pga_server_seeds_info(pga_server *s, int q, pga_seeds_meta *info)
+pga_slot *get_slot(pga_server *s, int slot)
 {
-    if (!s || !s->have_result || q < 0 || q >= s->nq) return s ? sfail(s, PGA_EINVAL, "no result for query %d", q) : PGA_EINVAL;
-    info->nlen = s->nseeds[q] * s->qd[q].s;
-    info->clen = s->cseeds[q] * s->qd[q].s;
-    info->s = s->qd[q].s;
-    info->ncontig = s->qd[q].ncomb;
-    return PGA_OK;
+    if (!s || slot < 0 || slot >= (int) s->slots.size()) return nullptr;
+    return s->slots[slot];
 }
 
-int pga_server_seeds_export(pga_server *s, int q, uint8_t *nseeds, uint8_t *cseeds, int64_t *nbuck, int64_t *cbuck)
+} // namespace
+
+extern "C" {
+
+int pga_server_create(const pga_opts *o, int nslots, pga_server **out)
 {
-    if (!s || !s->have_result || q < 0 || q >= s->nq) return s ? sfail(s, PGA_EINVAL, "no result for query %d", q) : PGA_EINVAL;
-    pga_ctx *Q = s->Q;
-    const QDev &d = s->qd[q];
-    if (nseeds && s->nseeds[q] > 0)
-        HIPCHK(Q, hipMemcpyAsync(nseeds, P<uint8_t>(s->seedN) + d.nseed_off, (size_t) (s->nseeds[q] * d.s), hipMemcpyDeviceToHost, Q->st));
-    if (cseeds && s->cseeds[q] > 0)
-        HIPCHK(Q, hipMemcpyAsync(cseeds, P<uint8_t>(s->seedC) + d.cseed_off, (size_t) (s->cseeds[q] * d.s), hipMemcpyDeviceToHost, Q->st));
-    if (nbuck) HIPCHK(Q, hipMemcpyAsync(nbuck, P<uint64_t>(s->nbuck) + d.buck_off, (size_t) d.ncomb * 8, hipMemcpyDeviceToHost, Q->st));
-    if (cbuck) HIPCHK(Q, hipMemcpyAsync(cbuck, P<uint64_t>(s->cbuck) + d.buck_off, (size_t) d.ncomb * 8, hipMemcpyDeviceToHost, Q->st));
-    HIPCHK(Q, hipStreamSynchronize(Q->st));
+    *out = nullptr;
+    if (nslots < 1 || nslots > MAX_SLOTS) return PGA_EINVAL;
+    pga_server *s = new (std::nothrow) pga_server();
+    if (!s) return PGA_ENOMEM;
+    if (o) s->o = *o; else pga_default_opts(&s->o);
+    if (s->o.freq > MAX_SERVE_FREQ) { delete s; return PGA_EINVAL; }
+    pga_opts to = s->o;
+    to.build_cum = 1;
+    int rc = pga_create(&to, &s->T);
+    if (rc) { delete s; return rc; }
+    s->T->keep_keys = true;
+    pga_opts qo = s->o;
+    qo.build_cum = 0;                  // query groups are located through the target's table
+    for (int k = 0; k < nslots && !rc; k++) {
+        pga_slot *sl = new (std::nothrow) pga_slot();
+        if (!sl) { rc = PGA_ENOMEM; break; }
+        s->slots.push_back(sl);
+        rc = pga_create(&qo, &sl->Q);
+        if (!rc) {
+            sl->Q->keep_keys = true;
+            for (auto &e : sl->ev)
+                if (hipEventCreate(&e) != hipSuccess) rc = PGA_EHIP;
+        }
+    }
+    if (rc) { pga_server_destroy(s); return rc; }
+    *out = s;
     return PGA_OK;
 }
 
-int pga_server_last_ms(pga_server *s, double *ms)
+void pga_server_destroy(pga_server *s)
+{
+    if (!s) return;
+    for (pga_slot *sl : s->slots) slot_free(sl, s->o.device);
+    pga_destroy(s->T);
+    delete s;
+}
+
+const char *pga_server_error(const pga_server *s, int slot)
+{
+    if (!s) return "no server";
+    if (slot >= 0 && slot < (int) s->slots.size()) {
+        const pga_slot *sl = s->slots[slot];
+        if (sl->err[0]) return sl->err;
+        if (sl->Q && sl->Q->err[0]) return sl->Q->err;
+        return "";
+    }
+    if (s->err[0]) return s->err;
+    if (s->T && s->T->err[0]) return s->T->err;
+    return "";
+}
+
+int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes, const int64_t *boff,
+                          const int64_t *clen, int ncontig)
 {
     if (!s) return PGA_EINVAL;
-    *ms = s->last_ms;
+    s->err[0] = 0;
+    s->T->err[0] = 0;
+    s->have_target = false;
+    for (pga_slot *sl : s->slots) sl->have_result = false;
+    TRY(from_ctx(s->err, s->T, load_batch_impl(s->T, 1, &bps, &bps_bytes, &boff, &clen, &ncontig, nullptr, SERVE_JBITS)));
+    TRY(from_ctx(s->err, s->T, pga_build_index(s->T)));
+    if (s->T->jbits != SERVE_JBITS) return sfail(s->err, PGA_EOVERFLOW, "a target contig is longer than 2^%d bases", SERVE_JBITS);
+    // every slot reads the target from its own stream
+    if (hipStreamSynchronize(s->T->st) != hipSuccess) return sfail(s->err, PGA_EHIP, "target build failed");
+    s->nT = ncontig;
+    s->maxctgT = s->T->hp[0].maxctg;
+    s->have_target = true;
+    return PGA_OK;
+}
+
+int pga_server_target_info(pga_server *s, pga_index_meta *info)
+{
+    if (!s) return PGA_EINVAL;
+    if (!s->have_target) return sfail(s->err, PGA_EINVAL, "no target");
+    return from_ctx(s->err, s->T, pga_index_info(s->T, 0, info));
+}
+
+int pga_server_footprint(pga_server *s, int64_t query_bases, int64_t *bytes)
+{
+    if (!s || query_bases < 0 || !bytes) return PGA_EINVAL;
+    // Per base: at most 2 records (one per strand) in practice 0.8; per record two
+    // 16 B partition buffers, 16 B kept keys, e <= 18 B entry, LCP, prefix, join
+    // position and flags (~70 B), plus up to ~2 seeds of s <= 19 B.
+    *bytes = query_bases * 120 + ((int64_t) 64 << 20);
+    return PGA_OK;
+}
+
+int pga_server_slots(pga_server *s) { return s ? (int) s->slots.size() : 0; }
+
+int pga_server_run(pga_server *s, int slot, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
+                   const int64_t *const *boff, const int64_t *const *clen, const int *ncontig)
+{
+    pga_slot *sl = get_slot(s, slot);
+    if (!sl) return PGA_EINVAL;
+    sl->err[0] = 0;
+    sl->Q->err[0] = 0;
+    sl->have_result = false;
+    if (!s->have_target) return sfail(sl->err, PGA_EINVAL, "no target");
+    if (nq <= 0) return sfail(sl->err, PGA_EINVAL, "no queries");
+    return slot_run(s, sl, nq, bps, bps_bytes, boff, clen, ncontig);
+}
+
+int pga_server_seeds_info(pga_server *s, int slot, int q, pga_seeds_meta *info)
+{
+    pga_slot *sl = get_slot(s, slot);
+    if (!sl) return PGA_EINVAL;
+    if (!sl->have_result || q < 0 || q >= sl->nq) return sfail(sl->err, PGA_EINVAL, "no result for query %d", q);
+    info->nlen = sl->nseeds[q] * sl->qd[q].s;
+    info->clen = sl->cseeds[q] * sl->qd[q].s;
+    info->s = sl->qd[q].s;
+    info->ncontig = sl->qd[q].ncomb;
+    return PGA_OK;
+}
+
+int pga_server_seeds_export(pga_server *s, int slot, int q, uint8_t *nseeds, uint8_t *cseeds, int64_t *nbuck,
+                            int64_t *cbuck)
+{
+    pga_slot *sl = get_slot(s, slot);
+    if (!sl) return PGA_EINVAL;
+    if (!sl->have_result || q < 0 || q >= sl->nq) return sfail(sl->err, PGA_EINVAL, "no result for query %d", q);
+    pga_ctx *Q = sl->Q;
+    const QDev &d = sl->qd[q];
+    HIPCHK(Q, hipSetDevice(s->o.device));
+    if (nseeds && sl->nseeds[q] > 0)
+        HIPCHK(Q, hipMemcpyAsync(nseeds, P<uint8_t>(sl->seedN) + d.nseed_off, (size_t) (sl->nseeds[q] * d.s),
+                                 hipMemcpyDeviceToHost, Q->st));
+    if (cseeds && sl->cseeds[q] > 0)
+        HIPCHK(Q, hipMemcpyAsync(cseeds, P<uint8_t>(sl->seedC) + d.cseed_off, (size_t) (sl->cseeds[q] * d.s),
+                                 hipMemcpyDeviceToHost, Q->st));
+    if (nbuck)
+        HIPCHK(Q, hipMemcpyAsync(nbuck, P<uint64_t>(sl->nbuck) + d.buck_off, (size_t) d.ncomb * 8,
+                                 hipMemcpyDeviceToHost, Q->st));
+    if (cbuck)
+        HIPCHK(Q, hipMemcpyAsync(cbuck, P<uint64_t>(sl->cbuck) + d.buck_off, (size_t) d.ncomb * 8,
+                                 hipMemcpyDeviceToHost, Q->st));
+    HIPCHK(Q, hipStreamSynchronize(Q->st));
+    return PGA_OK;
+}
+
+int pga_server_last_ms(pga_server *s, int slot, double *ms)
+{
+    pga_slot *sl = get_slot(s, slot);
+    if (!sl || !ms) return PGA_EINVAL;
+    *ms = sl->last_ms;
     return PGA_OK;
 }
 

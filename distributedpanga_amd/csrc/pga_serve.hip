@@ -23,10 +23,11 @@ constexpr int SEG_Q = 16;              // Q records per segment at most
 constexpr int SEGCAP = 1024;           // combined entries a segment can hold in LDS
 constexpr int JT = 256;                // threads of the join kernels
 constexpr int MAX_SERVE_FREQ = 24;     // largest FREQ a server accepts
-// A segment holds < 32W core target entries (W = FREQ+1; consecutive query
-// records of one segment sit < 2W target positions apart), FREQ+2 target
-// entries of context on each side, and at most SEG_Q + 2(FREQ+2) query records.
-static_assert(32 * (MAX_SERVE_FREQ + 1) + 4 * (MAX_SERVE_FREQ + 2) + SEG_Q <= SEGCAP, "segment exceeds LDS");
+// A segment holds < 33W core target entries (W = FREQ+1: consecutive query
+// records of one segment sit < 2W target positions apart, the core reaches W
+// before the first and at most 2W past the last), FREQ+2 target entries of
+// context on each side, and at most SEG_Q + 2(FREQ+2) query records.
+static_assert(33 * (MAX_SERVE_FREQ + 1) + 4 * (MAX_SERVE_FREQ + 2) + SEG_Q <= SEGCAP, "segment exceeds LDS");
 
 struct QSeg {
     int64_t ct_lo, ct_hi;              // core target range

@@ -141,12 +141,19 @@ int pga_sync(pga_ctx *ctx);
  * that combined GDB.  The buck vectors count those seeds per anchor contig
  * (target contigs first).  Each query is answered independently of the others
  * in its batch.  Limits: freq <= 24; target + query contigs <= 32767; contig
- * lengths < 2^32.  Calls on one server must not overlap. */
+ * lengths < 2^32.
+ *
+ * A server has 1..8 slots.  A slot answers one batch at a time on its own HIP
+ * stream; different slots may be driven from different host threads at once
+ * (their batches overlap on the device, all reading the one resident target).
+ * Calls on one slot must not overlap, and set_target must not overlap any run. */
 typedef struct pga_server pga_server;
 
-int         pga_server_create(const pga_opts *o, pga_server **out);
+int         pga_server_create(const pga_opts *o, int nslots, pga_server **out);
 void        pga_server_destroy(pga_server *s);
-const char *pga_server_error(const pga_server *s);
+int         pga_server_slots(pga_server *s);
+/* Message of the last failed call on `slot` (slot < 0: the target calls). */
+const char *pga_server_error(const pga_server *s, int slot);
 
 /* Index the target (same arguments as pga_load_gdb); it stays resident. */
 int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes,
@@ -157,18 +164,19 @@ int pga_server_target_info(pga_server *s, pga_index_meta *info);
  * (an upper estimate, for admission control). */
 int pga_server_footprint(pga_server *s, int64_t query_bases, int64_t *bytes);
 
-/* Answer nq queries (arrays of per-query pointers, as pga_load_batch). */
-int pga_server_run(pga_server *s, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
+/* Answer nq queries on `slot` (arrays of per-query pointers, as pga_load_batch).
+ * Returns when the answers are on the device. */
+int pga_server_run(pga_server *s, int slot, int nq, const uint8_t *const *bps, const int64_t *bps_bytes,
                    const int64_t *const *boff, const int64_t *const *clen, const int *ncontig);
 
-/* Sizes / copies of query q's answer from the last pga_server_run
+/* Sizes / copies of query q's answer from the slot's last run
  * (ncontig = target + query contigs = length of each buck vector). */
-int pga_server_seeds_info(pga_server *s, int q, pga_seeds_meta *info);
-int pga_server_seeds_export(pga_server *s, int q, uint8_t *nseeds, uint8_t *cseeds,
+int pga_server_seeds_info(pga_server *s, int slot, int q, pga_seeds_meta *info);
+int pga_server_seeds_export(pga_server *s, int slot, int q, uint8_t *nseeds, uint8_t *cseeds,
                             int64_t *nbuck, int64_t *cbuck);
 
-/* Device milliseconds of the last pga_server_run (load, index, join). */
-int pga_server_last_ms(pga_server *s, double *ms);
+/* Device milliseconds of the slot's last run (load, index, join). */
+int pga_server_last_ms(pga_server *s, int slot, double *ms);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,9 @@ query-side anchor or partner, byte for byte, and the same whether the query is
 answered alone or in a batch.  CPU: the request loop's admission and batching,
 and the host-side reduction.
 """
+import threading
+import time
+
 import numpy as np
 import pytest
 
@@ -16,35 +19,34 @@ from distributedpanga_amd import serve, synth
 # ---------------------------------------------------------------- CPU: host logic
 
 class FakeServer:
-    """Stands in for Server in the request loop: fixed device time per batch."""
+    """Stands in for Server in the request loop: each batch takes a fixed time."""
 
-    def __init__(self, clock, per_batch_s=0.010, fp=100):
-        self.clock, self.per_batch_s, self.fp = clock, per_batch_s, fp
+    def __init__(self, slots=1, per_batch_s=0.02, fp=100):
+        self.slots, self.per_batch_s, self.fp = slots, per_batch_s, fp
         self.batches = []
+        self.lock = threading.Lock()
+        self.running = 0
+        self.max_running = 0
+        self.sizes = {}
 
     def footprint(self, bases):
         return self.fp * bases
 
-    def run(self, queries):
-        self.batches.append(len(queries))
-        self.clock.t += self.per_batch_s
+    def run(self, queries, slot=0):
+        with self.lock:
+            self.batches.append(len(queries))
+            self.running += 1
+            self.max_running = max(self.max_running, self.running)
+            self.sizes[slot] = len(queries)
+        time.sleep(self.per_batch_s)
+        with self.lock:
+            self.running -= 1
 
-    def last_ms(self):
+    def last_ms(self, slot=0):
         return 1000.0 * self.per_batch_s
 
-    def seeds_info(self, q):
+    def seeds_info(self, q, slot=0):
         return dict(nlen=10 * 7, clen=0, s=7)
-
-
-class Clock:
-    def __init__(self):
-        self.t = 0.0
-
-    def __call__(self):
-        return self.t
-
-    def sleep(self, d):
-        self.t += d
 
 
 def _reqs(arrivals, bases=1000):
@@ -53,35 +55,49 @@ def _reqs(arrivals, bases=1000):
 
 
 def test_loop_batches_what_has_arrived():
-    clk = Clock()
-    srv = FakeServer(clk)
-    reqs = _reqs([0.0, 0.001, 0.002, 0.0105, 0.5])
-    st, _ = serve.serve(srv, reqs, budget_bytes=10 ** 9, clock=clk, sleep=clk.sleep)
-    # t=0: only request 0 has arrived; at t=0.010 requests 1, 2 have; then 3; then 4 after a wait
-    assert srv.batches == [1, 2, 1, 1]
-    assert st.requests == 5 and st.batches == 4 and st.max_batch == 2
-    lat = st.latencies_ms
-    assert abs(lat[0] - 10.0) < 1e-6 and abs(lat[1] - 19.0) < 1e-6 and abs(lat[4] - 10.0) < 1e-6
-    assert st.seeds == 50
+    srv = FakeServer(slots=1, per_batch_s=0.05)
+    reqs = _reqs([0.0, 0.005, 0.01, 0.2])
+    st, _ = serve.serve(srv, reqs, budget_bytes=10 ** 9)
+    # request 0 runs alone; 1 and 2 arrive while it runs and form the next batch; 3 comes later
+    assert srv.batches == [1, 2, 1]
+    assert st.requests == 4 and st.batches == 3 and st.max_batch == 2 and st.seeds == 40
+    assert all(r.done >= r.arrival + 0.05 for r in reqs)
+    assert reqs[1].start >= 0.05 and reqs[3].start >= 0.2
+
+
+def test_loop_overlaps_batches_on_free_slots():
+    srv = FakeServer(slots=3, per_batch_s=0.1)
+    reqs = _reqs([0.0, 0.01, 0.02, 0.03])
+    st, _ = serve.serve(srv, reqs, budget_bytes=10 ** 9)
+    # each arrival finds a free slot and starts at once; the fourth waits for one
+    assert srv.batches[:3] == [1, 1, 1]
+    assert srv.max_running == 3 and st.max_inflight == 3
+    assert reqs[3].start >= 0.1 - 1e-3
 
 
 def test_loop_respects_the_budget():
-    clk = Clock()
-    srv = FakeServer(clk, fp=100)
+    srv = FakeServer(slots=4, fp=100)
     reqs = _reqs([0.0] * 7, bases=1000)          # 100 kB each
-    serve.serve(srv, reqs, budget_bytes=250_000, clock=clk, sleep=clk.sleep)
-    assert srv.batches == [2, 2, 2, 1]
-    clk2 = Clock()
-    srv2 = FakeServer(clk2, fp=100)
-    serve.serve(srv2, _reqs([0.0] * 3, bases=10_000), budget_bytes=250_000, clock=clk2, sleep=clk2.sleep)
-    assert srv2.batches == [1, 1, 1]             # each alone exceeds the budget: runs alone
+    st, _ = serve.serve(srv, reqs, budget_bytes=250_000)
+    assert st.max_inflight_bytes <= 250_000
+    assert sum(srv.batches) == 7 and max(srv.batches) <= 2
+    srv2 = FakeServer(slots=2, fp=100)
+    st2, _ = serve.serve(srv2, _reqs([0.0] * 3, bases=10_000), budget_bytes=250_000)
+    assert srv2.batches == [1, 1, 1] and st2.max_inflight == 1   # each alone exceeds the budget
 
 
 def test_loop_max_batch():
-    clk = Clock()
-    srv = FakeServer(clk)
-    serve.serve(srv, _reqs([0.0] * 10), budget_bytes=10 ** 12, max_batch=4, clock=clk, sleep=clk.sleep)
+    srv = FakeServer(slots=1, per_batch_s=0.01)
+    serve.serve(srv, _reqs([0.0] * 10), budget_bytes=10 ** 12, max_batch=4)
     assert srv.batches == [4, 4, 2]
+
+
+def test_loop_surfaces_errors():
+    class Broken(FakeServer):
+        def run(self, queries, slot=0):
+            raise RuntimeError("device failure")
+    with pytest.raises(RuntimeError):
+        serve.serve(Broken(slots=2), _reqs([0.0, 0.0]), budget_bytes=10 ** 9)
 
 
 def test_poisson_arrivals_rate():
@@ -180,3 +196,21 @@ def test_server_rejects_large_freq():
     target = synth.workload(1, 0.01)
     with pytest.raises(Exception):
         serve.Server(target, freq=25)
+
+
+@pytest.mark.gpu
+def test_serve_loop_overlapping_slots_match_alone():
+    """The request loop with three slots driven concurrently: every answer
+    equals the oracle's and the query's answer when run alone."""
+    target = synth.workload(1, 0.1)
+    qs = [synth.query(target, 300 + i, 20_000 + 7_000 * (i % 5), 0.01 + 0.03 * (i % 4), seg_min=1_000,
+                      seg_max=9_000) for i in range(14)]
+    reqs = [serve.Request(rid=i, arrival=0.002 * i, query=q) for i, q in enumerate(qs)]
+    with serve.Server(target, slots=3) as srv:
+        st, answers = serve.serve(srv, reqs, budget_bytes=1 << 40, max_batch=4, export=True)
+        print(f"[serve-loop] {st.summary()}", flush=True)
+        assert st.requests == len(qs) and len(answers) == len(qs)
+        for i, q in enumerate(qs):
+            want = _expected(target, q, 10)
+            _same(answers[i], want, f"request {i} in the loop")
+            _same(srv.answer([q], slot=i % 3)[0], want, f"request {i} alone")
