@@ -12,6 +12,27 @@
  *     [0..7) k-mer bytes 3..9  [7] mask=0  [8] LCP  [9..e) pos LE, contig|strand LE
  *   seed (s = 1+2(P+C), src/pga-mpi-merge.c:68-76):
  *     [0] plen  [1..1+P+C) payload(i), strand bit cleared  [..s) payload(p)
+ *
+ * How each function is pinned (tests/test_oracle.py, tests/test_golden.py):
+ *   record_sizing     R-03 width table and E-13 (tests/CATALOG.md:415-423)
+ *   pack_forward/_rc  E-01, E-02 closed forms; RC == reverse-complement string
+ *   extract           E-06 known answer (3,106 records, right/left/tie 762/801/2),
+ *                     naive == rolling over many lengths, E-08/E-09/E-10, R-03;
+ *                     byte-equal to the compiled reference (oracle/_ref) at two
+ *                     scales and to the reference-written fixtures
+ *   sort_records      == qsort brute force with the full comparator, conservation;
+ *                     == the compiled reference after canonicalising equal k-mers
+ *   lcp_bases/recalc  S-01 hand table, S-02, R-04; == compiled reference
+ *   build_adapter     K-01..K-03; entries, prefix table and maxpre == the
+ *                     compiled reference and the fixtures
+ *   merge             closed forms on hand-built groups (plen, strand split,
+ *                     strand-bit clearing, FREQ cut at n = FREQ-1/FREQ/FREQ+1,
+ *                     no crossing of 24-bit groups) and two independent
+ *                     formulations (the reference's walk, mode 0, and the
+ *                     LCP-interval definition, mode 1) equal on random and
+ *                     workload inputs.  The compiled reference merge is not
+ *                     available here (its translation units need MPI), so no
+ *                     reference-run vector pins it.
  */
 #ifndef PGA_ORACLE_H
 #define PGA_ORACLE_H
