@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-from distributedpanga_amd import autotune, pga, synth  # noqa: E402
+from distributedpanga_amd import autotune, pga, serve, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "index+seed Gbases/s and % HBM peak, batched vs plain GPU vs CPU, 1 MI355X"
@@ -38,7 +38,7 @@ def time_steps(ctx, steps, warmup):
         ctx.merge()
     ctx.sync()
     stage = {k: 0.0 for k in ("extract_ms", "partition_ms", "tile_sort_ms", "table_ms", "merge_ms",
-                              "scatter_ms")}
+                              "scatter_ms", "merge_emit_ms")}
     scat_bytes = 0
     scat_launches = 0
     tiles = 0
@@ -63,6 +63,19 @@ def algorithmic_bytes(seqtot, nrec, e, s, nseed):
     (2 x read+write 16 B), LCP read, GIX entries written and read by the merge,
     seeds written."""
     return seqtot / 4 + 16 * nrec + 64 * nrec + 16 * nrec + 2 * e * nrec + s * nseed
+
+
+def measured_traffic(name, scale):
+    """HBM bytes per launch of the dominant kernel from its committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (valid for the full-size workload only)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if scale != 1.0 or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), "profiles/" + name
+    return None, None
 
 
 def subsample(g, target_bases):
@@ -92,6 +105,53 @@ def cpu_baseline(g, target_bases):
                 seconds=round(dt, 3))
 
 
+def serving_leg(args):
+    """Serving: genome A of the human-scale pair workload (at --serve-scale)
+    resident as the target; query genomes (lognormal 0.5-50 Mbp, from random
+    target segments at 1-20% divergence, segment order shuffled) arrive as a
+    Poisson process at 70% of the measured capacity and are answered by the
+    continuous-batching loop over --serve-slots slots."""
+    t = time.perf_counter()
+    pair = synth.workload(4, args.serve_scale)
+    target = synth.take(pair, 0, pair.n_a)
+    del pair
+    rng = np.random.default_rng(0x5EED0005)
+    sizes = np.clip(np.exp(rng.normal(np.log(5e6), 1.0, args.serve_pool)), 5e5, 5e7).astype(np.int64)
+    divs = rng.uniform(0.01, 0.20, args.serve_pool)
+    pool = [synth.query(target, 0x5EED0500 + i, int(sizes[i]), float(divs[i])) for i in range(args.serve_pool)]
+    log(f"[bench] serving: target {target.seqtot} bases in {target.ncontig} contigs, pool of {len(pool)} queries "
+        f"({sum(q.seqtot for q in pool)} bases) generated ({time.perf_counter() - t:.1f}s)")
+    budget = int(args.serve_budget_gb * 1e9)
+    with serve.Server(target, slots=args.serve_slots) as srv:
+        tinfo = srv.target_info()
+        for i in range(0, len(pool), args.serve_max_batch):            # warm every slot's buffers
+            srv.run(pool[i:i + args.serve_max_batch], slot=(i // args.serve_max_batch) % args.serve_slots)
+        # capacity: every request present at t=0, all slots busy
+        order = rng.integers(0, len(pool), args.serve_requests)
+        burst = [serve.Request(rid=i, arrival=0.0, query=pool[j]) for i, j in enumerate(order)]
+        cap, _ = serve.serve(srv, burst, budget, max_batch=args.serve_max_batch)
+        cap_bps = cap.bases / cap.wall_s
+        mean_bases = float(np.mean([pool[j].seqtot for j in order]))
+        rate = 0.7 * cap_bps / mean_bases
+        arr = serve.poisson_arrivals(args.serve_requests, rate, seed=5)
+        reqs = [serve.Request(rid=i, arrival=float(a), query=pool[j]) for i, (a, j) in enumerate(zip(arr, order))]
+        st, _ = serve.serve(srv, reqs, budget, max_batch=args.serve_max_batch)
+    sm = st.summary()
+    log(f"[bench] serving: capacity {cap_bps / 1e9:.2f} Gbases/s; at 70%: {sm['bases_per_s'] / 1e9:.2f} Gbases/s, "
+        f"p50 {sm['p50_ms']:.1f} ms, p99 {sm['p99_ms']:.1f} ms ({time.perf_counter() - t:.1f}s)")
+    return dict(workload=f"target = genome A of the human-scale pair workload at scale {args.serve_scale} "
+                         f"({target.seqtot} bases, {target.ncontig} contigs, {tinfo['nrec']} resident records); "
+                         f"{args.serve_requests} Poisson requests drawn from {len(pool)} query genomes "
+                         "(lognormal 0.5-50 Mbp, median 5 Mbp, 1-20% divergence), FREQ=10",
+                slots=args.serve_slots, max_batch=args.serve_max_batch, budget_gb=args.serve_budget_gb,
+                capacity_gbases_s=round(cap_bps / 1e9, 4), arrival_rate_per_s=round(rate, 3),
+                gbases_s=round(sm["bases_per_s"] / 1e9, 4), p50_ms=round(sm["p50_ms"], 3),
+                p99_ms=round(sm["p99_ms"], 3), mean_ms=round(sm["mean_ms"], 3), batches=sm["batches"],
+                max_batch_seen=sm["max_batch"], max_inflight=sm["max_inflight"],
+                device_busy_frac=round(sm["device_ms"] / 1e3 / sm["wall_s"], 3), seeds=sm["seeds"],
+                pcie_inclusive=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +165,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-bases", type=int, default=50_000_000)
     ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-scale", type=float, default=0.1, help="scale of the resident target (1.0 = 3.1 Gbp)")
+    ap.add_argument("--serve-pool", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=240)
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-max-batch", type=int, default=8)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-tune", action="store_true", help="use the fixed --tile/--chunk/--merge-tile")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
     args = ap.parse_args()
@@ -138,17 +205,24 @@ def main():
     log(f"[bench] plain: {ms_step:.2f} ms/step, {value:.2f} Gbases/s, {info['nrec']} records, {nseed} seeds, "
         f"stages {json.dumps({k: round(v, 3) for k, v in per.items()})}")
 
-    # dominant kernel: the LDS tile sort (one launch per step).  Algorithmic bytes:
-    # every record read once (16 B) and its GIX entry (e B), LCP/strand byte (1 B)
-    # and 24-bit prefix (4 B) written once.
+    # Dominant kernel: the merge's seed emission (longest launch of the step).
+    # Algorithmic bytes per launch: every GIX entry (e B) and LCP/strand byte
+    # read once, every seed (s B) and the two buck vectors written once.
+    emit_bytes = info["nrec"] * (info["e"] + 1) + nseed * info["s"] + 16 * info["ncontig"]
+    emit_gbs = emit_bytes / (per["merge_emit_ms"] * 1e-3) / 1e9
+    # the tile sort: records read once (16 B), GIX entry, LCP/strand byte and 24-bit prefix written once
     sort_bytes = info["nrec"] * (16 + info["e"] + 1 + 4)
     sort_gbs = sort_bytes / (per["tile_sort_ms"] * 1e-3) / 1e9
     scat_gbs = scat_bytes / (per["scatter_ms"] * 1e-3) / 1e9 if per["scatter_ms"] > 0 else 0.0
     balg = algorithmic_bytes(g.seqtot, info["nrec"], info["e"], info["s"], nseed)
     path_gbs = balg / (ms_step * 1e-3) / 1e9
-    roofline = dict(bound="hbm", kernel="k_tile_sort", achieved=round(sort_gbs, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(sort_gbs / HBM_PEAK_GBS, 4), traffic=None,
-                    bytes_per_launch=int(sort_bytes), avg_launch_ms=round(per["tile_sort_ms"], 4),
+    traffic, traffic_src = measured_traffic("merge_emit_workload2_hbm_traffic.md", args.scale)
+    roofline = dict(bound="hbm", kernel="k_merge_emit", achieved=round(emit_gbs, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(emit_gbs / HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=traffic_src, bytes_per_launch=int(emit_bytes),
+                    avg_launch_ms=round(per["merge_emit_ms"], 4),
+                    tile_sort=dict(achieved=round(sort_gbs, 1), frac=round(sort_gbs / HBM_PEAK_GBS, 4),
+                                   bytes_per_launch=int(sort_bytes), avg_launch_ms=round(per["tile_sort_ms"], 4)),
                     radix_scatter=dict(achieved=round(scat_gbs, 1), frac=round(scat_gbs / HBM_PEAK_GBS, 4),
                                        bytes_per_step=int(scat_bytes), launches_per_step=scat_launches,
                                        ms_per_step=round(per["scatter_ms"], 4)),
@@ -216,6 +290,9 @@ def main():
                               speedup=round(batched / plain, 3), batched_opts=bopts, plain_opts=popts,
                               batched_stages_ms={k: round(v, 3) for k, v in bper.items()})
         log(f"[bench] batched {batched:.2f} Gbases/s vs plain per pair {plain:.2f} Gbases/s")
+
+    if not args.no_serve:
+        out["serving"] = serving_leg(args)
 
     if not args.no_cpu:
         t = time.perf_counter()

@@ -97,6 +97,7 @@ struct pga_ctx {
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, bmax;
 
+    DBuf vout;                         // results of the small check kernels
     // sorted device keys kept after the build (serving)
     bool keep_keys = false;
     DBuf skeys;
@@ -370,7 +371,7 @@ void pga_destroy(pga_ctx *c)
     DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
-                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout,
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
@@ -572,10 +573,12 @@ int pga_build_index(pga_ctx *c)
 
     // ---- extraction: emit records in extraction order + level-1 digit counts
     if (nblk > 0 && R > 0) {
-        hipLaunchKernelGGL(k_extract_emit, dim3(grid_for(c, (const void *) k_extract_emit, 256, 0, nblk)), dim3(256),
+        const unsigned xg = grid_for(c, (const void *) k_extract_emit, 256, 0, nblk);
+        const int64_t xper = (nblk + xg - 1) / xg;
+        hipLaunchKernelGGL(k_extract_emit, dim3(xg), dim3(256),
                            0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, c->jbits,
                            (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
-                           P<unsigned long long>(c->hist1), nblk);
+                           P<unsigned long long>(c->hist1), nblk, xper);
         LAUNCHCHK(c);
     }
     TRY(record(c, 1));
@@ -743,6 +746,74 @@ int pga_index_export(pga_ctx *c, int pair, uint8_t *entries, int64_t *cum)
     return PGA_OK;
 }
 
+int pga_index_verify(pga_ctx *c, int pair, int64_t *bad, int64_t *first_bad, int64_t *lcp_bad)
+{
+    if (!c || !c->built || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no index for pair %d", pair) : PGA_EINVAL;
+    HIPCHK(c, hipSetDevice(c->o.device));
+    const PairDev &d = c->pairs[pair];
+    TRY(ensure(c, c->vout, 4 * sizeof(uint64_t)));
+    uint64_t init[3] = {0, ~0ull, 0}, res[3] = {0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(c->vout.p, init, sizeof(init), hipMemcpyHostToDevice, c->st));
+    if (d.nrec > 0) {
+        hipLaunchKernelGGL(k_verify, dim3(grid_for(c, (const void *) k_verify, 256, 0, (d.nrec + 255) / 256)), dim3(256),
+                           0, c->st, (const uint8_t *) P<uint8_t>(c->entries) + d.ent_off, d.e,
+                           (const uint32_t *) P<uint32_t>(c->pre), d.rec0, d.nrec, P<unsigned long long>(c->vout));
+        LAUNCHCHK(c);
+    }
+    HIPCHK(c, hipMemcpyAsync(res, c->vout.p, sizeof(res), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (bad) *bad = (int64_t) res[0];
+    if (first_bad) *first_bad = res[1] == ~0ull ? -1 : (int64_t) res[1];
+    if (lcp_bad) *lcp_bad = (int64_t) res[2];
+    return PGA_OK;
+}
+
+int pga_bucket_counts(pga_ctx *c, int pair, int64_t *counts)
+{
+    if (!c || !c->built || pair < 0 || pair >= c->npairs || !counts)
+        return c ? fail(c, PGA_EINVAL, "no index for pair %d", pair) : PGA_EINVAL;
+    if (!c->o.build_cum) return fail(c, PGA_EINVAL, "prefix table not built (opts.build_cum = 0)");
+    HIPCHK(c, hipSetDevice(c->o.device));
+    TRY(ensure(c, c->vout, 1024 * sizeof(int64_t)));
+    hipLaunchKernelGGL(k_bucket_counts, dim3(4), dim3(256), 0, c->st,
+                       (const int64_t *) P<int64_t>(c->cum) + c->pairs[pair].cum_off, P<long long>(c->vout));
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(counts, c->vout.p, 1024 * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+// The shard splitter of a multi-GPU build: FastGA's distribute() rule as the
+// reference states it (src/dsort.c:120-167) -- walk the cumulative bucket
+// counts; at each multiple t of total/world_size the shard boundary goes
+// before or after the crossing bucket, whichever is nearer to t.
+int pga_destination_map(const int64_t *global_counts, int world_size, int *ksplit, int *select)
+{
+    if (!global_counts || !ksplit || !select || world_size < 1) return PGA_EINVAL;
+    const int NB = 1024;
+    int64_t total = 0;
+    for (int b = 0; b < NB; b++) total += global_counts[b];
+    ksplit[0] = 0;
+    int shard = 0;                      // shards closed so far
+    int64_t target = total / world_size, run = 0;
+    for (int b = 0; b < NB; b++) {
+        const int64_t before = run;
+        run += global_counts[b];
+        if (run >= target) {
+            const bool next = run - target > target - before;   // nearer to close before b
+            select[b] = next ? shard + 1 : shard;
+            ksplit[shard + 1] = next ? b : b + 1;
+            shard = shard + 1 < world_size - 1 ? shard + 1 : world_size - 1;
+            target = (int64_t) (shard + 1) * total / world_size;
+        } else {
+            select[b] = shard;
+        }
+    }
+    ksplit[world_size] = NB;
+    for (int r = shard + 1; r < world_size; r++) ksplit[r] = NB;
+    return PGA_OK;
+}
+
 int pga_merge(pga_ctx *c)
 {
     if (!c || !c->built) return c ? fail(c, PGA_EINVAL, "pga_merge before pga_build_index") : PGA_EINVAL;
@@ -824,6 +895,7 @@ int pga_merge(pga_ctx *c)
     }
     const int64_t egrid = std::min<int64_t>(nmb, (int64_t) c->ncu * occ * 4);
     const int64_t eper = egrid > 0 ? (nmb + egrid - 1) / egrid : 0;
+    TRY(record(c, 7));
     if (nmb > 0) {
 #define PGA_EMIT(A_)                                                                                    \
     hipLaunchKernelGGL(k_merge_emit<A_>, dim3((unsigned) egrid), dim3(MTH), 0, c->st,                   \
@@ -844,6 +916,7 @@ int pga_merge(pga_ctx *c)
     TRY(record(c, 6));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->tm.merge_ms = ev_ms(c->ev[5], c->ev[6]);
+    c->tm.merge_emit_ms = ev_ms(c->ev[7], c->ev[6]);
     c->merged = true;
     return PGA_OK;
 }
@@ -898,12 +971,13 @@ int pga_timings_get(pga_ctx *c, pga_timings *t)
 // thread per slot), so batches overlap on the device.
 struct pga_slot {
     pga_ctx *Q = nullptr;
-    char err[512] = {0};
+    char err_msg[512] = {0};
     bool have_result = false;
     int nq = 0;
     std::vector<QDev> qd;
     std::vector<int64_t> nseeds, cseeds;
     DBuf tpos, flag, segidx, segstart, segs, qdev, cntN, cntC, ofsN, ofsC, gath, qrec, seedN, seedC, nbuck, cbuck;
+    DBuf jseg, cost, soff, pstart, err;
     hipEvent_t ev[2] = {};
     double last_ms = 0.0;
 };
@@ -945,7 +1019,8 @@ void slot_free(pga_slot *q, int device)
     (void) hipSetDevice(device);
     if (q->Q && q->Q->st) (void) hipStreamSynchronize(q->Q->st);
     DBuf *all[] = {&q->tpos, &q->flag, &q->segidx, &q->segstart, &q->segs, &q->qdev, &q->cntN, &q->cntC,
-                   &q->ofsN, &q->ofsC, &q->gath, &q->qrec, &q->seedN, &q->seedC, &q->nbuck, &q->cbuck};
+                   &q->ofsN, &q->ofsC, &q->gath, &q->qrec, &q->seedN, &q->seedC, &q->nbuck, &q->cbuck,
+                   &q->jseg, &q->cost, &q->soff, &q->pstart, &q->err};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
     for (auto &e : q->ev) if (e) (void) hipEventDestroy(e);
@@ -959,11 +1034,11 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
     pga_ctx *Q = sl->Q, *T = s->T;
     std::vector<int> base(nq, s->nT);
     for (int q = 0; q < nq; q++)
-        if ((int64_t) s->nT + ncontig[q] > 32767) return sfail(sl->err, PGA_EOVERFLOW, "query %d: too many contigs", q);
+        if ((int64_t) s->nT + ncontig[q] > 32767) return sfail(sl->err_msg, PGA_EOVERFLOW, "query %d: too many contigs", q);
     HIPCHK(Q, hipSetDevice(s->o.device));
     HIPCHK(Q, hipEventRecord(sl->ev[0], Q->st));
     TRY(load_batch_impl(Q, nq, bps, bps_bytes, boff, clen, ncontig, base.data(), T->jbits));
-    if (Q->jbits != T->jbits) return sfail(sl->err, PGA_EOVERFLOW, "a query contig is longer than 2^%d bases", T->jbits);
+    if (Q->jbits != T->jbits) return sfail(sl->err_msg, PGA_EOVERFLOW, "a query contig is longer than 2^%d bases", T->jbits);
     TRY(pga_build_index(Q));
     const int64_t RQ = Q->R;
     const int W = Q->o.freq + 1;
@@ -1020,12 +1095,18 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
     nseg = seg0[nq];
     for (int q = 0; q < nq; q++) sl->qd[q].seg0 = seg0[q];
     std::vector<int64_t> po(2 * (nq + 1), 0), sidx(nq + 1);
+    int64_t npack = 0;
     if (nseg > 0) {
         TRY(ensure(Q, sl->segs, (size_t) nseg * sizeof(QSeg)));
         TRY(ensure(Q, sl->cntN, (size_t) nseg * sizeof(uint32_t)));
         TRY(ensure(Q, sl->cntC, (size_t) nseg * sizeof(uint32_t)));
         TRY(ensure(Q, sl->ofsN, (size_t) (nseg + 1) * sizeof(int64_t)));
         TRY(ensure(Q, sl->ofsC, (size_t) (nseg + 1) * sizeof(int64_t)));
+        TRY(ensure(Q, sl->jseg, (size_t) nseg * sizeof(JSeg)));
+        TRY(ensure(Q, sl->cost, (size_t) nseg * sizeof(uint32_t)));
+        TRY(ensure(Q, sl->soff, (size_t) (nseg + 1) * sizeof(int64_t)));
+        TRY(ensure(Q, sl->err, 16));
+        TRY(fill_u64(Q, sl->err.p, 1, 0));
         HIPCHK(Q, hipMemcpyAsync(sl->qdev.p, sl->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
         const unsigned g = grid_for(Q, (const void *) k_join_segs, 256, 0, (RQ + 255) / 256);
         hipLaunchKernelGGL(k_join_segs, dim3(g), dim3(256), 0, Q->st, (const uint32_t *) P<uint32_t>(sl->flag),
@@ -1036,12 +1117,32 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
                            (const uint32_t *) P<uint32_t>(Q->pre), (const int64_t *) P<int64_t>(sl->tpos), tcum, RQ, W,
                            (const int64_t *) P<int64_t>(sl->segstart), P<QSeg>(sl->segs), nseg);
         LAUNCHCHK(Q);
-        const unsigned gj = grid_for(Q, (const void *) k_join_count, JT, 0, nseg);
-        hipLaunchKernelGGL(k_join_count, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(sl->segs), nseg,
-                           (const QDev *) P<QDev>(sl->qdev), (const Rec *) P<Rec>(T->skeys),
-                           (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre),
-                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq,
-                           P<uint32_t>(sl->cntN), P<uint32_t>(sl->cntC));
+        // slot ranges of the segments, then packs of consecutive segments
+        hipLaunchKernelGGL(k_join_size, dim3(grid_for(Q, (const void *) k_join_size, 256, 0, (nseg + 255) / 256)),
+                           dim3(256), 0, Q->st, (const QSeg *) P<QSeg>(sl->segs), nseg, (const QDev *) P<QDev>(sl->qdev),
+                           (const uint32_t *) P<uint32_t>(Q->pre), Q->o.freq, P<JSeg>(sl->jseg), P<uint32_t>(sl->cost),
+                           P<unsigned int>(sl->err));
+        LAUNCHCHK(Q);
+        TRY(scan_u32(Q, P<uint32_t>(sl->cost), nseg, P<int64_t>(sl->soff)));
+        int64_t head[2] = {0, 0};
+        HIPCHK(Q, hipMemcpyAsync(&head[0], P<int64_t>(sl->soff) + nseg, sizeof(int64_t), hipMemcpyDeviceToHost, Q->st));
+        HIPCHK(Q, hipMemcpyAsync(&head[1], sl->err.p, sizeof(unsigned int), hipMemcpyDeviceToHost, Q->st));
+        HIPCHK(Q, hipStreamSynchronize(Q->st));
+        if ((unsigned int) head[1]) return sfail(sl->err_msg, PGA_EINVAL, "query records out of order (%u segments)", (unsigned) head[1]);
+        const int64_t hp = PK_CAP - max_segment(Q->o.freq);
+        npack = (head[0] + hp - 1) / hp;
+        TRY(ensure(Q, sl->pstart, (size_t) (npack + 1) * sizeof(int64_t)));
+        hipLaunchKernelGGL(k_join_packs, dim3(grid_for(Q, (const void *) k_join_packs, 256, 0, (npack + 256) / 256)),
+                           dim3(256), 0, Q->st, (const int64_t *) P<int64_t>(sl->soff), nseg, hp, npack,
+                           P<int64_t>(sl->pstart));
+        LAUNCHCHK(Q);
+        const unsigned gj = grid_for(Q, (const void *) k_join_pack<false>, PK_T, 0, npack);
+        hipLaunchKernelGGL(k_join_pack<false>, dim3(gj), dim3(PK_T), 0, Q->st, (const int64_t *) P<int64_t>(sl->pstart),
+                           npack, (const int64_t *) P<int64_t>(sl->soff), (const JSeg *) P<JSeg>(sl->jseg),
+                           (const QSeg *) P<QSeg>(sl->segs), (const QDev *) P<QDev>(sl->qdev),
+                           (const Rec *) P<Rec>(T->skeys), (const Rec *) P<Rec>(Q->skeys),
+                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq, P<uint32_t>(sl->cntN),
+                           P<uint32_t>(sl->cntC), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
         LAUNCHCHK(Q);
         TRY(scan_u32(Q, P<uint32_t>(sl->cntN), nseg, P<int64_t>(sl->ofsN)));
         TRY(scan_u32(Q, P<uint32_t>(sl->cntC), nseg, P<int64_t>(sl->ofsC)));
@@ -1077,12 +1178,13 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
     TRY(fill_u64(Q, sl->nbuck.p, nb, 0));
     TRY(fill_u64(Q, sl->cbuck.p, nb, 0));
     HIPCHK(Q, hipMemcpyAsync(sl->qdev.p, sl->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
-    if (nseg > 0) {
-        const unsigned gj = grid_for(Q, (const void *) k_join_emit, JT, 0, nseg);
-        hipLaunchKernelGGL(k_join_emit, dim3(gj), dim3(JT), 0, Q->st, (const QSeg *) P<QSeg>(sl->segs), nseg,
-                           (const QDev *) P<QDev>(sl->qdev), (const Rec *) P<Rec>(T->skeys),
-                           (const Rec *) P<Rec>(Q->skeys), (const uint32_t *) P<uint32_t>(Q->pre),
-                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq,
+    if (nseg > 0 && npack > 0) {
+        const unsigned gj = grid_for(Q, (const void *) k_join_pack<true>, PK_T, 0, npack);
+        hipLaunchKernelGGL(k_join_pack<true>, dim3(gj), dim3(PK_T), 0, Q->st, (const int64_t *) P<int64_t>(sl->pstart),
+                           npack, (const int64_t *) P<int64_t>(sl->soff), (const JSeg *) P<JSeg>(sl->jseg),
+                           (const QSeg *) P<QSeg>(sl->segs), (const QDev *) P<QDev>(sl->qdev),
+                           (const Rec *) P<Rec>(T->skeys), (const Rec *) P<Rec>(Q->skeys),
+                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq, nullptr, nullptr,
                            (const int64_t *) P<int64_t>(sl->ofsN), (const int64_t *) P<int64_t>(sl->ofsC),
                            P<uint8_t>(sl->seedN), P<uint8_t>(sl->seedC), P<unsigned long long>(sl->nbuck),
                            P<unsigned long long>(sl->cbuck));
@@ -1149,7 +1251,7 @@ const char *pga_server_error(const pga_server *s, int slot)
     if (!s) return "no server";
     if (slot >= 0 && slot < (int) s->slots.size()) {
         const pga_slot *sl = s->slots[slot];
-        if (sl->err[0]) return sl->err;
+        if (sl->err_msg[0]) return sl->err_msg;
         if (sl->Q && sl->Q->err[0]) return sl->Q->err;
         return "";
     }
@@ -1201,11 +1303,11 @@ int pga_server_run(pga_server *s, int slot, int nq, const uint8_t *const *bps, c
 {
     pga_slot *sl = get_slot(s, slot);
     if (!sl) return PGA_EINVAL;
-    sl->err[0] = 0;
+    sl->err_msg[0] = 0;
     sl->Q->err[0] = 0;
     sl->have_result = false;
-    if (!s->have_target) return sfail(sl->err, PGA_EINVAL, "no target");
-    if (nq <= 0) return sfail(sl->err, PGA_EINVAL, "no queries");
+    if (!s->have_target) return sfail(sl->err_msg, PGA_EINVAL, "no target");
+    if (nq <= 0) return sfail(sl->err_msg, PGA_EINVAL, "no queries");
     return slot_run(s, sl, nq, bps, bps_bytes, boff, clen, ncontig);
 }
 
@@ -1213,7 +1315,7 @@ int pga_server_seeds_info(pga_server *s, int slot, int q, pga_seeds_meta *info)
 {
     pga_slot *sl = get_slot(s, slot);
     if (!sl) return PGA_EINVAL;
-    if (!sl->have_result || q < 0 || q >= sl->nq) return sfail(sl->err, PGA_EINVAL, "no result for query %d", q);
+    if (!sl->have_result || q < 0 || q >= sl->nq) return sfail(sl->err_msg, PGA_EINVAL, "no result for query %d", q);
     info->nlen = sl->nseeds[q] * sl->qd[q].s;
     info->clen = sl->cseeds[q] * sl->qd[q].s;
     info->s = sl->qd[q].s;
@@ -1226,7 +1328,7 @@ int pga_server_seeds_export(pga_server *s, int slot, int q, uint8_t *nseeds, uin
 {
     pga_slot *sl = get_slot(s, slot);
     if (!sl) return PGA_EINVAL;
-    if (!sl->have_result || q < 0 || q >= sl->nq) return sfail(sl->err, PGA_EINVAL, "no result for query %d", q);
+    if (!sl->have_result || q < 0 || q >= sl->nq) return sfail(sl->err_msg, PGA_EINVAL, "no result for query %d", q);
     pga_ctx *Q = sl->Q;
     const QDev &d = sl->qd[q];
     HIPCHK(Q, hipSetDevice(s->o.device));

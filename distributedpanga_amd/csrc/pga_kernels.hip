@@ -219,13 +219,17 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
 
 // Emit records in extraction order at blkoff[block] (staged in LDS, written
 // with coalesced 16-byte stores) and count the first (level-1) radix digit
-// per pair.  Every block lies inside one pair.
+// per pair.  Every logical block lies inside one pair; each workgroup takes a
+// contiguous run of `per` logical blocks and keeps its digit counts in LDS
+// until the pair changes, so the 256 per-pair counters see one atomic per
+// workgroup and pair rather than one per logical block.
 __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict__ bps64,
                                                      const ContigDev *__restrict__ ctg, int nctg,
                                                      int64_t F, int jbits,
                                                      const int64_t *__restrict__ blkoff,
                                                      Rec *__restrict__ out,
-                                                     unsigned long long *__restrict__ hist1, int64_t nlog)
+                                                     unsigned long long *__restrict__ hist1, int64_t nlog,
+                                                     int64_t per)
 {
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint32_t hist[256];
@@ -233,11 +237,21 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
     __shared__ Rec stage[2 * XT * XU];
     __shared__ int sc;
     load_tables(tl, tm);
-    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
     for (int t = threadIdx.x; t < 256; t += XT) hist[t] = 0;
+    const int64_t lb0 = (int64_t) blockIdx.x * per;
+    const int64_t lb1 = lb0 + per < nlog ? lb0 + per : nlog;
+    int cur = -1;
+    for (int64_t LB = lb0; LB < lb1; LB++) {
     const int64_t fb = LB * XT * XU;
     int c = block_contig(ctg, nctg, fb, &sc);
     const int pair0 = ctg[c].pair;
+    if (pair0 != cur) {                               // uniform: every thread sees the same pair0
+        if (cur >= 0)
+            for (int t = threadIdx.x; t < 256; t += XT)
+                if (hist[t]) { atomicAdd(&hist1[(size_t) cur * 256 + t], (unsigned long long) hist[t]); hist[t] = 0; }
+        cur = pair0;
+        __syncthreads();
+    }
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint64_t k0[2 * XU], k1[2 * XU];
     uint32_t valid = 0;
@@ -278,10 +292,11 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
     __syncthreads();
     Rec *dst = out + blkoff[LB];
     for (int i = threadIdx.x; i < (int) total; i += XT) dst[i] = stage[i];
-    for (int t = threadIdx.x; t < 256; t += XT)
-        if (hist[t]) atomicAdd(&hist1[(size_t) pair0 * 256 + t], (unsigned long long) hist[t]);
     __syncthreads();
     }
+    if (cur >= 0)
+        for (int t = threadIdx.x; t < 256; t += XT)
+            if (hist[t]) atomicAdd(&hist1[(size_t) cur * 256 + t], (unsigned long long) hist[t]);
 }
 
 // ---------------------------------------------------------------- scans
@@ -1001,8 +1016,8 @@ __device__ __forceinline__ int merge_block_pair(const PairDev *__restrict__ pair
 
 constexpr int MTH = 256;             // threads of the merge kernels
 constexpr int BUCK_LDS = 1024;       // per-pair contigs counted in LDS by the merge
-constexpr int STG_N = 16384;         // LDS seed staging per round: same-strand stream
-constexpr int STG_C = 4096;          //                             cross-strand stream
+constexpr int TBL_N = 4096;          // seeds described in LDS per round: same-strand stream
+constexpr int TBL_C = 1024;          //                                   cross-strand stream
 
 // LCP/strand bytes of window [w0, w0+n) of the pair into win (0 outside it).
 __device__ __forceinline__ void load_win(uint8_t *__restrict__ win, const uint8_t *__restrict__ ls,
@@ -1052,23 +1067,6 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
     }
 }
 
-// Copy nb staged bytes (starting at stg[shift]) to g, 16-byte aligned stores
-// inside, byte stores at the two ends.
-__device__ __forceinline__ void copy_out(uint8_t *__restrict__ g, const uint8_t *__restrict__ stg,
-                                         int shift, int nb)
-{
-    const int nchunks = (shift + nb + 15) >> 4;
-    for (int c = threadIdx.x; c < nchunks; c += MTH) {
-        const int b0 = c << 4;
-        if (b0 >= shift && b0 + 16 <= shift + nb) {
-            *reinterpret_cast<uint4 *>(g + b0) = *reinterpret_cast<const uint4 *>(stg + b0);
-        } else {
-            for (int b = b0; b < b0 + 16; b++)
-                if (b >= shift && b < shift + nb) g[b] = stg[b];
-        }
-    }
-}
-
 __device__ __forceinline__ void flush_buck(uint32_t *__restrict__ lb, int n, int64_t off,
                                            unsigned long long *__restrict__ nbuck,
                                            unsigned long long *__restrict__ cbuck)
@@ -1090,13 +1088,68 @@ __device__ __forceinline__ uint64_t payload_at(const uint8_t *__restrict__ ew, i
     return v;
 }
 
+// Write seeds [0, n) of a round straight to global memory, 16 bytes per lane.
+// Seed j is described by tbl[j] = anchor slot | partner slot << 11 | plen << 22
+// (slots index the LDS window); its bytes are [plen][anchor payload, strand bit
+// cleared][partner payload] (src/pga-mpi-merge.c:68-76).  Every lane assembles
+// one aligned 16-byte chunk of the output; chunks the round shares with its
+// neighbours in the stream are written byte by byte.
+__device__ __forceinline__ void emit_round(uint8_t *__restrict__ g, const uint32_t *__restrict__ tbl, int n,
+                                           const uint8_t *__restrict__ ew, int e, int pc, int s)
+{
+    if (n <= 0) return;
+    const uintptr_t base = (uintptr_t) g;
+    const uintptr_t a0 = base & ~(uintptr_t) 15;
+    const int64_t nb = (int64_t) n * s;
+    const int nchunks = (int) ((base + nb - a0 + 15) >> 4);
+    const uint8_t clear = 0x7F;                       // strand bit of the anchor payload's last byte
+    for (int c = threadIdx.x; c < nchunks; c += MTH) {
+        const int64_t o0 = (int64_t) (a0 + 16 * (uintptr_t) c) - (int64_t) base;   // may be < 0 in chunk 0
+        uint8_t v[16];
+        const int o = o0 < 0 ? 0 : (int) o0;           // < TBL_N * s
+        int j = o / s, t = o - j * s;
+        uint32_t d = j < n ? tbl[j] : 0;
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+            const int64_t ob = o0 + b;
+            uint8_t x = 0;
+            if (ob >= 0 && ob < nb) {
+                if (t == 0) {
+                    x = (uint8_t) (d >> 22);
+                } else if (t <= pc) {
+                    x = ew[(int) (d & 2047u) * e + 9 + (t - 1)];
+                    if (t == pc) x &= clear;
+                } else {
+                    x = ew[(int) ((d >> 11) & 2047u) * e + 9 + (t - 1 - pc)];
+                }
+                if (++t == s) { t = 0; j++; d = j < n ? tbl[j] : 0; }
+            }
+            v[b] = x;
+        }
+        uint8_t *dst = reinterpret_cast<uint8_t *>(a0 + 16 * (uintptr_t) c);
+        if (o0 >= 0 && o0 + 16 <= nb) {
+            uint4 w;
+            w.x = (uint32_t) v[0] | (uint32_t) v[1] << 8 | (uint32_t) v[2] << 16 | (uint32_t) v[3] << 24;
+            w.y = (uint32_t) v[4] | (uint32_t) v[5] << 8 | (uint32_t) v[6] << 16 | (uint32_t) v[7] << 24;
+            w.z = (uint32_t) v[8] | (uint32_t) v[9] << 8 | (uint32_t) v[10] << 16 | (uint32_t) v[11] << 24;
+            w.w = (uint32_t) v[12] | (uint32_t) v[13] << 8 | (uint32_t) v[14] << 16 | (uint32_t) v[15] << 24;
+            *reinterpret_cast<uint4 *>(dst) = w;
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; b++)
+                if (o0 + b >= 0 && o0 + b < nb) dst[b] = v[b];
+        }
+    }
+}
+
 // Seeds of a run of merge tiles (A*MTH anchors each, A consecutive per
 // thread).  Per tile: the LCP/strand bytes and raw GIX entries of the tile plus
 // a FREQ halo on each side go to LDS, each anchor finds its interval, block
 // scans place the anchors' seeds in the tile's slice of the N and C streams
-// (offsets from the count pass), and the seeds are staged in LDS and written
-// with aligned 16-byte stores.  Per-anchor-contig counts accumulate in LDS
-// while the pair stays the same.
+// (offsets from the count pass).  Each round, every seed is described by one
+// 32-bit LDS word (anchor, partner, plen) and the output bytes are assembled
+// and stored 16 at a time by all lanes alike.  Per-anchor-contig counts
+// accumulate in LDS while the pair stays the same.
 template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
                                                     const uint8_t *__restrict__ ls,
@@ -1110,10 +1163,11 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
 {
     constexpr int TA = MTH * A;
     constexpr int W = TA + 2 * MAX_FREQ + 4;
+    constexpr int FRONT = (MAX_FREQ + 2) * 16;   // room for entries left of a clipped window
     __shared__ uint8_t win[W];
-    __shared__ __align__(16) uint8_t ent[W * 16 + 16];
-    __shared__ __align__(16) uint8_t stn[STG_N + 16];
-    __shared__ __align__(16) uint8_t stc[STG_C + 16];
+    __shared__ __align__(16) uint8_t entp[FRONT + W * 16 + 16];
+    __shared__ uint32_t tblN[TBL_N];
+    __shared__ uint32_t tblC[TBL_C];
     __shared__ uint32_t lb[2 * BUCK_LDS];
     __shared__ uint32_t sh[MTH / 64 + 1];
     for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MTH) lb[c] = 0;
@@ -1134,7 +1188,7 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         const int64_t end = pd.rec0 + pd.nrec;
         const int H = freq + 1;
         const int pc = pd.P + pd.C, e = pd.e, s = pd.s;
-        // window [lo-H, lo+TA+H) clipped to the pair; raw entry bytes with 4-byte loads
+        // window [lo-H, lo+TA+H) clipped to the pair; raw entry bytes with aligned 4-byte loads and stores
         const int64_t wlo = lo - H > pd.rec0 ? lo - H : pd.rec0;
         const int64_t whi = lo + TA + H < end ? lo + TA + H : end;
         const int64_t b0 = pd.ent_off + (wlo - pd.rec0) * (int64_t) e;
@@ -1145,19 +1199,13 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
         {
             const uint32_t *src = reinterpret_cast<const uint32_t *>(entries + a0);
-            uint8_t *dst = ent + (int) (wlo - (lo - H)) * e;
-            for (int k = threadIdx.x; k < nwords; k += MTH) {
-                const uint32_t v = src[k];
-                dst[4 * k + 0] = (uint8_t) v;
-                dst[4 * k + 1] = (uint8_t) (v >> 8);
-                dst[4 * k + 2] = (uint8_t) (v >> 16);
-                dst[4 * k + 3] = (uint8_t) (v >> 24);
-            }
+            uint32_t *dst = reinterpret_cast<uint32_t *>(entp + FRONT);
+            for (int k = threadIdx.x; k < nwords; k += MTH) dst[k] = src[k];
         }
         __syncthreads();
         // window entry k (global lo-H+k) starts at ew + k*e; entries outside
         // [wlo, whi) are never read (their LCP byte is 0, a run boundary)
-        const uint8_t *ew = ent + (int) (b0 - a0);
+        const uint8_t *ew = entp + FRONT + (int) (b0 - a0) - (int) (wlo - (lo - H)) * e;
         int pl[A], aa[A], bb[A];
         uint32_t nN[A], nC[A];
         uint32_t sN = 0, sC = 0;
@@ -1181,46 +1229,33 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         uint32_t totN, totC;
         const uint32_t oN = block_excl_scan<MTH, uint32_t>(sN, sh, &totN);
         const uint32_t oC = block_excl_scan<MTH, uint32_t>(sC, sh, &totC);
-        const int capN = STG_N / s, capC = STG_C / s;
-        const int64_t rounds = max(((int64_t) totN + capN - 1) / capN, ((int64_t) totC + capC - 1) / capC);
+        const int64_t rounds = max(((int64_t) totN + TBL_N - 1) / TBL_N, ((int64_t) totC + TBL_C - 1) / TBL_C);
         uint8_t *gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
         uint8_t *gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
         for (int64_t r = 0; r < rounds; r++) {
-            const int64_t n0 = r * capN, n1 = min(n0 + capN, (int64_t) totN);
-            const int64_t c0 = r * capC, c1 = min(c0 + capC, (int64_t) totC);
-            const int shN = (int) ((uintptr_t) (gN + n0 * s) & 15);
-            const int shC = (int) ((uintptr_t) (gC + c0 * s) & 15);
+            const int64_t n0 = r * TBL_N, n1 = min(n0 + TBL_N, (int64_t) totN);
+            const int64_t c0 = r * TBL_C, c1 = min(c0 + TBL_C, (int64_t) totC);
             int64_t kn = oN, kc = oC;
 #pragma unroll
             for (int k = 0; k < A; k++) {
                 if (nN[k] + nC[k] == 0) continue;
                 const int gi = H + threadIdx.x * A + k;
                 const uint8_t si = win[gi] & 0x80;
-                // anchor payload with its strand bit cleared
-                const uint64_t pi = payload_at(ew, gi, e, pc) & ~(0x80ull << (8 * (pc - 1)));
+                const uint32_t head = (uint32_t) gi | (uint32_t) pl[k] << 22;
                 for (int q = aa[k]; q < bb[k]; q++) {
                     if (q == gi) continue;
-                    const bool same = (win[q] & 0x80) == si;
-                    uint8_t *d = nullptr;
-                    if (same) {
-                        if (kn >= n0 && kn < n1) d = stn + shN + (kn - n0) * s;
+                    if ((win[q] & 0x80) == si) {
+                        if (kn >= n0 && kn < n1) tblN[kn - n0] = head | (uint32_t) q << 11;
                         kn++;
                     } else {
-                        if (kc >= c0 && kc < c1) d = stc + shC + (kc - c0) * s;
+                        if (kc >= c0 && kc < c1) tblC[kc - c0] = head | (uint32_t) q << 11;
                         kc++;
-                    }
-                    if (d) {
-                        const uint64_t pq = payload_at(ew, q, e, pc);
-                        d[0] = (uint8_t) pl[k];
-#pragma unroll
-                        for (int t = 0; t < 7; t++)
-                            if (t < pc) { d[1 + t] = (uint8_t) (pi >> (8 * t)); d[1 + pc + t] = (uint8_t) (pq >> (8 * t)); }
                     }
                 }
             }
             __syncthreads();
-            if (n1 > n0) copy_out(gN + n0 * s - shN, stn, shN, (int) ((n1 - n0) * s));
-            if (c1 > c0) copy_out(gC + c0 * s - shC, stc, shC, (int) ((c1 - c0) * s));
+            emit_round(gN + n0 * s, tblN, (int) (n1 - n0), ew, e, pc, s);
+            emit_round(gC + c0 * s, tblC, (int) (c1 - c0), ew, e, pc, s);
             __syncthreads();
         }
         // per-anchor-contig counts (src/fastga_pipeline.c:403)
@@ -1241,6 +1276,63 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
     __syncthreads();
     if (cur_pair >= 0 && pairs[cur_pair].ncontig <= BUCK_LDS)
         flush_buck(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
+}
+
+// ---------------------------------------------------------------- checks
+
+// 80-bit k-mer of GIX entry i: 24-bit prefix, then the entry's 7 suffix bytes.
+__device__ __forceinline__ void entry_kmer(const uint8_t *__restrict__ ent, int e, const uint32_t *__restrict__ pre,
+                                           int64_t g, int64_t li, uint64_t &k0, uint64_t &k1)
+{
+    const uint8_t *x = ent + li * e;
+    uint64_t suf = 0;
+#pragma unroll
+    for (int b = 0; b < 7; b++) suf = (suf << 8) | x[b];
+    k0 = ((uint64_t) pre[g] << 40) | (suf >> 16);
+    k1 = (suf & 0xFFFFull) << 48;
+}
+
+// Sortedness and LCP check of one pair's index (src/dsort.c:229-246): bad
+// counts entries whose k-mer is smaller than its predecessor's; lcp_bad
+// counts entries whose LCP byte differs from the recomputed LCP with the
+// predecessor (0 for the first entry).  out = {bad, first_bad, lcp_bad}.
+__global__ void k_verify(const uint8_t *__restrict__ ent, int e, const uint32_t *__restrict__ pre, int64_t rec0,
+                         int64_t nrec, unsigned long long *__restrict__ out)
+{
+    __shared__ uint64_t red[256 / 64 + 1];
+    uint64_t bad = 0, lbad = 0, first = ~0ull;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nrec;
+         i += (int64_t) gridDim.x * blockDim.x) {
+        uint64_t b0, b1;
+        entry_kmer(ent, e, pre, rec0 + i, i, b0, b1);
+        int l = 0;
+        if (i > 0) {
+            uint64_t a0, a1;
+            entry_kmer(ent, e, pre, rec0 + i - 1, i - 1, a0, a1);
+            if (a0 > b0 || (a0 == b0 && a1 > b1)) { bad++; if ((uint64_t) i < first) first = (uint64_t) i; }
+            l = lcp_bases(a0, a1, b0, b1);
+        }
+        if (ent[i * e + 8] != (uint8_t) l) lbad++;
+    }
+    bad = block_sum_u64<256>(bad, red);
+    lbad = block_sum_u64<256>(lbad, red);
+    if (threadIdx.x == 0) {
+        if (bad) atomicAdd(&out[0], (unsigned long long) bad);
+        if (lbad) atomicAdd(&out[2], (unsigned long long) lbad);
+    }
+    if (first != ~0ull) atomicMin(&out[1], (unsigned long long) first);
+}
+
+// Records per 5-base bucket (src/dsort.h:30-33: the first 10 bits of the
+// k-mer) from a pair's cumulative 24-bit prefix table.
+__global__ void k_bucket_counts(const int64_t *__restrict__ cum, long long *__restrict__ out)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < 1024) {
+        const int64_t hi = cum[((b + 1) << 14) - 1];
+        const int64_t lo = b ? cum[(b << 14) - 1] : 0;
+        out[b] = (long long) (hi - lo);
+    }
 }
 
 #define PGA_MERGE_INST(A_)                                                                              \

@@ -10,7 +10,8 @@
 //      search inside the group);
 //   2. Q records are cut into segments whose cores (target entries within W of
 //      a Q record, plus the Q records) are disjoint and ordered;
-//   3. a segment's target and Q keys (core plus FREQ+2 entries of context on
+//   3. consecutive segments are packed into workgroup-sized packs; each
+//      segment's target and Q keys (core plus FREQ+2 entries of context on
 //      each side, or up to the group edge) are merged in LDS, LCPs recomputed,
 //      and every core anchor's interval evaluated exactly as in the index merge;
 //      seeds with a Q side are counted, then emitted in anchor order.
@@ -20,14 +21,7 @@
 namespace pga {
 
 constexpr int SEG_Q = 16;              // Q records per segment at most
-constexpr int SEGCAP = 1024;           // combined entries a segment can hold in LDS
-constexpr int JT = 256;                // threads of the join kernels
 constexpr int MAX_SERVE_FREQ = 24;     // largest FREQ a server accepts
-// A segment holds < 33W core target entries (W = FREQ+1: consecutive query
-// records of one segment sit < 2W target positions apart, the core reaches W
-// before the first and at most 2W past the last), FREQ+2 target entries of
-// context on each side, and at most SEG_Q + 2(FREQ+2) query records.
-static_assert(33 * (MAX_SERVE_FREQ + 1) + 4 * (MAX_SERVE_FREQ + 2) + SEG_Q <= SEGCAP, "segment exceeds LDS");
 
 struct QSeg {
     int64_t ct_lo, ct_hi;              // core target range
@@ -141,193 +135,352 @@ __device__ __forceinline__ uint64_t key_payload(const Rec &k, int jbits, int P, 
     return pos | (cs << (8 * P));
 }
 
-// Shared state of one segment in LDS.
-struct JoinLds {
-    Rec key[SEGCAP];
-    uint8_t win[SEGCAP + 2 * MAX_FREQ + 8];    // LCP | strand<<7, offset by H, 0 outside
-    uint8_t isq[SEGCAP];                        // bit0: query record, bit1: core anchor
-    int64_t qt[SEG_Q + 2 * (MAX_SERVE_FREQ + 2)]; // tpos of the loaded query records
-    int n, lo_idx, hi_idx;                      // combined entries; valid window [lo_idx, hi_idx]
+// ---------------------------------------------------------------- packed join
+//
+// Most segments hold one or two query records and a few dozen target entries,
+// so one workgroup works on a "pack" of consecutive segments at a time: up to
+// PK_CAP combined entries in LDS, each segment merged in place inside its own
+// slot range.  Segment slot ranges are padded to at least PK_MINC entries, so a
+// pack holds at most PK_CAP / PK_MINC segments.  Packs cut the running sum of
+// slot sizes at multiples of Hp = PK_CAP - (largest segment): pack p takes the
+// segments whose slot range starts in [p*Hp, (p+1)*Hp).
+
+constexpr int PK_CAP = 2048;           // combined entries of one pack
+constexpr int PK_MINC = 8;             // slots per segment at least
+constexpr int PK_SEG = PK_CAP / PK_MINC;
+constexpr int PK_T = 256;              // threads of the pack kernels
+constexpr int PK_PER = PK_CAP / PK_T;  // anchor slots per thread
+constexpr int PK_BUCK = 1024;          // anchor contigs counted in LDS (the pack's first query)
+
+// Largest segment in combined entries for a FREQ: 33W core target entries,
+// FREQ+2 of context each side, SEG_Q + 2(FREQ+2) query records.
+__host__ __device__ constexpr int max_segment(int freq)
+{
+    return 33 * (freq + 1) + 4 * (freq + 2) + SEG_Q;
+}
+static_assert(max_segment(MAX_SERVE_FREQ) < PK_CAP / 2, "pack too small for the largest segment");
+
+// Loaded ranges of one segment (relative offsets fit 16 bits: a segment is
+// smaller than PK_CAP).
+struct JSeg {
+    int64_t tlo, qlo;                  // first loaded target entry / query record
+    uint16_t nt, nq;                   // loaded counts
+    uint16_t ct0, ct1;                 // core target range, relative to tlo
+    uint16_t cq0, cq1;                 // core query range, relative to qlo
+    uint16_t trunc;                    // bit0 t_lo, bit1 t_hi, bit2 q_lo, bit3 q_hi truncated
+    uint16_t pad;
 };
 
-// Load and merge one segment; returns the number of combined entries (all threads).
-__device__ int join_load(JoinLds &L, const QSeg &sg, const QDev &d, const Rec *__restrict__ tkeys,
-                         const Rec *__restrict__ qkeys, const uint32_t *__restrict__ qpre,
-                         const int64_t *__restrict__ tpos, int freq, int H)
+// S1. Loaded ranges and slot cost of every segment.
+__global__ void k_join_size(const QSeg *__restrict__ segs, int64_t nseg, const QDev *__restrict__ qd,
+                            const uint32_t *__restrict__ qpre, int freq, JSeg *__restrict__ js,
+                            uint32_t *__restrict__ cost, unsigned int *__restrict__ err)
 {
     const int MG = freq + 2;
-    const uint32_t p = qpre[sg.cq_lo];
-    const int64_t tlo = max(sg.glo, sg.ct_lo - MG), thi = min(sg.ghi, sg.ct_hi + MG);
-    int64_t qlo = max(d.rec0, sg.cq_lo - MG), qhi = min(d.rec0 + d.nrec, sg.cq_hi + MG);
-    // clip the query context to the group (at most MG steps each way)
-    while (qlo < sg.cq_lo && qpre[qlo] != p) qlo++;
-    while (qhi > sg.cq_hi && qpre[qhi - 1] != p) qhi--;
-    const int nt = (int) (thi - tlo), nqq = (int) (qhi - qlo);
-    const bool t_lo_trunc = tlo > sg.glo, t_hi_trunc = thi < sg.ghi;
-    const bool q_lo_trunc = qlo > d.rec0 && qpre[qlo - 1] == p;
-    const bool q_hi_trunc = qhi < d.rec0 + d.nrec && qpre[qhi] == p;
-    const int n = nt + nqq;
-    // query records: combined index = (records before it in T) + (rank among loaded Q)
-    for (int i = threadIdx.x; i < nqq; i += JT) L.qt[i] = tpos[qlo + i];
-    __syncthreads();
-    for (int i = threadIdx.x; i < nqq; i += JT) {
-        int64_t t = L.qt[i] - tlo;
-        t = t < 0 ? 0 : (t > nt ? nt : t);
-        const int ci = (int) t + i;
-        L.key[ci] = qkeys[qlo + i];
-        L.isq[ci] = 1 | ((qlo + i >= sg.cq_lo && qlo + i < sg.cq_hi) ? 2 : 0);
+    const uint32_t cap = (uint32_t) max_segment(freq);
+    for (int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
+         s += (int64_t) gridDim.x * blockDim.x) {
+        const QSeg sg = segs[s];
+        const QDev d = qd[sg.q];
+        const uint32_t p = qpre[sg.cq_lo];
+        const int64_t tlo = max(sg.glo, sg.ct_lo - MG), thi = min(sg.ghi, sg.ct_hi + MG);
+        int64_t qlo = max(d.rec0, sg.cq_lo - MG), qhi = min(d.rec0 + d.nrec, sg.cq_hi + MG);
+        while (qlo < sg.cq_lo && qpre[qlo] != p) qlo++;
+        while (qhi > sg.cq_hi && qpre[qhi - 1] != p) qhi--;
+        JSeg j;
+        j.tlo = tlo;
+        j.qlo = qlo;
+        j.nt = (uint16_t) (thi - tlo);
+        j.nq = (uint16_t) (qhi - qlo);
+        j.ct0 = (uint16_t) (sg.ct_lo - tlo);
+        j.ct1 = (uint16_t) (sg.ct_hi - tlo);
+        j.cq0 = (uint16_t) (sg.cq_lo - qlo);
+        j.cq1 = (uint16_t) (sg.cq_hi - qlo);
+        j.trunc = (uint16_t) ((tlo > sg.glo ? 1 : 0) | (thi < sg.ghi ? 2 : 0) |
+                              ((qlo > d.rec0 && qpre[qlo - 1] == p) ? 4 : 0) |
+                              ((qhi < d.rec0 + d.nrec && qpre[qhi] == p) ? 8 : 0));
+        j.pad = 0;
+        js[s] = j;
+        uint32_t n = (uint32_t) (thi - tlo + qhi - qlo);
+        if (n > cap) {                     // cannot happen for sorted inputs; never overrun LDS
+            atomicAdd(err, 1u);
+            n = cap;
+            j.nt = 0;
+            j.nq = 0;
+            js[s] = j;
+        }
+        cost[s] = n > (uint32_t) PK_MINC ? n : (uint32_t) PK_MINC;
     }
-    for (int i = threadIdx.x; i < nt; i += JT) {
-        const int64_t k = tlo + i;
-        // loaded query records placed before T[k]: those with tpos <= k
-        int a = 0, b = nqq;
-        while (a < b) { int m = (a + b) >> 1; if (L.qt[m] <= k) a = m + 1; else b = m; }
-        const int ci = i + a;
-        L.key[ci] = tkeys[k];
-        L.isq[ci] = (k >= sg.ct_lo && k < sg.ct_hi) ? 2 : 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // the contiguous part of the combined order: clipped where a list was truncated
-        int lo = 0, hi = n - 1;
-        if (t_lo_trunc) { int a = 0, b = nqq; while (a < b) { int m = (a + b) >> 1; if (L.qt[m] <= tlo) a = m + 1; else b = m; } lo = max(lo, a); }
-        if (q_lo_trunc && nqq) { int64_t t = L.qt[0] - tlo; t = t < 0 ? 0 : (t > nt ? nt : t); lo = max(lo, (int) t); }
-        if (t_hi_trunc && nt) { int a = 0, b = nqq; while (a < b) { int m = (a + b) >> 1; if (L.qt[m] <= thi - 1) a = m + 1; else b = m; } hi = min(hi, nt - 1 + a); }
-        if (q_hi_trunc && nqq) { int64_t t = L.qt[nqq - 1] - tlo; t = t < 0 ? 0 : (t > nt ? nt : t); hi = min(hi, (int) t + nqq - 1); }
-        L.n = n;
-        L.lo_idx = lo;
-        L.hi_idx = hi;
-    }
-    __syncthreads();
-    // LCP / strand window; 0 outside [lo_idx, hi_idx] and at the group start
-    const int lo = L.lo_idx, hi = L.hi_idx;
-    for (int i = threadIdx.x; i < n + 2 * H + 2; i += JT) L.win[i] = 0;
-    __syncthreads();
-    for (int i = lo + threadIdx.x; i <= hi; i += JT) {
-        int l = 0;
-        if (i > lo) l = lcp_bases(L.key[i - 1].w0, L.key[i - 1].w1, L.key[i].w0, L.key[i].w1);
-        L.win[H + i] = (uint8_t) (l | ((L.key[i].w1 & 1ull) << 7));
-    }
-    __syncthreads();
-    return n;
 }
 
-// Counts (count pass) or seeds (emit pass) of one segment's core anchors.
-// A seed is kept when its anchor or its partner is a query record.
-template <bool EMIT>
-__device__ void join_anchors(JoinLds &L, const QDev &d, int jbits, int freq, int H,
-                             uint32_t *nN, uint32_t *nC, uint8_t *__restrict__ outN, uint8_t *__restrict__ outC,
-                             int64_t oN, int64_t oC, unsigned long long *__restrict__ nbuck,
-                             unsigned long long *__restrict__ cbuck)
+// S2. First segment of every pack (segments whose slot start lies in [p*Hp, (p+1)*Hp)).
+__global__ void k_join_packs(const int64_t *__restrict__ soff, int64_t nseg, int64_t hp, int64_t npack,
+                             int64_t *__restrict__ pstart)
 {
-    const int pc = d.P + d.C, s = d.s;
-    uint32_t cn = 0, cc = 0;
-    for (int i = threadIdx.x; i < L.n; i += JT) {
-        if (!(L.isq[i] & 2) || i < L.lo_idx || i > L.hi_idx) continue;
-        const int gi = H + i;
-        int plen, a, b;
-        if (!anchor_interval(L.win, gi, freq, plen, a, b)) continue;
-        const bool iq = L.isq[i] & 1;
-        const uint8_t si = L.win[gi] & 0x80;
-        uint32_t ln = 0, lc = 0;
-        for (int q = a; q < b; q++) {
-            if (q == gi) continue;
-            const int j = q - H;
-            if (!iq && !(L.isq[j] & 1)) continue;
-            if ((L.win[q] & 0x80) == si) ln++; else lc++;
+    for (int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; p <= npack;
+         p += (int64_t) gridDim.x * blockDim.x) {
+        if (p == npack) { pstart[p] = nseg; continue; }
+        const int64_t x = p * hp;
+        int64_t lo = 0, hi = nseg;
+        while (lo < hi) {
+            const int64_t m = (lo + hi) >> 1;
+            if (soff[m] < x) lo = m + 1; else hi = m;
         }
-        if (EMIT && (ln | lc)) {
-            // offsets of this anchor's seeds: the anchors before it in the segment
-            // were counted by the same loop order in a prefix pass (see caller)
-            uint64_t pi = key_payload(L.key[i], jbits, d.P, d.C) & ~(0x80ull << (8 * (pc - 1)));
-            int64_t kn = oN + nN[i], kc = oC + nC[i];
-            for (int q = a; q < b; q++) {
-                if (q == gi) continue;
-                const int j = q - H;
-                if (!iq && !(L.isq[j] & 1)) continue;
-                const bool same = (L.win[q] & 0x80) == si;
-                uint8_t *dst = same ? outN + (kn++) * s : outC + (kc++) * s;
-                const uint64_t pq = key_payload(L.key[j], jbits, d.P, d.C);
+        pstart[p] = lo;
+    }
+}
+
+struct PackLds {
+    Rec key[PK_CAP];
+    uint8_t win[PK_CAP + 2 * (MAX_SERVE_FREQ + 1) + 8];
+    uint8_t flg[PK_CAP];               // bit0 query record, bit1 core anchor
+    uint8_t sidx[PK_CAP];              // segment of each slot
+    uint16_t qtp[PK_CAP];              // query slots: insertion point relative to the segment's tlo
+    // per segment of the pack
+    int64_t tlo[PK_SEG], qlo[PK_SEG];
+    int64_t bN[PK_SEG], bC[PK_SEG];    // seed byte base: byte = b + (global seed index) * s
+    uint16_t so[PK_SEG + 1], nt[PK_SEG], nq[PK_SEG], lo[PK_SEG], hi[PK_SEG];
+    uint16_t ct0[PK_SEG], ct1[PK_SEG], cq0[PK_SEG], cq1[PK_SEG];
+    int32_t q[PK_SEG];
+    uint32_t cN[PK_SEG], cC[PK_SEG];   // seeds per segment (count pass)
+    uint8_t sbytes[PK_SEG], Pb[PK_SEG], Cb[PK_SEG];
+    uint32_t bk[2][PK_BUCK];           // buck counters of the pack's first query (emit pass)
+    uint32_t scan[PK_T / 64 + 1];
+    int ns, npos, qa;
+};
+
+__device__ __forceinline__ int pack_seg_of(const PackLds &L, int pos)
+{
+    int lo = 0, hi = L.ns;             // largest t with so[t] <= pos
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (L.so[m] <= pos) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+// Load pack p into LDS: segments, merged keys, flags, valid windows, LCP/strand window.
+__device__ void pack_load(PackLds &L, int64_t p, const int64_t *__restrict__ pstart,
+                          const int64_t *__restrict__ soff, const JSeg *__restrict__ js,
+                          const QSeg *__restrict__ segs, const QDev *__restrict__ qd,
+                          const Rec *__restrict__ tkeys, const Rec *__restrict__ qkeys,
+                          const int64_t *__restrict__ tpos, const int64_t *__restrict__ ofsN,
+                          const int64_t *__restrict__ ofsC, int H, bool emit)
+{
+    const int64_t s0 = pstart[p], s1 = pstart[p + 1];
+    const int ns = (int) (s1 - s0);
+    const int64_t base = s0 < s1 ? soff[s0] : 0;
+    for (int t = threadIdx.x; t < ns; t += PK_T) {
+        const JSeg j = js[s0 + t];
+        const int q = segs[s0 + t].q;
+        L.so[t] = (uint16_t) (soff[s0 + t] - base);
+        L.tlo[t] = j.tlo;
+        L.qlo[t] = j.qlo;
+        L.nt[t] = j.nt;
+        L.nq[t] = j.nq;
+        L.ct0[t] = j.ct0;
+        L.ct1[t] = j.ct1;
+        L.cq0[t] = j.cq0;
+        L.cq1[t] = j.cq1;
+        L.q[t] = q;
+        L.cN[t] = 0;
+        L.cC[t] = 0;
+        if (emit) {
+            const QDev d = qd[q];
+            L.sbytes[t] = (uint8_t) d.s;
+            L.Pb[t] = (uint8_t) d.P;
+            L.Cb[t] = (uint8_t) d.C;
+            L.bN[t] = d.nseed_off - ofsN[d.seg0] * (int64_t) d.s;
+            L.bC[t] = d.cseed_off - ofsC[d.seg0] * (int64_t) d.s;
+        }
+        // valid window needs the query insertion points: filled below
+        L.lo[t] = (uint16_t) j.trunc;            // trunc bits parked here until then
+    }
+    if (threadIdx.x == 0) {
+        L.ns = ns;
+        L.npos = ns > 0 ? (int) (soff[s1] - base) : 0;
+        L.so[ns] = (uint16_t) L.npos;
+        L.qa = ns > 0 ? segs[s0].q : -1;
+    }
+    __syncthreads();
+    const int npos = L.npos;
+    // query slots: insertion points relative to tlo, clipped to [0, nt]
+    for (int pos = threadIdx.x; pos < npos; pos += PK_T) {
+        const int s = pack_seg_of(L, pos);
+        L.sidx[pos] = (uint8_t) s;
+        L.flg[pos] = 0;
+        const int k = pos - L.so[s], nt = L.nt[s];
+        if (k >= nt && k < nt + L.nq[s]) {
+            int64_t r = tpos[L.qlo[s] + (k - nt)] - L.tlo[s];
+            r = r < 0 ? 0 : (r > nt ? nt : r);
+            L.qtp[pos] = (uint16_t) r;
+        }
+    }
+    __syncthreads();
+    // place target and query keys at their merged slot inside the segment
+    for (int pos = threadIdx.x; pos < npos; pos += PK_T) {
+        const int s = L.sidx[pos];
+        const int b = L.so[s], k = pos - b, nt = L.nt[s], nq = L.nq[s];
+        if (k < nt) {
+            int a = 0, e = nq;                           // query records placed before T[tlo+k]
+            while (a < e) { const int m = (a + e) >> 1; if (L.qtp[b + nt + m] <= k) a = m + 1; else e = m; }
+            L.key[b + k + a] = tkeys[L.tlo[s] + k];
+            L.flg[b + k + a] = (k >= L.ct0[s] && k < L.ct1[s]) ? 2 : 0;
+        } else if (k < nt + nq) {
+            const int i = k - nt;
+            const int m = L.qtp[pos] + i;
+            L.key[b + m] = qkeys[L.qlo[s] + i];
+            L.flg[b + m] = (uint8_t) (1 | ((i >= L.cq0[s] && i < L.cq1[s]) ? 2 : 0));
+        }
+    }
+    // valid window of each segment, from its truncation bits and query insertion points
+    for (int t = threadIdx.x; t < ns; t += PK_T) {
+        const int tr = L.lo[t];
+        const int b = L.so[t], nt = L.nt[t], nq = L.nq[t], n = nt + nq;
+        const uint16_t *qt = L.qtp + b + nt;
+        int lo = 0, hi = n - 1;
+        if ((tr & 1) && nq) {                        // query records at or before tlo come first
+            int a = 0, e = nq;
+            while (a < e) { const int m = (a + e) >> 1; if (qt[m] <= 0) a = m + 1; else e = m; }
+            lo = max(lo, a);
+        }
+        if ((tr & 4) && nq) lo = max(lo, (int) qt[0]);
+        if ((tr & 2) && nt) {
+            int a = 0, e = nq;
+            while (a < e) { const int m = (a + e) >> 1; if (qt[m] <= nt - 1) a = m + 1; else e = m; }
+            hi = min(hi, nt - 1 + a);
+        }
+        if ((tr & 8) && nq) hi = min(hi, (int) qt[nq - 1] + nq - 1);
+        L.lo[t] = (uint16_t) lo;
+        L.hi[t] = (uint16_t) (hi < lo ? 0 : hi);
+        if (hi < lo) L.lo[t] = 1;                    // empty window: lo > hi
+    }
+    for (int i = threadIdx.x; i < H; i += PK_T) { L.win[i] = 0; L.win[H + npos + i] = 0; }
+    if (threadIdx.x == 0) { L.win[2 * H + npos] = 0; L.win[2 * H + npos + 1] = 0; }
+    __syncthreads();
+    // LCP | strand<<7 inside each valid window, 0 elsewhere
+    for (int pos = threadIdx.x; pos < npos; pos += PK_T) {
+        const int s = L.sidx[pos];
+        const int i = pos - L.so[s];
+        uint8_t w = 0;
+        if (i >= L.lo[s] && i <= L.hi[s] && i < L.nt[s] + L.nq[s]) {
+            const Rec k = L.key[pos];
+            int l = 0;
+            if (i > L.lo[s]) { const Rec pk = L.key[pos - 1]; l = lcp_bases(pk.w0, pk.w1, k.w0, k.w1); }
+            w = (uint8_t) (l | ((k.w1 & 1ull) << 7));
+        } else {
+            L.flg[pos] = 0;                          // outside the window: never an anchor
+        }
+        L.win[H + pos] = w;
+    }
+    __syncthreads();
+}
+
+// S3/S4. Count (EMIT = false: seeds per segment) or emit the seeds of every pack.
+template <bool EMIT>
+__global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ pstart, int64_t npack,
+                                                    const int64_t *__restrict__ soff, const JSeg *__restrict__ js,
+                                                    const QSeg *__restrict__ segs, const QDev *__restrict__ qd,
+                                                    const Rec *__restrict__ tkeys, const Rec *__restrict__ qkeys,
+                                                    const int64_t *__restrict__ tpos, int jbits, int freq,
+                                                    uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC,
+                                                    const int64_t *__restrict__ ofsN, const int64_t *__restrict__ ofsC,
+                                                    uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
+                                                    unsigned long long *__restrict__ nbuck,
+                                                    unsigned long long *__restrict__ cbuck)
+{
+    __shared__ PackLds L;
+    const int H = freq + 1;
+    for (int64_t p = blockIdx.x; p < npack; p += gridDim.x) {
+        pack_load(L, p, pstart, soff, js, segs, qd, tkeys, qkeys, tpos, ofsN, ofsC, H, EMIT);
+        if (EMIT)
+            for (int c = threadIdx.x; c < PK_BUCK; c += PK_T) { L.bk[0][c] = 0; L.bk[1][c] = 0; }
+        const int npos = L.npos;
+        const int pos0 = threadIdx.x * PK_PER;
+        uint32_t tn = 0, tc = 0;
+        uint32_t mN = 0, mC = 0;             // anchors of this thread: bit k set when slot k emits
+#pragma unroll
+        for (int k = 0; k < PK_PER; k++) {
+            const int pos = pos0 + k;
+            if (pos >= npos || !(L.flg[pos] & 2)) continue;
+            const int gi = H + pos;
+            int plen, a, b;
+            if (!anchor_interval(L.win, gi, freq, plen, a, b)) continue;
+            const bool iq = L.flg[pos] & 1;
+            const uint8_t si = L.win[gi] & 0x80;
+            uint32_t ln = 0, lc = 0;
+            for (int x = a; x < b; x++) {
+                if (x == gi) continue;
+                if (!iq && !(L.flg[x - H] & 1)) continue;
+                if ((L.win[x] & 0x80) == si) ln++; else lc++;
+            }
+            tn += ln;
+            tc += lc;
+            if (ln) mN |= 1u << k;
+            if (lc) mC |= 1u << k;
+            if (!EMIT && (ln | lc)) {
+                const int s = L.sidx[pos];
+                if (ln) atomicAdd(&L.cN[s], ln);
+                if (lc) atomicAdd(&L.cC[s], lc);
+            }
+        }
+        if (!EMIT) {
+            __syncthreads();
+            const int64_t s0 = pstart[p];
+            for (int t = threadIdx.x; t < L.ns; t += PK_T) { cntN[s0 + t] = L.cN[t]; cntC[s0 + t] = L.cC[t]; }
+            __syncthreads();
+            continue;
+        }
+        // emit: this thread's first seed index in the batch-wide N / C order
+        uint32_t totN, totC;
+        const uint32_t rn = block_excl_scan<PK_T, uint32_t>(tn, L.scan, &totN);
+        const uint32_t rc = block_excl_scan<PK_T, uint32_t>(tc, L.scan, &totC);
+        const int64_t s0 = pstart[p];
+        int64_t gN = (L.ns > 0 ? ofsN[s0] : 0) + rn, gC = (L.ns > 0 ? ofsC[s0] : 0) + rc;
+        const int qa = L.qa;
+#pragma unroll 1
+        for (int k = 0; k < PK_PER; k++) {
+            if (!(((mN | mC) >> k) & 1u)) continue;
+            const int pos = pos0 + k;
+            const int gi = H + pos;
+            int plen, a, b;
+            anchor_interval(L.win, gi, freq, plen, a, b);
+            const int s = L.sidx[pos];
+            const int P = L.Pb[s], Cb = L.Cb[s], sb = L.sbytes[s], pc = P + Cb;
+            const bool iq = L.flg[pos] & 1;
+            const uint8_t si = L.win[gi] & 0x80;
+            const uint64_t pi = key_payload(L.key[pos], jbits, P, Cb) & ~(0x80ull << (8 * (pc - 1)));
+            uint32_t ln = 0, lc = 0;
+            for (int x = a; x < b; x++) {
+                if (x == gi) continue;
+                const int j = x - H;
+                if (!iq && !(L.flg[j] & 1)) continue;
+                const bool same = (L.win[x] & 0x80) == si;
+                uint8_t *dst = same ? seedN + L.bN[s] + (gN++) * sb : seedC + L.bC[s] + (gC++) * sb;
+                const uint64_t pq = key_payload(L.key[j], jbits, P, Cb);
                 dst[0] = (uint8_t) plen;
                 for (int t = 0; t < pc; t++) { dst[1 + t] = (uint8_t) (pi >> (8 * t)); dst[1 + pc + t] = (uint8_t) (pq >> (8 * t)); }
+                if (same) ln++; else lc++;
             }
-            const uint64_t icont = (pi >> (8 * d.P)) & ((1ull << (8 * d.C - 1)) - 1);
-            if (ln) atomicAdd(&nbuck[d.buck_off + icont], (unsigned long long) ln);
-            if (lc) atomicAdd(&cbuck[d.buck_off + icont], (unsigned long long) lc);
+            const int icont = (int) ((pi >> (8 * P)) & ((1ull << (8 * Cb - 1)) - 1));
+            if (L.q[s] == qa && icont < PK_BUCK) {
+                if (ln) atomicAdd(&L.bk[0][icont], ln);
+                if (lc) atomicAdd(&L.bk[1][icont], lc);
+            } else {
+                const int64_t bo = qd[L.q[s]].buck_off + icont;
+                if (ln) atomicAdd(&nbuck[bo], (unsigned long long) ln);
+                if (lc) atomicAdd(&cbuck[bo], (unsigned long long) lc);
+            }
         }
-        if (!EMIT) { nN[i] = ln; nC[i] = lc; }
-        cn += ln;
-        cc += lc;
-    }
-    if (!EMIT) { (void) outN; }
-    (void) cn; (void) cc;
-}
-
-// 3a. Seed counts per segment; per-entry counts kept in LDS for the exclusive
-// prefix within the segment.
-__global__ __launch_bounds__(JT) void k_join_count(const QSeg *__restrict__ segs, int64_t nseg,
-                                                   const QDev *__restrict__ qd, const Rec *__restrict__ tkeys,
-                                                   const Rec *__restrict__ qkeys, const uint32_t *__restrict__ qpre,
-                                                   const int64_t *__restrict__ tpos, int jbits, int freq,
-                                                   uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC)
-{
-    __shared__ JoinLds L;
-    __shared__ uint32_t nN[SEGCAP], nC[SEGCAP];
-    __shared__ uint64_t red[JT / 64 + 1];
-    const int H = freq + 1;
-    for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-        const QSeg sg = segs[s];
-        const QDev d = qd[sg.q];
-        const int n = join_load(L, sg, d, tkeys, qkeys, qpre, tpos, freq, H);
-        for (int i = threadIdx.x; i < n; i += JT) { nN[i] = 0; nC[i] = 0; }
         __syncthreads();
-        join_anchors<false>(L, d, jbits, freq, H, nN, nC, nullptr, nullptr, 0, 0, nullptr, nullptr);
-        __syncthreads();
-        uint64_t a = 0, b = 0;
-        for (int i = threadIdx.x; i < n; i += JT) { a += nN[i]; b += nC[i]; }
-        a = block_sum_u64<JT>(a, red);
-        b = block_sum_u64<JT>(b, red);
-        if (threadIdx.x == 0) { cntN[s] = (uint32_t) a; cntC[s] = (uint32_t) b; }
-        __syncthreads();
-    }
-}
-
-// 3b. Seeds: recount, exclusive prefix over the segment's entries, write.
-__global__ __launch_bounds__(JT) void k_join_emit(const QSeg *__restrict__ segs, int64_t nseg,
-                                                  const QDev *__restrict__ qd, const Rec *__restrict__ tkeys,
-                                                  const Rec *__restrict__ qkeys, const uint32_t *__restrict__ qpre,
-                                                  const int64_t *__restrict__ tpos, int jbits, int freq,
-                                                  const int64_t *__restrict__ ofsN, const int64_t *__restrict__ ofsC,
-                                                  uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
-                                                  unsigned long long *__restrict__ nbuck,
-                                                  unsigned long long *__restrict__ cbuck)
-{
-    __shared__ JoinLds L;
-    __shared__ uint32_t nN[SEGCAP], nC[SEGCAP];
-    __shared__ uint32_t sh[JT / 64 + 1];
-    const int H = freq + 1;
-    for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-        const QSeg sg = segs[s];
-        const QDev d = qd[sg.q];
-        (void) join_load(L, sg, d, tkeys, qkeys, qpre, tpos, freq, H);
-        for (int i = threadIdx.x; i < SEGCAP; i += JT) { nN[i] = 0; nC[i] = 0; }
-        __syncthreads();
-        join_anchors<false>(L, d, jbits, freq, H, nN, nC, nullptr, nullptr, 0, 0, nullptr, nullptr);
-        __syncthreads();
-        // exclusive prefix of the per-entry counts in entry order (JT threads x 4 entries)
-        constexpr int PER = SEGCAP / JT;
-        uint32_t vn[PER], vc[PER], tn = 0, tc = 0;
-#pragma unroll
-        for (int k = 0; k < PER; k++) { vn[k] = nN[threadIdx.x * PER + k]; vc[k] = nC[threadIdx.x * PER + k]; tn += vn[k]; tc += vc[k]; }
-        uint32_t totN, totC;
-        uint32_t rn = block_excl_scan<JT, uint32_t>(tn, sh, &totN);
-        uint32_t rc = block_excl_scan<JT, uint32_t>(tc, sh, &totC);
-#pragma unroll
-        for (int k = 0; k < PER; k++) { nN[threadIdx.x * PER + k] = rn; nC[threadIdx.x * PER + k] = rc; rn += vn[k]; rc += vc[k]; }
-        __syncthreads();
-        uint8_t *outN = seedN + d.nseed_off + (ofsN[s] - ofsN[d.seg0]) * (int64_t) d.s;
-        uint8_t *outC = seedC + d.cseed_off + (ofsC[s] - ofsC[d.seg0]) * (int64_t) d.s;
-        join_anchors<true>(L, d, jbits, freq, H, nN, nC, outN, outC, 0, 0, nbuck, cbuck);
+        if (qa >= 0) {
+            const QDev d = qd[qa];
+            const int lim = d.ncomb < PK_BUCK ? d.ncomb : PK_BUCK;
+            for (int c = threadIdx.x; c < lim; c += PK_T) {
+                if (L.bk[0][c]) atomicAdd(&nbuck[d.buck_off + c], (unsigned long long) L.bk[0][c]);
+                if (L.bk[1][c]) atomicAdd(&cbuck[d.buck_off + c], (unsigned long long) L.bk[1][c]);
+            }
+        }
         __syncthreads();
     }
 }

@@ -36,7 +36,8 @@ class Timings(C.Structure):
     _fields_ = [("extract_ms", C.c_double), ("partition_ms", C.c_double), ("tile_sort_ms", C.c_double),
                 ("table_ms", C.c_double), ("merge_ms", C.c_double), ("total_ms", C.c_double),
                 ("partition_levels", C.c_int), ("partition_launches", C.c_int), ("tiles", C.c_int64),
-                ("scatter_ms", C.c_double), ("scatter_bytes", C.c_int64), ("scatter_launches", C.c_int)]
+                ("scatter_ms", C.c_double), ("scatter_bytes", C.c_int64), ("scatter_launches", C.c_int),
+                ("merge_emit_ms", C.c_double)]
 
 
 _u8p = C.POINTER(C.c_uint8)
@@ -60,6 +61,9 @@ def _setup(L):
     L.pga_build_index.argtypes = [C.c_void_p]
     L.pga_index_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(IndexMeta)]
     L.pga_index_export.argtypes = [C.c_void_p, C.c_int, _u8p, _i64p]
+    L.pga_index_verify.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p, _i64p]
+    L.pga_bucket_counts.argtypes = [C.c_void_p, C.c_int, _i64p]
+    L.pga_destination_map.argtypes = [_i64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pga_merge.argtypes = [C.c_void_p]
     L.pga_seeds_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(SeedsMeta)]
     L.pga_seeds_export.argtypes = [C.c_void_p, C.c_int, _u8p, _u8p, _i64p, _i64p]
@@ -162,6 +166,17 @@ class Context:
                     "pga_index_export")
         return ent[: info["nrec"] * info["e"]].reshape(info["nrec"], info["e"]), cp, info
 
+    def verify(self, pair=0):
+        """(bad, first_bad, lcp_bad) of the pair's index, checked on the device."""
+        v = [C.c_int64(), C.c_int64(), C.c_int64()]
+        self._check(self._L.pga_index_verify(self._h, pair, *[C.byref(x) for x in v]), "pga_index_verify")
+        return tuple(x.value for x in v)
+
+    def bucket_counts(self, pair=0):
+        out = np.zeros(1024, np.int64)
+        self._check(self._L.pga_bucket_counts(self._h, pair, out.ctypes.data_as(_i64p)), "pga_bucket_counts")
+        return out
+
     # -- merge
     def merge(self):
         self._check(self._L.pga_merge(self._h), "pga_merge")
@@ -190,6 +205,19 @@ class Context:
 
     def sync(self):
         self._check(self._L.pga_sync(self._h), "pga_sync")
+
+
+def destination_map(global_counts, world_size):
+    """(ksplit[world_size+1], select[1024]): contiguous bucket ranges per shard."""
+    L = _setup(hip_lib())
+    g = np.ascontiguousarray(global_counts, np.int64)
+    if g.shape != (1024,):
+        raise ValueError("global_counts must have 1024 entries")
+    ks = (C.c_int * (world_size + 1))()
+    sel = (C.c_int * 1024)()
+    if L.pga_destination_map(g.ctypes.data_as(_i64p), int(world_size), ks, sel):
+        raise PgaError("pga_destination_map: invalid argument")
+    return np.array(ks[:], np.int64), np.array(sel[:], np.int64)
 
 
 def run_pair(gdb, **opts):

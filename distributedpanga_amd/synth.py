@@ -171,6 +171,13 @@ def concat(a, b):
     return Gdb(bps=bps, boff=boff, clen=clen, n_a=a.ncontig)
 
 
+def take(g, lo, hi):
+    """GDB of contigs [lo, hi) of g (e.g. genome A of a pair: take(g, 0, g.n_a))."""
+    b0, b1 = int(g.boff[lo]), int(g.boff[hi])
+    return Gdb(bps=g.bps[b0:b1].copy(), boff=(g.boff[lo:hi + 1] - b0).astype(np.int64),
+               clen=g.clen[lo:hi].copy(), n_a=hi - lo)
+
+
 def from_codes(contigs, n_a=None):
     """GDB from a list of numeric-base arrays (0..3) or ACGT strings."""
     L = _setup(host_lib())

@@ -83,6 +83,8 @@ typedef struct {
     double scatter_ms;
     int64_t scatter_bytes;
     int    scatter_launches;
+    /* the seed emission kernel of the merge alone (its own event pair) */
+    double merge_emit_ms;
 } pga_timings;
 
 void        pga_default_opts(pga_opts *o);
@@ -113,6 +115,22 @@ int pga_index_info(pga_ctx *ctx, int pair, pga_index_meta *info);
  * (src/kmer_adapter.c:55-80) under the canonical tie order (equal k-mers in
  * (contig, offset, forward-before-reverse) order). */
 int pga_index_export(pga_ctx *ctx, int pair, uint8_t *entries, int64_t *cum);
+
+/* Check pair's index on the device (verify_per_rank_sorted, src/dsort.c:229-246):
+ * *bad = entries whose k-mer is smaller than the previous entry's, *first_bad =
+ * the first such entry (-1 if none), *lcp_bad = entries whose LCP byte differs
+ * from the LCP recomputed from the k-mers (0 for the first entry). */
+int pga_index_verify(pga_ctx *ctx, int pair, int64_t *bad, int64_t *first_bad, int64_t *lcp_bad);
+
+/* Records of pair per 5-base bucket (the first 10 k-mer bits; record_bucket,
+ * src/dsort.h:30-33), counts[1024]: bucket_local_counts, src/dsort.c:108-118,
+ * read off the prefix table (needs opts.build_cum). */
+int pga_bucket_counts(pga_ctx *ctx, int pair, int64_t *counts);
+
+/* Host-only: contiguous bucket ranges per shard from summed bucket counts
+ * (compute_destination_map, src/dsort.c:120-167): ksplit[world_size+1] bucket
+ * boundaries, select[1024] shard of each bucket. */
+int pga_destination_map(const int64_t *global_counts, int world_size, int *ksplit, int *select);
 
 /* Self-merge every indexed GDB of the batch (frequency cut = opts.freq). */
 int pga_merge(pga_ctx *ctx);

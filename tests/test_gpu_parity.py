@@ -131,3 +131,17 @@ def test_context_reuse_smaller_second_call():
             want = oracle.pipeline(g.bps, g.boff[:-1], g.clen)
             assert np.array_equal(ent, want["entries"]) and np.array_equal(cum, want["cum"])
             assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"])
+
+
+def test_verify_and_bucket_counts():
+    """Device sortedness/LCP check is clean on a built index, and the 5-base
+    bucket counts equal the oracle's record_bucket tally (src/dsort.h:30-33)."""
+    g = synth.workload(1, 0.2)
+    want = oracle.pipeline(g.bps, g.boff[:-1], g.clen)
+    rec = want["records"]
+    wb = np.bincount((rec[:, 1].astype(np.int64) << 2) | (rec[:, 2].astype(np.int64) >> 6), minlength=1024)
+    with pga.Context() as ctx:
+        ctx.load_gdb(g)
+        ctx.build_index()
+        assert ctx.verify(0) == (0, -1, 0)
+        assert np.array_equal(ctx.bucket_counts(0), wb)
