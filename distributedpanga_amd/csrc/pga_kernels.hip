@@ -1016,8 +1016,8 @@ __device__ __forceinline__ int merge_block_pair(const PairDev *__restrict__ pair
 
 constexpr int MTH = 256;             // threads of the merge kernels
 constexpr int BUCK_LDS = 1024;       // per-pair contigs counted in LDS by the merge
-constexpr int TBL_N = 4096;          // seeds described in LDS per round: same-strand stream
-constexpr int TBL_C = 1024;          //                                   cross-strand stream
+constexpr int SLOT_N = 768;          // seeds staged in LDS per round: same-strand stream
+constexpr int SLOT_C = 768;          //                                cross-strand stream
 
 // LCP/strand bytes of window [w0, w0+n) of the pair into win (0 outside it).
 __device__ __forceinline__ void load_win(uint8_t *__restrict__ win, const uint8_t *__restrict__ ls,
@@ -1088,68 +1088,56 @@ __device__ __forceinline__ uint64_t payload_at(const uint8_t *__restrict__ ew, i
     return v;
 }
 
+typedef unsigned __int128 u128;
+
 // Write seeds [0, n) of a round straight to global memory, 16 bytes per lane.
-// Seed j is described by tbl[j] = anchor slot | partner slot << 11 | plen << 22
-// (slots index the LDS window); its bytes are [plen][anchor payload, strand bit
-// cleared][partner payload] (src/pga-mpi-merge.c:68-76).  Every lane assembles
-// one aligned 16-byte chunk of the output; chunks the round shares with its
-// neighbours in the stream are written byte by byte.
-__device__ __forceinline__ void emit_round(uint8_t *__restrict__ g, const uint32_t *__restrict__ tbl, int n,
-                                           const uint8_t *__restrict__ ew, int e, int pc, int s)
+// Seed j sits in LDS slot j as one 128-bit little-endian value holding its s
+// bytes: [plen][anchor payload, strand bit cleared][partner payload]
+// (src/pga-mpi-merge.c:68-76).  Every lane assembles one aligned 16-byte chunk
+// of the output from the (at most three) slots it overlaps; chunks the round
+// shares with its neighbours in the stream are written byte by byte.
+__device__ __forceinline__ void emit_round(uint8_t *__restrict__ g, const u128 *__restrict__ slot, int n, int s)
 {
     if (n <= 0) return;
     const uintptr_t base = (uintptr_t) g;
     const uintptr_t a0 = base & ~(uintptr_t) 15;
-    const int64_t nb = (int64_t) n * s;
-    const int nchunks = (int) ((base + nb - a0 + 15) >> 4);
-    const uint8_t clear = 0x7F;                       // strand bit of the anchor payload's last byte
+    const int nb = n * s;
+    const int lead = (int) (base - a0);                 // stream bytes start at this offset of chunk 0
+    const int nchunks = (lead + nb + 15) >> 4;
     for (int c = threadIdx.x; c < nchunks; c += MTH) {
-        const int64_t o0 = (int64_t) (a0 + 16 * (uintptr_t) c) - (int64_t) base;   // may be < 0 in chunk 0
-        uint8_t v[16];
-        const int o = o0 < 0 ? 0 : (int) o0;           // < TBL_N * s
-        int j = o / s, t = o - j * s;
-        uint32_t d = j < n ? tbl[j] : 0;
-#pragma unroll
-        for (int b = 0; b < 16; b++) {
-            const int64_t ob = o0 + b;
-            uint8_t x = 0;
-            if (ob >= 0 && ob < nb) {
-                if (t == 0) {
-                    x = (uint8_t) (d >> 22);
-                } else if (t <= pc) {
-                    x = ew[(int) (d & 2047u) * e + 9 + (t - 1)];
-                    if (t == pc) x &= clear;
-                } else {
-                    x = ew[(int) ((d >> 11) & 2047u) * e + 9 + (t - 1 - pc)];
-                }
-                if (++t == s) { t = 0; j++; d = j < n ? tbl[j] : 0; }
-            }
-            v[b] = x;
+        const int o0 = 16 * c - lead;                   // stream offset of the chunk's first byte
+        const int o = o0 < 0 ? 0 : o0;
+        int j = o / s;
+        const int t = o - j * s;
+        u128 acc = slot[j] >> (8 * t);
+        int filled = s - t;
+        for (j++; filled < 16 && j < n; j++) {
+            acc |= slot[j] << (8 * filled);
+            filled += s;
         }
+        if (o0 < 0) acc <<= 8 * (-o0);                  // chunk 0 starts before the stream
         uint8_t *dst = reinterpret_cast<uint8_t *>(a0 + 16 * (uintptr_t) c);
         if (o0 >= 0 && o0 + 16 <= nb) {
             uint4 w;
-            w.x = (uint32_t) v[0] | (uint32_t) v[1] << 8 | (uint32_t) v[2] << 16 | (uint32_t) v[3] << 24;
-            w.y = (uint32_t) v[4] | (uint32_t) v[5] << 8 | (uint32_t) v[6] << 16 | (uint32_t) v[7] << 24;
-            w.z = (uint32_t) v[8] | (uint32_t) v[9] << 8 | (uint32_t) v[10] << 16 | (uint32_t) v[11] << 24;
-            w.w = (uint32_t) v[12] | (uint32_t) v[13] << 8 | (uint32_t) v[14] << 16 | (uint32_t) v[15] << 24;
+            const uint64_t lo = (uint64_t) acc, hi = (uint64_t) (acc >> 64);
+            w.x = (uint32_t) lo; w.y = (uint32_t) (lo >> 32); w.z = (uint32_t) hi; w.w = (uint32_t) (hi >> 32);
             *reinterpret_cast<uint4 *>(dst) = w;
         } else {
 #pragma unroll
             for (int b = 0; b < 16; b++)
-                if (o0 + b >= 0 && o0 + b < nb) dst[b] = v[b];
+                if (o0 + b >= 0 && o0 + b < nb) dst[b] = (uint8_t) (acc >> (8 * b));
         }
     }
 }
 
 // Seeds of a run of merge tiles (A*MTH anchors each, A consecutive per
 // thread).  Per tile: the LCP/strand bytes and raw GIX entries of the tile plus
-// a FREQ halo on each side go to LDS, each anchor finds its interval, block
-// scans place the anchors' seeds in the tile's slice of the N and C streams
-// (offsets from the count pass).  Each round, every seed is described by one
-// 32-bit LDS word (anchor, partner, plen) and the output bytes are assembled
-// and stored 16 at a time by all lanes alike.  Per-anchor-contig counts
-// accumulate in LDS while the pair stays the same.
+// a FREQ halo on each side go to LDS, then one 64-bit payload per entry; each
+// anchor finds its interval, block scans place the anchors' seeds in the
+// tile's slice of the N and C streams (offsets from the count pass).  Each
+// round, every seed is composed in a 16-byte LDS slot by its anchor's thread,
+// then all lanes alike assemble and store the output 16 bytes at a time.
+// Per-anchor-contig counts accumulate in LDS while the pair stays the same.
 template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
                                                     const uint8_t *__restrict__ ls,
@@ -1164,12 +1152,17 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
     constexpr int TA = MTH * A;
     constexpr int W = TA + 2 * MAX_FREQ + 4;
     constexpr int FRONT = (MAX_FREQ + 2) * 16;   // room for entries left of a clipped window
+    constexpr int ENT_BYTES = FRONT + W * 16 + 16;
+    constexpr int SLOT_BYTES = (SLOT_N + SLOT_C) * 16;
     __shared__ uint8_t win[W];
-    __shared__ __align__(16) uint8_t entp[FRONT + W * 16 + 16];
-    __shared__ uint32_t tblN[TBL_N];
-    __shared__ uint32_t tblC[TBL_C];
+    __shared__ uint64_t pay[W];
+    // raw entry bytes until the payloads are extracted, then the seed slots
+    __shared__ __align__(16) uint8_t shared_buf[ENT_BYTES > SLOT_BYTES ? ENT_BYTES : SLOT_BYTES];
     __shared__ uint32_t lb[2 * BUCK_LDS];
     __shared__ uint32_t sh[MTH / 64 + 1];
+    uint8_t *entp = shared_buf;
+    u128 *slotN = reinterpret_cast<u128 *>(shared_buf);
+    u128 *slotC = slotN + SLOT_N;
     for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MTH) lb[c] = 0;
     const int64_t t0 = (int64_t) blockIdx.x * per_block;
     const int64_t t1 = t0 + per_block < nmb ? t0 + per_block : nmb;
@@ -1203,9 +1196,14 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
             for (int k = threadIdx.x; k < nwords; k += MTH) dst[k] = src[k];
         }
         __syncthreads();
-        // window entry k (global lo-H+k) starts at ew + k*e; entries outside
-        // [wlo, whi) are never read (their LCP byte is 0, a run boundary)
-        const uint8_t *ew = entp + FRONT + (int) (b0 - a0) - (int) (wlo - (lo - H)) * e;
+        {
+            // window entry k (global lo-H+k) starts at ew + k*e; only [wlo, whi) is present
+            const uint8_t *ew = entp + FRONT + (int) (b0 - a0) - (int) (wlo - (lo - H)) * e;
+            const int k0 = (int) (wlo - (lo - H)), k1 = (int) (whi - (lo - H));
+            for (int k = threadIdx.x; k < TA + 2 * H; k += MTH)
+                pay[k] = (k >= k0 && k < k1) ? payload_at(ew, k, e, pc) : 0;
+        }
+        __syncthreads();                             // entry bytes are dead from here: slots reuse them
         int pl[A], aa[A], bb[A];
         uint32_t nN[A], nC[A];
         uint32_t sN = 0, sC = 0;
@@ -1229,33 +1227,35 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         uint32_t totN, totC;
         const uint32_t oN = block_excl_scan<MTH, uint32_t>(sN, sh, &totN);
         const uint32_t oC = block_excl_scan<MTH, uint32_t>(sC, sh, &totC);
-        const int64_t rounds = max(((int64_t) totN + TBL_N - 1) / TBL_N, ((int64_t) totC + TBL_C - 1) / TBL_C);
+        const int64_t rounds = max(((int64_t) totN + SLOT_N - 1) / SLOT_N, ((int64_t) totC + SLOT_C - 1) / SLOT_C);
         uint8_t *gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
         uint8_t *gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
+        const uint64_t clear = ~(0x80ull << (8 * (pc - 1)));
         for (int64_t r = 0; r < rounds; r++) {
-            const int64_t n0 = r * TBL_N, n1 = min(n0 + TBL_N, (int64_t) totN);
-            const int64_t c0 = r * TBL_C, c1 = min(c0 + TBL_C, (int64_t) totC);
+            const int64_t n0 = r * SLOT_N, n1 = min(n0 + SLOT_N, (int64_t) totN);
+            const int64_t c0 = r * SLOT_C, c1 = min(c0 + SLOT_C, (int64_t) totC);
             int64_t kn = oN, kc = oC;
 #pragma unroll
             for (int k = 0; k < A; k++) {
                 if (nN[k] + nC[k] == 0) continue;
                 const int gi = H + threadIdx.x * A + k;
                 const uint8_t si = win[gi] & 0x80;
-                const uint32_t head = (uint32_t) gi | (uint32_t) pl[k] << 22;
+                const u128 head = (u128) (uint32_t) pl[k] | ((u128) (pay[gi] & clear) << 8);
+                const int sh2 = 8 + 8 * pc;
                 for (int q = aa[k]; q < bb[k]; q++) {
                     if (q == gi) continue;
                     if ((win[q] & 0x80) == si) {
-                        if (kn >= n0 && kn < n1) tblN[kn - n0] = head | (uint32_t) q << 11;
+                        if (kn >= n0 && kn < n1) slotN[kn - n0] = head | ((u128) pay[q] << sh2);
                         kn++;
                     } else {
-                        if (kc >= c0 && kc < c1) tblC[kc - c0] = head | (uint32_t) q << 11;
+                        if (kc >= c0 && kc < c1) slotC[kc - c0] = head | ((u128) pay[q] << sh2);
                         kc++;
                     }
                 }
             }
             __syncthreads();
-            emit_round(gN + n0 * s, tblN, (int) (n1 - n0), ew, e, pc, s);
-            emit_round(gC + c0 * s, tblC, (int) (c1 - c0), ew, e, pc, s);
+            emit_round(gN + n0 * s, slotN, (int) (n1 - n0), s);
+            emit_round(gC + c0 * s, slotC, (int) (c1 - c0), s);
             __syncthreads();
         }
         // per-anchor-contig counts (src/fastga_pipeline.c:403)
@@ -1263,7 +1263,7 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         for (int k = 0; k < A; k++) {
             if (nN[k] + nC[k] == 0) continue;
             const int gi = H + threadIdx.x * A + k;
-            const uint64_t icont = (payload_at(ew, gi, e, pc) >> (8 * pd.P)) & ((1ull << (8 * pd.C - 1)) - 1);
+            const uint64_t icont = (pay[gi] >> (8 * pd.P)) & ((1ull << (8 * pd.C - 1)) - 1);
             if (lds_buck) {
                 if (nN[k]) atomicAdd(&lb[icont], nN[k]);
                 if (nC[k]) atomicAdd(&lb[BUCK_LDS + icont], nC[k]);

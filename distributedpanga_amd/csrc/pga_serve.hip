@@ -148,7 +148,7 @@ __device__ __forceinline__ uint64_t key_payload(const Rec &k, int jbits, int P, 
 constexpr int PK_CAP = 2048;           // combined entries of one pack
 constexpr int PK_MINC = 8;             // slots per segment at least
 constexpr int PK_SEG = PK_CAP / PK_MINC;
-constexpr int PK_T = 256;              // threads of the pack kernels
+constexpr int PK_T = 512;              // threads of the pack kernels
 constexpr int PK_PER = PK_CAP / PK_T;  // anchor slots per thread
 constexpr int PK_BUCK = 1024;          // anchor contigs counted in LDS (the pack's first query)
 
