@@ -578,6 +578,8 @@ __device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, 
     return 40;
 }
 
+constexpr int FB_BIN = 128;          // largest bin the binned tile sort ranks directly
+
 #define PGA_CE(i, j)                                                             \
     do {                                                                          \
         bool sw_ = (a0[j] < a0[i]) || (a0[j] == a0[i] && a1[j] < a1[i]);          \
@@ -669,7 +671,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     uint32_t cnt[BPT], sum = 0, mx = 0;
 #pragma unroll
     for (int q = 0; q < BPT; q++) { cnt[q] = bh[tid * BPT + q]; sum += cnt[q]; mx = cnt[q] > mx ? cnt[q] : mx; }
-    if (mx > 32) fallback = 1;                      // a crowded bin (repeats): merge sort instead
+    if (mx > FB_BIN) fallback = 1;                  // a crowded bin (repeats): merge sort instead
     uint32_t tot;
     uint32_t run = block_excl_scan<T, uint32_t>(sum, scan_sh, &tot);
 #pragma unroll
@@ -677,26 +679,30 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     __syncthreads();
     const bool binned = !fallback;
     if (binned) {
+        // scatter into bins, then every record counts the smaller keys of its
+        // bin (keys are unique) and moves to bin start + that rank
+        uint32_t bn[E];
 #pragma unroll
         for (int k = 0; k < E; k++)
-            if (k * T + tid < n) sk[bh[bin_of(r[k].w0, r[k].w1)] + rk[k]] = r[k];
+            if (k * T + tid < n) { bn[k] = bin_of(r[k].w0, r[k].w1); sk[bh[bn[k]] + rk[k]] = r[k]; }
+        __syncthreads();
+        uint32_t pos[E];
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            if (k * T + tid >= n) continue;
+            const uint32_t st = bh[bn[k]];
+            const uint32_t en = bn[k] + 1 < (uint32_t) NB ? bh[bn[k] + 1] : (uint32_t) n;
+            uint32_t rank = 0;
+            for (uint32_t i = st; i < en; i++) {
+                const Rec y = sk[i];
+                rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && y.w1 < r[k].w1)) ? 1u : 0u;
+            }
+            pos[k] = st + rank;
+        }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < BPT; q++) {
-            const int b0 = (int) bh[tid * BPT + q];
-            const int b1 = b0 + (int) cnt[q];
-            for (int i = b0 + 1; i < b1; i++) {
-                Rec x = sk[i];
-                int j = i - 1;
-                while (j >= b0) {
-                    Rec y = sk[j];
-                    if (key_le(y.w0, y.w1, x.w0, x.w1)) break;
-                    sk[j + 1] = y;
-                    j--;
-                }
-                sk[j + 1] = x;
-            }
-        }
+        for (int k = 0; k < E; k++)
+            if (k * T + tid < n) sk[pos[k]] = r[k];
     } else {
 #pragma unroll
         for (int k = 0; k < E; k++) {
