@@ -98,6 +98,7 @@ struct pga_ctx {
         mblk, pofs, seedN, seedC, nbuck, cbuck, bmax;
 
     DBuf vout;                         // results of the small check kernels
+    DBuf minfo;                        // per-anchor interval + seed counts (count pass -> emit pass)
     // sorted device keys kept after the build (serving)
     bool keep_keys = false;
     DBuf skeys;
@@ -371,7 +372,7 @@ void pga_destroy(pga_ctx *c)
     DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
-                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo,
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
@@ -838,12 +839,19 @@ int pga_merge(pga_ctx *c)
     HIPCHK(c, hipMemcpyAsync(c->mblk.p, mblk.data(), mblk.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
     std::vector<int64_t> po(2 * (npairs + 1), 0);
+    // the count pass hands each anchor's interval and seed counts to the emit
+    // pass when they fit the 32-bit record (intervals of fewer than 32 entries)
+    uint32_t *minfo = nullptr;
+    if (c->o.freq <= 31 && c->R > 0) {
+        TRY(ensure(c, c->minfo, (size_t) c->R * sizeof(uint32_t)));
+        minfo = P<uint32_t>(c->minfo);
+    }
     if (nmb > 0) {
 #define PGA_COUNT(A_)                                                                                   \
     hipLaunchKernelGGL(k_merge_count<A_>, dim3(grid_for(c, (const void *) k_merge_count<A_>, MTH, 0, nmb)), \
                        dim3(MTH), 0, c->st,                                                             \
                        (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
-                       c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC), nmb)
+                       c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC), minfo, nmb)
         switch (MT) {
         case 512: PGA_COUNT(2); break;
         case 1024: PGA_COUNT(4); break;
@@ -903,7 +911,7 @@ int pga_merge(pga_ctx *c)
                        (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
                        (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
                        P<uint8_t>(c->seedN), P<uint8_t>(c->seedC), P<unsigned long long>(c->nbuck),     \
-                       P<unsigned long long>(c->cbuck), nmb, eper)
+                       P<unsigned long long>(c->cbuck), (const uint32_t *) minfo, nmb, eper)
         switch (MT) {
         case 512: PGA_EMIT(2); break;
         case 1024: PGA_EMIT(4); break;

@@ -1040,7 +1040,7 @@ template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,
                                                      const uint8_t *__restrict__ ls, int freq,
                                                      uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC,
-                                                     int64_t nlog)
+                                                     uint32_t *__restrict__ minfo, int64_t nlog)
 {
     constexpr int TA = MTH * A;
     __shared__ uint8_t win[TA + 2 * MAX_FREQ + 4];
@@ -1057,14 +1057,21 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
 #pragma unroll
     for (int k = 0; k < A; k++) {
         const int gi = H + threadIdx.x * A + k;
+        const int64_t g = lo + threadIdx.x * A + k;
         int plen, a, b;
-        if (lo + threadIdx.x * A + k < end && anchor_interval(win, gi, freq, plen, a, b)) {
+        uint32_t info = 0;
+        if (g < end && anchor_interval(win, gi, freq, plen, a, b)) {
             const uint8_t si = win[gi] & 0x80;
+            uint32_t xn = 0, xc = 0;
             for (int q = a; q < b; q++) {
                 if (q == gi) continue;
-                if ((win[q] & 0x80) == si) nN++; else nC++;
+                if ((win[q] & 0x80) == si) xn++; else xc++;
             }
+            nN += xn;
+            nC += xc;
+            info = (uint32_t) plen | (uint32_t) (gi - a) << 6 | (uint32_t) (b - gi) << 11 | xn << 16 | xc << 21;
         }
+        if (minfo && g < end) minfo[g] = info;
     }
     uint64_t sn = block_sum_u64<MTH>(nN, red);
     uint64_t sc = block_sum_u64<MTH>(nC, red);
@@ -1153,6 +1160,7 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
                                                     uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
                                                     unsigned long long *__restrict__ nbuck,
                                                     unsigned long long *__restrict__ cbuck,
+                                                    const uint32_t *__restrict__ minfo,
                                                     int64_t nmb, int64_t per_block)
 {
     constexpr int TA = MTH * A;
@@ -1216,9 +1224,19 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
 #pragma unroll
         for (int k = 0; k < A; k++) {
             const int gi = H + threadIdx.x * A + k;
+            const int64_t g = lo + threadIdx.x * A + k;
             nN[k] = nC[k] = 0;
             pl[k] = 0; aa[k] = bb[k] = gi;
-            if (lo + threadIdx.x * A + k < end && anchor_interval(win, gi, freq, pl[k], aa[k], bb[k])) {
+            if (minfo) {                                 // intervals found by the count pass
+                const uint32_t v = g < end ? minfo[g] : 0u;
+                if (v) {
+                    pl[k] = (int) (v & 63u);
+                    aa[k] = gi - (int) ((v >> 6) & 31u);
+                    bb[k] = gi + (int) ((v >> 11) & 31u);
+                    nN[k] = (v >> 16) & 31u;
+                    nC[k] = (v >> 21) & 31u;
+                }
+            } else if (g < end && anchor_interval(win, gi, freq, pl[k], aa[k], bb[k])) {
                 const uint8_t si = win[gi] & 0x80;
                 for (int q = aa[k]; q < bb[k]; q++) {
                     if (q == gi) continue;
@@ -1343,11 +1361,11 @@ __global__ void k_bucket_counts(const int64_t *__restrict__ cum, long long *__re
 
 #define PGA_MERGE_INST(A_)                                                                              \
     template __global__ void k_merge_count<A_>(const PairDev *, int, const uint8_t *, int, uint32_t *, \
-                                               uint32_t *, int64_t);                                    \
+                                               uint32_t *, uint32_t *, int64_t);                        \
     template __global__ void k_merge_emit<A_>(const PairDev *, int, const uint8_t *, const uint8_t *,   \
                                               int, const int64_t *, const int64_t *, uint8_t *,         \
                                               uint8_t *, unsigned long long *, unsigned long long *,    \
-                                              int64_t, int64_t);
+                                              const uint32_t *, int64_t, int64_t);
 PGA_MERGE_INST(1)
 PGA_MERGE_INST(2)
 PGA_MERGE_INST(4)
