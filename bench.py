@@ -152,6 +152,39 @@ def serving_leg(args):
                 pcie_inclusive=True)
 
 
+def human_leg(args):
+    """The human-scale pair (3.1 Gbp A + B, 24 contigs of 50-250 Mbp, 45%
+    repeats, satellites; B with 2% SNP, 0.3% indel, 200 rearrangements) at
+    --human-scale: index build + merge with the input resident in HBM, the
+    device sortedness/LCP check, and the memory the context holds."""
+    t = time.perf_counter()
+    log(f"[bench] generating the human-scale pair at scale {args.human_scale}")
+    g = synth.workload(4, args.human_scale)
+    log(f"[bench] human-scale pair at scale {args.human_scale}: {g.ncontig} contigs, {g.seqtot} bases "
+        f"generated ({time.perf_counter() - t:.1f}s)")
+    ctx = pga.Context(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+    ctx.load_gdb(g)
+    steps = max(1, args.human_steps)
+    dt, per, scat_bytes, _, _ = time_steps(ctx, steps, 1)
+    info = ctx.index_info(0)
+    sinfo = ctx.seeds_info(0)
+    nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
+    bad, first_bad, lcp_bad = ctx.verify(0)
+    ms = dt / steps * 1e3
+    scat_gbs = scat_bytes / (per["scatter_ms"] * 1e-3) / 1e9 if per["scatter_ms"] > 0 else 0.0
+    out = dict(workload=f"human-scale pair at scale {args.human_scale}", bases=g.seqtot, contigs=g.ncontig,
+               records=info["nrec"], seeds=int(nseed), ms_per_step=round(ms, 3),
+               gbases_s=round(g.seqtot / (ms * 1e-3) / 1e9, 4), steps=steps,
+               stages_ms={k: round(v, 3) for k, v in per.items()},
+               radix_scatter_gbs=round(scat_gbs, 1), radix_scatter_frac=round(scat_gbs / HBM_PEAK_GBS, 4),
+               verify=dict(kmer_order_violations=bad, first_bad=first_bad, lcp_mismatches=lcp_bad),
+               P=info["P"], C=info["C"], maxpre=info["maxpre"])
+    ctx.close()
+    log(f"[bench] human-scale: {ms:.1f} ms/step, {out['gbases_s']} Gbases/s, {info['nrec']} records, "
+        f"{nseed} seeds, verify {out['verify']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +198,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-bases", type=int, default=50_000_000)
     ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--human-scale", type=float, default=0.0,
+                    help="also run the human-scale pair at this scale (1.0 = 3.1 Gbp A + B); 0 skips it")
+    ap.add_argument("--human-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-scale", type=float, default=0.1, help="scale of the resident target (1.0 = 3.1 Gbp)")
     ap.add_argument("--serve-pool", type=int, default=48)
@@ -293,6 +329,9 @@ def main():
 
     if not args.no_serve:
         out["serving"] = serving_leg(args)
+
+    if args.human_scale > 0:
+        out["human_scale"] = human_leg(args)
 
     if not args.no_cpu:
         t = time.perf_counter()

@@ -99,6 +99,10 @@ struct pga_ctx {
 
     DBuf vout;                         // results of the small check kernels
     DBuf minfo;                        // per-anchor interval + seed counts (count pass -> emit pass)
+    // where the last merge put its outputs: inside the record buffers (dead once
+    // the index is built) when they fit there, else in minfo / seedN / seedC
+    uint32_t *minfo_p = nullptr;
+    uint8_t *seedN_p = nullptr, *seedC_p = nullptr;
     // sorted device keys kept after the build (serving)
     bool keep_keys = false;
     DBuf skeys;
@@ -154,7 +158,8 @@ int ensure(pga_ctx *c, DBuf &b, size_t bytes)
         b.cap = 0;
         if (e != hipSuccess) return fail(c, PGA_EHIP, "hipFree: %s", hipGetErrorString(e));
     }
-    size_t want = bytes + bytes / 8;                 // headroom so small growth does not reallocate
+    // headroom so small growth does not reallocate (bounded: buffers can be tens of GB)
+    size_t want = bytes + std::min(bytes / 8, (size_t) 1 << 30);
     if (hipMalloc(&b.p, want) != hipSuccess) {
         (void) hipGetLastError();
         if (hipMalloc(&b.p, bytes) != hipSuccess) {
@@ -839,13 +844,31 @@ int pga_merge(pga_ctx *c)
     HIPCHK(c, hipMemcpyAsync(c->mblk.p, mblk.data(), mblk.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice, c->st));
     std::vector<int64_t> po(2 * (npairs + 1), 0);
+    // Merge outputs go into the two record buffers when they fit: after the
+    // build the records live on only as GIX entries (and, for serving, as the
+    // kept keys), so their memory is free.  That keeps the human-scale pair
+    // (2 x 77 GB of records) within one GPU.
+    struct Arena {
+        uint8_t *p[2];
+        size_t cap[2], used[2];
+        void *take(size_t bytes) {
+            bytes = (bytes + 255) & ~(size_t) 255;
+            for (int k = 0; k < 2; k++)
+                if (p[k] && cap[k] - used[k] >= bytes) { void *r = p[k] + used[k]; used[k] += bytes; return r; }
+            return nullptr;
+        }
+    } arena{{P<uint8_t>(c->rec[0]), P<uint8_t>(c->rec[1])}, {c->rec[0].cap, c->rec[1].cap}, {0, 0}};
     // the count pass hands each anchor's interval and seed counts to the emit
     // pass when they fit the 32-bit record (intervals of fewer than 32 entries)
     uint32_t *minfo = nullptr;
     if (c->o.freq <= 31 && c->R > 0) {
-        TRY(ensure(c, c->minfo, (size_t) c->R * sizeof(uint32_t)));
-        minfo = P<uint32_t>(c->minfo);
+        minfo = static_cast<uint32_t *>(arena.take((size_t) c->R * sizeof(uint32_t)));
+        if (!minfo) {
+            TRY(ensure(c, c->minfo, (size_t) c->R * sizeof(uint32_t)));
+            minfo = P<uint32_t>(c->minfo);
+        }
     }
+    c->minfo_p = minfo;
     if (nmb > 0) {
 #define PGA_COUNT(A_)                                                                                   \
     hipLaunchKernelGGL(k_merge_count<A_>, dim3(grid_for(c, (const void *) k_merge_count<A_>, MTH, 0, nmb)), \
@@ -883,8 +906,16 @@ int pga_merge(pga_ctx *c)
         nbytes += c->nseeds[p] * d.s;
         cbytes += c->cseeds[p] * d.s;
     }
-    TRY(ensure(c, c->seedN, (size_t) nbytes + 16));
-    TRY(ensure(c, c->seedC, (size_t) cbytes + 16));
+    c->seedN_p = static_cast<uint8_t *>(arena.take((size_t) nbytes + 16));
+    if (!c->seedN_p) {
+        TRY(ensure(c, c->seedN, (size_t) nbytes + 16));
+        c->seedN_p = P<uint8_t>(c->seedN);
+    }
+    c->seedC_p = static_cast<uint8_t *>(arena.take((size_t) cbytes + 16));
+    if (!c->seedC_p) {
+        TRY(ensure(c, c->seedC, (size_t) cbytes + 16));
+        c->seedC_p = P<uint8_t>(c->seedC);
+    }
     const int64_t nb_elems = std::max(c->nctg, 1);
     TRY(ensure(c, c->nbuck, (size_t) nb_elems * sizeof(uint64_t)));
     TRY(ensure(c, c->cbuck, (size_t) nb_elems * sizeof(uint64_t)));
@@ -910,7 +941,7 @@ int pga_merge(pga_ctx *c)
                        (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
                        (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
                        (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
-                       P<uint8_t>(c->seedN), P<uint8_t>(c->seedC), P<unsigned long long>(c->nbuck),     \
+                       c->seedN_p, c->seedC_p, P<unsigned long long>(c->nbuck),                         \
                        P<unsigned long long>(c->cbuck), (const uint32_t *) minfo, nmb, eper)
         switch (MT) {
         case 512: PGA_EMIT(2); break;
@@ -945,10 +976,10 @@ int pga_seeds_export(pga_ctx *c, int pair, uint8_t *nseeds, uint8_t *cseeds, int
     if (!c || !c->merged || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no seeds for pair %d", pair) : PGA_EINVAL;
     const PairDev &d = c->pairs[pair];
     if (nseeds && c->nseeds[pair] > 0)
-        HIPCHK(c, hipMemcpyAsync(nseeds, P<uint8_t>(c->seedN) + d.nseed_off, (size_t) (c->nseeds[pair] * d.s),
+        HIPCHK(c, hipMemcpyAsync(nseeds, c->seedN_p + d.nseed_off, (size_t) (c->nseeds[pair] * d.s),
                                  hipMemcpyDeviceToHost, c->st));
     if (cseeds && c->cseeds[pair] > 0)
-        HIPCHK(c, hipMemcpyAsync(cseeds, P<uint8_t>(c->seedC) + d.cseed_off, (size_t) (c->cseeds[pair] * d.s),
+        HIPCHK(c, hipMemcpyAsync(cseeds, c->seedC_p + d.cseed_off, (size_t) (c->cseeds[pair] * d.s),
                                  hipMemcpyDeviceToHost, c->st));
     const int64_t row = 0;
     if (nbuck && d.ncontig > 0)

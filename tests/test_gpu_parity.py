@@ -145,3 +145,10 @@ def test_verify_and_bucket_counts():
         ctx.build_index()
         assert ctx.verify(0) == (0, -1, 0)
         assert np.array_equal(ctx.bucket_counts(0), wb)
+
+
+def test_human_scale_workload_reduced():
+    """The human-scale pair's shape (long contigs, 45% repeats from 500
+    families, satellites) at 1/100 of its length."""
+    got, _ = _compare(synth.workload(4, 0.01))
+    assert got["info"]["nrec"] > 40_000_000
