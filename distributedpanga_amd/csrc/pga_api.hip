@@ -239,13 +239,14 @@ int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *src, const Se
 int launch_seg_scatter(pga_ctx *c, int per, unsigned nlog, const Rec *src, Rec *dst, const Seg *segs,
                        int nseg, const int64_t *chk0, unsigned long long *cursor)
 {
-#define PGA_SCAT(PER)                                                                                  \
-    hipLaunchKernelGGL(k_seg_scatter<PER>, dim3(grid_for(c, (const void *) k_seg_scatter<PER>, 256, 0, nlog)), \
-                       dim3(256), 0, c->st, src, dst, segs, nseg, chk0, cursor, (int64_t) nlog)
+    // chunks of 2048 / 4096 records: 256 threads; 8192: 512 threads of 16 records
+#define PGA_SCAT(TT, PER)                                                                              \
+    hipLaunchKernelGGL((k_seg_scatter<TT, PER>), dim3(grid_for(c, (const void *) k_seg_scatter<TT, PER>, TT, 0, nlog)), \
+                       dim3(TT), 0, c->st, src, dst, segs, nseg, chk0, cursor, (int64_t) nlog)
     switch (per) {
-    case 8:  PGA_SCAT(8); break;
-    case 32: PGA_SCAT(32); break;
-    default: PGA_SCAT(16); break;
+    case 8:  PGA_SCAT(256, 8); break;
+    case 32: PGA_SCAT(512, 16); break;
+    default: PGA_SCAT(256, 16); break;
     }
 #undef PGA_SCAT
     LAUNCHCHK(c);
@@ -688,7 +689,17 @@ int pga_build_index(pga_ctx *c)
                 LAUNCHCHK(c);
             }
             const int64_t nmp = (R + MAXPRE_SPAN - 1) / MAXPRE_SPAN;
-            TRY(ensure(c, c->bmax, (size_t) nmp * sizeof(uint64_t)));
+            TRY(ensure(c, c->bmax, (size_t) std::max<int64_t>(nmp, 1024) * sizeof(uint64_t)));
+            if (npairs == 1 && R > NPRE / 2) {
+                // one large pair: the largest group from the finished table
+                hipLaunchKernelGGL(k_maxpre_cum, dim3(1024), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->cum),
+                                   P<unsigned long long>(c->bmax));
+                LAUNCHCHK(c);
+                hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c->st,
+                                   (const unsigned long long *) P<unsigned long long>(c->bmax), 1024,
+                                   P<unsigned long long>(c->maxpre));
+                LAUNCHCHK(c);
+            } else {
             hipLaunchKernelGGL(k_maxpre, dim3((unsigned) nmp), dim3(256), 0, c->st,
                                (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
                                (const int64_t *) P<int64_t>(c->cum), P<unsigned long long>(c->bmax),
@@ -698,6 +709,7 @@ int pga_build_index(pga_ctx *c)
                                (const PairDev *) P<PairDev>(c->dpairs), (const unsigned long long *) P<unsigned long long>(c->bmax),
                                P<unsigned long long>(c->maxpre));
             LAUNCHCHK(c);
+            }
         }
     }
     TRY(record(c, 4));

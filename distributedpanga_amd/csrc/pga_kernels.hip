@@ -498,20 +498,22 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
     }
 }
 
-template <int PER>
-__global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src, Rec *__restrict__ dst,
-                                                    const Seg *__restrict__ segs, int nseg,
-                                                    const int64_t *__restrict__ chk0,
-                                                    unsigned long long *__restrict__ cursor, int64_t nlog)
+template <int T, int PER>
+__global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, Rec *__restrict__ dst,
+                                                   const Seg *__restrict__ segs, int nseg,
+                                                   const int64_t *__restrict__ chk0,
+                                                   unsigned long long *__restrict__ cursor, int64_t nlog)
 {
-    constexpr int CH = PT * PER;
+    static_assert(T >= 256, "one thread per digit at least");
+    constexpr int CH = T * PER;
     __shared__ Rec stage[CH];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t loff[256];
     __shared__ int64_t gbase[256];
-    __shared__ uint32_t sh[PT / 64 + 1];
+    __shared__ uint32_t sh[T / 64 + 1];
+    const int tid = threadIdx.x;
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
-    hist[threadIdx.x] = 0;
+    if (tid < 256) hist[tid] = 0;
     __syncthreads();
     int s = seg_of_chunk(chk0, nseg, LB);
     const Seg sg = segs[s];
@@ -522,29 +524,30 @@ __global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src,
     uint32_t rk[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        int i = k * PT + threadIdx.x;
+        int i = k * T + tid;
         if (i < n) {
             r[k] = p[i];
             rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, sg.level)], 1u);
         }
     }
     __syncthreads();
-    uint32_t h = hist[threadIdx.x];
+    const uint32_t h = tid < 256 ? hist[tid] : 0u;
     uint32_t tot;
-    uint32_t lo = block_excl_scan<PT, uint32_t>(h, sh, &tot);
-    loff[threadIdx.x] = lo;
-    if (h) gbase[threadIdx.x] = (int64_t) atomicAdd(&cursor[(size_t) s * 256 + threadIdx.x],
-                                                    (unsigned long long) h);
+    const uint32_t lo = block_excl_scan<T, uint32_t>(h, sh, &tot);
+    if (tid < 256) {
+        loff[tid] = lo;
+        if (h) gbase[tid] = (int64_t) atomicAdd(&cursor[(size_t) s * 256 + tid], (unsigned long long) h);
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        int i = k * PT + threadIdx.x;
+        int i = k * T + tid;
         if (i < n) stage[loff[digit_at(r[k].w0, r[k].w1, sg.level)] + rk[k]] = r[k];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        int i = k * PT + threadIdx.x;
+        int i = k * T + tid;
         if (i < n) {
             Rec x = stage[i];
             uint32_t d = digit_at(x.w0, x.w1, sg.level);
@@ -558,9 +561,9 @@ __global__ __launch_bounds__(PT) void k_seg_scatter(const Rec *__restrict__ src,
 template __global__ void k_seg_hist<16>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
 template __global__ void k_seg_hist<8>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
 template __global__ void k_seg_hist<32>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<8>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<32>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<256, 8>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<256, 16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<512, 16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
 
 // ---------------------------------------------------------------- tile sort
 
@@ -983,6 +986,53 @@ __global__ __launch_bounds__(256) void k_maxpre_reduce(const PairDev *__restrict
         unsigned long long x = red[0];
         for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
         if (x) atomicMax(&maxpre[blockIdx.x], x);
+    }
+}
+
+// Largest prefix group of one pair from its finished cumulative table:
+// max over p of cum[p] - cum[p-1].  Reads the 128 MiB table instead of four
+// bytes per record; used when the pair has more records than the table has
+// half-entries.  Per-block maxima go to bmax, k_max_reduce folds them.
+__global__ __launch_bounds__(256) void k_maxpre_cum(const int64_t *__restrict__ cum, unsigned long long *__restrict__ bmax)
+{
+    __shared__ unsigned long long red[4];
+    unsigned long long m = 0;
+    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < (1 << 24);
+         q += (int64_t) gridDim.x * blockDim.x) {
+        const unsigned long long v = (unsigned long long) (cum[q] - (q ? cum[q - 1] : 0));
+        m = v > m ? v : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long x = __shfl_xor(m, o, 64);
+        m = x > m ? x : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = red[0];
+        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
+        bmax[blockIdx.x] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_max_reduce(const unsigned long long *__restrict__ bmax, int n,
+                                                    unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long red[4];
+    unsigned long long m = 0;
+    for (int i = threadIdx.x; i < n; i += 256) m = bmax[i] > m ? bmax[i] : m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long x = __shfl_xor(m, o, 64);
+        m = x > m ? x : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = red[0];
+        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
+        *out = x;
     }
 }
 
