@@ -95,7 +95,7 @@ struct pga_ctx {
     // device buffers
     DBuf bps, ctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
-        mblk, pofs, seedN, seedC, nbuck, cbuck, bmax;
+        mblk, pofs, seedN, seedC, nbuck, cbuck, bmax, sel;
 
     DBuf vout;                         // results of the small check kernels
     DBuf minfo;                        // per-anchor interval + seed counts (count pass -> emit pass)
@@ -378,7 +378,7 @@ void pga_destroy(pga_ctx *c)
     DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
-                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel,
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
@@ -514,10 +514,11 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->blkcnt, (size_t) std::max<int64_t>(nblk, 1) * sizeof(uint32_t)));
     TRY(ensure(c, c->blkoff, (size_t) (nblk + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->sel, (size_t) std::max<int64_t>(nblk, 1) * 256));    // 2 bits per position
     if (nblk > 0) {
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, nblk)), dim3(256),
                            0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, P<uint32_t>(c->blkcnt),
-                           nblk);
+                           P<uint8_t>(c->sel), nblk);
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
     } else {
@@ -584,7 +585,7 @@ int pga_build_index(pga_ctx *c)
         const int64_t xper = (nblk + xg - 1) / xg;
         hipLaunchKernelGGL(k_extract_emit, dim3(xg), dim3(256),
                            0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, c->jbits,
-                           (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
+                           (const uint8_t *) P<uint8_t>(c->sel), (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
                            P<unsigned long long>(c->hist1), nblk, xper);
         LAUNCHCHK(c);
     }

@@ -179,6 +179,27 @@ __device__ __forceinline__ int scan_position(const uint64_t *__restrict__ bps64,
     return (int) out.fwd + (int) out.rev;
 }
 
+// Keys of the records at an offset already known to be selected (e.fwd / e.rev set).
+__device__ __forceinline__ void record_keys(const uint64_t *__restrict__ bps64, const ContigDev &c, int64_t j,
+                                            int jbits, XEmit &out)
+{
+    const uint64_t pay = ((uint64_t) c.local << (jbits + 1)) | ((uint64_t) j << 1);
+    if (out.fwd) {
+        uint64_t W0;
+        uint32_t W1;
+        window80(bps64, c.bitoff + 2 * j, W0, W1);
+        out.f0 = field_rev64(W0);
+        out.f1 = ((uint64_t) field_rev16(W1) << 48) | pay;
+    }
+    if (out.rev) {
+        uint64_t V0;
+        uint32_t V1;
+        window80(bps64, c.bitoff + 2 * (j - 28), V0, V1);
+        out.r0 = ~(((uint64_t) V1 << 48) | (V0 >> 16));
+        out.r1 = ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull;
+    }
+}
+
 // Contig of the block's first position (one binary search per block); every
 // thread then steps forward from it.  Contigs hold >= 1 candidate position or
 // none, so a thread of XU positions crosses few contig starts.
@@ -191,7 +212,8 @@ __device__ __forceinline__ int block_contig(const ContigDev *__restrict__ ctg, i
 
 __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
                                                       const ContigDev *__restrict__ ctg, int nctg,
-                                                      int64_t F, uint32_t *__restrict__ blkcnt, int64_t nlog)
+                                                      int64_t F, uint32_t *__restrict__ blkcnt,
+                                                      uint8_t *__restrict__ sel, int64_t nlog)
 {
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint64_t red[XT / 64 + 1];
@@ -201,8 +223,8 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
     const int64_t fb = LB * XT * XU;
     int c = block_contig(ctg, nctg, fb, &sc);
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
-    uint32_t cnt = 0;
-    XEmit dummy;
+    uint32_t cnt = 0, bits = 0;
+    XEmit x;
 #pragma unroll
     for (int u = 0; u < XU; u++) {
         const int64_t f = f0 + u;
@@ -210,8 +232,10 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
         while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
         const ContigDev cc = ctg[c];
         if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
-        cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, dummy);
+        cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, x);
+        bits |= ((x.fwd ? 1u : 0u) | (x.rev ? 2u : 0u)) << (2 * u);
     }
+    sel[fb / XU + threadIdx.x] = (uint8_t) bits;       // the emit pass reads the selection back
     uint64_t tot = block_sum_u64<XT>(cnt, red);
     if (threadIdx.x == 0) blkcnt[LB] = (uint32_t) tot;
     }
@@ -225,7 +249,7 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
 // workgroup and pair rather than one per logical block.
 __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict__ bps64,
                                                      const ContigDev *__restrict__ ctg, int nctg,
-                                                     int64_t F, int jbits,
+                                                     int64_t F, int jbits, const uint8_t *__restrict__ sel,
                                                      const int64_t *__restrict__ blkoff,
                                                      Rec *__restrict__ out,
                                                      unsigned long long *__restrict__ hist1, int64_t nlog,
@@ -255,15 +279,17 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint64_t k0[2 * XU], k1[2 * XU];
     uint32_t valid = 0;
+    const uint32_t bits = sel[fb / XU + threadIdx.x];  // selection found by the count pass
 #pragma unroll
     for (int u = 0; u < XU; u++) {
         const int64_t f = f0 + u;
-        if (f >= F) break;
+        if (!((bits >> (2 * u)) & 3u)) continue;
         while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
         const ContigDev cc = ctg[c];
-        if (f >= cc.pos0 + cc.ncand) continue;
         XEmit e;
-        scan_position(bps64, cc, f - cc.pos0, jbits, tl, tm, true, e);
+        e.fwd = (bits >> (2 * u)) & 1u;
+        e.rev = (bits >> (2 * u + 1)) & 1u;
+        record_keys(bps64, cc, f - cc.pos0, jbits, e);
         if (e.fwd) {
             k0[2 * u] = e.f0; k1[2 * u] = e.f1; valid |= 1u << (2 * u);
             uint32_t d = (uint32_t) (e.f0 >> 56);
