@@ -95,7 +95,7 @@ struct pga_ctx {
     // device buffers
     DBuf bps, ctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
-        mblk, pofs, seedN, seedC, nbuck, cbuck, bmax, sel;
+        mblk, pofs, seedN, seedC, nbuck, cbuck, bmax, sel, plist, gmax;
 
     DBuf vout;                         // results of the small check kernels
     DBuf minfo;                        // per-anchor interval + seed counts (count pass -> emit pass)
@@ -378,7 +378,7 @@ void pga_destroy(pga_ctx *c)
     DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
-                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax,
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
@@ -551,6 +551,7 @@ int pga_build_index(pga_ctx *c)
         ent += d.nrec * d.e;
         d.cum_off = (int64_t) p * NPRE;
         d.buck_off = h.ctg0;
+        d.dense_cum = d.nrec > 0 && d.nrec <= 2 * (int64_t) NPRE;    // at most ~2 records per prefix
     }
     const size_t nrec_alloc = (size_t) std::max<int64_t>(R, 1);
     TRY(ensure(c, c->rec[0], nrec_alloc * sizeof(Rec)));
@@ -673,10 +674,30 @@ int pga_build_index(pga_ctx *c)
     if (c->o.build_cum) {
         TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
         TRY(ensure(c, c->longs, (size_t) npairs * 65536 * sizeof(LongFill)));
-        for (int p = 0; p < npairs; p++)
+        std::vector<int> dense;
+        for (int p = 0; p < npairs; p++) {
             if (c->pairs[p].nrec == 0) TRY(fill_u64(c, P<int64_t>(c->cum) + (int64_t) p * NPRE, NPRE, 0));
+            else if (c->pairs[p].dense_cum) dense.push_back(p);
+        }
         HIPCHK(c, hipMemsetAsync(dcount + 2, 0, sizeof(int), c->st));
-        if (R > 0) {
+        if (!dense.empty()) {
+            const int nrange = NPRE / CD_SPAN;
+            TRY(ensure(c, c->plist, dense.size() * sizeof(int)));
+            TRY(ensure(c, c->gmax, dense.size() * nrange * sizeof(uint32_t)));
+            HIPCHK(c, hipMemcpyAsync(c->plist.p, dense.data(), dense.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+            hipLaunchKernelGGL(k_cum_dense, dim3(nrange, (unsigned) dense.size()), dim3(256), 0, c->st,
+                               (const PairDev *) P<PairDev>(c->dpairs), (const int *) P<int>(c->plist),
+                               (const uint32_t *) P<uint32_t>(c->pre), P<int64_t>(c->cum), P<uint32_t>(c->gmax));
+            LAUNCHCHK(c);
+            hipLaunchKernelGGL(k_cum_dense_max, dim3((unsigned) dense.size()), dim3(256), 0, c->st,
+                               (const int *) P<int>(c->plist), (const uint32_t *) P<uint32_t>(c->gmax), nrange,
+                               P<unsigned long long>(c->maxpre));
+            LAUNCHCHK(c);
+        }
+        // pairs with many records per prefix: group-end fills and the maximum over group ends
+        int nsparse = 0;
+        for (int p = 0; p < npairs; p++) nsparse += c->pairs[p].nrec > 0 && !c->pairs[p].dense_cum;
+        if (R > 0 && nsparse > 0) {
             hipLaunchKernelGGL(k_cum_fill, dim3(grid_for(c, (const void *) k_cum_fill, 256, 0, (R + 255) / 256)),
                                dim3(256), 0, c->st,
                                (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),

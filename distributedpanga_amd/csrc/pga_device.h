@@ -28,7 +28,8 @@ struct PairDev {
     int64_t nseed_off, cseed_off;// byte offsets of its seed streams
     int64_t mblk0;               // first merge block
     int32_t P, C, e, s;
-    int32_t ncontig, pad;
+    int32_t ncontig;
+    int32_t dense_cum;           // 1: prefix table filled prefix by prefix (few records per prefix)
 };
 
 struct Seg {                     // a key range still to be partitioned

@@ -924,6 +924,7 @@ __global__ void k_cum_fill(const PairDev *__restrict__ pairs, int npairs, int64_
     for (int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; g < R;
          g += (int64_t) gridDim.x * blockDim.x) {
         const int p = pair_of(pairs, npairs, g);
+        if (pairs[p].dense_cum) continue;           // filled by k_cum_dense
         const int64_t rec0 = pairs[p].rec0, nrec = pairs[p].nrec;
         const int64_t li = g - rec0;
         int64_t *cp = cum + pairs[p].cum_off;
@@ -931,6 +932,88 @@ __global__ void k_cum_fill(const PairDev *__restrict__ pairs, int npairs, int64_
         const int64_t nxt = (li + 1 < nrec) ? (int64_t) pre[g + 1] : (int64_t) (1 << 24);
         if (li == 0 && cur > 0) fill_range(cp, 0, cur, 0, longs, nlong);
         if (cur != nxt) fill_range(cp, cur, nxt, li + 1, longs, nlong);
+    }
+}
+
+// Prefix table of a pair with few records per prefix, written prefix by
+// prefix (coalesced): block (x, y) covers prefixes [x*CD_SPAN, (x+1)*CD_SPAN)
+// of pair plist[y].  It finds the pair's records in that prefix range, counts
+// them per prefix in LDS, and writes the running count (records with prefix
+// <= p) for every prefix of the range.
+constexpr int CD_SPAN = 4096;
+
+__global__ __launch_bounds__(256) void k_cum_dense(const PairDev *__restrict__ pairs, const int *__restrict__ plist,
+                                                   const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
+                                                   uint32_t *__restrict__ gmax)
+{
+    constexpr int PER = CD_SPAN / 256;
+    __shared__ uint32_t cntp[CD_SPAN];
+    __shared__ uint32_t sh[256 / 64 + 1];
+    __shared__ uint32_t gm;                              // the range's largest per-prefix count
+    __shared__ int64_t rng[2];
+    const PairDev pd = pairs[plist[blockIdx.y]];
+    const uint32_t p0 = blockIdx.x * (uint32_t) CD_SPAN;
+    const uint32_t *pp = pre + pd.rec0;
+    for (int k = threadIdx.x; k < CD_SPAN; k += 256) cntp[k] = 0;
+    if (threadIdx.x == 0) gm = 0;
+    // first record with prefix >= p0 and >= p0 + CD_SPAN: 256-way search, each
+    // round every thread tests one sample (log_256 of the pair's records rounds)
+    for (int w = 0; w < 2; w++) {
+        const uint32_t key = p0 + w * (uint32_t) CD_SPAN;
+        int64_t lo = 0, hi = pd.nrec;                  // answer in [lo, hi]
+        while (hi > lo) {
+            const int64_t step = (hi - lo + 255) / 256;
+            const int64_t idx = lo + (int64_t) threadIdx.x * step;
+            const int below = (idx < hi && pp[idx] < key) ? 1 : 0;
+            const int k = __syncthreads_count(below);  // samples below key: a prefix of the samples
+            if (k == 0) { hi = lo; break; }           // pp[lo] >= key (lo is sample 0)
+            const int64_t nlo = lo + (int64_t) (k - 1) * step + 1;   // first index after the last sample below
+            const int64_t nhi = lo + (int64_t) k * step < hi ? lo + (int64_t) k * step : hi;
+            lo = nlo;
+            hi = nhi;
+        }
+        if (threadIdx.x == 0) rng[w] = lo;
+    }
+    __syncthreads();
+    const int64_t lo = rng[0], cnt = rng[1] - rng[0];
+    for (int64_t i = threadIdx.x; i < cnt; i += 256) atomicAdd(&cntp[pp[lo + i] - p0], 1u);
+    __syncthreads();
+    // inclusive running count over the range: PER consecutive prefixes per thread;
+    // the largest per-prefix count is this range's largest prefix group
+    uint32_t v[PER], sum = 0, mx = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) { v[k] = cntp[threadIdx.x * PER + k]; sum += v[k]; mx = v[k] > mx ? v[k] : mx; }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t x = __shfl_xor(mx, o, 64); mx = x > mx ? x : mx; }
+    if ((threadIdx.x & 63) == 0) atomicMax(&gm, mx);
+    uint32_t tot;
+    uint32_t run = block_excl_scan<256, uint32_t>(sum, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < PER; k++) { run += v[k]; cntp[threadIdx.x * PER + k] = run; }
+    __syncthreads();
+    int64_t *cp = cum + pd.cum_off + p0;
+    for (int k = threadIdx.x; k < CD_SPAN; k += 256) cp[k] = lo + (int64_t) cntp[k];
+    if (threadIdx.x == 0) gmax[(size_t) blockIdx.y * gridDim.x + blockIdx.x] = gm;
+}
+
+// Largest prefix group of each densely filled pair: the maximum of its ranges' maxima.
+__global__ __launch_bounds__(256) void k_cum_dense_max(const int *__restrict__ plist, const uint32_t *__restrict__ gmax,
+                                                       int nrange, unsigned long long *__restrict__ maxpre)
+{
+    __shared__ uint32_t red[4];
+    uint32_t m = 0;
+    for (int i = threadIdx.x; i < nrange; i += 256) {
+        const uint32_t x = gmax[(size_t) blockIdx.x * nrange + i];
+        m = x > m ? x : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t x = __shfl_xor(m, o, 64); m = x > m ? x : m; }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t x = red[0];
+        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
+        maxpre[plist[blockIdx.x]] = x;
     }
 }
 

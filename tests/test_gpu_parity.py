@@ -152,3 +152,22 @@ def test_human_scale_workload_reduced():
     families, satellites) at 1/100 of its length."""
     got, _ = _compare(synth.workload(4, 0.01))
     assert got["info"]["nrec"] > 40_000_000
+
+
+def test_batch_mixed_table_paths():
+    """One pair with many records per prefix (its table filled from group
+    ends) batched with small pairs (tables filled prefix by prefix)."""
+    gdbs = [synth.workload(1, 0.05), synth.workload(4, 0.01), synth.workload(1, 0.02)]
+    with pga.Context() as ctx:
+        ctx.load_batch(gdbs)
+        ctx.build_index()
+        ctx.merge()
+        for p, g in enumerate(gdbs):
+            print(f"[mixed] pair {p}", flush=True)
+            ent, cum, info = ctx.index_export(p)
+            N, Cs, nb, cb = ctx.seeds_export(p)
+            want = oracle.pipeline(g.bps, g.boff[:-1], g.clen)
+            assert np.array_equal(ent, want["entries"])
+            assert np.array_equal(cum, want["cum"]) and info["maxpre"] == want["maxpre"]
+            assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"])
+            assert np.array_equal(nb, want["nbuck"]) and np.array_equal(cb, want["cbuck"])
