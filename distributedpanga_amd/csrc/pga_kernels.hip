@@ -135,6 +135,47 @@ __device__ __forceinline__ T block_excl_scan(T v, T *sh, T *total)
     return r;
 }
 
+typedef unsigned __int128 u128;
+
+// Write n fixed-width items (s <= 16 bytes each, item j held little-endian in
+// slot[j]) as one packed byte run starting at g: every lane assembles one
+// aligned 16-byte chunk of the run from the (at most three) slots it
+// overlaps; the partial chunks at the two ends are written byte by byte.
+template <int NT>
+__device__ __forceinline__ void write_packed(uint8_t *__restrict__ g, const u128 *__restrict__ slot, int n, int s)
+{
+    if (n <= 0) return;
+    const uintptr_t base = (uintptr_t) g;
+    const uintptr_t a0 = base & ~(uintptr_t) 15;
+    const int nb = n * s;
+    const int lead = (int) (base - a0);                 // the run starts at this offset of chunk 0
+    const int nchunks = (lead + nb + 15) >> 4;
+    for (int c = threadIdx.x; c < nchunks; c += NT) {
+        const int o0 = 16 * c - lead;                   // run offset of the chunk's first byte
+        const int o = o0 < 0 ? 0 : o0;
+        int j = o / s;
+        const int t = o - j * s;
+        u128 acc = slot[j] >> (8 * t);
+        int filled = s - t;
+        for (j++; filled < 16 && j < n; j++) {
+            acc |= slot[j] << (8 * filled);
+            filled += s;
+        }
+        if (o0 < 0) acc <<= 8 * (-o0);                  // chunk 0 starts before the run
+        uint8_t *dst = reinterpret_cast<uint8_t *>(a0 + 16 * (uintptr_t) c);
+        if (o0 >= 0 && o0 + 16 <= nb) {
+            uint4 w;
+            const uint64_t lo = (uint64_t) acc, hi = (uint64_t) (acc >> 64);
+            w.x = (uint32_t) lo; w.y = (uint32_t) (lo >> 32); w.z = (uint32_t) hi; w.w = (uint32_t) (hi >> 32);
+            *reinterpret_cast<uint4 *>(dst) = w;
+        } else {
+#pragma unroll
+            for (int b = 0; b < 16; b++)
+                if (o0 + b >= 0 && o0 + b < nb) dst[b] = (uint8_t) (acc >> (8 * b));
+        }
+    }
+}
+
 // ---------------------------------------------------------------- extraction
 
 // Positions per thread and threads per block of the syncmer scan.  Flat
@@ -788,71 +829,49 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     for (int k = 0; k < E; k++) { Rec x; x.w0 = a0[k]; x.w1 = a1[k]; sk[tid * E + k] = x; }
     __syncthreads();
     }
-    // Output, striped (record k*T + tid): conflict-free LDS reads, coalesced
-    // LCP/prefix stores.  The LCP of the tile's first record is set by k_tile_fix.
-    uint64_t c0[E], c1[E];
-    uint32_t lc[E];
+    // Output.  LCPs with the predecessor first (striped, record k*T + tid), into
+    // the bins' LDS, which is free now; the LCP of the tile's first record is
+    // set by k_tile_fix.
+    uint8_t *lcb = reinterpret_cast<uint8_t *>(bh);
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const int idx = k * T + tid;
-        c0[k] = c1[k] = 0;
-        lc[k] = 0;
         if (idx < n) {
             const Rec x = sk[idx];
-            c0[k] = x.w0;
-            c1[k] = x.w1;
-            if (idx > 0) {
-                const Rec y = sk[idx - 1];
-                lc[k] = (uint32_t) lcp_bases(y.w0, y.w1, x.w0, x.w1);
-            }
+            uint32_t l = 0;
+            if (idx > 0) { const Rec y = sk[idx - 1]; l = (uint32_t) lcp_bases(y.w0, y.w1, x.w0, x.w1); }
+            lcb[idx] = (uint8_t) l;
         }
     }
-    __syncthreads();                                // LDS is reused for entry bytes below
-
+    __syncthreads();
+    // Each record's GIX entry (src/kmer_adapter.c:55-62 layout) replaces it in
+    // its own 16-byte slot: k-mer bytes 3..9, mask 0, LCP, pos LE, contig|strand LE.
     const int e = pd.e, P = pd.P, C = pd.C;
-    const int64_t ebase = pd.ent_off + (tl.start - pd.rec0) * (int64_t) e;
-    const int shift = (int) (ebase & 15);
-    uint8_t *sb = smem_raw;
     const uint64_t jmask = (1ull << jbits) - 1ull;
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const int idx = k * T + tid;
         if (idx < n) {
-            const uint64_t w0 = c0[k], w1 = c1[k];
-            const uint32_t lcp = lc[k];
-            const uint32_t strand = (uint32_t) (w1 & 1ull);
-            const uint64_t j = (w1 >> 1) & jmask;
-            const uint64_t contig = (w1 & 0xFFFFFFFFFFFFull) >> (jbits + 1);
+            const Rec x = sk[idx];
+            const uint64_t lcp = lcb[idx];
+            const uint64_t strand = x.w1 & 1ull;
+            const uint64_t j = (x.w1 >> 1) & jmask;
+            const uint64_t contig = (x.w1 & 0xFFFFFFFFFFFFull) >> (jbits + 1);
             const uint64_t pos = j + 12ull * strand;
-            const uint64_t cs = contig | ((uint64_t) strand << (8 * C - 1));
-            uint8_t *d = sb + shift + idx * e;
-            d[0] = (uint8_t) (w0 >> 32); d[1] = (uint8_t) (w0 >> 24); d[2] = (uint8_t) (w0 >> 16);
-            d[3] = (uint8_t) (w0 >> 8);  d[4] = (uint8_t) w0;
-            d[5] = (uint8_t) (w1 >> 56); d[6] = (uint8_t) (w1 >> 48);
-            d[7] = 0;
-            d[8] = (uint8_t) lcp;
-            for (int b = 0; b < P; b++) d[9 + b] = (uint8_t) (pos >> (8 * b));
-            for (int b = 0; b < C; b++) d[9 + P + b] = (uint8_t) (cs >> (8 * b));
+            const uint64_t cs = contig | (strand << (8 * C - 1));
+            // the last 7 k-mer bytes, big-endian -> little-endian bytes 0..6, byte 7 (mask) 0
+            const uint64_t low56 = ((x.w0 & 0xFFFFFFFFFFull) << 16) | (x.w1 >> 48);
+            Rec ent;
+            ent.w0 = __builtin_bswap64(low56 << 8);
+            ent.w1 = lcp | (pos << 8) | (cs << (8 + 8 * P));
             ls[tl.start + idx] = (uint8_t) (lcp | (strand << 7));
-            pre[tl.start + idx] = (uint32_t) (w0 >> 40);
-            if (skeys) { Rec x; x.w0 = w0; x.w1 = w1; skeys[tl.start + idx] = x; }
+            pre[tl.start + idx] = (uint32_t) (x.w0 >> 40);
+            if (skeys) skeys[tl.start + idx] = x;
+            sk[idx] = ent;
         }
     }
     __syncthreads();
-    // coalesced copy of the tile's entry bytes: 16-byte aligned chunks, byte
-    // stores only for the partial first and last chunk
-    const int nb = n * e;
-    const int nchunks = (shift + nb + 15) >> 4;
-    uint8_t *g = entries + (ebase - shift);
-    for (int c = tid; c < nchunks; c += T) {
-        int b0 = c << 4;
-        if (b0 >= shift && b0 + 16 <= shift + nb) {
-            *reinterpret_cast<uint4 *>(g + b0) = *reinterpret_cast<const uint4 *>(sb + b0);
-        } else {
-            for (int b = b0; b < b0 + 16; b++)
-                if (b >= shift && b < shift + nb) g[b] = sb[b];
-        }
-    }
+    write_packed<T>(entries + pd.ent_off + (tl.start - pd.rec0) * (int64_t) e, reinterpret_cast<const u128 *>(sk), n, e);
     __syncthreads();
     }
 }
@@ -1260,48 +1279,6 @@ __device__ __forceinline__ uint64_t payload_at(const uint8_t *__restrict__ ew, i
     return v;
 }
 
-typedef unsigned __int128 u128;
-
-// Write seeds [0, n) of a round straight to global memory, 16 bytes per lane.
-// Seed j sits in LDS slot j as one 128-bit little-endian value holding its s
-// bytes: [plen][anchor payload, strand bit cleared][partner payload]
-// (src/pga-mpi-merge.c:68-76).  Every lane assembles one aligned 16-byte chunk
-// of the output from the (at most three) slots it overlaps; chunks the round
-// shares with its neighbours in the stream are written byte by byte.
-__device__ __forceinline__ void emit_round(uint8_t *__restrict__ g, const u128 *__restrict__ slot, int n, int s)
-{
-    if (n <= 0) return;
-    const uintptr_t base = (uintptr_t) g;
-    const uintptr_t a0 = base & ~(uintptr_t) 15;
-    const int nb = n * s;
-    const int lead = (int) (base - a0);                 // stream bytes start at this offset of chunk 0
-    const int nchunks = (lead + nb + 15) >> 4;
-    for (int c = threadIdx.x; c < nchunks; c += MTH) {
-        const int o0 = 16 * c - lead;                   // stream offset of the chunk's first byte
-        const int o = o0 < 0 ? 0 : o0;
-        int j = o / s;
-        const int t = o - j * s;
-        u128 acc = slot[j] >> (8 * t);
-        int filled = s - t;
-        for (j++; filled < 16 && j < n; j++) {
-            acc |= slot[j] << (8 * filled);
-            filled += s;
-        }
-        if (o0 < 0) acc <<= 8 * (-o0);                  // chunk 0 starts before the stream
-        uint8_t *dst = reinterpret_cast<uint8_t *>(a0 + 16 * (uintptr_t) c);
-        if (o0 >= 0 && o0 + 16 <= nb) {
-            uint4 w;
-            const uint64_t lo = (uint64_t) acc, hi = (uint64_t) (acc >> 64);
-            w.x = (uint32_t) lo; w.y = (uint32_t) (lo >> 32); w.z = (uint32_t) hi; w.w = (uint32_t) (hi >> 32);
-            *reinterpret_cast<uint4 *>(dst) = w;
-        } else {
-#pragma unroll
-            for (int b = 0; b < 16; b++)
-                if (o0 + b >= 0 && o0 + b < nb) dst[b] = (uint8_t) (acc >> (8 * b));
-        }
-    }
-}
-
 // Seeds of a run of merge tiles (A*MTH anchors each, A consecutive per
 // thread).  Per tile: the LCP/strand bytes and raw GIX entries of the tile plus
 // a FREQ halo on each side go to LDS, then one 64-bit payload per entry; each
@@ -1437,8 +1414,8 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
                 }
             }
             __syncthreads();
-            emit_round(gN + n0 * s, slotN, (int) (n1 - n0), s);
-            emit_round(gC + c0 * s, slotC, (int) (c1 - c0), s);
+            write_packed<MTH>(gN + n0 * s, slotN, (int) (n1 - n0), s);
+            write_packed<MTH>(gC + c0 * s, slotC, (int) (c1 - c0), s);
             __syncthreads();
         }
         // per-anchor-contig counts (src/fastga_pipeline.c:403)
