@@ -98,6 +98,11 @@ struct pga_ctx {
         mblk, pofs, seedN, seedC, nbuck, cbuck, bmax, sel, plist, gmax;
 
     DBuf vout;                         // results of the small check kernels
+    // seed sort: keys (ping-pong), their pointer pair, the packed records
+    DBuf ssk[2], ssbufs, ssout;
+    int64_t ss_n = 0;
+    int ss_swide = 0;
+    bool ss_ready = false;
     DBuf minfo;                        // per-anchor interval + seed counts (count pass -> emit pass)
     // where the last merge put its outputs: inside the record buffers (dead once
     // the index is built) when they fit there, else in minfo / seedN / seedC
@@ -253,17 +258,18 @@ int launch_seg_scatter(pga_ctx *c, int per, unsigned nlog, const Rec *src, Rec *
     return PGA_OK;
 }
 
-int launch_tile_sort(pga_ctx *c, unsigned nlog)
+// bufs: device array {buffer 0, buffer 1} the tiles point into; entries ==
+// nullptr sorts keys only (into skeys).
+int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entries, uint8_t *ls, uint32_t *pre,
+                     Rec *skeys)
 {
     const int T = c->o.tile_records / 8;
     const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
-    Rec *const *bufs = (Rec *const *) c->bufs.p;
     const Tile *tiles = P<Tile>(c->tiles);
     const PairDev *pairs = P<PairDev>(c->dpairs);
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
-                       dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, P<uint8_t>(c->entries),      \
-                       P<uint8_t>(c->ls), P<uint32_t>(c->pre), c->keep_keys ? P<Rec>(c->skeys) : nullptr,  \
+                       dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, entries, ls, pre, skeys,         \
                        (int64_t) nlog)
     switch (T) {
     case 256: PGA_TSORT(256); break;
@@ -378,7 +384,7 @@ void pga_destroy(pga_ctx *c)
     DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
-                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
@@ -481,6 +487,79 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
     return PGA_OK;
 }
 
+namespace {
+
+// MSD radix partition: the segments seg0 (records in buf0) are split on 8-bit
+// digits, level after level, until every bucket fits a sort tile; tiles go to
+// c->tiles after the tile0 ones (c->ntiles is set).  Level-1 digit counts come
+// from hist1 (one 256-row per pair) when given, else they are counted here.
+// The tiles' `buf` index refers to {buf0, buf1}.
+int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, const std::vector<Tile> &tile0,
+              const unsigned long long *hist1, int64_t seg_cap, int64_t tile_cap, int *levels_out)
+{
+    const int CAP = cap_of(c);
+    const int per = chunk_per(c);
+    const int CH = per * 256;
+    Rec *buf[2] = {buf0, buf1};
+    int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles, [2] long fills
+    int hcount[4] = {0, (int) tile0.size(), 0, 0};
+    HIPCHK(c, hipMemcpyAsync(dcount, hcount, sizeof(hcount), hipMemcpyHostToDevice, c->st));
+    if (!tile0.empty())
+        HIPCHK(c, hipMemcpyAsync(c->tiles.p, tile0.data(), tile0.size() * sizeof(Tile), hipMemcpyHostToDevice, c->st));
+    int nseg = (int) seg0.size();
+    if (nseg)
+        HIPCHK(c, hipMemcpyAsync(c->segs[0].p, seg0.data(), seg0.size() * sizeof(Seg), hipMemcpyHostToDevice, c->st));
+    int cur = 0, level = 1;
+    while (nseg > 0) {
+        if (level > MAX_LEVELS) return fail(c, PGA_EINVAL, "radix partition did not converge");
+        hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
+                           nseg, CH, P<int64_t>(c->chk0));
+        LAUNCHCHK(c);
+        int64_t ck[2] = {0, 0};
+        HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        const int64_t nchunks = ck[0];
+        c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * ck[1];   // each record read and written once
+        const Rec *src = buf[(level - 1) & 1];
+        Rec *dst = buf[level & 1];
+        const bool counted = level == 1 && hist1;
+        if (!counted) {
+            TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * 256, 0));
+            TRY(launch_seg_hist(c, per, (unsigned) nchunks, src, P<Seg>(c->segs[cur]), nseg,
+                                P<int64_t>(c->chk0), P<unsigned long long>(c->seghist)));
+        }
+        HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(int), c->st));
+        hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
+                           counted ? hist1 : (const unsigned long long *) P<unsigned long long>(c->seghist),
+                           counted ? 1 : 0, CAP, level & 1, P<unsigned long long>(c->cursor),
+                           P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
+                           (int) tile_cap);
+        LAUNCHCHK(c);
+        hipEvent_t se[2];
+        TRY(scatter_event(c, se));
+        HIPCHK(c, hipEventRecord(se[0], c->st));
+        TRY(launch_seg_scatter(c, per, (unsigned) nchunks, src, dst, P<Seg>(c->segs[cur]), nseg,
+                               P<int64_t>(c->chk0), P<unsigned long long>(c->cursor)));
+        HIPCHK(c, hipEventRecord(se[1], c->st));
+        c->nscatter++;
+        HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        nseg = hcount[0];
+        cur ^= 1;
+        level++;
+        if (nseg > seg_cap) return fail(c, PGA_EOVERFLOW, "segment list overflow");
+        if (hcount[1] > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
+    }
+    HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ntiles = hcount[1];
+    if (c->ntiles > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
+    *levels_out = level - 1;
+    return PGA_OK;
+}
+
+} // namespace
+
 extern "C" {
 
 int pga_load_batch(pga_ctx *c, int npairs, const uint8_t *const *bps, const int64_t *bps_bytes,
@@ -499,14 +578,12 @@ int pga_build_index(pga_ctx *c)
 {
     if (!c || !c->loaded) return c ? fail(c, PGA_EINVAL, "no GDB loaded") : PGA_EINVAL;
     HIPCHK(c, hipSetDevice(c->o.device));
-    c->built = c->merged = false;
+    c->built = c->merged = c->ss_ready = false;
     c->nscatter = 0;
     c->scatter_bytes = 0;
     memset(&c->tm, 0, sizeof(c->tm));
     const int npairs = c->npairs;
     const int CAP = cap_of(c);
-    const int per = chunk_per(c);
-    const int CH = per * 256;
 
     TRY(record(c, 0));
     // ---- extraction: count, scan, pair record ranges
@@ -600,65 +677,18 @@ int pga_build_index(pga_ctx *c)
         if (d.nrec > CAP) seg0.push_back(Seg{d.rec0, d.nrec, p, 1});
         else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 0, 1});
     }
-    int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles, [2] long fills
-    int hcount[4] = {0, (int) tile0.size(), 0, 0};
-    HIPCHK(c, hipMemcpyAsync(dcount, hcount, sizeof(hcount), hipMemcpyHostToDevice, c->st));
-    if (!tile0.empty())
-        HIPCHK(c, hipMemcpyAsync(c->tiles.p, tile0.data(), tile0.size() * sizeof(Tile), hipMemcpyHostToDevice, c->st));
-    int nseg = (int) seg0.size();
-    if (nseg)
-        HIPCHK(c, hipMemcpyAsync(c->segs[0].p, seg0.data(), seg0.size() * sizeof(Seg), hipMemcpyHostToDevice, c->st));
-    int cur = 0, level = 1;
-    while (nseg > 0) {
-        if (level > MAX_LEVELS) return fail(c, PGA_EINVAL, "radix partition did not converge");
-        hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
-                           nseg, CH, P<int64_t>(c->chk0));
-        LAUNCHCHK(c);
-        int64_t ck[2] = {0, 0};
-        HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        const int64_t nchunks = ck[0];
-        c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * ck[1];   // each record read and written once
-        const Rec *src = P<Rec>(c->rec[(level - 1) & 1]);
-        Rec *dst = P<Rec>(c->rec[level & 1]);
-        if (level > 1) {
-            TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * 256, 0));
-            TRY(launch_seg_hist(c, per, (unsigned) nchunks, src, P<Seg>(c->segs[cur]), nseg,
-                                P<int64_t>(c->chk0), P<unsigned long long>(c->seghist)));
-        }
-        HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(int), c->st));
-        hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
-                           (const unsigned long long *) (level == 1 ? P<unsigned long long>(c->hist1)
-                                                                     : P<unsigned long long>(c->seghist)),
-                           level == 1 ? 1 : 0, CAP, level & 1, P<unsigned long long>(c->cursor),
-                           P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
-                           (int) tile_cap);
-        LAUNCHCHK(c);
-        hipEvent_t se[2];
-        TRY(scatter_event(c, se));
-        HIPCHK(c, hipEventRecord(se[0], c->st));
-        TRY(launch_seg_scatter(c, per, (unsigned) nchunks, src, dst, P<Seg>(c->segs[cur]), nseg,
-                               P<int64_t>(c->chk0), P<unsigned long long>(c->cursor)));
-        HIPCHK(c, hipEventRecord(se[1], c->st));
-        c->nscatter++;
-        HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        nseg = hcount[0];
-        cur ^= 1;
-        level++;
-        if (nseg > seg_cap) return fail(c, PGA_EOVERFLOW, "segment list overflow");
-        if (hcount[1] > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
-    }
-    c->tm.partition_levels = level - 1;
+    int levels = 0;
+    TRY(partition(c, P<Rec>(c->rec[0]), P<Rec>(c->rec[1]), seg0, tile0, P<unsigned long long>(c->hist1), seg_cap,
+                  tile_cap, &levels));
+    int *dcount = P<int>(c->counters);          // [2]: long fills of the prefix table
+    c->tm.partition_levels = levels;
     c->tm.partition_launches = c->nscatter;
-    HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    c->ntiles = hcount[1];
-    if (c->ntiles > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
     TRY(record(c, 2));
 
     // ---- per-tile LDS sort, LCP, GIX entries
-    if (c->ntiles > 0) TRY(launch_tile_sort(c, (unsigned) c->ntiles));
+    if (c->ntiles > 0)
+        TRY(launch_tile_sort(c, (unsigned) c->ntiles, (Rec *const *) c->bufs.p, P<uint8_t>(c->entries),
+                             P<uint8_t>(c->ls), P<uint32_t>(c->pre), c->keep_keys ? P<Rec>(c->skeys) : nullptr));
     TRY(record(c, 3));
     if (c->ntiles > 0) {
         hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->ntiles + 255) / 256)),
@@ -1022,6 +1052,87 @@ int pga_seeds_export(pga_ctx *c, int pair, uint8_t *nseeds, uint8_t *cseeds, int
     if (cbuck && d.ncontig > 0)
         HIPCHK(c, hipMemcpyAsync(cbuck, P<uint64_t>(c->cbuck) + row + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
                                  hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+// Diagonal-sorted seed records of one stream of one pair (see k_seed_keys).
+int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
+{
+    if (!c || !c->merged || pair < 0 || pair >= c->npairs || (comp != 0 && comp != 1))
+        return c ? fail(c, PGA_EINVAL, "no seeds for pair %d / stream %d", pair, comp) : PGA_EINVAL;
+    HIPCHK(c, hipSetDevice(c->o.device));
+    const PairDev &d = c->pairs[pair];
+    const int64_t n = comp ? c->cseeds[pair] : c->nseeds[pair];
+    const uint8_t *stream = comp ? c->seedC_p + d.cseed_off : c->seedN_p + d.nseed_off;
+    const int64_t maxctg = c->hp[pair].maxctg;
+    int dbyte = 0;                                       // src/pga-mpi-merge.c:415-425
+    for (int64_t cum = 1; cum < 2 * maxctg; cum *= 256) dbyte++;
+    const int swide = 2 * dbyte + d.C + 2;
+    const int kbits = 8 * swide + bits_for(std::max(d.ncontig - 1, 0));
+    if (kbits > 128) return fail(c, PGA_EOVERFLOW, "seed sort key of %d bits exceeds 128", kbits);
+    const int shift = 128 - kbits;
+    c->ss_n = n;
+    c->ss_swide = swide;
+    c->ss_ready = false;
+    const size_t n1 = (size_t) std::max<int64_t>(n, 1);
+    TRY(ensure(c, c->ssk[0], n1 * sizeof(Rec)));
+    TRY(ensure(c, c->ssk[1], n1 * sizeof(Rec)));
+    TRY(ensure(c, c->ssbufs, 2 * sizeof(Rec *)));
+    TRY(ensure(c, c->ssout, n1 * (size_t) swide + 16));
+    {
+        Rec *bp[2] = {P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1])};
+        HIPCHK(c, hipMemcpyAsync(c->ssbufs.p, bp, sizeof(bp), hipMemcpyHostToDevice, c->st));
+    }
+    if (n > 0) {
+        hipLaunchKernelGGL(k_seed_keys, dim3(grid_for(c, (const void *) k_seed_keys, 256, 0, (n + 255) / 256)), dim3(256),
+                           0, c->st, stream, n, d.P, d.C, maxctg, dbyte, swide, comp, shift, P<Rec>(c->ssk[0]));
+        LAUNCHCHK(c);
+        // partition + tile sort, the keys ending in ssk[0]; the build's statistics are kept
+        const int CAP = cap_of(c);
+        std::vector<Seg> seg0;
+        std::vector<Tile> tile0;
+        if (n > CAP) seg0.push_back(Seg{0, n, pair, 1});
+        else tile0.push_back(Tile{0, (int32_t) n, pair, 0, 1});
+        const int64_t seg_cap = n / CAP + 16;
+        const int64_t tile_cap = 34 * (n / CAP) + 1024;
+        TRY(ensure(c, c->segs[0], (size_t) seg_cap * sizeof(Seg)));
+        TRY(ensure(c, c->segs[1], (size_t) seg_cap * sizeof(Seg)));
+        TRY(ensure(c, c->seghist, (size_t) seg_cap * 256 * sizeof(uint64_t)));
+        TRY(ensure(c, c->cursor, (size_t) seg_cap * 256 * sizeof(uint64_t)));
+        TRY(ensure(c, c->chk0, (size_t) (seg_cap + 2) * sizeof(int64_t)));
+        TRY(ensure(c, c->tiles, (size_t) tile_cap * sizeof(Tile)));
+        const int ns = c->nscatter;
+        const int64_t sb = c->scatter_bytes;
+        const int64_t nt = c->ntiles;
+        int levels = 0;
+        const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0, nullptr, seg_cap, tile_cap,
+                                 &levels);
+        const int64_t sort_tiles = c->ntiles;
+        c->nscatter = ns;
+        c->scatter_bytes = sb;
+        c->ntiles = nt;
+        TRY(rc);
+        if (sort_tiles > 0)
+            TRY(launch_tile_sort(c, (unsigned) sort_tiles, (Rec *const *) c->ssbufs.p, nullptr, nullptr, nullptr,
+                                 P<Rec>(c->ssk[0])));
+        const int64_t nb = (n + SP_T * SP_PER - 1) / (SP_T * SP_PER);
+        hipLaunchKernelGGL(k_seed_pack, dim3(grid_for(c, (const void *) k_seed_pack, SP_T, 0, nb)), dim3(SP_T), 0,
+                           c->st, (const Rec *) P<Rec>(c->ssk[0]), n, shift, swide, P<uint8_t>(c->ssout));
+        LAUNCHCHK(c);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ss_ready = true;
+    if (nels) *nels = n;
+    if (swide_out) *swide_out = swide;
+    return PGA_OK;
+}
+
+int pga_seed_sort_export(pga_ctx *c, uint8_t *records)
+{
+    if (!c || !c->ss_ready) return c ? fail(c, PGA_EINVAL, "no seed sort to export") : PGA_EINVAL;
+    if (records && c->ss_n > 0)
+        HIPCHK(c, hipMemcpyAsync(records, c->ssout.p, (size_t) (c->ss_n * c->ss_swide), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return PGA_OK;
 }

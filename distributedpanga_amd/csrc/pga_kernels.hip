@@ -749,8 +749,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     __syncthreads();
     const bool binned = !fallback;
     if (binned) {
-        // scatter into bins, then every record counts the smaller keys of its
-        // bin (keys are unique) and moves to bin start + that rank
+        // scatter into bins, then every record counts the keys of its bin that
+        // come before it (smaller, or equal and scattered earlier: the seed sort
+        // has equal keys) and moves to bin start + that rank
         uint32_t bn[E];
 #pragma unroll
         for (int k = 0; k < E; k++)
@@ -762,10 +763,12 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             if (k * T + tid >= n) continue;
             const uint32_t st = bh[bn[k]];
             const uint32_t en = bn[k] + 1 < (uint32_t) NB ? bh[bn[k] + 1] : (uint32_t) n;
+            const uint32_t me = st + rk[k];
             uint32_t rank = 0;
             for (uint32_t i = st; i < en; i++) {
                 const Rec y = sk[i];
-                rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && y.w1 < r[k].w1)) ? 1u : 0u;
+                rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && (y.w1 < r[k].w1 || (y.w1 == r[k].w1 && i < me))))
+                        ? 1u : 0u;
             }
             pos[k] = st + rank;
         }
@@ -828,6 +831,16 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
 #pragma unroll
     for (int k = 0; k < E; k++) { Rec x; x.w0 = a0[k]; x.w1 = a1[k]; sk[tid * E + k] = x; }
     __syncthreads();
+    }
+    if (!entries) {
+        // keys only (the seed sort): sorted records to skeys at their positions
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const int idx = k * T + tid;
+            if (idx < n) skeys[tl.start + idx] = sk[idx];
+        }
+        __syncthreads();
+        continue;
     }
     // Output.  LCPs with the predecessor first (striped, record k*T + tid), into
     // the bins' LDS, which is free now; the LCP of the tile's first record is
@@ -1492,6 +1505,74 @@ __global__ void k_bucket_counts(const int64_t *__restrict__ cum, long long *__re
         const int64_t hi = cum[((b + 1) << 14) - 1];
         const int64_t lo = b ? cum[(b << 14) - 1] : 0;
         out[b] = (long long) (hi - lo);
+    }
+}
+
+// ---------------------------------------------------------------- seed sort
+//
+// The diagonal records of one seed stream (reimport_thread,
+// src/fastga_pipeline.c:1391-1450): [lcp][diag&63][anti DBYTE][band DBYTE]
+// [jcont C], all little-endian.  rmsd_sort orders each anchor-contig panel on
+// the whole record read from its last byte (lib/RSDsort.c:292-377), i.e. on
+// the record as one little-endian number; prefixing that number with the
+// anchor contig gives one key whose plain sort is the panel-by-panel order.
+// Keys are left-aligned in 128 bits so that the MSD digits start at once.
+
+__device__ __forceinline__ uint64_t le_bytes(const uint8_t *p, int n)
+{
+    uint64_t v = 0;
+    for (int k = n - 1; k >= 0; k--) v = (v << 8) | p[k];
+    return v;
+}
+
+__global__ void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P, int C, int64_t maxctg, int dbyte,
+                            int swide, int comp, int shift, Rec *__restrict__ keys)
+{
+    const int s = 1 + 2 * (P + C);
+    const int64_t amx = maxctg, bmx = maxctg, maxdag = amx + bmx;
+    const uint64_t cmask = (1ull << (8 * C - 1)) - 1ull;
+    const uint64_t dmask = dbyte >= 8 ? ~0ull : (1ull << (8 * dbyte)) - 1ull;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const uint8_t *b = seeds + i * s;
+        const uint64_t lcp = b[0];
+        const int64_t ipost = (int64_t) le_bytes(b + 1, P);
+        const uint64_t icont = le_bytes(b + 1 + P, C);
+        const int64_t jpost = (int64_t) le_bytes(b + 1 + P + C, P);
+        const uint64_t jcont = le_bytes(b + 1 + 2 * P + C, C) & cmask;
+        int64_t diag, anti;
+        if (comp) { diag = maxdag - (ipost + jpost); anti = amx - (ipost - jpost); }
+        else      { diag = bmx + (ipost - jpost);    anti = ipost + jpost; }
+        const int64_t band = diag >> 6;                               // BUCK_SHIFT
+        u128 r = (u128) lcp | ((u128) (uint64_t) (diag & 63) << 8) | ((u128) ((uint64_t) anti & dmask) << 16)
+               | ((u128) ((uint64_t) band & dmask) << (16 + 8 * dbyte))
+               | ((u128) jcont << (16 + 16 * dbyte));
+        const u128 k = (((u128) icont << (8 * swide)) | r) << shift;
+        Rec x;
+        x.w0 = (uint64_t) (k >> 64);
+        x.w1 = (uint64_t) k;
+        keys[i] = x;
+    }
+}
+
+// Sorted keys back to packed swide-byte records (the record is the key's low
+// 8*swide bits once the left alignment is undone).
+constexpr int SP_T = 256, SP_PER = 16;
+
+__global__ __launch_bounds__(SP_T) void k_seed_pack(const Rec *__restrict__ keys, int64_t n, int shift, int swide,
+                                                    uint8_t *__restrict__ out)
+{
+    __shared__ u128 slot[SP_T * SP_PER];
+    const u128 rmask = swide >= 16 ? ~(u128) 0 : (((u128) 1) << (8 * swide)) - 1;
+    for (int64_t b0 = (int64_t) blockIdx.x * SP_T * SP_PER; b0 < n; b0 += (int64_t) gridDim.x * SP_T * SP_PER) {
+        const int cnt = (int) min((int64_t) SP_T * SP_PER, n - b0);
+        for (int i = threadIdx.x; i < cnt; i += SP_T) {
+            const Rec x = keys[b0 + i];
+            const u128 k = ((u128) x.w0 << 64) | x.w1;
+            slot[i] = (k >> shift) & rmask;
+        }
+        __syncthreads();
+        write_packed<SP_T>(out + b0 * swide, slot, cnt, swide);
+        __syncthreads();
     }
 }
 

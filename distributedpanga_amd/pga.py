@@ -68,6 +68,8 @@ def _setup(L):
     L.pga_seeds_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(SeedsMeta)]
     L.pga_seeds_export.argtypes = [C.c_void_p, C.c_int, _u8p, _u8p, _i64p, _i64p]
     L.pga_timings_get.argtypes = [C.c_void_p, C.POINTER(Timings)]
+    L.pga_seed_sort.argtypes = [C.c_void_p, C.c_int, C.c_int, _i64p, C.POINTER(C.c_int)]
+    L.pga_seed_sort_export.argtypes = [C.c_void_p, _u8p]
     L.pga_sync.argtypes = [C.c_void_p]
     L._pga_ready = True
     return L
@@ -197,6 +199,15 @@ class Context:
                     "pga_seeds_export")
         k = info["ncontig"]
         return N[: info["nlen"]], Cs[: info["clen"]], nb[:k], cb[:k]
+
+    def seed_sort(self, pair=0, comp=0):
+        """Diagonal-sorted seed records of one stream (0: N, 1: C): array [n, swide]."""
+        n = C.c_int64()
+        sw = C.c_int()
+        self._check(self._L.pga_seed_sort(self._h, pair, comp, C.byref(n), C.byref(sw)), "pga_seed_sort")
+        out = np.zeros(max(n.value * sw.value, 1), np.uint8)
+        self._check(self._L.pga_seed_sort_export(self._h, out.ctypes.data_as(_u8p)), "pga_seed_sort_export")
+        return out[: n.value * sw.value].reshape(n.value, sw.value)
 
     def timings(self):
         t = Timings()

@@ -21,6 +21,11 @@
  *   pga_merge                       self_adaptamer_merge src/fastga_pipeline.h:18
  *   pga_seeds_info / _export        Recv_N_Data/Recv_C_Data + N/C_Units[0].buck,
  *                                   src/pga-mpi-merge.c:523-547 (1-rank shortcut)
+ *   pga_seed_sort / _export         reimport_thread + rmsd_sort as pair_sort_search
+ *                                   drives them, src/fastga_pipeline.c:1391-1450, 1711-1822
+ *   pga_index_verify                verify_per_rank_sorted, src/dsort.c:229-246
+ *   pga_bucket_counts               bucket_local_counts, src/dsort.c:108-118
+ *   pga_destination_map             compute_destination_map, src/dsort.c:120-167
  */
 #ifndef LIBPGA_HIP_H
 #define LIBPGA_HIP_H
@@ -142,6 +147,20 @@ int pga_seeds_info(pga_ctx *ctx, int pair, pga_seeds_meta *info);
  * anchors in index order, partners ascending, 1+2(P+C) bytes per seed. */
 int pga_seeds_export(pga_ctx *ctx, int pair, uint8_t *nseeds, uint8_t *cseeds,
                      int64_t *nbuck, int64_t *cbuck);
+
+/* The seed array pair_sort_search sorts before its alignment search
+ * (src/fastga_pipeline.c:1711-1822): every seed of one stream of pair (comp 0:
+ * the N stream, 1: the C stream) reimported as a diagonal record
+ * (reimport_thread, src/fastga_pipeline.c:1391-1450)
+ *     [lcp][diag & 63][anti, DBYTE bytes LE][band = diag >> 6, DBYTE LE][jcont, C bytes LE]
+ * (N: diag = M + i - j, anti = i + j; C: diag = 2M - (i + j), anti = M - (i - j);
+ * M = the longest contig, DBYTE = bytes for 2M, src/pga-mpi-merge.c:415-425),
+ * placed in anchor-contig panels (contig order, panel sizes = the buck vector)
+ * and each panel sorted on the whole record read from its last byte (rmsd_sort,
+ * lib/RSDsort.c:292-377).  *nels = records, *swide = 2*DBYTE + C + 2 bytes each.
+ * Needs pga_merge; the records stay on the device until pga_seed_sort_export. */
+int pga_seed_sort(pga_ctx *ctx, int pair, int comp, int64_t *nels, int *swide);
+int pga_seed_sort_export(pga_ctx *ctx, uint8_t *records);
 
 int pga_timings_get(pga_ctx *ctx, pga_timings *t);
 

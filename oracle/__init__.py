@@ -76,6 +76,9 @@ def lib():
         L.oracle_pipeline.argtypes = [_u8p, _i64p, _i64p, C.c_int, C.c_int, C.POINTER(OracleResult)]
         L.oracle_result_free.argtypes = [C.POINTER(OracleResult)]
         L.oracle_free.argtypes = [C.c_void_p]
+        L.oracle_dbyte.argtypes = [C.c_int64]
+        L.oracle_seed_sort.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int,
+                                       C.POINTER(_u8p), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -93,6 +96,7 @@ def ref():
         L.ref_run.argtypes = [_u8p, C.c_int64, _i64p, _i64p, C.c_int, C.c_int, C.POINTER(RefResult)]
         L.ref_result_free.argtypes = [C.POINTER(RefResult)]
         L.ref_lcp_bases.argtypes = [_u8p, _u8p]
+        L.ref_rmsd_sort.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, _i64p]
         _ref = L
     return _ref
 
@@ -224,3 +228,38 @@ def ref_run(bps, boff, clen, do_index=True):
                    cum=_take(r.cum, NPRE, np.int64), maxpre=r.maxpre)
     ref().ref_result_free(C.byref(r))
     return out
+
+
+def dbyte(maxctg):
+    """DBYTE of a GDB whose longest contig is maxctg (src/pga-mpi-merge.c:415-425)."""
+    return lib().oracle_dbyte(int(maxctg))
+
+
+def seed_sort(seeds, P, Cb, ncontig, maxctg, comp, sort=True):
+    """Diagonal records of one seed stream, panels in contig order, each sorted
+    on the whole record from its last byte.  Returns (records [n, swide], swide)."""
+    L = lib()
+    s = 1 + 2 * (P + Cb)
+    seeds = np.ascontiguousarray(seeds, np.uint8)
+    n = seeds.size // s
+    out = _u8p()
+    sw = C.c_int()
+    buf = seeds if seeds.size else np.zeros(1, np.uint8)
+    if L.oracle_seed_sort(_u8(buf), n, P, Cb, ncontig, int(maxctg), int(comp), int(bool(sort)), C.byref(out),
+                          C.byref(sw)):
+        raise MemoryError("oracle_seed_sort failed")
+    recs = _take(out, n * sw.value).reshape(n, sw.value)
+    L.oracle_free(out)
+    return recs, sw.value
+
+
+def ref_rmsd_sort(records, panel_sizes):
+    """The reference's own rmsd_sort (lib/RSDsort.c) over reimported records,
+    one panel per contig (panel_sizes in records)."""
+    R = ref()
+    recs = np.ascontiguousarray(records, np.uint8).copy()
+    n, sw = recs.shape
+    panel = np.ascontiguousarray(np.asarray(panel_sizes, np.int64) * sw)
+    if n:
+        R.ref_rmsd_sort(_u8(recs), n, sw, len(panel), _i64(panel))
+    return recs

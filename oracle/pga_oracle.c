@@ -553,3 +553,85 @@ void oracle_result_free(oracle_result *r)
     free(r->nseeds); free(r->cseeds); free(r->nbuck); free(r->cbuck);
     memset(r, 0, sizeof(*r));
 }
+
+
+/* ---------------------------------------------------------------- seed sort */
+
+/* src/pga-mpi-merge.c:415-425: DBYTE = bytes for MAXDAG = AMXPOS + BMXPOS. */
+int oracle_dbyte(int64_t maxctg)
+{
+    const int64_t maxdag = 2 * maxctg;
+    int d = 0;
+    int64_t cum = 1;
+    while (cum < maxdag) { cum *= 256; d++; }
+    return d;
+}
+
+static int O_SWIDE;
+
+/* rmsd_sort's order: the whole record as a number, last byte most significant. */
+static int cmp_from_end(const void *a, const void *b)
+{
+    const uint8_t *x = (const uint8_t *) a, *y = (const uint8_t *) b;
+    for (int k = O_SWIDE - 1; k >= 0; k--)
+        if (x[k] != y[k]) return x[k] < y[k] ? -1 : 1;
+    return 0;
+}
+
+static int64_t le_get(const uint8_t *p, int n)
+{
+    int64_t v = 0;
+    for (int k = n - 1; k >= 0; k--) v = (v << 8) | p[k];
+    return v;
+}
+
+static void le_put(uint8_t *p, int64_t v, int n)
+{
+    for (int k = 0; k < n; k++) p[k] = (uint8_t) (v >> (8 * k));
+}
+
+int oracle_seed_sort(const uint8_t *seeds, int64_t nseeds, int P, int C, int ncontig, int64_t maxctg,
+                     int comp, int sort, uint8_t **out, int *swide_out)
+{
+    const int s = 1 + 2 * (P + C);
+    const int dbyte = oracle_dbyte(maxctg);
+    const int swide = 2 * dbyte + C + 2;
+    const int64_t amx = maxctg, bmx = maxctg, maxdag = amx + bmx;
+    const int64_t mask = ((int64_t) 1 << (8 * C - 1)) - 1;
+    *swide_out = swide;
+    int64_t *start = (int64_t *) calloc((size_t) ncontig + 1, sizeof(int64_t));
+    uint8_t *arr = (uint8_t *) malloc((size_t) (nseeds * swide + 1));
+    if (!start || !arr) { free(start); free(arr); return -1; }
+    /* panel sizes, then offsets (the bucket prefix of pair_sort_search) */
+    for (int64_t i = 0; i < nseeds; i++) start[le_get(seeds + i * s + 1 + P, C) + 1]++;
+    for (int c = 0; c < ncontig; c++) start[c + 1] += start[c];
+    int64_t *fill = (int64_t *) malloc(sizeof(int64_t) * (size_t) (ncontig + 1));
+    if (!fill) { free(start); free(arr); return -1; }
+    memcpy(fill, start, sizeof(int64_t) * (size_t) (ncontig + 1));
+    for (int64_t i = 0; i < nseeds; i++) {
+        const uint8_t *b = seeds + i * s;
+        const int lcp = b[0];
+        const int64_t ipost = le_get(b + 1, P), icont = le_get(b + 1 + P, C);
+        const int64_t jpost = le_get(b + 1 + P + C, P), jcont = le_get(b + 1 + 2 * P + C, C) & mask;
+        int64_t diag, anti;
+        if (comp) { diag = maxdag - (ipost + jpost); anti = amx - (ipost - jpost); }
+        else      { diag = bmx + (ipost - jpost);    anti = ipost + jpost; }
+        const int64_t band = diag >> 6;                       /* BUCK_SHIFT */
+        uint8_t *x = arr + swide * fill[icont]++;
+        x[0] = (uint8_t) lcp;
+        x[1] = (uint8_t) (diag - (band << 6));
+        le_put(x + 2, anti, dbyte);
+        le_put(x + 2 + dbyte, band, dbyte);
+        le_put(x + 2 + 2 * dbyte, jcont, C);
+    }
+    if (sort) {
+        O_SWIDE = swide;
+        for (int c = 0; c < ncontig; c++)
+            if (start[c + 1] - start[c] > 1)
+                qsort(arr + start[c] * swide, (size_t) (start[c + 1] - start[c]), (size_t) swide, cmp_from_end);
+    }
+    free(start);
+    free(fill);
+    *out = arr;
+    return 0;
+}

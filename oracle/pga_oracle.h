@@ -33,6 +33,9 @@
  *                     workload inputs.  The compiled reference merge is not
  *                     available here (its translation units need MPI), so no
  *                     reference-run vector pins it.
+ *   seed_sort         reimport closed forms on hand-built seeds (both streams);
+ *                     the per-panel sort == the compiled reference rmsd_sort
+ *                     (lib/RSDsort.c) run on the same reimported records.
  */
 #ifndef PGA_ORACLE_H
 #define PGA_ORACLE_H
@@ -105,6 +108,18 @@ int  oracle_build_adapter(const uint8_t *rec, int64_t n, int P, int C,
 int  oracle_merge(const uint8_t *entries, int64_t n, int P, int C, int ncontig,
                   int freq, int mode, uint8_t **nseeds, int64_t *nlen,
                   uint8_t **cseeds, int64_t *clen, int64_t *nbuck, int64_t *cbuck);
+
+/* Diagonal records of one seed stream, sorted per anchor-contig panel: the
+ * reimport of src/fastga_pipeline.c:1391-1450 (comp 0 for the N stream,
+ * comp 1 for the C stream) with AMXPOS = BMXPOS = maxctg, MAXDAG = 2*maxctg
+ * and DBYTE from src/pga-mpi-merge.c:415-425, then each panel sorted on the
+ * whole record read from its last byte (rmsd_sort, lib/RSDsort.c:292-377, as
+ * pair_sort_search drives it, src/fastga_pipeline.c:1779-1822).  Records:
+ * [lcp][diag&63][anti DBYTE LE][band DBYTE LE][jcont C LE], swide = 2*DBYTE+C+2.
+ * *out is malloc'd (nseeds*swide bytes, panels in contig order). */
+int  oracle_dbyte(int64_t maxctg);
+int  oracle_seed_sort(const uint8_t *seeds, int64_t nseeds, int P, int C, int ncontig, int64_t maxctg,
+                      int comp, int sort, uint8_t **out, int *swide);
 
 /* Whole path: extract (rolling) -> sort -> LCP -> adapter -> merge. */
 int  oracle_pipeline(const uint8_t *bps, const int64_t *boff, const int64_t *clen, int ncontig,

@@ -4,7 +4,7 @@
 # oracle/_ref/libpgaref.so.  Test infrastructure only; the output is git-ignored.
 #
 # Files: src/{extract,pack,tables,work_split,record,sort,kmer_adapter}.c and the
-# vendored lib/{GDB,gene_core,ONElib,ANO,MSDsort}.c, with the reference
+# vendored lib/{GDB,gene_core,ONElib,ANO,MSDsort,RSDsort}.c, with the reference
 # Makefile's flags (-O3 -fno-strict-aliasing -DLCPs).  The merge
 # (src/fastga_pipeline.c) is left out: it links src/seed_buffer.c, which
 # includes <mpi.h>, and no MPI is installed.
@@ -24,7 +24,7 @@ gcc -O3 -fPIC -shared -w -fno-strict-aliasing -DLCPs \
     "$REF/src/extract.c" "$REF/src/pack.c" "$REF/src/tables.c" "$REF/src/work_split.c" \
     "$REF/src/record.c" "$REF/src/sort.c" "$REF/src/kmer_adapter.c" \
     "$REF/lib/GDB.c" "$REF/lib/gene_core.c" "$REF/lib/ONElib.c" "$REF/lib/ANO.c" \
-    "$REF/lib/MSDsort.c" \
+    "$REF/lib/MSDsort.c" "$REF/lib/RSDsort.c" \
     -lpthread -lm -lz
 mv "$OUT/libpgaref.so.tmp" "$OUT/libpgaref.so"
 echo "ref_build: $OUT/libpgaref.so"

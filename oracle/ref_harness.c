@@ -113,3 +113,23 @@ void ref_result_free(ref_result *r)
 
 /* The reference's packers and LCP on raw bytes, for the known-answer tests. */
 int ref_lcp_bases(const uint8_t *a, const uint8_t *b) { return simple_kmer_lcp_bases(a, b, 10); }
+
+
+/* The reference's seed sort (lib/RSDsort.c), declared the way its own caller
+ * declares it (src/fastga_pipeline.c:27-28, 124-128), run on records the
+ * oracle reimported.  panel_bytes[c] = bytes of contig c's panel. */
+typedef struct {
+    int   beg;
+    int   end;
+    int64 off;
+} Range;
+
+extern int rmsd_sort(uint8 *array, int64 nelem, int rsize, int ksize, int nparts, int64 *part,
+                     int nthreads, Range *parms);
+
+int ref_rmsd_sort(uint8_t *arr, int64_t nelem, int swide, int nparts, int64_t *panel_bytes)
+{
+    Range r[1];
+    if (nelem <= 0) return 0;
+    return rmsd_sort(arr, nelem, swide, swide, nparts, panel_bytes, 1, r);
+}
