@@ -264,6 +264,21 @@ def main():
                                        ms_per_step=round(per["scatter_ms"], 4)),
                     path=dict(algorithmic_bytes=int(balg), achieved=round(path_gbs, 1),
                               frac=round(path_gbs / HBM_PEAK_GBS, 4)))
+    # the next step downstream: the diagonal-sorted seed array of both streams
+    ss = {}
+    for comp, name in ((0, "N"), (1, "C")):
+        ctx.seed_sort(0, comp)                      # warm the buffers
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            n = ctx._seed_sort_only(0, comp)
+        ss[name] = dict(records=n, ms=round((time.perf_counter() - t0) / reps * 1e3, 3))
+    ss_seeds = sum(v["records"] for v in ss.values())
+    ss_ms = sum(v["ms"] for v in ss.values())
+    seed_sort = dict(streams=ss, records=ss_seeds, ms=round(ss_ms, 3),
+                     mrecords_s=round(ss_seeds / (ss_ms * 1e-3) / 1e6, 1) if ss_ms > 0 else None,
+                     note="reimport + per-panel sort of both seed streams, records left on the device")
+    log(f"[bench] seed sort: {ss_seeds} records in {ss_ms:.2f} ms")
     ctx.close()
 
     out = {"metric": METRIC, "value": round(value, 4), "unit": "Gbases/s", "n_gpus": 1,
@@ -280,7 +295,8 @@ def main():
            "stages_ms": {k: round(v, 4) for k, v in per.items()},
            "h2d_load_s": round(h2d_s, 4),
            "pcie_inclusive_gbases_s": round(g.seqtot / (ms_step * 1e-3 + h2d_s) / 1e9, 4),
-           "roofline": roofline}
+           "roofline": roofline,
+           "seed_sort": seed_sort}
 
     if not args.no_batch:
         t = time.perf_counter()

@@ -209,6 +209,13 @@ class Context:
         self._check(self._L.pga_seed_sort_export(self._h, out.ctypes.data_as(_u8p)), "pga_seed_sort_export")
         return out[: n.value * sw.value].reshape(n.value, sw.value)
 
+    def _seed_sort_only(self, pair=0, comp=0):
+        """pga_seed_sort without the copy to the host (timing); returns the record count."""
+        n = C.c_int64()
+        sw = C.c_int()
+        self._check(self._L.pga_seed_sort(self._h, pair, comp, C.byref(n), C.byref(sw)), "pga_seed_sort")
+        return n.value
+
     def timings(self):
         t = Timings()
         self._check(self._L.pga_timings_get(self._h, C.byref(t)), "pga_timings_get")
