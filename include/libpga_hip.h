@@ -178,7 +178,7 @@ int pga_sync(pga_ctx *ctx);
  * that combined GDB.  The buck vectors count those seeds per anchor contig
  * (target contigs first).  Each query is answered independently of the others
  * in its batch.  Limits: freq <= 24; target + query contigs <= 32767; contig
- * lengths < 2^32.
+ * lengths < 2^31.
  *
  * A server has 1..8 slots.  A slot answers one batch at a time on its own HIP
  * stream; different slots may be driven from different host threads at once
