@@ -1314,16 +1314,12 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
 {
     constexpr int TA = MTH * A;
     constexpr int W = TA + 2 * MAX_FREQ + 4;
-    constexpr int FRONT = (MAX_FREQ + 2) * 16;   // room for entries left of a clipped window
-    constexpr int ENT_BYTES = FRONT + W * 16 + 16;
     constexpr int SLOT_BYTES = (SLOT_N + SLOT_C) * 16;
     __shared__ uint8_t win[W];
     __shared__ uint64_t pay[W];
-    // raw entry bytes until the payloads are extracted, then the seed slots
-    __shared__ __align__(16) uint8_t shared_buf[ENT_BYTES > SLOT_BYTES ? ENT_BYTES : SLOT_BYTES];
+    __shared__ __align__(16) uint8_t shared_buf[SLOT_BYTES];
     __shared__ uint32_t lb[2 * BUCK_LDS];
     __shared__ uint32_t sh[MTH / 64 + 1];
-    uint8_t *entp = shared_buf;
     u128 *slotN = reinterpret_cast<u128 *>(shared_buf);
     u128 *slotC = slotN + SLOT_N;
     for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MTH) lb[c] = 0;
@@ -1344,29 +1340,20 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         const int64_t end = pd.rec0 + pd.nrec;
         const int H = freq + 1;
         const int pc = pd.P + pd.C, e = pd.e, s = pd.s;
-        // window [lo-H, lo+TA+H) clipped to the pair; raw entry bytes with aligned 4-byte loads and stores
+        // window [lo-H, lo+TA+H) clipped to the pair: LCP/strand bytes, and one
+        // payload per entry read straight from the GIX entries (L2-resident slices)
         const int64_t wlo = lo - H > pd.rec0 ? lo - H : pd.rec0;
         const int64_t whi = lo + TA + H < end ? lo + TA + H : end;
-        const int64_t b0 = pd.ent_off + (wlo - pd.rec0) * (int64_t) e;
-        const int64_t b1 = pd.ent_off + (whi - pd.rec0) * (int64_t) e;
-        const int64_t a0 = b0 & ~(int64_t) 3;
-        const int nwords = (int) ((b1 - a0 + 3) >> 2);
         __syncthreads();
         load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
         {
-            const uint32_t *src = reinterpret_cast<const uint32_t *>(entries + a0);
-            uint32_t *dst = reinterpret_cast<uint32_t *>(entp + FRONT);
-            for (int k = threadIdx.x; k < nwords; k += MTH) dst[k] = src[k];
-        }
-        __syncthreads();
-        {
-            // window entry k (global lo-H+k) starts at ew + k*e; only [wlo, whi) is present
-            const uint8_t *ew = entp + FRONT + (int) (b0 - a0) - (int) (wlo - (lo - H)) * e;
+            // window entry k (global lo-H+k) starts at ew + k*e; only [wlo, whi) is read
+            const uint8_t *ew = entries + pd.ent_off + (lo - H - pd.rec0) * (int64_t) e;
             const int k0 = (int) (wlo - (lo - H)), k1 = (int) (whi - (lo - H));
             for (int k = threadIdx.x; k < TA + 2 * H; k += MTH)
                 pay[k] = (k >= k0 && k < k1) ? payload_at(ew, k, e, pc) : 0;
         }
-        __syncthreads();                             // entry bytes are dead from here: slots reuse them
+        __syncthreads();
         int pl[A], aa[A], bb[A];
         uint32_t nN[A], nC[A];
         uint32_t sN = 0, sC = 0;
