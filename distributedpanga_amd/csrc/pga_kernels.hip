@@ -752,17 +752,16 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         // scatter into bins, then every record counts the keys of its bin that
         // come before it (smaller, or equal and scattered earlier: the seed sort
         // has equal keys) and moves to bin start + that rank
-        uint32_t bn[E];
 #pragma unroll
         for (int k = 0; k < E; k++)
-            if (k * T + tid < n) { bn[k] = bin_of(r[k].w0, r[k].w1); sk[bh[bn[k]] + rk[k]] = r[k]; }
+            if (k * T + tid < n) sk[bh[bin_of(r[k].w0, r[k].w1)] + rk[k]] = r[k];
         __syncthreads();
-        uint32_t pos[E];
 #pragma unroll
         for (int k = 0; k < E; k++) {
             if (k * T + tid >= n) continue;
-            const uint32_t st = bh[bn[k]];
-            const uint32_t en = bn[k] + 1 < (uint32_t) NB ? bh[bn[k] + 1] : (uint32_t) n;
+            const uint32_t b = bin_of(r[k].w0, r[k].w1);
+            const uint32_t st = bh[b];
+            const uint32_t en = b + 1 < (uint32_t) NB ? bh[b + 1] : (uint32_t) n;
             const uint32_t me = st + rk[k];
             uint32_t rank = 0;
             for (uint32_t i = st; i < en; i++) {
@@ -770,12 +769,12 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                 rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && (y.w1 < r[k].w1 || (y.w1 == r[k].w1 && i < me))))
                         ? 1u : 0u;
             }
-            pos[k] = st + rank;
+            rk[k] = st + rank;                          // the record's place in the tile
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < E; k++)
-            if (k * T + tid < n) sk[pos[k]] = r[k];
+            if (k * T + tid < n) sk[rk[k]] = r[k];
     } else {
 #pragma unroll
         for (int k = 0; k < E; k++) {
