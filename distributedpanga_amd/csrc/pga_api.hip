@@ -923,9 +923,9 @@ int pga_merge(pga_ctx *c)
         }
     } arena{{P<uint8_t>(c->rec[0]), P<uint8_t>(c->rec[1])}, {c->rec[0].cap, c->rec[1].cap}, {0, 0}};
     // the count pass hands each anchor's interval and seed counts to the emit
-    // pass when they fit the 32-bit record (intervals of fewer than 32 entries)
+    // pass in a 32-bit record per anchor
     uint32_t *minfo = nullptr;
-    if (c->o.freq <= 31 && c->R > 0) {
+    if (c->R > 0) {
         minfo = static_cast<uint32_t *>(arena.take((size_t) c->R * sizeof(uint32_t)));
         if (!minfo) {
             TRY(ensure(c, c->minfo, (size_t) c->R * sizeof(uint32_t)));
@@ -1002,7 +1002,7 @@ int pga_merge(pga_ctx *c)
     if (nmb > 0) {
 #define PGA_EMIT(A_)                                                                                    \
     hipLaunchKernelGGL(k_merge_emit<A_>, dim3((unsigned) egrid), dim3(MTH), 0, c->st,                   \
-                       (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
+                       (const PairDev *) P<PairDev>(c->dpairs), npairs,                                 \
                        (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
                        (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
                        c->seedN_p, c->seedC_p, P<unsigned long long>(c->nbuck),                         \

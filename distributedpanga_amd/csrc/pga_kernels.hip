@@ -137,6 +137,19 @@ __device__ __forceinline__ T block_excl_scan(T v, T *sh, T *total)
 
 typedef unsigned __int128 u128;
 
+// A record through a pointer that was itself read from device memory (the
+// compiler cannot tell its address space, and would use a flat load, which
+// also counts against the LDS wait counter): load it as global memory.
+__device__ __forceinline__ Rec load_rec_global(const Rec *p)
+{
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+    const u64x2 v = *(const __attribute__((address_space(1))) u64x2 *) p;
+    Rec r;
+    r.w0 = v.x;
+    r.w1 = v.y;
+    return r;
+}
+
 // Write n fixed-width items (s <= 16 bytes each, item j held little-endian in
 // slot[j]) as one packed byte run starting at g: every lane assembles one
 // aligned 16-byte chunk of the run from the (at most three) slots it
@@ -145,10 +158,12 @@ template <int NT>
 __device__ __forceinline__ void write_packed(uint8_t *__restrict__ g, const u128 *__restrict__ slot, int n, int s)
 {
     if (n <= 0) return;
-    const uintptr_t base = (uintptr_t) g;
-    const uintptr_t a0 = base & ~(uintptr_t) 15;
     const int nb = n * s;
-    const int lead = (int) (base - a0);                 // the run starts at this offset of chunk 0
+    const int lead = (int) ((uintptr_t) g & 15);        // the run starts at this offset of chunk 0
+    // Addresses derive from g by pointer arithmetic (never from an integer) so
+    // that the stores stay global ones: flat stores would also count against
+    // the LDS wait counter and make every LDS wait wait for them.
+    uint8_t *const a0 = g - lead;
     const int nchunks = (lead + nb + 15) >> 4;
     for (int c = threadIdx.x; c < nchunks; c += NT) {
         const int o0 = 16 * c - lead;                   // run offset of the chunk's first byte
@@ -162,7 +177,7 @@ __device__ __forceinline__ void write_packed(uint8_t *__restrict__ g, const u128
             filled += s;
         }
         if (o0 < 0) acc <<= 8 * (-o0);                  // chunk 0 starts before the run
-        uint8_t *dst = reinterpret_cast<uint8_t *>(a0 + 16 * (uintptr_t) c);
+        uint8_t *dst = a0 + 16 * c;
         if (o0 >= 0 && o0 + 16 <= nb) {
             uint4 w;
             const uint64_t lo = (uint64_t) acc, hi = (uint64_t) (acc >> 64);
@@ -694,7 +709,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     for (int k = 0; k < E; k++) {
         const int i = k * T + tid;
         if (i < n) {
-            r[k] = src[i];
+            r[k] = load_rec_global(src + i);
             if (r[k].w0 < lo0 || (r[k].w0 == lo0 && r[k].w1 < lo1)) { lo0 = r[k].w0; lo1 = r[k].w1; }
             if (r[k].w0 > hi0 || (r[k].w0 == hi0 && r[k].w1 > hi1)) { hi0 = r[k].w0; hi1 = r[k].w1; }
         }
@@ -1212,8 +1227,6 @@ __device__ __forceinline__ int merge_block_pair(const PairDev *__restrict__ pair
 
 constexpr int MTH = 256;             // threads of the merge kernels
 constexpr int BUCK_LDS = 1024;       // per-pair contigs counted in LDS by the merge
-constexpr int SLOT_N = 768;          // seeds staged in LDS per round: same-strand stream
-constexpr int SLOT_C = 768;          //                                cross-strand stream
 
 // LCP/strand bytes of window [w0, w0+n) of the pair into win (0 outside it).
 __device__ __forceinline__ void load_win(uint8_t *__restrict__ win, const uint8_t *__restrict__ ls,
@@ -1223,6 +1236,15 @@ __device__ __forceinline__ void load_win(uint8_t *__restrict__ win, const uint8_
         const int64_t g = w0 + k;
         win[k] = (g >= rec0 && g < end) ? ls[g] : 0;
     }
+}
+
+// Per-anchor interval record handed from the count pass to the emit pass, six
+// bits a field (FREQ <= 64 keeps every field below 64): plen, the interval's
+// start and end relative to the anchor (gi - a, b - gi), and the anchor's
+// same-strand and cross-strand partner counts.  0 = the anchor emits nothing.
+__device__ __forceinline__ uint32_t minfo_pack(int plen, int aoff, int boff, uint32_t nn, uint32_t nc)
+{
+    return (uint32_t) plen | (uint32_t) aoff << 6 | (uint32_t) boff << 12 | nn << 18 | nc << 24;
 }
 
 // Seed counts of the tiles of A*MTH anchors (A consecutive anchors per thread).
@@ -1259,9 +1281,9 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
             }
             nN += xn;
             nC += xc;
-            info = (uint32_t) plen | (uint32_t) (gi - a) << 6 | (uint32_t) (b - gi) << 11 | xn << 16 | xc << 21;
+            info = minfo_pack(plen, gi - a, b - gi, xn, xc);
         }
-        if (minfo && g < end) minfo[g] = info;
+        if (g < end) minfo[g] = info;
     }
     uint64_t sn = block_sum_u64<MTH>(nN, red);
     uint64_t sc = block_sum_u64<MTH>(nC, red);
@@ -1280,28 +1302,50 @@ __device__ __forceinline__ void flush_buck(uint32_t *__restrict__ lb, int n, int
     }
 }
 
-// Payload (pc <= 7 bytes, little-endian) of window entry k.
-__device__ __forceinline__ uint64_t payload_at(const uint8_t *__restrict__ ew, int k, int e, int pc)
+constexpr int OB = 12288;            // bytes of each LDS output run (N and C streams)
+
+// Bytes [s-8, s) and [0, 8) of a seed whose first 1+pc bytes are hd (plen,
+// then the anchor payload) and whose last pc bytes are pq (the partner
+// payload), for pc >= 4.  Both 8-byte words lie inside the seed, so two
+// stores of them write exactly its s = 1 + 2 pc bytes.
+__device__ __forceinline__ void seed_words(uint64_t hd, uint64_t pq, int pc, uint64_t &w0, uint64_t &w1)
 {
-    const uint8_t *p = ew + k * e + 9;
-    uint64_t v = 0;
+    w0 = hd | ((pq << (8 * pc)) << 8);
+    w1 = (hd >> (8 * (2 * pc - 7))) | (pq << (8 * (8 - pc)));
+}
+
+// Copy the packed run of nb bytes staged at ob + lead (lead = g & 15) to g:
+// aligned 16-byte chunks go out as vector stores, the partial chunks at the
+// two ends byte by byte (their other bytes belong to neighbouring runs).
+__device__ __forceinline__ void copy_run(uint8_t *__restrict__ g, const uint8_t *__restrict__ ob, int lead, int nb)
+{
+    if (nb <= 0) return;
+    uint8_t *const a0 = g - lead;
+    const int end = lead + nb;
+    const int nchunks = (end + 15) >> 4;
+    for (int c = threadIdx.x; c < nchunks; c += MTH) {
+        const int o = 16 * c;
+        if (o >= lead && o + 16 <= end) {
+            *reinterpret_cast<uint4 *>(a0 + o) = *reinterpret_cast<const uint4 *>(ob + o);
+        } else {
 #pragma unroll
-    for (int t = 0; t < 7; t++)
-        if (t < pc) v |= (uint64_t) p[t] << (8 * t);
-    return v;
+            for (int b = 0; b < 16; b++)
+                if (o + b >= lead && o + b < end) a0[o + b] = ob[o + b];
+        }
+    }
 }
 
 // Seeds of a run of merge tiles (A*MTH anchors each, A consecutive per
-// thread).  Per tile: the LCP/strand bytes and raw GIX entries of the tile plus
-// a FREQ halo on each side go to LDS, then one 64-bit payload per entry; each
-// anchor finds its interval, block scans place the anchors' seeds in the
-// tile's slice of the N and C streams (offsets from the count pass).  Each
-// round, every seed is composed in a 16-byte LDS slot by its anchor's thread,
-// then all lanes alike assemble and store the output 16 bytes at a time.
+// thread), from the interval records of the count pass.  Per tile: one 64-bit
+// payload per entry of the tile plus a FREQ halo on each side goes to LDS
+// (the strand is the payload's top bit); a block scan places each thread's
+// seeds in the tile's slices of the N and C streams (tile offsets from the
+// count pass).  Each round, every anchor's thread writes its seeds' exact
+// bytes into two LDS runs laid out with the same 16-byte phase as their
+// destinations, and all lanes copy the runs out in aligned 16-byte stores.
 // Per-anchor-contig counts accumulate in LDS while the pair stays the same.
 template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
-                                                    const uint8_t *__restrict__ ls,
                                                     const uint8_t *__restrict__ entries, int freq,
                                                     const int64_t *__restrict__ ofsN,
                                                     const int64_t *__restrict__ ofsC,
@@ -1313,14 +1357,10 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
 {
     constexpr int TA = MTH * A;
     constexpr int W = TA + 2 * MAX_FREQ + 4;
-    constexpr int SLOT_BYTES = (SLOT_N + SLOT_C) * 16;
-    __shared__ uint8_t win[W];
     __shared__ uint64_t pay[W];
-    __shared__ __align__(16) uint8_t shared_buf[SLOT_BYTES];
+    __shared__ __align__(16) uint8_t obuf[2][OB + 16];
     __shared__ uint32_t lb[2 * BUCK_LDS];
     __shared__ uint32_t sh[MTH / 64 + 1];
-    u128 *slotN = reinterpret_cast<u128 *>(shared_buf);
-    u128 *slotC = slotN + SLOT_N;
     for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MTH) lb[c] = 0;
     const int64_t t0 = (int64_t) blockIdx.x * per_block;
     const int64_t t1 = t0 + per_block < nmb ? t0 + per_block : nmb;
@@ -1339,96 +1379,99 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         const int64_t end = pd.rec0 + pd.nrec;
         const int H = freq + 1;
         const int pc = pd.P + pd.C, e = pd.e, s = pd.s;
-        // window [lo-H, lo+TA+H) clipped to the pair: LCP/strand bytes, and one
-        // payload per entry read straight from the GIX entries (L2-resident slices)
-        const int64_t wlo = lo - H > pd.rec0 ? lo - H : pd.rec0;
-        const int64_t whi = lo + TA + H < end ? lo + TA + H : end;
-        __syncthreads();
-        load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
-        {
-            // window entry k (global lo-H+k) starts at ew + k*e; only [wlo, whi) is read
-            const uint8_t *ew = entries + pd.ent_off + (lo - H - pd.rec0) * (int64_t) e;
-            const int k0 = (int) (wlo - (lo - H)), k1 = (int) (whi - (lo - H));
-            for (int k = threadIdx.x; k < TA + 2 * H; k += MTH)
-                pay[k] = (k >= k0 && k < k1) ? payload_at(ew, k, e, pc) : 0;
+        const int sbit = 8 * pc - 1;
+        const uint64_t pmask = (1ull << (8 * pc)) - 1;
+        // interval records of this thread's anchors (0 past the pair's end)
+        uint32_t v[A];
+#pragma unroll
+        for (int k = 0; k < A; k++) {
+            const int64_t g = lo + threadIdx.x * A + k;
+            v[k] = g < end ? minfo[g] : 0u;
         }
-        __syncthreads();
-        int pl[A], aa[A], bb[A];
-        uint32_t nN[A], nC[A];
+        __syncthreads();                                 // the previous tile is done with pay[]
+        {
+            // window entry k is global entry lo-H+k; only those inside the pair
+            // are read (intervals never leave the pair).  The 8-byte load at
+            // the payload may run past an entry: the entries buffer is padded.
+            const int64_t wlo = lo - H > pd.rec0 ? lo - H : pd.rec0;
+            const int64_t whi = lo + TA + H < end ? lo + TA + H : end;
+            const uint8_t *ew = entries + pd.ent_off + (lo - H - pd.rec0) * (int64_t) e + 9;
+            const int k0 = (int) (wlo - (lo - H)), k1 = (int) (whi - (lo - H));
+            for (int k = k0 + (int) threadIdx.x; k < k1; k += MTH) {
+                uint64_t x;
+                __builtin_memcpy(&x, ew + (int64_t) k * e, 8);
+                pay[k] = x & pmask;
+            }
+        }
         uint32_t sN = 0, sC = 0;
 #pragma unroll
         for (int k = 0; k < A; k++) {
-            const int gi = H + threadIdx.x * A + k;
-            const int64_t g = lo + threadIdx.x * A + k;
-            nN[k] = nC[k] = 0;
-            pl[k] = 0; aa[k] = bb[k] = gi;
-            if (minfo) {                                 // intervals found by the count pass
-                const uint32_t v = g < end ? minfo[g] : 0u;
-                if (v) {
-                    pl[k] = (int) (v & 63u);
-                    aa[k] = gi - (int) ((v >> 6) & 31u);
-                    bb[k] = gi + (int) ((v >> 11) & 31u);
-                    nN[k] = (v >> 16) & 31u;
-                    nC[k] = (v >> 21) & 31u;
-                }
-            } else if (g < end && anchor_interval(win, gi, freq, pl[k], aa[k], bb[k])) {
-                const uint8_t si = win[gi] & 0x80;
-                for (int q = aa[k]; q < bb[k]; q++) {
-                    if (q == gi) continue;
-                    if ((win[q] & 0x80) == si) nN[k]++; else nC[k]++;
-                }
-            } else {
-                aa[k] = bb[k] = gi;
-            }
-            sN += nN[k];
-            sC += nC[k];
+            sN += (v[k] >> 18) & 63u;
+            sC += (v[k] >> 24) & 63u;
         }
-        uint32_t totN, totC;
-        const uint32_t oN = block_excl_scan<MTH, uint32_t>(sN, sh, &totN);
-        const uint32_t oC = block_excl_scan<MTH, uint32_t>(sC, sh, &totC);
-        const int64_t rounds = max(((int64_t) totN + SLOT_N - 1) / SLOT_N, ((int64_t) totC + SLOT_C - 1) / SLOT_C);
+        uint32_t tot;
+        const uint32_t off = block_excl_scan<MTH, uint32_t>(sN | sC << 16, sh, &tot);   // also orders pay[]
+        const int totN = (int) (tot & 0xFFFFu), totC = (int) (tot >> 16);
+        const int capN = OB / s;
+        const int rounds = max((totN + capN - 1) / capN, (totC + capN - 1) / capN);
         uint8_t *gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
         uint8_t *gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
-        const uint64_t clear = ~(0x80ull << (8 * (pc - 1)));
-        for (int64_t r = 0; r < rounds; r++) {
-            const int64_t n0 = r * SLOT_N, n1 = min(n0 + SLOT_N, (int64_t) totN);
-            const int64_t c0 = r * SLOT_C, c1 = min(c0 + SLOT_C, (int64_t) totC);
-            int64_t kn = oN, kc = oC;
+        const uint64_t clear = ~(1ull << sbit);
+        for (int r = 0; r < rounds; r++) {
+            const int n0 = r * capN, n1 = min(n0 + capN, totN);
+            const int c0 = r * capN, c1 = min(c0 + capN, totC);
+            const int leadN = (int) ((uintptr_t) (gN + (int64_t) n0 * s) & 15);
+            const int leadC = (int) ((uintptr_t) (gC + (int64_t) c0 * s) & 15);
+            int kn = (int) (off & 0xFFFFu), kc = (int) (off >> 16);
 #pragma unroll
             for (int k = 0; k < A; k++) {
-                if (nN[k] + nC[k] == 0) continue;
+                if (!(v[k] >> 18)) continue;
                 const int gi = H + threadIdx.x * A + k;
-                const uint8_t si = win[gi] & 0x80;
-                const u128 head = (u128) (uint32_t) pl[k] | ((u128) (pay[gi] & clear) << 8);
-                const int sh2 = 8 + 8 * pc;
-                for (int q = aa[k]; q < bb[k]; q++) {
+                const uint64_t pi = pay[gi];
+                const uint64_t si = (pi >> sbit) & 1;
+                const uint64_t hd = (uint64_t) (v[k] & 63u) | ((pi & clear) << 8);
+                const int qa = gi - (int) ((v[k] >> 6) & 63u), qb = gi + (int) ((v[k] >> 12) & 63u);
+                for (int q = qa; q < qb; q++) {
                     if (q == gi) continue;
-                    if ((win[q] & 0x80) == si) {
-                        if (kn >= n0 && kn < n1) slotN[kn - n0] = head | ((u128) pay[q] << sh2);
-                        kn++;
+                    const uint64_t pq = pay[q];
+                    const bool same = ((pq >> sbit) & 1) == si;
+                    const int idx = same ? kn - n0 : kc - c0;
+                    const int lim = same ? n1 - n0 : c1 - c0;
+                    kn += same;
+                    kc += !same;
+                    if (idx < 0 || idx >= lim) continue;
+                    uint8_t *dst = obuf[same ? 0 : 1] + (same ? leadN : leadC) + idx * s;
+                    if (pc >= 4) {
+                        uint64_t w0, w1;
+                        seed_words(hd, pq, pc, w0, w1);
+                        __builtin_memcpy(dst, &w0, 8);
+                        __builtin_memcpy(dst + s - 8, &w1, 8);
                     } else {
-                        if (kc >= c0 && kc < c1) slotC[kc - c0] = head | ((u128) pay[q] << sh2);
-                        kc++;
+                        const uint64_t w = hd | (pq << (8 * (1 + pc)));
+                        const uint32_t x0 = (uint32_t) w, x1 = (uint32_t) (w >> (8 * (s - 4)));
+                        __builtin_memcpy(dst, &x0, 4);
+                        __builtin_memcpy(dst + s - 4, &x1, 4);
                     }
                 }
             }
             __syncthreads();
-            write_packed<MTH>(gN + n0 * s, slotN, (int) (n1 - n0), s);
-            write_packed<MTH>(gC + c0 * s, slotC, (int) (c1 - c0), s);
+            copy_run(gN + (int64_t) n0 * s, obuf[0], leadN, (n1 - n0) * s);
+            copy_run(gC + (int64_t) c0 * s, obuf[1], leadC, (c1 - c0) * s);
             __syncthreads();
         }
         // per-anchor-contig counts (src/fastga_pipeline.c:403)
 #pragma unroll
         for (int k = 0; k < A; k++) {
-            if (nN[k] + nC[k] == 0) continue;
+            if (!(v[k] >> 18)) continue;
+            const uint32_t nn = (v[k] >> 18) & 63u, nc = (v[k] >> 24) & 63u;
             const int gi = H + threadIdx.x * A + k;
             const uint64_t icont = (pay[gi] >> (8 * pd.P)) & ((1ull << (8 * pd.C - 1)) - 1);
             if (lds_buck) {
-                if (nN[k]) atomicAdd(&lb[icont], nN[k]);
-                if (nC[k]) atomicAdd(&lb[BUCK_LDS + icont], nC[k]);
+                if (nn) atomicAdd(&lb[icont], nn);
+                if (nc) atomicAdd(&lb[BUCK_LDS + icont], nc);
             } else {
-                if (nN[k]) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nN[k]);
-                if (nC[k]) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nC[k]);
+                if (nn) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nn);
+                if (nc) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nc);
             }
         }
     }
@@ -1565,7 +1608,7 @@ __global__ __launch_bounds__(SP_T) void k_seed_pack(const Rec *__restrict__ keys
 #define PGA_MERGE_INST(A_)                                                                              \
     template __global__ void k_merge_count<A_>(const PairDev *, int, const uint8_t *, int, uint32_t *, \
                                                uint32_t *, uint32_t *, int64_t);                        \
-    template __global__ void k_merge_emit<A_>(const PairDev *, int, const uint8_t *, const uint8_t *,   \
+    template __global__ void k_merge_emit<A_>(const PairDev *, int, const uint8_t *,                    \
                                               int, const int64_t *, const int64_t *, uint8_t *,         \
                                               uint8_t *, unsigned long long *, unsigned long long *,    \
                                               const uint32_t *, int64_t, int64_t);
