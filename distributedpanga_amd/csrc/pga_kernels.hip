@@ -1247,6 +1247,61 @@ __device__ __forceinline__ uint32_t minfo_pack(int plen, int aoff, int boff, uin
     return (uint32_t) plen | (uint32_t) aoff << 6 | (uint32_t) boff << 12 | nn << 18 | nc << 24;
 }
 
+// 8 window bytes win[i..i+8) as a little-endian word (LDS reads need no alignment).
+__device__ __forceinline__ uint64_t win8(const uint8_t *__restrict__ win, int i)
+{
+    uint64_t w;
+    __builtin_memcpy(&w, win + i, 8);
+    return w;
+}
+
+// Bit 7 of byte k is set iff the LCP (bits 0..5) of byte k of w is below plen
+// (12 <= plen <= 63): (LCP | 0x80) - plen keeps bit 7 exactly when LCP >= plen,
+// and no byte borrows from its neighbour.
+__device__ __forceinline__ uint64_t lcp_below(uint64_t w, uint32_t plen)
+{
+    const uint64_t t = ((w & 0x3F3F3F3F3F3F3F3Full) | 0x8080808080808080ull) - 0x0101010101010101ull * plen;
+    return ~t & 0x8080808080808080ull;
+}
+
+// anchor_interval plus the partner counts, for FREQ <= 16, without loops: the
+// 16 window bytes on each side of the anchor are compared with plen eight at a
+// time; the run lengths sharing >= plen bases come from leading/trailing zero
+// counts and the strand split from popcounts of the strand bits (bit 7).
+// Needs 15 window bytes before gi and 16 after it.  Returns the interval
+// record (0 = no seeds).
+__device__ __forceinline__ uint32_t anchor_info16(const uint8_t *__restrict__ win, int gi, int freq,
+                                                  uint32_t &xn, uint32_t &xc)
+{
+    const uint64_t b0 = win8(win, gi - 7), b1 = win8(win, gi - 15);   // gi-7..gi, gi-15..gi-8
+    const uint64_t f0 = win8(win, gi + 1), f1 = win8(win, gi + 9);    // gi+1..gi+8, gi+9..gi+16
+    const uint32_t Li = (uint32_t) (b0 >> 56) & 63u, Ln = (uint32_t) f0 & 63u;
+    const uint32_t li = Li >= 12 ? Li : 0, ln = Ln >= 12 ? Ln : 0;
+    const uint32_t plen = li > ln ? li : ln;
+    xn = xc = 0;
+    if (plen < 12) return 0;
+    const uint64_t lb0 = lcp_below(b0, plen), lb1 = lcp_below(b1, plen);
+    const uint64_t lf0 = lcp_below(f0, plen), lf1 = lcp_below(f1, plen);
+    // entries gi, gi-1, ... sharing >= plen (cb), and gi+1, gi+2, ... (cf):
+    // the interval is [gi - cb, gi + cf + 1)
+    const int cb = lb0 ? __builtin_clzll(lb0) >> 3 : 8 + (lb1 ? __builtin_clzll(lb1) >> 3 : 8);
+    const int cf = lf0 ? __builtin_ctzll(lf0) >> 3 : 8 + (lf1 ? __builtin_ctzll(lf1) >> 3 : 8);
+    const int m = cb + 1 + cf;
+    if (m >= freq) return 0;                     // so cb, cf <= 14 below
+    // strand bits of the partners gi-cb..gi-1 and gi+1..gi+cf
+    const uint64_t HI = 0x8080808080808080ull;
+    const uint64_t mb0 = 0x00FFFFFFFFFFFFFFull & ~((1ull << (cb >= 7 ? 0 : 8 * (7 - cb))) - 1);
+    const uint64_t mb1 = cb >= 8 ? ~((1ull << (8 * (15 - cb))) - 1) : 0ull;
+    const uint64_t mf0 = cf >= 8 ? ~0ull : (1ull << (8 * cf)) - 1;
+    const uint64_t mf1 = cf > 8 ? (1ull << (8 * (cf - 8))) - 1 : 0ull;
+    const uint32_t ones = __popcll(b0 & mb0 & HI) + __popcll(b1 & mb1 & HI) +
+                          __popcll(f0 & mf0 & HI) + __popcll(f1 & mf1 & HI);
+    const uint32_t part = (uint32_t) m - 1;
+    xn = (b0 >> 63) ? ones : part - ones;
+    xc = part - xn;
+    return minfo_pack((int) plen, cb, cf + 1, xn, xc);
+}
+
 // Seed counts of the tiles of A*MTH anchors (A consecutive anchors per thread).
 template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,
@@ -1262,7 +1317,8 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
     const PairDev pd = pairs[p];
     const int64_t lo = pd.rec0 + (LB - pd.mblk0) * (int64_t) TA;
     const int64_t end = pd.rec0 + pd.nrec;
-    const int H = freq + 1;
+    const bool swar = freq <= 16;
+    const int H = swar ? 17 : freq + 1;          // window halo on each side
     load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
     __syncthreads();
     uint32_t nN = 0, nC = 0;
@@ -1270,24 +1326,28 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
     for (int k = 0; k < A; k++) {
         const int gi = H + threadIdx.x * A + k;
         const int64_t g = lo + threadIdx.x * A + k;
-        int plen, a, b;
-        uint32_t info = 0;
-        if (g < end && anchor_interval(win, gi, freq, plen, a, b)) {
-            const uint8_t si = win[gi] & 0x80;
-            uint32_t xn = 0, xc = 0;
-            for (int q = a; q < b; q++) {
-                if (q == gi) continue;
-                if ((win[q] & 0x80) == si) xn++; else xc++;
+        uint32_t info = 0, xn = 0, xc = 0;
+        if (swar) {
+            info = anchor_info16(win, gi, freq, xn, xc);
+        } else {
+            int plen, a, b;
+            if (anchor_interval(win, gi, freq, plen, a, b)) {
+                const uint8_t si = win[gi] & 0x80;
+                for (int q = a; q < b; q++) {
+                    if (q == gi) continue;
+                    if ((win[q] & 0x80) == si) xn++; else xc++;
+                }
+                info = minfo_pack(plen, gi - a, b - gi, xn, xc);
             }
+        }
+        if (g < end) {
+            minfo[g] = info;
             nN += xn;
             nC += xc;
-            info = minfo_pack(plen, gi - a, b - gi, xn, xc);
         }
-        if (g < end) minfo[g] = info;
     }
-    uint64_t sn = block_sum_u64<MTH>(nN, red);
-    uint64_t sc = block_sum_u64<MTH>(nC, red);
-    if (threadIdx.x == 0) { cntN[LB] = (uint32_t) sn; cntC[LB] = (uint32_t) sc; }
+    const uint64_t sum = block_sum_u64<MTH>((uint64_t) nN | (uint64_t) nC << 32, red);
+    if (threadIdx.x == 0) { cntN[LB] = (uint32_t) sum; cntC[LB] = (uint32_t) (sum >> 32); }
     __syncthreads();
     }
 }
@@ -1302,7 +1362,7 @@ __device__ __forceinline__ void flush_buck(uint32_t *__restrict__ lb, int n, int
     }
 }
 
-constexpr int OB = 12288;            // bytes of each LDS output run (N and C streams)
+constexpr int OB = 8192;             // bytes of each LDS output run (N and C streams)
 
 // Bytes [s-8, s) and [0, 8) of a seed whose first 1+pc bytes are hd (plen,
 // then the anchor payload) and whose last pc bytes are pq (the partner
