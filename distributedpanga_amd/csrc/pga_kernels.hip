@@ -550,36 +550,50 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
     off[threadIdx.x] = o;
     cursor[(size_t) blockIdx.x * 256 + threadIdx.x] = (unsigned long long) (sg.start + o);
     __syncthreads();
+    // One lane groups the digits (a serial greedy over 256 counts) into LDS
+    // lists; the segment then reserves its slots with one atomic per list.
+    __shared__ Tile ltile[256];
+    __shared__ Seg lnext[256];
+    __shared__ int nt, nn, tbase, nbase;
     if (threadIdx.x == 0) {
+        int t = 0, q = 0;
         int64_t ts = 0, tc = 0;
         for (int d = 0; d < 256; d++) {
-            int64_t k = cnt[d];
+            const int64_t k = cnt[d];
             if (k == 0) continue;
             if (k > cap) {
                 if (tc) {
-                    int t = atomicAdd(ntiles, 1);
-                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
+                    ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
                     tc = 0;
                 }
-                int q = atomicAdd(nnext, 1);
-                if (q < next_cap) next[q] = Seg{sg.start + off[d], k, sg.pair, sg.level + 1};
+                lnext[q++] = Seg{sg.start + off[d], k, sg.pair, sg.level + 1};
             } else {
                 if (tc + k > cap) {
-                    int t = atomicAdd(ntiles, 1);
-                    if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
+                    ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
                     tc = 0;
                 }
                 if (tc == 0) ts = sg.start + off[d];
                 tc += k;
             }
         }
-        if (tc) {
-            int t = atomicAdd(ntiles, 1);
-            if (t < tile_cap) tiles[t] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
-        }
+        if (tc) ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
+        nt = t;
+        nn = q;
+        tbase = t ? atomicAdd(ntiles, t) : 0;
+        nbase = q ? atomicAdd(nnext, q) : 0;
     }
+    __syncthreads();
+    // slots past the capacity are counted but not written (the host sees the
+    // count and grows the lists)
+    if ((int) threadIdx.x < nt && tbase + (int) threadIdx.x < tile_cap) tiles[tbase + threadIdx.x] = ltile[threadIdx.x];
+    if ((int) threadIdx.x < nn && nbase + (int) threadIdx.x < next_cap) next[nbase + threadIdx.x] = lnext[threadIdx.x];
 }
 
+// Scatter one level: each chunk of CH records is ranked by its digit in LDS,
+// staged in digit order, and written as one run per digit at the digit's
+// cursor (reserved with one atomic per digit and chunk).  The next chunk's
+// records are loaded into registers before the current chunk is written out,
+// so its reads overlap the writes.
 template <int T, int PER>
 __global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, Rec *__restrict__ dst,
                                                    const Seg *__restrict__ segs, int nseg,
@@ -594,23 +608,31 @@ __global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, 
     __shared__ int64_t gbase[256];
     __shared__ uint32_t sh[T / 64 + 1];
     const int tid = threadIdx.x;
+    Rec r[PER];
+    int s = 0, n = 0, level = 1;
+    // records of chunk lb into r[] (segment s, n records, digit level)
+    auto fetch = [&](int64_t lb) {
+        s = seg_of_chunk(chk0, nseg, lb);
+        const Seg sg = segs[s];
+        const int64_t off = (lb - chk0[s]) * (int64_t) CH;
+        n = (int) min((int64_t) CH, sg.count - off);
+        level = sg.level;
+        const Rec *p = src + sg.start + off;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = k * T + tid;
+            if (i < n) r[k] = p[i];
+        }
+    };
+    if ((int64_t) blockIdx.x < nlog) fetch(blockIdx.x);
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
     if (tid < 256) hist[tid] = 0;
     __syncthreads();
-    int s = seg_of_chunk(chk0, nseg, LB);
-    const Seg sg = segs[s];
-    int64_t off = (LB - chk0[s]) * (int64_t) CH;
-    int n = (int) min((int64_t) CH, sg.count - off);
-    const Rec *p = src + sg.start + off;
-    Rec r[PER];
     uint32_t rk[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         int i = k * T + tid;
-        if (i < n) {
-            r[k] = p[i];
-            rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, sg.level)], 1u);
-        }
+        if (i < n) rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, level)], 1u);
     }
     __syncthreads();
     const uint32_t h = tid < 256 ? hist[tid] : 0u;
@@ -624,15 +646,17 @@ __global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, 
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         int i = k * T + tid;
-        if (i < n) stage[loff[digit_at(r[k].w0, r[k].w1, sg.level)] + rk[k]] = r[k];
+        if (i < n) stage[loff[digit_at(r[k].w0, r[k].w1, level)] + rk[k]] = r[k];
     }
     __syncthreads();
+    const int ncur = n, lcur = level;
+    if (LB + gridDim.x < nlog) fetch(LB + gridDim.x);
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         int i = k * T + tid;
-        if (i < n) {
+        if (i < ncur) {
             Rec x = stage[i];
-            uint32_t d = digit_at(x.w0, x.w1, sg.level);
+            uint32_t d = digit_at(x.w0, x.w1, lcur);
             dst[gbase[d] + (i - (int64_t) loff[d])] = x;
         }
     }
