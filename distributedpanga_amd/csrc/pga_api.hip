@@ -93,7 +93,7 @@ struct pga_ctx {
     std::vector<int64_t> nseeds, cseeds;
 
     // device buffers
-    DBuf bps, ctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
+    DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, bmax, sel, plist, gmax;
 
@@ -381,7 +381,7 @@ void pga_destroy(pga_ctx *c)
     if (!c) return;
     (void) hipSetDevice(c->o.device);
     if (c->st) (void) hipStreamSynchronize(c->st);
-    DBuf *all[] = {&c->bps, &c->ctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
+    DBuf *all[] = {&c->bps, &c->ctg, &c->bctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
@@ -477,6 +477,15 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
                                      hipMemcpyHostToDevice, c->st));
     TRY(ensure(c, c->ctg, tab.size() * sizeof(ContigDev)));
     HIPCHK(c, hipMemcpyAsync(c->ctg.p, tab.data(), tab.size() * sizeof(ContigDev), hipMemcpyHostToDevice, c->st));
+    // contig of every extraction block's first position: the last contig whose
+    // first candidate position is <= it (what a binary search over pos0 gives)
+    std::vector<int32_t> bctg((size_t) std::max<int64_t>(c->nblk, 1), 0);
+    for (int64_t b = 0, i = 0; b < c->nblk; b++) {
+        while (i + 1 < nctg && tab[i + 1].pos0 <= b * XBLOCK) i++;
+        bctg[b] = (int32_t) i;
+    }
+    TRY(ensure(c, c->bctg, bctg.size() * sizeof(int32_t)));
+    HIPCHK(c, hipMemcpyAsync(c->bctg.p, bctg.data(), bctg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
     std::vector<int64_t> pblk(npairs + 1);
     for (int p = 0; p < npairs; p++) pblk[p] = c->hp[p].blk0;
     pblk[npairs] = c->nblk;
@@ -594,7 +603,8 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->sel, (size_t) std::max<int64_t>(nblk, 1) * 256));    // 2 bits per position
     if (nblk > 0) {
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, nblk)), dim3(256),
-                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, P<uint32_t>(c->blkcnt),
+                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), (const int32_t *) P<int32_t>(c->bctg),
+                           c->nctg, c->F, P<uint32_t>(c->blkcnt),
                            P<uint8_t>(c->sel), nblk);
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
@@ -662,7 +672,8 @@ int pga_build_index(pga_ctx *c)
         const unsigned xg = grid_for(c, (const void *) k_extract_emit, 256, 0, nblk);
         const int64_t xper = (nblk + xg - 1) / xg;
         hipLaunchKernelGGL(k_extract_emit, dim3(xg), dim3(256),
-                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), c->nctg, c->F, c->jbits,
+                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), (const int32_t *) P<int32_t>(c->bctg),
+                           c->nctg, c->F, c->jbits,
                            (const uint8_t *) P<uint8_t>(c->sel), (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
                            P<unsigned long long>(c->hist1), nblk, xper);
         LAUNCHCHK(c);

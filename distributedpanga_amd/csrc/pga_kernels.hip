@@ -256,28 +256,21 @@ __device__ __forceinline__ void record_keys(const uint64_t *__restrict__ bps64, 
     }
 }
 
-// Contig of the block's first position (one binary search per block); every
-// thread then steps forward from it.  Contigs hold >= 1 candidate position or
-// none, so a thread of XU positions crosses few contig starts.
-__device__ __forceinline__ int block_contig(const ContigDev *__restrict__ ctg, int nctg, int64_t fb, int *sc)
-{
-    if (threadIdx.x == 0) *sc = locate_contig(ctg, nctg, fb);
-    __syncthreads();
-    return *sc;
-}
-
+// bctg[LB] is the contig of logical block LB's first position; every thread
+// steps forward from it (a thread of XU positions crosses few contig starts).
 __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
-                                                      const ContigDev *__restrict__ ctg, int nctg,
+                                                      const ContigDev *__restrict__ ctg,
+                                                      const int32_t *__restrict__ bctg, int nctg,
                                                       int64_t F, uint32_t *__restrict__ blkcnt,
                                                       uint8_t *__restrict__ sel, int64_t nlog)
 {
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint64_t red[XT / 64 + 1];
-    __shared__ int sc;
     load_tables(tl, tm);
+    __syncthreads();
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
     const int64_t fb = LB * XT * XU;
-    int c = block_contig(ctg, nctg, fb, &sc);
+    int c = bctg[LB];
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint32_t cnt = 0, bits = 0;
     XEmit x;
@@ -304,7 +297,8 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
 // until the pair changes, so the 256 per-pair counters see one atomic per
 // workgroup and pair rather than one per logical block.
 __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict__ bps64,
-                                                     const ContigDev *__restrict__ ctg, int nctg,
+                                                     const ContigDev *__restrict__ ctg,
+                                                     const int32_t *__restrict__ bctg, int nctg,
                                                      int64_t F, int jbits, const uint8_t *__restrict__ sel,
                                                      const int64_t *__restrict__ blkoff,
                                                      Rec *__restrict__ out,
@@ -315,15 +309,15 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
     __shared__ uint32_t hist[256];
     __shared__ uint32_t scan_sh[XT / 64 + 1];
     __shared__ Rec stage[2 * XT * XU];
-    __shared__ int sc;
     load_tables(tl, tm);
     for (int t = threadIdx.x; t < 256; t += XT) hist[t] = 0;
+    __syncthreads();
     const int64_t lb0 = (int64_t) blockIdx.x * per;
     const int64_t lb1 = lb0 + per < nlog ? lb0 + per : nlog;
     int cur = -1;
     for (int64_t LB = lb0; LB < lb1; LB++) {
     const int64_t fb = LB * XT * XU;
-    int c = block_contig(ctg, nctg, fb, &sc);
+    int c = bctg[LB];
     const int pair0 = ctg[c].pair;
     if (pair0 != cur) {                               // uniform: every thread sees the same pair0
         if (cur >= 0)
