@@ -65,16 +65,20 @@ def algorithmic_bytes(seqtot, nrec, e, s, nseed):
     return seqtot / 4 + 16 * nrec + 64 * nrec + 16 * nrec + 2 * e * nrec + s * nseed
 
 
-def measured_traffic(name, scale):
-    """HBM bytes per launch of the dominant kernel from its committed rocprofv3
+TRAFFIC_PROFILE = "index_merge_workload2_hbm_traffic.md"
+
+
+def measured_traffic(kernel, scale):
+    """HBM bytes per launch of one kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (valid for the full-size workload only)."""
-    path = os.path.join(ROOT, "profiles", name)
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE)
     if scale != 1.0 or not os.path.exists(path):
         return None, None
+    key = f"traffic_bytes_per_launch[{kernel}]:"
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1]), "profiles/" + name
+            if line.startswith(key):
+                return int(line.split(":")[1]), "profiles/" + TRAFFIC_PROFILE
     return None, None
 
 
@@ -202,6 +206,7 @@ def main():
                     help="also run the human-scale pair at this scale (1.0 = 3.1 Gbp A + B); 0 skips it")
     ap.add_argument("--human-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-seed-sort", action="store_true", help="skip the seed-sort leg (keeps profiles to the step)")
     ap.add_argument("--serve-scale", type=float, default=0.1, help="scale of the resident target (1.0 = 3.1 Gbp)")
     ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--serve-requests", type=int, default=240)
@@ -241,24 +246,27 @@ def main():
     log(f"[bench] plain: {ms_step:.2f} ms/step, {value:.2f} Gbases/s, {info['nrec']} records, {nseed} seeds, "
         f"stages {json.dumps({k: round(v, 3) for k, v in per.items()})}")
 
-    # Dominant kernel: the merge's seed emission (longest launch of the step).
-    # Algorithmic bytes per launch: every GIX entry (e B) and LCP/strand byte
-    # read once, every seed (s B) and the two buck vectors written once.
-    emit_bytes = info["nrec"] * (info["e"] + 1) + nseed * info["s"] + 16 * info["ncontig"]
-    emit_gbs = emit_bytes / (per["merge_emit_ms"] * 1e-3) / 1e9
-    # the tile sort: records read once (16 B), GIX entry, LCP/strand byte and 24-bit prefix written once
+    # Dominant kernel: the tile sort (longest launch of the step).  Algorithmic
+    # bytes per launch: every record read once (16 B); its GIX entry (e B),
+    # LCP/strand byte and 24-bit prefix (4 B) written once.
     sort_bytes = info["nrec"] * (16 + info["e"] + 1 + 4)
     sort_gbs = sort_bytes / (per["tile_sort_ms"] * 1e-3) / 1e9
+    # The merge's seed emission: every GIX entry read once (e B), every seed
+    # (s B) and the two buck vectors written once.
+    emit_bytes = info["nrec"] * info["e"] + nseed * info["s"] + 16 * info["ncontig"]
+    emit_gbs = emit_bytes / (per["merge_emit_ms"] * 1e-3) / 1e9
     scat_gbs = scat_bytes / (per["scatter_ms"] * 1e-3) / 1e9 if per["scatter_ms"] > 0 else 0.0
     balg = algorithmic_bytes(g.seqtot, info["nrec"], info["e"], info["s"], nseed)
     path_gbs = balg / (ms_step * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic("merge_emit_workload2_hbm_traffic.md", args.scale)
-    roofline = dict(bound="hbm", kernel="k_merge_emit", achieved=round(emit_gbs, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(emit_gbs / HBM_PEAK_GBS, 4), traffic=traffic,
-                    traffic_source=traffic_src, bytes_per_launch=int(emit_bytes),
-                    avg_launch_ms=round(per["merge_emit_ms"], 4),
-                    tile_sort=dict(achieved=round(sort_gbs, 1), frac=round(sort_gbs / HBM_PEAK_GBS, 4),
-                                   bytes_per_launch=int(sort_bytes), avg_launch_ms=round(per["tile_sort_ms"], 4)),
+    traffic, traffic_src = measured_traffic("k_tile_sort", args.scale)
+    emit_traffic, _ = measured_traffic("k_merge_emit", args.scale)
+    roofline = dict(bound="hbm", kernel="k_tile_sort", achieved=round(sort_gbs, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(sort_gbs / HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=traffic_src, bytes_per_launch=int(sort_bytes),
+                    avg_launch_ms=round(per["tile_sort_ms"], 4),
+                    merge_emit=dict(achieved=round(emit_gbs, 1), frac=round(emit_gbs / HBM_PEAK_GBS, 4),
+                                    traffic=emit_traffic, bytes_per_launch=int(emit_bytes),
+                                    avg_launch_ms=round(per["merge_emit_ms"], 4)),
                     radix_scatter=dict(achieved=round(scat_gbs, 1), frac=round(scat_gbs / HBM_PEAK_GBS, 4),
                                        bytes_per_step=int(scat_bytes), launches_per_step=scat_launches,
                                        ms_per_step=round(per["scatter_ms"], 4)),
@@ -266,7 +274,7 @@ def main():
                               frac=round(path_gbs / HBM_PEAK_GBS, 4)))
     # the next step downstream: the diagonal-sorted seed array of both streams
     ss = {}
-    for comp, name in ((0, "N"), (1, "C")):
+    for comp, name in (() if args.no_seed_sort else ((0, "N"), (1, "C"))):
         ctx.seed_sort(0, comp)                      # warm the buffers
         t0 = time.perf_counter()
         reps = 3
@@ -278,7 +286,8 @@ def main():
     seed_sort = dict(streams=ss, records=ss_seeds, ms=round(ss_ms, 3),
                      mrecords_s=round(ss_seeds / (ss_ms * 1e-3) / 1e6, 1) if ss_ms > 0 else None,
                      note="reimport + per-panel sort of both seed streams, records left on the device")
-    log(f"[bench] seed sort: {ss_seeds} records in {ss_ms:.2f} ms")
+    if ss:
+        log(f"[bench] seed sort: {ss_seeds} records in {ss_ms:.2f} ms")
     ctx.close()
 
     out = {"metric": METRIC, "value": round(value, 4), "unit": "Gbases/s", "n_gpus": 1,
