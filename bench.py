@@ -200,7 +200,7 @@ def main():
     ap.add_argument("--merge-tile", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-bases", type=int, default=50_000_000)
+    ap.add_argument("--cpu-bases", type=int, default=150_000_000)
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--human-scale", type=float, default=0.0,
                     help="also run the human-scale pair at this scale (1.0 = 3.1 Gbp A + B); 0 skips it")
