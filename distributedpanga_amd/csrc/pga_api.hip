@@ -93,9 +93,9 @@ struct pga_ctx {
     std::vector<int64_t> nseeds, cseeds;
 
     // device buffers
-    DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum, longs,
+    DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
-        mblk, pofs, seedN, seedC, nbuck, cbuck, bmax, sel, plist, gmax;
+        mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax;
 
     DBuf vout;                         // results of the small check kernels
     // seed sort: keys (ping-pong), their pointer pair, the packed records
@@ -382,10 +382,10 @@ void pga_destroy(pga_ctx *c)
     (void) hipSetDevice(c->o.device);
     if (c->st) (void) hipStreamSynchronize(c->st);
     DBuf *all[] = {&c->bps, &c->ctg, &c->bctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
-                   &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum, &c->longs,
+                   &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
-                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->bmax, &c->skeys};
+                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->skeys};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
@@ -510,7 +510,7 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
     const int per = chunk_per(c);
     const int CH = per * 256;
     Rec *buf[2] = {buf0, buf1};
-    int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles, [2] long fills
+    int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles
     int hcount[4] = {0, (int) tile0.size(), 0, 0};
     HIPCHK(c, hipMemcpyAsync(dcount, hcount, sizeof(hcount), hipMemcpyHostToDevice, c->st));
     if (!tile0.empty())
@@ -638,7 +638,6 @@ int pga_build_index(pga_ctx *c)
         ent += d.nrec * d.e;
         d.cum_off = (int64_t) p * NPRE;
         d.buck_off = h.ctg0;
-        d.dense_cum = d.nrec > 0 && d.nrec <= 2 * (int64_t) NPRE;    // at most ~2 records per prefix
     }
     const size_t nrec_alloc = (size_t) std::max<int64_t>(R, 1);
     TRY(ensure(c, c->rec[0], nrec_alloc * sizeof(Rec)));
@@ -691,7 +690,6 @@ int pga_build_index(pga_ctx *c)
     int levels = 0;
     TRY(partition(c, P<Rec>(c->rec[0]), P<Rec>(c->rec[1]), seg0, tile0, P<unsigned long long>(c->hist1), seg_cap,
                   tile_cap, &levels));
-    int *dcount = P<int>(c->counters);          // [2]: long fills of the prefix table
     c->tm.partition_levels = levels;
     c->tm.partition_launches = c->nscatter;
     TRY(record(c, 2));
@@ -714,65 +712,25 @@ int pga_build_index(pga_ctx *c)
     TRY(fill_u64(c, c->maxpre.p, npairs, 0));
     if (c->o.build_cum) {
         TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
-        TRY(ensure(c, c->longs, (size_t) npairs * 65536 * sizeof(LongFill)));
-        std::vector<int> dense;
+        std::vector<int> live;
         for (int p = 0; p < npairs; p++) {
             if (c->pairs[p].nrec == 0) TRY(fill_u64(c, P<int64_t>(c->cum) + (int64_t) p * NPRE, NPRE, 0));
-            else if (c->pairs[p].dense_cum) dense.push_back(p);
+            else live.push_back(p);
         }
-        HIPCHK(c, hipMemsetAsync(dcount + 2, 0, sizeof(int), c->st));
-        if (!dense.empty()) {
-            const int nrange = NPRE / CD_SPAN;
-            TRY(ensure(c, c->plist, dense.size() * sizeof(int)));
-            TRY(ensure(c, c->gmax, dense.size() * nrange * sizeof(uint32_t)));
-            HIPCHK(c, hipMemcpyAsync(c->plist.p, dense.data(), dense.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
-            hipLaunchKernelGGL(k_cum_dense, dim3(nrange, (unsigned) dense.size()), dim3(256), 0, c->st,
+        if (!live.empty()) {
+            const int nrange = NPRE / CR_SPAN;
+            TRY(ensure(c, c->plist, live.size() * sizeof(int)));
+            TRY(ensure(c, c->gmax, live.size() * nrange * sizeof(uint64_t)));
+            HIPCHK(c, hipMemcpyAsync(c->plist.p, live.data(), live.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+            hipLaunchKernelGGL(k_cum_ranges, dim3(nrange, (unsigned) live.size()), dim3(256), 0, c->st,
                                (const PairDev *) P<PairDev>(c->dpairs), (const int *) P<int>(c->plist),
-                               (const uint32_t *) P<uint32_t>(c->pre), P<int64_t>(c->cum), P<uint32_t>(c->gmax));
+                               (const uint32_t *) P<uint32_t>(c->pre), P<int64_t>(c->cum),
+                               P<unsigned long long>(c->gmax));
             LAUNCHCHK(c);
-            hipLaunchKernelGGL(k_cum_dense_max, dim3((unsigned) dense.size()), dim3(256), 0, c->st,
-                               (const int *) P<int>(c->plist), (const uint32_t *) P<uint32_t>(c->gmax), nrange,
-                               P<unsigned long long>(c->maxpre));
+            hipLaunchKernelGGL(k_cum_max, dim3((unsigned) live.size()), dim3(256), 0, c->st,
+                               (const int *) P<int>(c->plist), (const unsigned long long *) P<unsigned long long>(c->gmax),
+                               nrange, P<unsigned long long>(c->maxpre));
             LAUNCHCHK(c);
-        }
-        // pairs with many records per prefix: group-end fills and the maximum over group ends
-        int nsparse = 0;
-        for (int p = 0; p < npairs; p++) nsparse += c->pairs[p].nrec > 0 && !c->pairs[p].dense_cum;
-        if (R > 0 && nsparse > 0) {
-            hipLaunchKernelGGL(k_cum_fill, dim3(grid_for(c, (const void *) k_cum_fill, 256, 0, (R + 255) / 256)),
-                               dim3(256), 0, c->st,
-                               (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
-                               P<int64_t>(c->cum), P<LongFill>(c->longs), dcount + 2);
-            LAUNCHCHK(c);
-            int nlong = 0;
-            HIPCHK(c, hipMemcpyAsync(&nlong, dcount + 2, sizeof(int), hipMemcpyDeviceToHost, c->st));
-            HIPCHK(c, hipStreamSynchronize(c->st));
-            if (nlong > 0) {
-                hipLaunchKernelGGL(k_long_fill, dim3(nlong), dim3(256), 0, c->st, (const LongFill *) P<LongFill>(c->longs));
-                LAUNCHCHK(c);
-            }
-            const int64_t nmp = (R + MAXPRE_SPAN - 1) / MAXPRE_SPAN;
-            TRY(ensure(c, c->bmax, (size_t) std::max<int64_t>(nmp, 1024) * sizeof(uint64_t)));
-            if (npairs == 1 && R > NPRE / 2) {
-                // one large pair: the largest group from the finished table
-                hipLaunchKernelGGL(k_maxpre_cum, dim3(1024), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->cum),
-                                   P<unsigned long long>(c->bmax));
-                LAUNCHCHK(c);
-                hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c->st,
-                                   (const unsigned long long *) P<unsigned long long>(c->bmax), 1024,
-                                   P<unsigned long long>(c->maxpre));
-                LAUNCHCHK(c);
-            } else {
-            hipLaunchKernelGGL(k_maxpre, dim3((unsigned) nmp), dim3(256), 0, c->st,
-                               (const PairDev *) P<PairDev>(c->dpairs), npairs, R, (const uint32_t *) P<uint32_t>(c->pre),
-                               (const int64_t *) P<int64_t>(c->cum), P<unsigned long long>(c->bmax),
-                               P<unsigned long long>(c->maxpre));
-            LAUNCHCHK(c);
-            hipLaunchKernelGGL(k_maxpre_reduce, dim3(npairs), dim3(256), 0, c->st,
-                               (const PairDev *) P<PairDev>(c->dpairs), (const unsigned long long *) P<unsigned long long>(c->bmax),
-                               P<unsigned long long>(c->maxpre));
-            LAUNCHCHK(c);
-            }
         }
     }
     TRY(record(c, 4));

@@ -29,7 +29,7 @@ struct PairDev {
     int64_t mblk0;               // first merge block
     int32_t P, C, e, s;
     int32_t ncontig;
-    int32_t dense_cum;           // 1: prefix table filled prefix by prefix (few records per prefix)
+    int32_t pad;
 };
 
 struct Seg {                     // a key range still to be partitioned
@@ -41,11 +41,6 @@ struct Tile {                    // a bucket run small enough for one workgroup 
     int64_t start;
     int32_t count, pair, buf;
     int32_t level;               // radix digit (1..16) its buckets differ in; all share the digits above
-};
-
-struct LongFill {
-    int64_t *cum;
-    int64_t from, to, v;
 };
 
 } // namespace pga

@@ -966,246 +966,134 @@ __global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const Pai
 
 // ---------------------------------------------------------------- prefix table
 
-constexpr int LONG_FILL = 256;
+// Prefix table of a pair (src/kmer_adapter.c:64-80: cum[p] = entries with
+// prefix <= p), one block per 4096 prefixes: block (x, y) covers prefixes
+// [x*CR_SPAN, (x+1)*CR_SPAN) of pair plist[y].  It finds the pair's records in
+// that range; the last record of each prefix group writes its running count
+// at its prefix in LDS (one writer per prefix), a block max-scan carries the
+// counts over the empty prefixes, and the range goes out in coalesced 8-byte
+// stores.  The range's largest prefix group (the largest step of the table)
+// goes to gmax.
+constexpr int CR_SPAN = 4096;
 
-__device__ __forceinline__ void fill_range(int64_t *__restrict__ cum, int64_t from, int64_t to, int64_t v,
-                                           LongFill *__restrict__ longs, int *__restrict__ nlong)
+__global__ __launch_bounds__(256) void k_cum_ranges(const PairDev *__restrict__ pairs, const int *__restrict__ plist,
+                                                    const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
+                                                    unsigned long long *__restrict__ gmax)
 {
-    int64_t end = to - from > LONG_FILL ? from + LONG_FILL : to;
-    for (int64_t q = from; q < end; q++) cum[q] = v;
-    if (end < to) {
-        int k = atomicAdd(nlong, 1);
-        longs[k] = LongFill{cum, end, to, v};
-    }
-}
-
-// cum[p] = number of entries of the pair with prefix <= p: the last record of
-// each prefix group writes its (index+1) over [its prefix, next prefix).
-__global__ void k_cum_fill(const PairDev *__restrict__ pairs, int npairs, int64_t R,
-                           const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
-                           LongFill *__restrict__ longs, int *__restrict__ nlong)
-{
-    for (int64_t g = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; g < R;
-         g += (int64_t) gridDim.x * blockDim.x) {
-        const int p = pair_of(pairs, npairs, g);
-        if (pairs[p].dense_cum) continue;           // filled by k_cum_dense
-        const int64_t rec0 = pairs[p].rec0, nrec = pairs[p].nrec;
-        const int64_t li = g - rec0;
-        int64_t *cp = cum + pairs[p].cum_off;
-        const int64_t cur = pre[g];
-        const int64_t nxt = (li + 1 < nrec) ? (int64_t) pre[g + 1] : (int64_t) (1 << 24);
-        if (li == 0 && cur > 0) fill_range(cp, 0, cur, 0, longs, nlong);
-        if (cur != nxt) fill_range(cp, cur, nxt, li + 1, longs, nlong);
-    }
-}
-
-// Prefix table of a pair with few records per prefix, written prefix by
-// prefix (coalesced): block (x, y) covers prefixes [x*CD_SPAN, (x+1)*CD_SPAN)
-// of pair plist[y].  It finds the pair's records in that prefix range, counts
-// them per prefix in LDS, and writes the running count (records with prefix
-// <= p) for every prefix of the range.
-constexpr int CD_SPAN = 4096;
-
-__global__ __launch_bounds__(256) void k_cum_dense(const PairDev *__restrict__ pairs, const int *__restrict__ plist,
-                                                   const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
-                                                   uint32_t *__restrict__ gmax)
-{
-    constexpr int PER = CD_SPAN / 256;
-    __shared__ uint32_t cntp[CD_SPAN];
-    __shared__ uint32_t sh[256 / 64 + 1];
-    __shared__ uint32_t gm;                              // the range's largest per-prefix count
+    constexpr int PER = CR_SPAN / 256;
+    __shared__ uint64_t val[CR_SPAN];
+    __shared__ uint64_t sh[256 / 64 + 1];
+    __shared__ unsigned long long gm;
     __shared__ int64_t rng[2];
     const PairDev pd = pairs[plist[blockIdx.y]];
-    const uint32_t p0 = blockIdx.x * (uint32_t) CD_SPAN;
+    const uint32_t p0 = blockIdx.x * (uint32_t) CR_SPAN;
     const uint32_t *pp = pre + pd.rec0;
-    for (int k = threadIdx.x; k < CD_SPAN; k += 256) cntp[k] = 0;
+    for (int k = threadIdx.x; k < CR_SPAN; k += 256) val[k] = 0;
     if (threadIdx.x == 0) gm = 0;
-    // first record with prefix >= p0 and >= p0 + CD_SPAN: 256-way search, each
-    // round every thread tests one sample (log_256 of the pair's records rounds)
-    for (int w = 0; w < 2; w++) {
-        const uint32_t key = p0 + w * (uint32_t) CD_SPAN;
+    // first record with prefix >= p0 (threads 0..127) and >= p0 + CR_SPAN
+    // (threads 128..255), both at once: 128-way searches, each round every
+    // thread tests one sample of its half's interval
+    {
+        __shared__ int hcount[2];
+        const int w = threadIdx.x >> 7, t = threadIdx.x & 127;
+        const uint32_t key = p0 + w * (uint32_t) CR_SPAN;
         int64_t lo = 0, hi = pd.nrec;                  // answer in [lo, hi]
-        while (hi > lo) {
-            const int64_t step = (hi - lo + 255) / 256;
-            const int64_t idx = lo + (int64_t) threadIdx.x * step;
-            const int below = (idx < hi && pp[idx] < key) ? 1 : 0;
-            const int k = __syncthreads_count(below);  // samples below key: a prefix of the samples
-            if (k == 0) { hi = lo; break; }           // pp[lo] >= key (lo is sample 0)
-            const int64_t nlo = lo + (int64_t) (k - 1) * step + 1;   // first index after the last sample below
-            const int64_t nhi = lo + (int64_t) k * step < hi ? lo + (int64_t) k * step : hi;
-            lo = nlo;
-            hi = nhi;
+        bool done = false;
+        while (true) {
+            const bool act = !done && hi > lo;
+            if (threadIdx.x < 2) hcount[threadIdx.x] = 0;
+            const int any = __syncthreads_or(act ? 1 : 0);
+            if (!any) break;
+            const int64_t step = act ? (hi - lo + 127) / 128 : 1;
+            const int64_t idx = lo + (int64_t) t * step;
+            if (act && idx < hi && pp[idx] < key) atomicAdd(&hcount[w], 1);
+            __syncthreads();
+            if (act) {
+                const int k = hcount[w];              // samples below key: a prefix of the samples
+                if (k == 0) {
+                    hi = lo;                          // pp[lo] >= key (lo is sample 0)
+                } else {
+                    const int64_t nlo = lo + (int64_t) (k - 1) * step + 1;
+                    const int64_t nhi = lo + (int64_t) k * step < hi ? lo + (int64_t) k * step : hi;
+                    lo = nlo;
+                    hi = nhi;
+                }
+                done = hi <= lo;
+            }
+            __syncthreads();
         }
-        if (threadIdx.x == 0) rng[w] = lo;
+        if (t == 0) rng[w] = lo;
     }
     __syncthreads();
     const int64_t lo = rng[0], cnt = rng[1] - rng[0];
-    for (int64_t i = threadIdx.x; i < cnt; i += 256) atomicAdd(&cntp[pp[lo + i] - p0], 1u);
+    // group ends, 8 records per thread per round with the loads issued first
+    constexpr int U = 8;
+    for (int64_t base = 0; base < cnt; base += 256 * U) {
+        uint32_t cur[U], nxt[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + u * 256 + threadIdx.x;
+            cur[u] = i < cnt ? pp[lo + i] : 0u;
+            nxt[u] = i + 1 < cnt ? pp[lo + i + 1] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + u * 256 + threadIdx.x;
+            if (i < cnt && nxt[u] != cur[u]) val[cur[u] - p0] = (uint64_t) (i + 1);
+        }
+    }
     __syncthreads();
-    // inclusive running count over the range: PER consecutive prefixes per thread;
-    // the largest per-prefix count is this range's largest prefix group
-    uint32_t v[PER], sum = 0, mx = 0;
+    // inclusive max-scan over the range, PER consecutive prefixes per thread
+    uint64_t v[PER], m = 0;
 #pragma unroll
-    for (int k = 0; k < PER; k++) { v[k] = cntp[threadIdx.x * PER + k]; sum += v[k]; mx = v[k] > mx ? v[k] : mx; }
+    for (int k = 0; k < PER; k++) { const uint64_t x = val[threadIdx.x * PER + k]; m = x > m ? x : m; v[k] = m; }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t inc = m;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { const uint32_t x = __shfl_xor(mx, o, 64); mx = x > mx ? x : mx; }
-    if ((threadIdx.x & 63) == 0) atomicMax(&gm, mx);
-    uint32_t tot;
-    uint32_t run = block_excl_scan<256, uint32_t>(sum, sh, &tot);
-#pragma unroll
-    for (int k = 0; k < PER; k++) { run += v[k]; cntp[threadIdx.x * PER + k] = run; }
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc = y > inc ? y : inc;
+    }
+    if (lane == 63) sh[wv] = inc;
     __syncthreads();
+    uint64_t carry = __shfl_up(inc, 1, 64);            // the running count before this thread's prefixes
+    if (lane == 0) carry = 0;
+    for (int w = 0; w < wv; w++) carry = sh[w] > carry ? sh[w] : carry;
+    uint64_t prev = carry, big = 0;
     int64_t *cp = cum + pd.cum_off + p0;
-    for (int k = threadIdx.x; k < CD_SPAN; k += 256) cp[k] = lo + (int64_t) cntp[k];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint64_t x = v[k] > carry ? v[k] : carry;
+        big = x - prev > big ? x - prev : big;
+        prev = x;
+        val[threadIdx.x * PER + k] = x;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { const uint64_t x = __shfl_xor(big, o, 64); big = x > big ? x : big; }
+    if (lane == 0) atomicMax(&gm, (unsigned long long) big);
+    __syncthreads();
+    for (int k = threadIdx.x; k < CR_SPAN; k += 256) cp[k] = lo + (int64_t) val[k];
     if (threadIdx.x == 0) gmax[(size_t) blockIdx.y * gridDim.x + blockIdx.x] = gm;
 }
 
-// Largest prefix group of each densely filled pair: the maximum of its ranges' maxima.
-__global__ __launch_bounds__(256) void k_cum_dense_max(const int *__restrict__ plist, const uint32_t *__restrict__ gmax,
-                                                       int nrange, unsigned long long *__restrict__ maxpre)
+// Largest prefix group of each pair: the maximum of its ranges' maxima.
+__global__ __launch_bounds__(256) void k_cum_max(const int *__restrict__ plist, const unsigned long long *__restrict__ gmax,
+                                                 int nrange, unsigned long long *__restrict__ maxpre)
 {
-    __shared__ uint32_t red[4];
-    uint32_t m = 0;
+    __shared__ uint64_t red[4];
+    uint64_t m = 0;
     for (int i = threadIdx.x; i < nrange; i += 256) {
-        const uint32_t x = gmax[(size_t) blockIdx.x * nrange + i];
+        const uint64_t x = gmax[(size_t) blockIdx.x * nrange + i];
         m = x > m ? x : m;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { const uint32_t x = __shfl_xor(m, o, 64); m = x > m ? x : m; }
+    for (int o = 32; o > 0; o >>= 1) { const uint64_t x = __shfl_xor(m, o, 64); m = x > m ? x : m; }
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t x = red[0];
+        uint64_t x = red[0];
         for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
         maxpre[plist[blockIdx.x]] = x;
-    }
-}
-
-__global__ void k_long_fill(const LongFill *__restrict__ longs)
-{
-    const LongFill lf = longs[blockIdx.x];
-    for (int64_t q = lf.from + threadIdx.x; q < lf.to; q += blockDim.x) lf.cum[q] = lf.v;
-}
-
-// Largest prefix group per pair, without same-address atomics: each block of
-// MAXPRE_SPAN records writes the largest group ending in it (for the pair of
-// its first record) to bmax[block]; k_maxpre_reduce takes the maximum over each
-// pair's blocks.  Groups of a pair that starts inside a block are rare (one
-// block per pair boundary) and go through an atomic.
-constexpr int MAXPRE_PER = 16;
-constexpr int MAXPRE_SPAN = 256 * MAXPRE_PER;
-
-__global__ __launch_bounds__(256) void k_maxpre(const PairDev *__restrict__ pairs, int npairs, int64_t R,
-                                                const uint32_t *__restrict__ pre,
-                                                const int64_t *__restrict__ cum,
-                                                unsigned long long *__restrict__ bmax,
-                                                unsigned long long *__restrict__ maxpre)
-{
-    __shared__ unsigned long long red[4];
-    const int64_t base = (int64_t) blockIdx.x * MAXPRE_SPAN;
-    const int p0 = pair_of(pairs, npairs, base);
-    const PairDev pd0 = pairs[p0];
-    unsigned long long m = 0;
-    for (int k = 0; k < MAXPRE_PER; k++) {
-        const int64_t g = base + (int64_t) k * 256 + threadIdx.x;
-        if (g >= R) break;
-        const bool in0 = g < pd0.rec0 + pd0.nrec;
-        const int p = in0 ? p0 : pair_of(pairs, npairs, g);
-        const int64_t rec0 = in0 ? pd0.rec0 : pairs[p].rec0;
-        const int64_t nrec = in0 ? pd0.nrec : pairs[p].nrec;
-        const int64_t coff = in0 ? pd0.cum_off : pairs[p].cum_off;
-        const uint32_t cur = pre[g];
-        if (!((g + 1 == rec0 + nrec) || pre[g + 1] != cur)) continue;
-        const int64_t *c = cum + coff;
-        const unsigned long long size = (unsigned long long) (c[cur] - (cur > 0 ? c[cur - 1] : 0));
-        if (in0) m = size > m ? size : m;
-        else atomicMax(&maxpre[p], size);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long x = __shfl_xor(m, o, 64);
-        m = x > m ? x : m;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long x = red[0];
-        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
-        bmax[blockIdx.x] = x;
-    }
-}
-
-// One block per pair: maximum of bmax over the blocks whose first record is the pair's.
-__global__ __launch_bounds__(256) void k_maxpre_reduce(const PairDev *__restrict__ pairs,
-                                                       const unsigned long long *__restrict__ bmax,
-                                                       unsigned long long *__restrict__ maxpre)
-{
-    __shared__ unsigned long long red[4];
-    const PairDev pd = pairs[blockIdx.x];
-    unsigned long long m = 0;
-    if (pd.nrec > 0) {
-        const int64_t b0 = (pd.rec0 + MAXPRE_SPAN - 1) / MAXPRE_SPAN;
-        const int64_t b1 = (pd.rec0 + pd.nrec - 1) / MAXPRE_SPAN;
-        for (int64_t b = b0 + threadIdx.x; b <= b1; b += 256) m = bmax[b] > m ? bmax[b] : m;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long x = __shfl_xor(m, o, 64);
-        m = x > m ? x : m;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long x = red[0];
-        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
-        if (x) atomicMax(&maxpre[blockIdx.x], x);
-    }
-}
-
-// Largest prefix group of one pair from its finished cumulative table:
-// max over p of cum[p] - cum[p-1].  Reads the 128 MiB table instead of four
-// bytes per record; used when the pair has more records than the table has
-// half-entries.  Per-block maxima go to bmax, k_max_reduce folds them.
-__global__ __launch_bounds__(256) void k_maxpre_cum(const int64_t *__restrict__ cum, unsigned long long *__restrict__ bmax)
-{
-    __shared__ unsigned long long red[4];
-    unsigned long long m = 0;
-    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < (1 << 24);
-         q += (int64_t) gridDim.x * blockDim.x) {
-        const unsigned long long v = (unsigned long long) (cum[q] - (q ? cum[q - 1] : 0));
-        m = v > m ? v : m;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long x = __shfl_xor(m, o, 64);
-        m = x > m ? x : m;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long x = red[0];
-        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
-        bmax[blockIdx.x] = x;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_max_reduce(const unsigned long long *__restrict__ bmax, int n,
-                                                    unsigned long long *__restrict__ out)
-{
-    __shared__ unsigned long long red[4];
-    unsigned long long m = 0;
-    for (int i = threadIdx.x; i < n; i += 256) m = bmax[i] > m ? bmax[i] : m;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long x = __shfl_xor(m, o, 64);
-        m = x > m ? x : m;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long x = red[0];
-        for (int w = 1; w < 4; w++) x = red[w] > x ? red[w] : x;
-        *out = x;
     }
 }
 
