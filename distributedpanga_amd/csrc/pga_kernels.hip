@@ -784,18 +784,21 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     if (binned) {
         // scatter into bins, then every record counts the keys of its bin that
         // come before it (smaller, or equal and scattered earlier: the seed sort
-        // has equal keys) and moves to bin start + that rank
+        // has equal keys) and moves to bin start + that rank.  The ranking goes
+        // in bin order (lane p takes the record scattered to p), so a wave's
+        // lanes share bins and only the waves over a crowded bin loop long.
 #pragma unroll
         for (int k = 0; k < E; k++)
             if (k * T + tid < n) sk[bh[bin_of(r[k].w0, r[k].w1)] + rk[k]] = r[k];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < E; k++) {
-            if (k * T + tid >= n) continue;
+            const uint32_t me = (uint32_t) (k * T + tid);
+            if (me >= (uint32_t) n) continue;
+            r[k] = sk[me];
             const uint32_t b = bin_of(r[k].w0, r[k].w1);
             const uint32_t st = bh[b];
             const uint32_t en = b + 1 < (uint32_t) NB ? bh[b + 1] : (uint32_t) n;
-            const uint32_t me = st + rk[k];
             uint32_t rank = 0;
             for (uint32_t i = st; i < en; i++) {
                 const Rec y = sk[i];
