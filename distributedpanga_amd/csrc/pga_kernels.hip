@@ -308,7 +308,7 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t scan_sh[XT / 64 + 1];
-    __shared__ Rec stage[2 * XT * XU];
+    __shared__ Rec stage[XT * XU];                  // one round's records (a block emits up to 2 per position)
     load_tables(tl, tm);
     for (int t = threadIdx.x; t < 256; t += XT) hist[t] = 0;
     __syncthreads();
@@ -327,9 +327,14 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
         __syncthreads();
     }
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
-    uint64_t k0[2 * XU], k1[2 * XU];
-    uint32_t valid = 0;
     const uint32_t bits = sel[fb / XU + threadIdx.x];  // selection found by the count pass
+    // places first (the selection bits give the counts), then each record's
+    // keys go straight to its slot: the block's first XT*XU records are
+    // staged for coalesced stores; the rest, when the block emits both
+    // strands at most of its positions, go out directly
+    uint32_t total;
+    uint32_t k = block_excl_scan<XT, uint32_t>((uint32_t) __builtin_popcount(bits), scan_sh, &total);
+    Rec *dst = out + blkoff[LB];
 #pragma unroll
     for (int u = 0; u < XU; u++) {
         const int64_t f = f0 + u;
@@ -341,33 +346,31 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
         e.rev = (bits >> (2 * u + 1)) & 1u;
         record_keys(bps64, cc, f - cc.pos0, jbits, e);
         if (e.fwd) {
-            k0[2 * u] = e.f0; k1[2 * u] = e.f1; valid |= 1u << (2 * u);
-            uint32_t d = (uint32_t) (e.f0 >> 56);
+            Rec r;
+            r.w0 = e.f0;
+            r.w1 = e.f1;
+            if (k < (uint32_t) (XT * XU)) stage[k] = r;
+            else dst[k] = r;
+            k++;
+            const uint32_t d = (uint32_t) (e.f0 >> 56);
             if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
             else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
         }
         if (e.rev) {
-            k0[2 * u + 1] = e.r0; k1[2 * u + 1] = e.r1; valid |= 1u << (2 * u + 1);
-            uint32_t d = (uint32_t) (e.r0 >> 56);
+            Rec r;
+            r.w0 = e.r0;
+            r.w1 = e.r1;
+            if (k < (uint32_t) (XT * XU)) stage[k] = r;
+            else dst[k] = r;
+            k++;
+            const uint32_t d = (uint32_t) (e.r0 >> 56);
             if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
             else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
         }
     }
-    uint32_t total;
-    uint32_t off = block_excl_scan<XT, uint32_t>((uint32_t) __builtin_popcount(valid), scan_sh, &total);
-    int k = 0;
-#pragma unroll
-    for (int s = 0; s < 2 * XU; s++) {
-        if ((valid >> s) & 1u) {
-            Rec r;
-            r.w0 = k0[s];
-            r.w1 = k1[s];
-            stage[off + k++] = r;
-        }
-    }
     __syncthreads();
-    Rec *dst = out + blkoff[LB];
-    for (int i = threadIdx.x; i < (int) total; i += XT) dst[i] = stage[i];
+    const int nr = (int) min(total, (uint32_t) (XT * XU));
+    for (int i = threadIdx.x; i < nr; i += XT) dst[i] = stage[i];
     __syncthreads();
     }
     if (cur >= 0)

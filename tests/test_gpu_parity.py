@@ -78,7 +78,11 @@ def test_no_records():
 
 
 def test_low_complexity_deep_partition():
-    """Identical k-mers beyond one tile force every partition level down to the payload bits."""
+    """Identical k-mers beyond one tile force every partition level down to the payload bits.
+
+    The poly-A contig selects both strands at almost every position (1.97
+    records per position), so its extraction blocks emit more records than the
+    emit pass stages in LDS and the rest take the direct-store path."""
     contigs = [np.zeros(30000, np.uint8), np.tile(np.array([0, 1], np.uint8), 12000),
                np.tile(np.array([2, 2, 3], np.uint8), 5000)]
     rng = np.random.default_rng(3)
