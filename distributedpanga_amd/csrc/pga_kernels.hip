@@ -710,12 +710,22 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     constexpr int CAP = T * E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Rec *sk = reinterpret_cast<Rec *>(smem_raw);
+    const int tid = threadIdx.x;
+    // a tile's records (striped, coalesced) into registers; the next tile's
+    // are loaded while the current one writes its output
+    Rec r[E];
+    auto load_tile = [&](int64_t lb) {
+        const Tile t = tiles[lb];
+        const Rec *src = bufs[t.buf] + t.start;
+#pragma unroll
+        for (int k = 0; k < E; k++)
+            if (k * T + tid < t.count) r[k] = load_rec_global(src + k * T + tid);
+    };
+    if ((int64_t) blockIdx.x < nlog) load_tile(blockIdx.x);
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
     const Tile tl = tiles[LB];
     const PairDev pd = pairs[tl.pair];
-    const Rec *src = bufs[tl.buf] + tl.start;
     const int n = tl.count;
-    const int tid = threadIdx.x;
 
     constexpr int NB = CAP / 2;                     // bins of the binned sort
     uint32_t *bh = reinterpret_cast<uint32_t *>(smem_raw + (size_t) CAP * sizeof(Rec));
@@ -723,14 +733,12 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     __shared__ uint32_t scan_sh[T / 64 + 1];
     __shared__ int fallback;
 
-    // Records into registers (striped, coalesced); min and max key of the tile.
-    Rec r[E];
+    // min and max key of the tile
     uint64_t lo0 = ~0ull, lo1 = ~0ull, hi0 = 0, hi1 = 0;
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const int i = k * T + tid;
         if (i < n) {
-            r[k] = load_rec_global(src + i);
             if (r[k].w0 < lo0 || (r[k].w0 == lo0 && r[k].w1 < lo1)) { lo0 = r[k].w0; lo1 = r[k].w1; }
             if (r[k].w0 > hi0 || (r[k].w0 == hi0 && r[k].w1 > hi1)) { hi0 = r[k].w0; hi1 = r[k].w1; }
         }
@@ -870,6 +878,8 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     for (int k = 0; k < E; k++) { Rec x; x.w0 = a0[k]; x.w1 = a1[k]; sk[tid * E + k] = x; }
     __syncthreads();
     }
+    // the tile is sorted in LDS and r[] is free: start loading the next tile
+    if (LB + gridDim.x < nlog) load_tile(LB + gridDim.x);
     if (!entries) {
         // keys only (the seed sort): sorted records to skeys at their positions
 #pragma unroll
