@@ -23,7 +23,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
              "-Wall", "-Wno-unused-function", "-I" + INC, "-I" + CSRC]
 CC_FLAGS = ["-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + INC]
 
-HOST_SRCS = ["pga_synth.c", "pga_gix.c"]
+HOST_SRCS = ["pga_synth.c", "pga_gix.c", "pga_gdbfile.c"]
 
 
 def _run(cmd):

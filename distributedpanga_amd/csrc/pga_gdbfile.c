@@ -1,0 +1,288 @@
+/* pga_gdbfile.c -- reader and writer of GDB genome files: the `<root>.gdb`
+ * skeleton in ONE text form and the `.<root>.bps` sequence file (layout in
+ * pga_gdbfile.h; lib/GDB.c:1171-1400 reads them, lib/GDB.c:1519-1607 writes them). */
+#include "pga_gdbfile.h"
+
+#include <ctype.h>
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* The skeleton's schema as the reference writes it into every file
+ * (lib/GDB.c:37-47): kept byte-alike so that ONE readers accept the files. */
+static const char *const SCHEMA_LINES[] = {
+    "~ D f 4 4 REAL 4 REAL 4 REAL 4 REAL   global: base frequency vector",
+    "~ D u 0                               global: upper case when displayed (Deprecated)",
+    "~ O S 1 6 STRING                      id for a scaffold",
+    "~ D G 1 3 INT                         gap of given length",
+    "~ D C 1 3 INT                         contig of given length",
+    "~ D M 1 8 INT_LIST                    mask pair list for a contig",
+};
+
+/* ".../<root>.gdb" -> ".../.<root>.bps" (NULL unless path ends in ".gdb"). */
+static char *bps_path(const char *path)
+{
+    const size_t n = strlen(path);
+    if (n < 5 || strcmp(path + n - 4, ".gdb") != 0) return NULL;
+    const char *slash = strrchr(path, '/');
+    const size_t dir = slash ? (size_t) (slash - path) + 1 : 0;
+    const size_t root = n - 4 - dir;
+    char *out = (char *) malloc(dir + root + 6);
+    if (!out) return NULL;
+    memcpy(out, path, dir);
+    out[dir] = '.';
+    memcpy(out + dir + 1, path + dir, root);
+    memcpy(out + dir + 1 + root, ".bps", 5);
+    return out;
+}
+
+static int slurp(const char *path, char **buf, size_t *len)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    size_t cap = 1 << 16, n = 0;
+    char *b = (char *) malloc(cap + 1);
+    while (b) {
+        const size_t got = fread(b + n, 1, cap - n, f);
+        n += got;
+        if (n < cap) break;
+        cap *= 2;
+        char *nb = (char *) realloc(b, cap + 1);
+        if (!nb) { free(b); b = NULL; }
+        else b = nb;
+    }
+    const int bad = !b || ferror(f);
+    fclose(f);
+    if (bad) { free(b); return -1; }
+    b[n] = '\0';
+    *buf = b;
+    *len = n;
+    return 0;
+}
+
+/* A decimal integer at *p (after optional blanks); advances *p. */
+static int get_int(const char **p, const char *end, int64_t *v)
+{
+    const char *s = *p;
+    while (s < end && *s == ' ') s++;
+    if (s >= end || !(isdigit((unsigned char) *s) || *s == '-')) return -1;
+    char *e;
+    errno = 0;
+    const long long x = strtoll(s, &e, 10);
+    if (errno || e == s || e > end) return -1;
+    *v = x;
+    *p = e;
+    return 0;
+}
+
+/* Grow-only arrays: *a holds at least need elements afterwards. */
+#define DEFINE_GROW(name, T)                                                    \
+    static int name(T **a, size_t *cap, size_t need)                            \
+    {                                                                           \
+        if (need <= *cap) return 0;                                             \
+        size_t c = *cap ? *cap : 64;                                            \
+        while (c < need) c *= 2;                                                \
+        T *q = (T *) realloc(*a, c * sizeof(T));                                \
+        if (!q) return -1;                                                      \
+        *a = q;                                                                 \
+        *cap = c;                                                               \
+        return 0;                                                               \
+    }
+DEFINE_GROW(grow_i64, int64_t)
+DEFINE_GROW(grow_int, int)
+DEFINE_GROW(grow_chr, char)
+
+void pga_gdbfile_free(pga_gdbfile *g)
+{
+    if (!g) return;
+    free(g->slen); free(g->fctg); free(g->hoff); free(g->headers);
+    free(g->sbeg); free(g->clen); free(g->boff);
+    memset(g, 0, sizeof(*g));
+}
+
+int pga_gdbfile_read(const char *path, pga_gdbfile *g)
+{
+    if (!path || !g) return -3;
+    memset(g, 0, sizeof(*g));
+    char *buf;
+    size_t len;
+    if (slurp(path, &buf, &len)) return -1;
+    const char *p = buf, *end = buf + len;
+    int rc = -3;
+    size_t cs = 0, cf = 0, ch = 0, cb = 0, cc = 0, co = 0, ct = 0;   /* capacities */
+    int64_t spos = 0, boff = 0;
+    int first = 1;
+    while (p < end) {
+        const char *eol = memchr(p, '\n', (size_t) (end - p));
+        if (!eol) eol = end;
+        const char t = *p;
+        const char *q = p + 1;
+        if (first) {
+            /* "1 <n> gdb <major> <minor>" */
+            int64_t n;
+            if (t != '1' || get_int(&q, eol, &n) || n != 3 || q + 4 > eol || memcmp(q, " gdb", 4) != 0) goto done;
+            first = 0;
+        } else if (t == '$') {
+            rc = -2;                               /* the binary lines follow */
+            goto done;
+        } else if (t == 'S') {
+            int64_t n;
+            if (get_int(&q, eol, &n) || n < 0 || q + 1 + n > eol || *q != ' ') goto done;
+            q++;
+            if (g->nscaff > 0) g->slen[g->nscaff - 1] = spos;
+            if (grow_i64(&g->slen, &cs, (size_t) g->nscaff + 1) || grow_int(&g->fctg, &cf, (size_t) g->nscaff + 2) ||
+                grow_i64(&g->hoff, &ch, (size_t) g->nscaff + 1) ||
+                grow_chr(&g->headers, &ct, (size_t) (g->hdrtot + n + 1))) { rc = -1; goto done; }
+            g->fctg[g->nscaff] = g->ncontig;
+            g->hoff[g->nscaff] = g->hdrtot;
+            memcpy(g->headers + g->hdrtot, q, (size_t) n);
+            g->hdrtot += n;
+            g->headers[g->hdrtot++] = '\0';
+            g->nscaff++;
+            spos = 0;
+        } else if (t == 'G' || t == 'C') {
+            int64_t v;
+            if (g->nscaff == 0 || get_int(&q, eol, &v) || v < 0) goto done;
+            if (t == 'C') {
+                if (grow_i64(&g->sbeg, &cb, (size_t) g->ncontig + 1) || grow_i64(&g->clen, &cc, (size_t) g->ncontig + 1) ||
+                    grow_i64(&g->boff, &co, (size_t) g->ncontig + 2)) { rc = -1; goto done; }
+                g->sbeg[g->ncontig] = spos;
+                g->clen[g->ncontig] = v;
+                g->boff[g->ncontig] = boff;
+                g->ncontig++;
+                g->seqtot += v;
+                if (v > g->maxctg) g->maxctg = v;
+                boff += (v + 3) >> 2;
+            }
+            spos += v;
+        } else if (t == 'f') {
+            for (int i = 0; i < 4; i++) {
+                while (q < eol && *q == ' ') q++;
+                char *e;
+                const double x = strtod(q, &e);
+                if (e == q || e > eol) goto done;
+                g->freq[i] = (float) x;
+                q = e;
+            }
+        } else if (t == 'u' || t == 'M') {
+            /* ignored, as Read_GDB does */
+        } else if (isalpha((unsigned char) t)) {
+            goto done;                             /* not a gdb line type */
+        }
+        /* anything else is a header, schema or spacer line */
+        p = eol + 1;
+    }
+    if (first) goto done;
+    if (g->nscaff > 0) g->slen[g->nscaff - 1] = spos;
+    if (grow_int(&g->fctg, &cf, (size_t) g->nscaff + 1) || grow_i64(&g->boff, &co, (size_t) g->ncontig + 1)) {
+        rc = -1;
+        goto done;
+    }
+    g->fctg[g->nscaff] = g->ncontig;
+    g->boff[g->ncontig] = boff;
+    rc = 0;
+done:
+    free(buf);
+    if (rc) pga_gdbfile_free(g);
+    return rc;
+}
+
+int pga_gdbfile_read_bps(const char *path, const pga_gdbfile *g, uint8_t *bps)
+{
+    if (!path || !g || (!bps && g->ncontig > 0 && g->boff[g->ncontig] > 0)) return -3;
+    char *bp = bps_path(path);
+    if (!bp) return -3;
+    FILE *f = fopen(bp, "rb");
+    free(bp);
+    if (!f) return -1;
+    const int64_t n = g->boff ? g->boff[g->ncontig] : 0;
+    const int bad = n > 0 && fread(bps, 1, (size_t) n, f) != (size_t) n;
+    fclose(f);
+    return bad ? -1 : 0;
+}
+
+int pga_gdbfile_write(const char *path, const pga_gdbfile *g, const uint8_t *bps)
+{
+    if (!path || !g || g->nscaff < 0 || g->ncontig < 0 || (g->nscaff > 0 && (!g->fctg || !g->slen || !g->hoff || !g->headers)))
+        return -3;
+    if (g->ncontig > 0 && (!g->clen || !g->sbeg)) return -3;
+    if (g->nscaff == 0 ? g->ncontig != 0 : (g->fctg[0] != 0 || g->fctg[g->nscaff] != g->ncontig)) return -3;
+    int64_t nbytes = 0;
+    for (int c = 0; c < g->ncontig; c++) {
+        if (g->clen[c] < 0) return -3;
+        nbytes += (g->clen[c] + 3) >> 2;
+    }
+    /* contigs sit in order inside their scaffold */
+    for (int s = 0; s < g->nscaff; s++) {
+        int64_t spos = 0;
+        for (int c = g->fctg[s]; c < g->fctg[s + 1]; c++) {
+            if (g->sbeg[c] < spos) return -3;
+            spos = g->sbeg[c] + g->clen[c];
+        }
+        if (g->fctg[s + 1] < g->fctg[s] || g->slen[s] < spos) return -3;
+    }
+    if (nbytes > 0 && !bps) return -3;
+    char *bp = bps_path(path);
+    if (!bp) return -3;
+    FILE *f = fopen(bp, "wb");
+    free(bp);
+    if (!f) return -1;
+    int bad = nbytes > 0 && fwrite(bps, 1, (size_t) nbytes, f) != (size_t) nbytes;
+    if (fclose(f)) bad = 1;
+    if (bad) return -1;
+
+    f = fopen(path, "w");
+    if (!f) return -1;
+    char date[64];
+    const time_t now = time(NULL);
+    struct tm tmv;
+    localtime_r(&now, &tmv);
+    strftime(date, sizeof(date), "%Y-%m-%d_%H:%M:%S", &tmv);
+    const char *prog = "distributedpanga_amd", *ver = "1.0", *cmd = "pga_gdbfile_write", *src = "unknown";
+    fprintf(f, "1 3 gdb 2 1\n");
+    fprintf(f, "! 4 %zu %s %zu %s %zu %s %zu %s\n", strlen(prog), prog, strlen(ver), ver, strlen(cmd), cmd,
+            strlen(date), date);
+    fprintf(f, ".\n< %zu %s 1\n.\n", strlen(src), src);
+    for (size_t i = 0; i < sizeof(SCHEMA_LINES) / sizeof(SCHEMA_LINES[0]); i++) fprintf(f, "%s\n", SCHEMA_LINES[i]);
+    /* line counts ('#'), longest ('@') and total ('+') scaffold id: ONE
+     * readers size their arrays and string buffers from them (Read_GDB does,
+     * lib/GDB.c:1260-1262) */
+    int64_t ngap = 0, hmax = 0, htot = 0;
+    for (int s = 0; s < g->nscaff; s++) {
+        const int64_t h = (int64_t) strlen(g->headers + g->hoff[s]);
+        hmax = h > hmax ? h : hmax;
+        htot += h;
+        int64_t spos = 0;
+        for (int c = g->fctg[s]; c < g->fctg[s + 1]; c++) {
+            ngap += g->sbeg[c] > spos;
+            spos = g->sbeg[c] + g->clen[c];
+        }
+        ngap += g->slen[s] > spos;
+    }
+    fprintf(f, ".\n# f 1\n");
+    if (g->nscaff > 0) {
+        fprintf(f, "# S %d\n", g->nscaff);
+        if (hmax > 0) fprintf(f, "@ S %lld\n", (long long) hmax);
+        if (htot > 0) fprintf(f, "+ S %lld\n", (long long) htot);
+    }
+    if (ngap > 0) fprintf(f, "# G %lld\n", (long long) ngap);
+    if (g->ncontig > 0) fprintf(f, "# C %d\n", g->ncontig);
+    fprintf(f, ".\n");
+    fprintf(f, "f %f %f %f %f\n", g->freq[0], g->freq[1], g->freq[2], g->freq[3]);
+    for (int s = 0; s < g->nscaff; s++) {
+        const char *h = g->headers + g->hoff[s];
+        fprintf(f, "S %zu %s\n", strlen(h), h);
+        int64_t spos = 0;
+        for (int c = g->fctg[s]; c < g->fctg[s + 1]; c++) {
+            if (g->sbeg[c] > spos) fprintf(f, "G %lld\n", (long long) (g->sbeg[c] - spos));
+            fprintf(f, "C %lld\n", (long long) g->clen[c]);
+            spos = g->sbeg[c] + g->clen[c];
+        }
+        if (g->slen[s] > spos) fprintf(f, "G %lld\n", (long long) (g->slen[s] - spos));
+    }
+    bad = ferror(f);
+    if (fclose(f)) bad = 1;
+    return bad ? -1 : 0;
+}

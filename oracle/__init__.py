@@ -97,6 +97,10 @@ def ref():
         L.ref_result_free.argtypes = [C.POINTER(RefResult)]
         L.ref_lcp_bases.argtypes = [_u8p, _u8p]
         L.ref_rmsd_sort.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, _i64p]
+        _ip, _fp = C.POINTER(C.c_int), C.POINTER(C.c_float)
+        L.ref_gdb_write.argtypes = [C.c_char_p, C.c_int, C.c_char_p, _i64p, _i64p, _ip, C.c_int, _i64p, _i64p, _fp]
+        L.ref_gdb_read.argtypes = [C.c_char_p, _ip, _ip, _i64p, _ip, _i64p, _i64p, _i64p, C.c_char_p, C.c_int64,
+                                   _fp, C.c_int, C.c_int]
         _ref = L
     return _ref
 
@@ -263,3 +267,53 @@ def ref_rmsd_sort(records, panel_sizes):
     if n:
         R.ref_rmsd_sort(_u8(recs), n, sw, len(panel), _i64(panel))
     return recs
+
+
+def ref_gdb_write(path, ids, slen, fctg, sbeg, clen, freq=(0.25, 0.25, 0.25, 0.25)):
+    """The reference's Write_GDB (lib/GDB.c:1519-1607): path ends in ".gdb"
+    (ONE text) or ".1gdb" (ONE binary); dir/.<root>.bps must exist."""
+    R = ref()
+    hdr = b"".join(s.encode() + b"\0" for s in ids)
+    hoff = np.zeros(max(len(ids), 1), np.int64)
+    o = 0
+    for i, s in enumerate(ids):
+        hoff[i] = o
+        o += len(s.encode()) + 1
+    slen = np.ascontiguousarray(slen, np.int64)
+    fctg = np.ascontiguousarray(fctg, np.int32)
+    sbeg = np.ascontiguousarray(sbeg, np.int64)
+    clen = np.ascontiguousarray(clen, np.int64)
+    fr = np.ascontiguousarray(freq, np.float32)
+    rc = R.ref_gdb_write(str(path).encode(), len(ids), hdr, _i64(hoff), _i64(slen),
+                         fctg.ctypes.data_as(C.POINTER(C.c_int)), int(clen.size), _i64(sbeg), _i64(clen),
+                         fr.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc:
+        raise OSError(f"Write_GDB failed ({rc})")
+
+
+def ref_gdb_read(path, cap=1 << 16):
+    """The reference's Read_GDB (lib/GDB.c:1171-1400): scaffold ids, lengths and
+    first contigs, contig starts, lengths and .bps offsets, base frequencies."""
+    R = ref()
+    ns, nc = C.c_int(), C.c_int()
+    slen = np.zeros(cap, np.int64)
+    fctg = np.zeros(cap + 1, np.int32)
+    sbeg = np.zeros(cap, np.int64)
+    clen = np.zeros(cap, np.int64)
+    boff = np.zeros(cap + 1, np.int64)
+    hdr = C.create_string_buffer(cap * 64)
+    fr = np.zeros(4, np.float32)
+    rc = R.ref_gdb_read(str(path).encode(), C.byref(ns), C.byref(nc), _i64(slen),
+                        fctg.ctypes.data_as(C.POINTER(C.c_int)), _i64(sbeg), _i64(clen), _i64(boff), hdr,
+                        cap * 64, fr.ctypes.data_as(C.POINTER(C.c_float)), cap, cap)
+    if rc:
+        raise OSError(f"Read_GDB failed ({rc})")
+    ns, nc = ns.value, nc.value
+    raw = hdr.raw
+    ids, o = [], 0
+    for _ in range(ns):
+        e = raw.index(b"\0", o)
+        ids.append(raw[o:e].decode())
+        o = e + 1
+    return dict(ids=ids, slen=slen[:ns].copy(), fctg=fctg[:ns + 1].copy(), sbeg=sbeg[:nc].copy(),
+                clen=clen[:nc].copy(), boff=boff[:nc + 1].copy(), freq=tuple(float(x) for x in fr))

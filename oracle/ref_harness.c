@@ -133,3 +133,86 @@ int ref_rmsd_sort(uint8_t *arr, int64_t nelem, int swide, int nparts, int64_t *p
     if (nelem <= 0) return 0;
     return rmsd_sort(arr, nelem, swide, swide, nparts, panel_bytes, 1, r);
 }
+
+/* The reference's own GDB skeleton writer and reader (Write_GDB
+ * lib/GDB.c:1519-1607, Read_GDB lib/GDB.c:1171-1400), for pinning the
+ * package's `.gdb` text reader/writer.  path ends in ".gdb" (ONE text) or
+ * ".1gdb" (ONE binary); Write_GDB needs dir/.<root>.bps to exist.  Scaffold s
+ * holds contigs [fctg[s], fctg[s+1]); sbeg/clen give each contig's place. */
+int ref_gdb_write(const char *path, int nscaff, const char *headers, const int64_t *hoff,
+                  const int64_t *slen, const int *fctg, int ncontig, const int64_t *sbeg,
+                  const int64_t *clen, const float *freq)
+{
+    GDB g;
+    int64_t boff = 0, hdrtot = 0;
+    memset(&g, 0, sizeof(g));
+    if (Prog_Name == NULL) Prog_Name = strdup("ref_harness");
+    if (Command_Line == NULL) Command_Line = strdup("ref_harness");
+    for (int s = 0; s < nscaff; s++) hdrtot = hoff[s] + (int64_t) strlen(headers + hoff[s]) + 1;
+    g.nscaff = nscaff;
+    g.scaffolds = (GDB_SCAFFOLD *) calloc((size_t) nscaff + 1, sizeof(GDB_SCAFFOLD));
+    g.contigs = (GDB_CONTIG *) calloc((size_t) ncontig + 1, sizeof(GDB_CONTIG));
+    g.headers = (char *) malloc((size_t) hdrtot + 1);
+    if (!g.scaffolds || !g.contigs || !g.headers) return -1;
+    memcpy(g.headers, headers, (size_t) hdrtot);
+    g.hdrtot = (int) hdrtot;
+    for (int s = 0; s < nscaff; s++) {
+        g.scaffolds[s].slen = slen[s];
+        g.scaffolds[s].fctg = fctg[s];
+        g.scaffolds[s].ectg = fctg[s + 1];
+        g.scaffolds[s].hoff = hoff[s];
+        for (int c = fctg[s]; c < fctg[s + 1]; c++) g.contigs[c].scaf = s;
+    }
+    for (int c = 0; c < ncontig; c++) {
+        g.contigs[c].clen = clen[c];
+        g.contigs[c].sbeg = sbeg[c];
+        g.contigs[c].boff = boff;
+        boff += COMPRESSED_LEN(clen[c]);
+        g.seqtot += clen[c];
+        if (clen[c] > g.maxctg) g.maxctg = clen[c];
+    }
+    g.contigs[ncontig].boff = boff;
+    g.ncontig = ncontig;
+    g.seqstate = EXTERNAL;
+    g.srcpath = strdup("synthetic.fa");
+    for (int i = 0; i < 4; i++) g.freq[i] = freq[i];
+    char *p = strdup(path);
+    const int rc = Write_GDB(&g, p);
+    free(p);
+    free(g.srcpath);
+    free(g.headers);
+    free(g.contigs);
+    free(g.scaffolds);
+    return rc;
+}
+
+/* Read_GDB on path; copies out what fits (nscaff/ncontig report the totals). */
+int ref_gdb_read(const char *path, int *nscaff, int *ncontig, int64_t *slen, int *fctg, int64_t *sbeg,
+                 int64_t *clen, int64_t *boff, char *headers, int64_t hcap, float *freq, int cap_s, int cap_c)
+{
+    GDB g;
+    char *p = strdup(path);
+    memset(&g, 0, sizeof(g));
+    if (Read_GDB(&g, p)) return -1;
+    *nscaff = g.nscaff;
+    *ncontig = g.ncontig;
+    for (int s = 0; s < g.nscaff && s < cap_s; s++) {
+        slen[s] = g.scaffolds[s].slen;
+        fctg[s] = g.scaffolds[s].fctg;
+    }
+    if (g.nscaff < cap_s) fctg[g.nscaff] = g.ncontig;
+    for (int c = 0; c < g.ncontig && c < cap_c; c++) {
+        sbeg[c] = g.contigs[c].sbeg;
+        clen[c] = g.contigs[c].clen;
+        boff[c] = g.contigs[c].boff;
+    }
+    if (g.ncontig < cap_c) boff[g.ncontig] = g.contigs[g.ncontig].boff;
+    if (headers && hcap > 0) {
+        const int64_t n = g.hdrtot < hcap ? g.hdrtot : hcap;
+        memcpy(headers, g.headers, (size_t) n);
+    }
+    for (int i = 0; i < 4; i++) freq[i] = g.freq[i];
+    Close_GDB(&g);
+    free(p);
+    return 0;
+}

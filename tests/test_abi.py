@@ -19,7 +19,7 @@ def declared(header):
 
 
 @pytest.mark.parametrize("header,loader", [("libpga_hip.h", _lib.hip_lib), ("pga_synth.h", _lib.host_lib),
-                                           ("pga_gix.h", _lib.host_lib)])
+                                           ("pga_gix.h", _lib.host_lib), ("pga_gdbfile.h", _lib.host_lib)])
 def test_exports_every_declared_symbol(header, loader):
     names = declared(header)
     assert len(names) >= 3

@@ -1,0 +1,116 @@
+"""GDB genome files (`<root>.gdb` ONE text skeleton + `.<root>.bps`): the
+package's reader and writer against each other and against the reference's
+own Read_GDB / Write_GDB (lib/GDB.c:1171-1400, 1519-1607), compiled into
+oracle/_ref.  A genome read from disk then runs through the device path."""
+import numpy as np
+import pytest
+
+import oracle
+from distributedpanga_amd import gdbfile, pga, synth
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference sources)")
+
+
+def _toy():
+    """Three scaffolds: gaps before, between and after contigs, an id with
+    spaces, and a scaffold that is all gap."""
+    clen = np.array([1000, 300, 77, 5, 12], np.int64)
+    sbeg = np.array([0, 1050, 20, 100, 110], np.int64)
+    fctg = np.array([0, 2, 2, 5], np.int32)
+    slen = np.array([1350, 40, 150], np.int64)
+    boff = np.concatenate([[0], np.cumsum((clen + 3) // 4)]).astype(np.int64)
+    rng = np.random.default_rng(5)
+    bps = rng.integers(0, 256, int(boff[-1])).astype(np.uint8)
+    return gdbfile.GdbGenome(ids=["chr1 test scaffold", "gap_only", "s3"], slen=slen, fctg=fctg, sbeg=sbeg,
+                             clen=clen, boff=boff, bps=bps, freq=(0.3, 0.2, 0.2, 0.3))
+
+
+def _same(a, b, bases=True):
+    assert a.ids == b.ids
+    for k in ("slen", "fctg", "sbeg", "clen", "boff"):
+        assert np.array_equal(np.asarray(getattr(a, k)), np.asarray(getattr(b, k))), k
+    assert np.allclose(a.freq, b.freq, atol=1e-6)
+    if bases:
+        assert np.array_equal(a.bps, b.bps)
+
+
+def test_round_trip(tmp_path):
+    g = _toy()
+    gdbfile.write(tmp_path / "toy.gdb", g)
+    assert (tmp_path / ".toy.bps").stat().st_size == int(g.boff[-1])
+    _same(gdbfile.read(tmp_path / "toy.gdb"), g)
+
+
+def test_genome_through_files(tmp_path):
+    """A synthetic pair written and read back loads the same bases and offsets."""
+    g = synth.workload(1, 0.02)
+    assert np.array_equal(np.diff(g.boff[:g.ncontig + 1]), (np.asarray(g.clen) + 3) // 4)
+    gdbfile.write(tmp_path / "pair.gdb", gdbfile.from_genome(g))
+    h = gdbfile.read(tmp_path / "pair.gdb")
+    assert h.ncontig == g.ncontig and h.seqtot == g.seqtot
+    assert np.array_equal(h.boff, np.asarray(g.boff)[:g.ncontig + 1])
+    assert np.array_equal(h.bps, np.asarray(g.bps)[:int(h.boff[-1])])
+
+
+@needs_ref
+def test_reference_reads_ours(tmp_path):
+    g = _toy()
+    gdbfile.write(tmp_path / "ours.gdb", g)
+    r = oracle.ref_gdb_read(tmp_path / "ours.gdb")
+    assert r["ids"] == g.ids
+    for k in ("slen", "fctg", "sbeg", "clen", "boff"):
+        assert np.array_equal(r[k], getattr(g, k)), k
+    assert np.allclose(r["freq"], g.freq, atol=1e-6)
+
+
+@needs_ref
+def test_we_read_the_references(tmp_path):
+    g = _toy()
+    (tmp_path / ".ref.bps").write_bytes(g.bps.tobytes())
+    oracle.ref_gdb_write(tmp_path / "ref.gdb", g.ids, g.slen, g.fctg, g.sbeg, g.clen, g.freq)
+    _same(gdbfile.read(tmp_path / "ref.gdb"), g)
+
+
+@needs_ref
+def test_binary_one_file_is_refused(tmp_path):
+    g = _toy()
+    (tmp_path / ".bin.bps").write_bytes(g.bps.tobytes())
+    oracle.ref_gdb_write(tmp_path / "bin.1gdb", g.ids, g.slen, g.fctg, g.sbeg, g.clen, g.freq)
+    (tmp_path / "bin.gdb").write_bytes((tmp_path / "bin.1gdb").read_bytes())
+    with pytest.raises(OSError, match="binary ONE"):
+        gdbfile.read(tmp_path / "bin.gdb")
+
+
+@pytest.mark.parametrize("text", [
+    "1 3 seq 2 1\n.\nS 1 a\nC 10\n",          # not a gdb file
+    "1 3 gdb 2 1\n.\nC 10\n",                  # contig before any scaffold
+    "1 3 gdb 2 1\n.\nS 9 ab\nC 10\n",          # scaffold id shorter than its length
+    "1 3 gdb 2 1\n.\nS 1 a\nC x\n",            # not a number
+    "1 3 gdb 2 1\n.\nS 1 a\nQ 3\n",            # not a gdb line type
+    "",                                         # empty
+])
+def test_bad_files(tmp_path, text):
+    (tmp_path / "bad.gdb").write_text(text)
+    with pytest.raises(OSError, match="bad argument or file contents"):
+        gdbfile.read(tmp_path / "bad.gdb", with_bases=False)
+
+
+def test_missing_file(tmp_path):
+    with pytest.raises(OSError, match="I/O error"):
+        gdbfile.read(tmp_path / "none.gdb")
+
+
+@pytest.mark.gpu
+def test_gpu_index_and_merge_of_a_gdb_from_disk(tmp_path):
+    g = synth.workload(1, 0.05)
+    gdbfile.write(tmp_path / "pair.gdb", gdbfile.from_genome(g))
+    h = gdbfile.read(tmp_path / "pair.gdb")
+    print(f"[gdbfile] ncontig={h.ncontig} bases={h.seqtot}", flush=True)
+    got = pga.run_pair(h)
+    want = oracle.pipeline(h.bps, h.boff[:-1], h.clen, freq=10)
+    assert np.array_equal(got["entries"], want["entries"])
+    assert np.array_equal(got["cum"], want["cum"])
+    assert np.array_equal(got["nseeds"], want["nseeds"])
+    assert np.array_equal(got["cseeds"], want["cseeds"])
+    assert np.array_equal(got["nbuck"], want["nbuck"])
+    assert np.array_equal(got["cbuck"], want["cbuck"])
