@@ -1038,9 +1038,16 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
     int dbyte = 0;                                       // src/pga-mpi-merge.c:415-425
     for (int64_t cum = 1; cum < 2 * maxctg; cum *= 256) dbyte++;
     const int swide = 2 * dbyte + d.C + 2;
-    const int kbits = 8 * swide + bits_for(std::max(d.ncontig - 1, 0));
+    // key fields at their exact widths: contig ids, band (diag >> 6), anti
+    // (stored in dbyte bytes), then diag&63 and lcp in 6 bits each
+    const int64_t dmax = dbyte >= 8 ? INT64_MAX : ((int64_t) 1 << (8 * dbyte)) - 1;
+    SeedKeyFields kf;
+    kf.bc = bits_for(std::max(d.ncontig - 1, 0));
+    kf.bb = bits_for(std::min<int64_t>((2 * maxctg) >> 6, dmax));
+    kf.ba = bits_for(std::min<int64_t>(2 * maxctg, dmax));
+    const int kbits = 2 * kf.bc + kf.bb + kf.ba + 12;
     if (kbits > 128) return fail(c, PGA_EOVERFLOW, "seed sort key of %d bits exceeds 128", kbits);
-    const int shift = 128 - kbits;
+    kf.shift = 128 - kbits;
     c->ss_n = n;
     c->ss_swide = swide;
     c->ss_ready = false;
@@ -1055,7 +1062,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
     }
     if (n > 0) {
         hipLaunchKernelGGL(k_seed_keys, dim3(grid_for(c, (const void *) k_seed_keys, 256, 0, (n + 255) / 256)), dim3(256),
-                           0, c->st, stream, n, d.P, d.C, maxctg, dbyte, swide, comp, shift, P<Rec>(c->ssk[0]));
+                           0, c->st, stream, n, d.P, d.C, maxctg, dbyte, comp, kf, P<Rec>(c->ssk[0]));
         LAUNCHCHK(c);
         // partition + tile sort, the keys ending in ssk[0]; the build's statistics are kept
         const int CAP = cap_of(c);
@@ -1087,7 +1094,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
                                  P<Rec>(c->ssk[0])));
         const int64_t nb = (n + SP_T * SP_PER - 1) / (SP_T * SP_PER);
         hipLaunchKernelGGL(k_seed_pack, dim3(grid_for(c, (const void *) k_seed_pack, SP_T, 0, nb)), dim3(SP_T), 0,
-                           c->st, (const Rec *) P<Rec>(c->ssk[0]), n, shift, swide, P<uint8_t>(c->ssout));
+                           c->st, (const Rec *) P<Rec>(c->ssk[0]), n, kf, dbyte, swide, P<uint8_t>(c->ssout));
         LAUNCHCHK(c);
     }
     HIPCHK(c, hipStreamSynchronize(c->st));

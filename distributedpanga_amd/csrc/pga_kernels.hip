@@ -1527,7 +1527,15 @@ __global__ void k_bucket_counts(const int64_t *__restrict__ cum, long long *__re
 // the whole record read from its last byte (lib/RSDsort.c:292-377), i.e. on
 // the record as one little-endian number; prefixing that number with the
 // anchor contig gives one key whose plain sort is the panel-by-panel order.
-// Keys are left-aligned in 128 bits so that the MSD digits start at once.
+// The key holds every field at its exact width (leading zero bits dropped,
+// the order unchanged): anchor contig, partner contig (bc bits each), band
+// (bb), anti (ba), diag&63 (6), lcp (6, at most 40); left-aligned in 128 bits
+// so that every MSD digit carries information.
+
+struct SeedKeyFields {
+    int bc, bb, ba;            // bits of the contig ids, the band and the anti-diagonal
+    int shift;                 // 128 - key bits
+};
 
 __device__ __forceinline__ uint64_t le_bytes(const uint8_t *p, int n)
 {
@@ -1537,7 +1545,7 @@ __device__ __forceinline__ uint64_t le_bytes(const uint8_t *p, int n)
 }
 
 __global__ void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P, int C, int64_t maxctg, int dbyte,
-                            int swide, int comp, int shift, Rec *__restrict__ keys)
+                            int comp, SeedKeyFields kf, Rec *__restrict__ keys)
 {
     const int s = 1 + 2 * (P + C);
     const int64_t amx = maxctg, bmx = maxctg, maxdag = amx + bmx;
@@ -1554,10 +1562,9 @@ __global__ void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P,
         if (comp) { diag = maxdag - (ipost + jpost); anti = amx - (ipost - jpost); }
         else      { diag = bmx + (ipost - jpost);    anti = ipost + jpost; }
         const int64_t band = diag >> 6;                               // BUCK_SHIFT
-        u128 r = (u128) lcp | ((u128) (uint64_t) (diag & 63) << 8) | ((u128) ((uint64_t) anti & dmask) << 16)
-               | ((u128) ((uint64_t) band & dmask) << (16 + 8 * dbyte))
-               | ((u128) jcont << (16 + 16 * dbyte));
-        const u128 k = (((u128) icont << (8 * swide)) | r) << shift;
+        const u128 low = (u128) lcp | ((u128) (uint64_t) (diag & 63) << 6) | ((u128) ((uint64_t) anti & dmask) << 12);
+        const u128 k = ((((((u128) icont << kf.bc) | jcont) << kf.bb | ((uint64_t) band & dmask)) << (kf.ba + 12))
+                        | low) << kf.shift;
         Rec x;
         x.w0 = (uint64_t) (k >> 64);
         x.w1 = (uint64_t) k;
@@ -1565,21 +1572,26 @@ __global__ void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P,
     }
 }
 
-// Sorted keys back to packed swide-byte records (the record is the key's low
-// 8*swide bits once the left alignment is undone).
+// Sorted keys back to packed swide-byte records: the fields of each key are
+// laid out again as [lcp][diag&63][anti DBYTE][band DBYTE][jcont C], LE.
 constexpr int SP_T = 256, SP_PER = 16;
 
-__global__ __launch_bounds__(SP_T) void k_seed_pack(const Rec *__restrict__ keys, int64_t n, int shift, int swide,
-                                                    uint8_t *__restrict__ out)
+__global__ __launch_bounds__(SP_T) void k_seed_pack(const Rec *__restrict__ keys, int64_t n, SeedKeyFields kf,
+                                                    int dbyte, int swide, uint8_t *__restrict__ out)
 {
     __shared__ u128 slot[SP_T * SP_PER];
-    const u128 rmask = swide >= 16 ? ~(u128) 0 : (((u128) 1) << (8 * swide)) - 1;
     for (int64_t b0 = (int64_t) blockIdx.x * SP_T * SP_PER; b0 < n; b0 += (int64_t) gridDim.x * SP_T * SP_PER) {
         const int cnt = (int) min((int64_t) SP_T * SP_PER, n - b0);
         for (int i = threadIdx.x; i < cnt; i += SP_T) {
             const Rec x = keys[b0 + i];
-            const u128 k = ((u128) x.w0 << 64) | x.w1;
-            slot[i] = (k >> shift) & rmask;
+            const u128 k = (((u128) x.w0 << 64) | x.w1) >> kf.shift;
+            const uint64_t lcp = (uint64_t) k & 63u;
+            const uint64_t d6 = (uint64_t) (k >> 6) & 63u;
+            const uint64_t anti = (uint64_t) (k >> 12) & ((1ull << kf.ba) - 1);
+            const uint64_t band = (uint64_t) (k >> (12 + kf.ba)) & ((1ull << kf.bb) - 1);
+            const uint64_t jcont = (uint64_t) (k >> (12 + kf.ba + kf.bb)) & ((1ull << kf.bc) - 1);
+            slot[i] = (u128) lcp | ((u128) d6 << 8) | ((u128) anti << 16) | ((u128) band << (16 + 8 * dbyte))
+                    | ((u128) jcont << (16 + 16 * dbyte));
         }
         __syncthreads();
         write_packed<SP_T>(out + b0 * swide, slot, cnt, swide);
