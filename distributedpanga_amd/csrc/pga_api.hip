@@ -225,12 +225,12 @@ unsigned grid_for(const pga_ctx *c, const void *fn, int threads, size_t lds, int
 
 int chunk_per(const pga_ctx *c) { return c->o.chunk_records / 256; }
 
-int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *src, const Seg *segs, int nseg,
+int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *buf0, const Rec *buf1, const Seg *segs, int nseg,
                     const int64_t *chk0, unsigned long long *hist)
 {
 #define PGA_HIST(PER)                                                                                  \
     hipLaunchKernelGGL(k_seg_hist<PER>, dim3(grid_for(c, (const void *) k_seg_hist<PER>, 256, 0, nlog)), \
-                       dim3(256), 0, c->st, src, segs, nseg, chk0, hist, (int64_t) nlog)
+                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, hist, (int64_t) nlog)
     switch (per) {
     case 8:  PGA_HIST(8); break;
     case 32: PGA_HIST(32); break;
@@ -241,13 +241,13 @@ int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *src, const Se
     return PGA_OK;
 }
 
-int launch_seg_scatter(pga_ctx *c, int per, unsigned nlog, const Rec *src, Rec *dst, const Seg *segs,
+int launch_seg_scatter(pga_ctx *c, int per, unsigned nlog, Rec *buf0, Rec *buf1, const Seg *segs,
                        int nseg, const int64_t *chk0, unsigned long long *cursor)
 {
     // chunks of 2048 / 4096 records: 256 threads; 8192: 512 threads of 16 records
 #define PGA_SCAT(TT, PER)                                                                              \
     hipLaunchKernelGGL((k_seg_scatter<TT, PER>), dim3(grid_for(c, (const void *) k_seg_scatter<TT, PER>, TT, 0, nlog)), \
-                       dim3(TT), 0, c->st, src, dst, segs, nseg, chk0, cursor, (int64_t) nlog)
+                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, cursor, (int64_t) nlog)
     switch (per) {
     case 8:  PGA_SCAT(256, 8); break;
     case 32: PGA_SCAT(512, 16); break;
@@ -510,7 +510,8 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
     const int per = chunk_per(c);
     const int CH = per * 256;
     Rec *buf[2] = {buf0, buf1};
-    int *dcount = P<int>(c->counters);          // [0] next segs, [1] tiles
+    // [0] next segs, [1] tiles, [2..3] records left in place by the level (u64)
+    int *dcount = P<int>(c->counters);
     int hcount[4] = {0, (int) tile0.size(), 0, 0};
     HIPCHK(c, hipMemcpyAsync(dcount, hcount, sizeof(hcount), hipMemcpyHostToDevice, c->st));
     if (!tile0.empty())
@@ -528,31 +529,32 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         const int64_t nchunks = ck[0];
-        c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * ck[1];   // each record read and written once
-        const Rec *src = buf[(level - 1) & 1];
-        Rec *dst = buf[level & 1];
         const bool counted = level == 1 && hist1;
         if (!counted) {
             TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * 256, 0));
-            TRY(launch_seg_hist(c, per, (unsigned) nchunks, src, P<Seg>(c->segs[cur]), nseg,
+            TRY(launch_seg_hist(c, per, (unsigned) nchunks, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg,
                                 P<int64_t>(c->chk0), P<unsigned long long>(c->seghist)));
         }
         HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(int), c->st));
-        hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
+        HIPCHK(c, hipMemsetAsync(dcount + 2, 0, 2 * sizeof(int), c->st));
+        hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, P<Seg>(c->segs[cur]),
                            counted ? hist1 : (const unsigned long long *) P<unsigned long long>(c->seghist),
-                           counted ? 1 : 0, CAP, level & 1, P<unsigned long long>(c->cursor),
+                           counted ? 1 : 0, CAP, P<unsigned long long>(c->cursor),
                            P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
-                           (int) tile_cap);
+                           (int) tile_cap, reinterpret_cast<unsigned long long *>(dcount + 2));
         LAUNCHCHK(c);
         hipEvent_t se[2];
         TRY(scatter_event(c, se));
         HIPCHK(c, hipEventRecord(se[0], c->st));
-        TRY(launch_seg_scatter(c, per, (unsigned) nchunks, src, dst, P<Seg>(c->segs[cur]), nseg,
+        TRY(launch_seg_scatter(c, per, (unsigned) nchunks, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg,
                                P<int64_t>(c->chk0), P<unsigned long long>(c->cursor)));
         HIPCHK(c, hipEventRecord(se[1], c->st));
         c->nscatter++;
-        HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(hcount, dcount, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
+        uint64_t kept;
+        memcpy(&kept, &hcount[2], sizeof(kept));
+        c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * (ck[1] - (int64_t) kept);   // moved records, read + write
         nseg = hcount[0];
         cur ^= 1;
         level++;

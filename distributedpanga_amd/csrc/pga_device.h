@@ -35,6 +35,8 @@ struct PairDev {
 struct Seg {                     // a key range still to be partitioned
     int64_t start, count;
     int32_t pair, level;         // level = digit (1..16) to split on next
+    int32_t buf;                 // record buffer (0/1) holding its records
+    int32_t skip;                // set by the plan: one digit only, left in place
 };
 
 struct Tile {                    // a bucket run small enough for one workgroup sort

@@ -497,7 +497,8 @@ __global__ __launch_bounds__(1024) void k_chunk_prefix(const Seg *__restrict__ s
 constexpr int PT = 256;   // threads of the partition kernels
 
 template <int PER>
-__global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ src, const Seg *__restrict__ segs,
+__global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, const Rec *__restrict__ buf1,
+                                                 const Seg *__restrict__ segs,
                                                  int nseg, const int64_t *__restrict__ chk0,
                                                  unsigned long long *__restrict__ seghist, int64_t nlog)
 {
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ src, co
         const Seg sg = segs[s];
         const int64_t off = (LB - chk0[s]) * (int64_t) CH;
         const int n = (int) min((int64_t) CH, sg.count - off);
-        const Rec *p = src + sg.start + off;
+        const Rec *p = (sg.buf ? buf1 : buf0) + sg.start + off;
 #pragma unroll 4
         for (int k = 0; k < PER; k++) {
             const int i = k * PT + threadIdx.x;
@@ -528,19 +529,39 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ src, co
 
 // Per segment: digit offsets (cursors) and the children.  Children of at most
 // `cap` records are grouped into sort tiles (runs of consecutive non-empty
-// buckets); larger ones become segments of the next level.
-__global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
+// buckets); larger ones become segments of the next level.  Children go to the
+// other record buffer, except when the segment has one non-empty bucket: then
+// its records are already in bucket order, the scatter skips it, and its child
+// (segment or tile) stays where it is.
+__global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
                                                   const unsigned long long *__restrict__ seghist,
-                                                  int hist_by_pair, int cap, int dstbuf,
+                                                  int hist_by_pair, int cap,
                                                   unsigned long long *__restrict__ cursor,
                                                   Seg *__restrict__ next, int *__restrict__ nnext, int next_cap,
-                                                  Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap)
+                                                  Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap,
+                                                  unsigned long long *__restrict__ kept)
 {
     __shared__ int64_t sh[256 / 64 + 1];
     __shared__ int64_t cnt[256], off[256];
     const Seg sg = segs[blockIdx.x];
     size_t row = hist_by_pair ? (size_t) sg.pair : (size_t) blockIdx.x;
     int64_t c = (int64_t) seghist[row * 256 + threadIdx.x];
+    if (__syncthreads_count(c != 0) == 1) {
+        // a single bucket: no scatter; the child keeps the buffer and the range
+        if (threadIdx.x == 0) {
+            segs[blockIdx.x].skip = 1;
+            if (sg.count > cap) {
+                const int q = atomicAdd(nnext, 1);
+                if (q < next_cap) next[q] = Seg{sg.start, sg.count, sg.pair, sg.level + 1, sg.buf, 0};
+            } else {
+                const int t = atomicAdd(ntiles, 1);
+                if (t < tile_cap) tiles[t] = Tile{sg.start, (int32_t) sg.count, sg.pair, sg.buf, sg.level};
+            }
+            atomicAdd(kept, (unsigned long long) sg.count);
+        }
+        return;
+    }
+    const int dstbuf = sg.buf ^ 1;
     int64_t tot;
     int64_t o = block_excl_scan<256, int64_t>(c, sh, &tot);
     cnt[threadIdx.x] = c;
@@ -563,7 +584,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
                     ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
                     tc = 0;
                 }
-                lnext[q++] = Seg{sg.start + off[d], k, sg.pair, sg.level + 1};
+                lnext[q++] = Seg{sg.start + off[d], k, sg.pair, sg.level + 1, dstbuf, 0};
             } else {
                 if (tc + k > cap) {
                     ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
@@ -592,7 +613,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(const Seg *__restrict__ segs,
 // records are loaded into registers before the current chunk is written out,
 // so its reads overlap the writes.
 template <int T, int PER>
-__global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, Rec *__restrict__ dst,
+__global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *__restrict__ buf1,
                                                    const Seg *__restrict__ segs, int nseg,
                                                    const int64_t *__restrict__ chk0,
                                                    unsigned long long *__restrict__ cursor, int64_t nlog)
@@ -606,15 +627,17 @@ __global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, 
     __shared__ uint32_t sh[T / 64 + 1];
     const int tid = threadIdx.x;
     Rec r[PER];
-    int s = 0, n = 0, level = 1;
-    // records of chunk lb into r[] (segment s, n records, digit level)
+    int s = 0, n = 0, level = 1, from = 0;
+    // records of chunk lb into r[] (segment s, n records, digit level, in
+    // buffer `from`); chunks of skipped segments have no records to move
     auto fetch = [&](int64_t lb) {
         s = seg_of_chunk(chk0, nseg, lb);
         const Seg sg = segs[s];
         const int64_t off = (lb - chk0[s]) * (int64_t) CH;
-        n = (int) min((int64_t) CH, sg.count - off);
+        n = sg.skip ? 0 : (int) min((int64_t) CH, sg.count - off);
         level = sg.level;
-        const Rec *p = src + sg.start + off;
+        from = sg.buf;
+        const Rec *p = (from ? buf1 : buf0) + sg.start + off;
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int i = k * T + tid;
@@ -647,6 +670,7 @@ __global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, 
     }
     __syncthreads();
     const int ncur = n, lcur = level;
+    Rec *const dst = from ? buf0 : buf1;
     if (LB + gridDim.x < nlog) fetch(LB + gridDim.x);
 #pragma unroll
     for (int k = 0; k < PER; k++) {
@@ -661,12 +685,12 @@ __global__ __launch_bounds__(T) void k_seg_scatter(const Rec *__restrict__ src, 
     }
 }
 
-template __global__ void k_seg_hist<16>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_hist<8>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_hist<32>(const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<256, 8>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<256, 16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<512, 16>(const Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
 
 // ---------------------------------------------------------------- tile sort
 

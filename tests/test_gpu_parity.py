@@ -90,6 +90,18 @@ def test_low_complexity_deep_partition():
     _compare(synth.from_codes(contigs), tile_records=2048)
 
 
+def test_identical_kmers_stay_in_place():
+    """30 kb of poly-A: the forward records share one k-mer, the reverse ones
+    another, and each group is beyond a tile.  Below the first digit every
+    segment has one non-empty bucket until the payload digits, so those levels
+    leave the records where they are: only levels that split move records."""
+    g = synth.from_codes([np.zeros(30000, np.uint8)])
+    got, _ = _compare(g, tile_records=2048)
+    t, nrec = got["timings"], got["info"]["nrec"]
+    assert t["partition_levels"] >= 12
+    assert t["scatter_bytes"] <= 4 * 2 * 16 * nrec
+
+
 def test_many_contigs_two_byte_ids():
     rng = np.random.default_rng(8)
     contigs = [rng.integers(0, 4, int(n)).astype(np.uint8) for n in rng.integers(12, 400, 300)]
