@@ -156,17 +156,18 @@ def serving_leg(args):
                 pcie_inclusive=True)
 
 
-def human_leg(args):
+def human_leg(args, opts):
     """The human-scale pair (3.1 Gbp A + B, 24 contigs of 50-250 Mbp, 45%
     repeats, satellites; B with 2% SNP, 0.3% indel, 200 rearrangements) at
     --human-scale: index build + merge with the input resident in HBM, the
-    device sortedness/LCP check, and the memory the context holds."""
+    device sortedness/LCP check, and the memory the context holds.  It runs
+    with the plain leg's shape options (tuned or given)."""
     t = time.perf_counter()
     log(f"[bench] generating the human-scale pair at scale {args.human_scale}")
     g = synth.workload(4, args.human_scale)
     log(f"[bench] human-scale pair at scale {args.human_scale}: {g.ncontig} contigs, {g.seqtot} bases "
         f"generated ({time.perf_counter() - t:.1f}s)")
-    ctx = pga.Context(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+    ctx = pga.Context(**opts)
     ctx.load_gdb(g)
     steps = max(1, args.human_steps)
     dt, per, scat_bytes, _, _ = time_steps(ctx, steps, 1)
@@ -182,7 +183,7 @@ def human_leg(args):
                stages_ms={k: round(v, 3) for k, v in per.items()},
                radix_scatter_gbs=round(scat_gbs, 1), radix_scatter_frac=round(scat_gbs / HBM_PEAK_GBS, 4),
                verify=dict(kmer_order_violations=bad, first_bad=first_bad, lcp_mismatches=lcp_bad),
-               P=info["P"], C=info["C"], maxpre=info["maxpre"])
+               P=info["P"], C=info["C"], maxpre=info["maxpre"], opts=dict(opts))
     ctx.close()
     log(f"[bench] human-scale: {ms:.1f} ms/step, {out['gbases_s']} Gbases/s, {info['nrec']} records, "
         f"{nseed} seeds, verify {out['verify']}")
@@ -356,7 +357,7 @@ def main():
         out["serving"] = serving_leg(args)
 
     if args.human_scale > 0:
-        out["human_scale"] = human_leg(args)
+        out["human_scale"] = human_leg(args, opts)
 
     if not args.no_cpu:
         t = time.perf_counter()
