@@ -1219,19 +1219,29 @@ __device__ __forceinline__ uint64_t lcp_below(uint64_t w, uint32_t plen)
 __device__ __forceinline__ uint32_t anchor_info16(const uint8_t *__restrict__ win, int gi, int freq,
                                                   uint32_t &xn, uint32_t &xc)
 {
-    const uint64_t b0 = win8(win, gi - 7), b1 = win8(win, gi - 15);   // gi-7..gi, gi-15..gi-8
-    const uint64_t f0 = win8(win, gi + 1), f1 = win8(win, gi + 9);    // gi+1..gi+8, gi+9..gi+16
+    const uint64_t b0 = win8(win, gi - 7), f0 = win8(win, gi + 1);    // gi-7..gi, gi+1..gi+8
     const uint32_t Li = (uint32_t) (b0 >> 56) & 63u, Ln = (uint32_t) f0 & 63u;
     const uint32_t li = Li >= 12 ? Li : 0, ln = Ln >= 12 ? Ln : 0;
     const uint32_t plen = li > ln ? li : ln;
     xn = xc = 0;
     if (plen < 12) return 0;
-    const uint64_t lb0 = lcp_below(b0, plen), lb1 = lcp_below(b1, plen);
-    const uint64_t lf0 = lcp_below(f0, plen), lf1 = lcp_below(f1, plen);
+    const uint64_t lb0 = lcp_below(b0, plen), lf0 = lcp_below(f0, plen);
     // entries gi, gi-1, ... sharing >= plen (cb), and gi+1, gi+2, ... (cf):
-    // the interval is [gi - cb, gi + cf + 1)
-    const int cb = lb0 ? __builtin_clzll(lb0) >> 3 : 8 + (lb1 ? __builtin_clzll(lb1) >> 3 : 8);
-    const int cf = lf0 ? __builtin_ctzll(lf0) >> 3 : 8 + (lf1 ? __builtin_ctzll(lf1) >> 3 : 8);
+    // the interval is [gi - cb, gi + cf + 1).  The next 8 bytes on a side
+    // (gi-15..gi-8, gi+9..gi+16) are read only when its run reaches them.
+    uint64_t b1 = 0, f1 = 0;
+    int cb = lb0 ? __builtin_clzll(lb0) >> 3 : 8;
+    if (!lb0) {
+        b1 = win8(win, gi - 15);
+        const uint64_t lb1 = lcp_below(b1, plen);
+        cb = 8 + (lb1 ? __builtin_clzll(lb1) >> 3 : 8);
+    }
+    int cf = lf0 ? __builtin_ctzll(lf0) >> 3 : 8;
+    if (!lf0) {
+        f1 = win8(win, gi + 9);
+        const uint64_t lf1 = lcp_below(f1, plen);
+        cf = 8 + (lf1 ? __builtin_ctzll(lf1) >> 3 : 8);
+    }
     const int m = cb + 1 + cf;
     if (m >= freq) return 0;                     // so cb, cf <= 14 below
     // strand bits of the partners gi-cb..gi-1 and gi+1..gi+cf
