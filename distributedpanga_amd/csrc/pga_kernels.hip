@@ -52,20 +52,26 @@ __device__ __forceinline__ void window80(const uint64_t *__restrict__ b, int64_t
     }
 }
 
-// Closed (12,8)-syncmer test on the 12 bases in bits 0..23 of W (LSB-first).
+// Canonical hash of the 8-mer at base t of W (LSB-first), src/extract.c:85-249:
+// min(forward, reverse complement) of (TMap[4-mer] << 8 | TMap[4-mer]).
 // tl[x] = TMap[field-reversed x], tm[x] = TMap[x]; the reverse-complement
 // 4-mer of an LSB-first byte x, packed MSB-first, is ~x.
+__device__ __forceinline__ uint32_t hash8(uint64_t W, int t, const uint32_t *tl, const uint32_t *tm)
+{
+    const uint32_t lo = (uint32_t) (W >> (2 * t)) & 0xFFu;
+    const uint32_t hi = (uint32_t) (W >> (2 * t + 8)) & 0xFFu;
+    const uint32_t fw = (tl[lo] << 8) | tl[hi];
+    const uint32_t rv = (tm[hi ^ 0xFFu] << 8) | tm[lo ^ 0xFFu];
+    return fw < rv ? fw : rv;
+}
+
+// Closed (12,8)-syncmer test on the 12 bases in bits 0..23 of W (LSB-first):
+// the earliest or the last of the five 8-mer hashes is the minimum.
 __device__ __forceinline__ bool syncmer(uint64_t W, const uint32_t *tl, const uint32_t *tm)
 {
     uint32_t h[5];
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        uint32_t lo = (uint32_t) (W >> (2 * t)) & 0xFFu;
-        uint32_t hi = (uint32_t) (W >> (2 * t + 8)) & 0xFFu;
-        uint32_t fw = (tl[lo] << 8) | tl[hi];
-        uint32_t rv = (tm[hi ^ 0xFFu] << 8) | tm[lo ^ 0xFFu];
-        h[t] = fw < rv ? fw : rv;
-    }
+    for (int t = 0; t < 5; t++) h[t] = hash8(W, t, tl, tm);
     uint32_t m = min(min(min(h[0], h[1]), min(h[2], h[3])), h[4]);
     return h[0] == m || h[4] == m;
 }
@@ -273,16 +279,39 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
     int c = bctg[LB];
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint32_t cnt = 0, bits = 0;
-    XEmit x;
+    while (c + 1 < nctg && ctg[c + 1].pos0 <= f0) c++;
+    const ContigDev c0 = ctg[c];
+    if (f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand) {
+        // the thread's XU positions lie in one contig: the 8-mers at
+        // j0 .. j0+XU+3 are hashed once and shared by the XU windows
+        const int64_t j0 = f0 - c0.pos0;
+        uint64_t W0;
+        uint32_t W1;
+        window80(bps64, c0.bitoff + 2 * j0, W0, W1);
+        uint32_t h[XU + 4];
 #pragma unroll
-    for (int u = 0; u < XU; u++) {
-        const int64_t f = f0 + u;
-        if (f >= F) break;
-        while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
-        const ContigDev cc = ctg[c];
-        if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
-        cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, x);
-        bits |= ((x.fwd ? 1u : 0u) | (x.rev ? 2u : 0u)) << (2 * u);
+        for (int t = 0; t < XU + 4; t++) h[t] = hash8(W0, t, tl, tm);
+#pragma unroll
+        for (int u = 0; u < XU; u++) {
+            const uint32_t m = min(min(min(h[u], h[u + 1]), min(h[u + 2], h[u + 3])), h[u + 4]);
+            if (h[u] != m && h[u + 4] != m) continue;
+            const int64_t j = j0 + u;
+            const bool fwd = j <= (int64_t) c0.clen - 40, rev = j >= 28;
+            cnt += (uint32_t) fwd + (uint32_t) rev;
+            bits |= ((fwd ? 1u : 0u) | (rev ? 2u : 0u)) << (2 * u);
+        }
+    } else {
+        XEmit x;
+#pragma unroll
+        for (int u = 0; u < XU; u++) {
+            const int64_t f = f0 + u;
+            if (f >= F) break;
+            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+            const ContigDev cc = ctg[c];
+            if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
+            cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, x);
+            bits |= ((x.fwd ? 1u : 0u) | (x.rev ? 2u : 0u)) << (2 * u);
+        }
     }
     sel[fb / XU + threadIdx.x] = (uint8_t) bits;       // the emit pass reads the selection back
     uint64_t tot = block_sum_u64<XT>(cnt, red);
