@@ -35,6 +35,22 @@ __device__ __forceinline__ uint32_t field_rev16(uint32_t x)
     return ((y >> 1) & 0x5555u) | ((y & 0x5555u) << 1);
 }
 
+// 128 bits (64 bases, LSB-first as in .bps) starting at bit offset `bit`.
+__device__ __forceinline__ void window128(const uint64_t *__restrict__ b, int64_t bit, uint64_t &lo, uint64_t &hi)
+{
+    const int64_t w = bit >> 6;
+    const int s = (int) (bit & 63);
+    const uint64_t x = b[w], y = b[w + 1];
+    if (s) {
+        const uint64_t z = b[w + 2];
+        lo = (x >> s) | (y << (64 - s));
+        hi = (y >> s) | (z << (64 - s));
+    } else {
+        lo = x;
+        hi = y;
+    }
+}
+
 // 80 bits (40 bases, LSB-first as in .bps) starting at bit offset `bit`.
 __device__ __forceinline__ void window80(const uint64_t *__restrict__ b, int64_t bit,
                                          uint64_t &lo, uint32_t &hi)
@@ -364,37 +380,57 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
     uint32_t total;
     uint32_t k = block_excl_scan<XT, uint32_t>((uint32_t) __builtin_popcount(bits), scan_sh, &total);
     Rec *dst = out + blkoff[LB];
+    auto put = [&](uint64_t w0, uint64_t w1, int pair) {
+        Rec r;
+        r.w0 = w0;
+        r.w1 = w1;
+        if (k < (uint32_t) (XT * XU)) stage[k] = r;
+        else dst[k] = r;
+        k++;
+        const uint32_t d = (uint32_t) (w0 >> 56);
+        if (pair == pair0) atomicAdd(&hist[d], 1u);
+        else atomicAdd(&hist1[(size_t) pair * 256 + d], 1ull);
+    };
+    while (c + 1 < nctg && ctg[c + 1].pos0 <= f0) c++;
+    const ContigDev c0 = ctg[c];
+    if (bits && f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand && f0 - c0.pos0 >= 28) {
+        // the thread's XU positions lie in one contig, past its first 28
+        // bases: their forward windows come from one 128-bit span at j0 and
+        // their reverse-complement windows from one at j0-28
+        const int64_t j0 = f0 - c0.pos0;
+        uint64_t A0, A1, B0, B1;
+        window128(bps64, c0.bitoff + 2 * j0, A0, A1);
+        window128(bps64, c0.bitoff + 2 * (j0 - 28), B0, B1);
 #pragma unroll
-    for (int u = 0; u < XU; u++) {
-        const int64_t f = f0 + u;
-        if (!((bits >> (2 * u)) & 3u)) continue;
-        while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
-        const ContigDev cc = ctg[c];
-        XEmit e;
-        e.fwd = (bits >> (2 * u)) & 1u;
-        e.rev = (bits >> (2 * u + 1)) & 1u;
-        record_keys(bps64, cc, f - cc.pos0, jbits, e);
-        if (e.fwd) {
-            Rec r;
-            r.w0 = e.f0;
-            r.w1 = e.f1;
-            if (k < (uint32_t) (XT * XU)) stage[k] = r;
-            else dst[k] = r;
-            k++;
-            const uint32_t d = (uint32_t) (e.f0 >> 56);
-            if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
-            else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
+        for (int u = 0; u < XU; u++) {
+            if (!((bits >> (2 * u)) & 3u)) continue;
+            const uint64_t pay = ((uint64_t) c0.local << (jbits + 1)) | ((uint64_t) (j0 + u) << 1);
+            const int sh = 2 * u;
+            if ((bits >> (2 * u)) & 1u) {
+                const uint64_t W0 = sh ? (A0 >> sh) | (A1 << (64 - sh)) : A0;
+                const uint32_t W1 = (uint32_t) (A1 >> sh) & 0xFFFFu;
+                put(field_rev64(W0), ((uint64_t) field_rev16(W1) << 48) | pay, c0.pair);
+            }
+            if ((bits >> (2 * u + 1)) & 1u) {
+                const uint64_t V0 = sh ? (B0 >> sh) | (B1 << (64 - sh)) : B0;
+                const uint32_t V1 = (uint32_t) (B1 >> sh) & 0xFFFFu;
+                put(~(((uint64_t) V1 << 48) | (V0 >> 16)), ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull,
+                    c0.pair);
+            }
         }
-        if (e.rev) {
-            Rec r;
-            r.w0 = e.r0;
-            r.w1 = e.r1;
-            if (k < (uint32_t) (XT * XU)) stage[k] = r;
-            else dst[k] = r;
-            k++;
-            const uint32_t d = (uint32_t) (e.r0 >> 56);
-            if (cc.pair == pair0) atomicAdd(&hist[d], 1u);
-            else atomicAdd(&hist1[(size_t) cc.pair * 256 + d], 1ull);
+    } else {
+#pragma unroll
+        for (int u = 0; u < XU; u++) {
+            const int64_t f = f0 + u;
+            if (!((bits >> (2 * u)) & 3u)) continue;
+            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+            const ContigDev cc = ctg[c];
+            XEmit e;
+            e.fwd = (bits >> (2 * u)) & 1u;
+            e.rev = (bits >> (2 * u + 1)) & 1u;
+            record_keys(bps64, cc, f - cc.pos0, jbits, e);
+            if (e.fwd) put(e.f0, e.f1, cc.pair);
+            if (e.rev) put(e.r0, e.r1, cc.pair);
         }
     }
     __syncthreads();
