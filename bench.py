@@ -199,6 +199,7 @@ def main():
     ap.add_argument("--tile", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=4096)
     ap.add_argument("--merge-tile", type=int, default=256)
+    ap.add_argument("--digit2", type=int, default=10, help="bits of the partition's second digit (8..10)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-bases", type=int, default=150_000_000)
@@ -214,7 +215,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-max-batch", type=int, default=8)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--no-tune", action="store_true", help="use the fixed --tile/--chunk/--merge-tile")
+    ap.add_argument("--no-tune", action="store_true", help="use the fixed --tile/--chunk/--merge-tile/--digit2")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
     args = ap.parse_args()
 
@@ -223,7 +224,8 @@ def main():
               "(the multi-GPU exchange is a design, not a run)", file=sys.stderr)
         sys.exit(2)
 
-    opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+    opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
+                digit2_bits=args.digit2)
     t = time.perf_counter()
     g = synth.workload(2, args.scale)
     log(f"[bench] workload 2 generated: {g.ncontig} contigs, {g.seqtot} bases ({time.perf_counter() - t:.1f}s)")
@@ -301,6 +303,7 @@ def main():
                       "bases": g.seqtot, "contigs": g.ncontig, "records": info["nrec"], "seeds": int(nseed),
                       "scale": args.scale, "tile_records": opts["tile_records"],
                       "chunk_records": opts["chunk_records"], "merge_tile": opts["merge_tile"],
+                      "digit2_bits": opts["digit2_bits"],
                       "autotuned": tuned is not None, "parallelism": "1 GPU"},
            "stages_ms": {k: round(v, 4) for k, v in per.items()},
            "h2d_load_s": round(h2d_s, 4),
@@ -313,7 +316,8 @@ def main():
         gdbs = synth.batch(args.batch_pairs, scale=args.scale)
         tot = sum(x.seqtot for x in gdbs)
         log(f"[bench] batch workload generated: {len(gdbs)} pairs, {tot} bases ({time.perf_counter() - t:.1f}s)")
-        bopts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+        bopts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
+                     digit2_bits=args.digit2)
         bctx = pga.Context(**bopts)
         bctx.load_batch(gdbs)
         if not args.no_tune:
@@ -325,7 +329,8 @@ def main():
         batched = tot * bsteps / bdt / 1e9
         # plain form: the same pairs one at a time (input resident, one build + merge per pair),
         # tuned on the median pair
-        popts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile)
+        popts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
+                     digit2_bits=args.digit2)
         pctx = pga.Context(**popts)
         if not args.no_tune:
             med = sorted(gdbs, key=lambda x: x.seqtot)[len(gdbs) // 2]

@@ -6,8 +6,11 @@ Tunables (all bit-exact-neutral: every setting gives the same output bytes):
   chunk_records  records per radix-scatter workgroup (2K / 4K / 8K): the run
                  length per digit, i.e. write coalescing, against occupancy
   merge_tile     anchors per merge workgroup (256 / 512 / 1024)
+  digit2_bits    width of the partition's second digit (8 / 9 / 10): wider
+                 digits split the second level finer, so with smaller tiles
+                 fewer records need a third pass
 
-The search is exhaustive (27 points, a couple of steps each, on the loaded
+The search is exhaustive (81 points, a couple of steps each, on the loaded
 batch) and the winner is cached in a JSON file keyed by the batch shape.
 """
 import itertools
@@ -16,7 +19,8 @@ import math
 import os
 import time
 
-SPACE = dict(tile_records=(2048, 4096, 8192), chunk_records=(2048, 4096, 8192), merge_tile=(256, 512, 1024))
+SPACE = dict(tile_records=(2048, 4096, 8192), chunk_records=(2048, 4096, 8192), merge_tile=(256, 512, 1024),
+             digit2_bits=(8, 9, 10))
 
 
 def cache_path():
@@ -72,12 +76,14 @@ def tune(ctx, gdbs, steps=2, use_cache=True, log=None):
         ctx.set_opts(**best)
         return best, cache[key].get("table", [])
     table = []
-    for t, c, m in itertools.product(*SPACE.values()):
-        ctx.set_opts(tile_records=t, chunk_records=c, merge_tile=m)
+    for point in itertools.product(*SPACE.values()):
+        o = dict(zip(SPACE, point))
+        ctx.set_opts(**o)
         dt = _time(ctx, steps)
-        table.append(dict(tile_records=t, chunk_records=c, merge_tile=m, ms=round(dt * 1e3, 3)))
+        table.append(dict(o, ms=round(dt * 1e3, 3)))
         if log:
-            log(f"[tune] {key} tile={t} chunk={c} merge={m}: {dt * 1e3:.2f} ms")
+            log(f"[tune] {key} tile={o['tile_records']} chunk={o['chunk_records']} merge={o['merge_tile']} "
+                f"digit2={o['digit2_bits']}: {dt * 1e3:.2f} ms")
     best = min(table, key=lambda r: r["ms"])
     opts = {k: best[k] for k in SPACE}
     ctx.set_opts(**opts)

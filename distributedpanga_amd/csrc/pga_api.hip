@@ -225,12 +225,16 @@ unsigned grid_for(const pga_ctx *c, const void *fn, int threads, size_t lds, int
 
 int chunk_per(const pga_ctx *c) { return c->o.chunk_records / 256; }
 
-int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *buf0, const Rec *buf1, const Seg *segs, int nseg,
-                    const int64_t *chk0, unsigned long long *hist)
+// The partition kernels loop over the chunks counted in chk0[nseg] on a grid
+// of one resident wave of blocks; blocks past the chunk count exit at once.
+constexpr int64_t ANY_CHUNKS = (int64_t) 1 << 40;
+
+int launch_seg_hist(pga_ctx *c, int per, const Rec *buf0, const Rec *buf1, const Seg *segs, int nseg,
+                    const int64_t *chk0, int level, unsigned long long *hist)
 {
 #define PGA_HIST(PER)                                                                                  \
-    hipLaunchKernelGGL(k_seg_hist<PER>, dim3(grid_for(c, (const void *) k_seg_hist<PER>, 256, 0, nlog)), \
-                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, hist, (int64_t) nlog)
+    hipLaunchKernelGGL(k_seg_hist<PER>, dim3(grid_for(c, (const void *) k_seg_hist<PER>, 256, 0, ANY_CHUNKS)), \
+                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, hist)
     switch (per) {
     case 8:  PGA_HIST(8); break;
     case 32: PGA_HIST(32); break;
@@ -241,13 +245,14 @@ int launch_seg_hist(pga_ctx *c, int per, unsigned nlog, const Rec *buf0, const R
     return PGA_OK;
 }
 
-int launch_seg_scatter(pga_ctx *c, int per, unsigned nlog, Rec *buf0, Rec *buf1, const Seg *segs,
-                       int nseg, const int64_t *chk0, unsigned long long *cursor)
+int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *segs,
+                       int nseg, const int64_t *chk0, int level, unsigned long long *cursor)
 {
     // chunks of 2048 / 4096 records: 256 threads; 8192: 512 threads of 16 records
 #define PGA_SCAT(TT, PER)                                                                              \
-    hipLaunchKernelGGL((k_seg_scatter<TT, PER>), dim3(grid_for(c, (const void *) k_seg_scatter<TT, PER>, TT, 0, nlog)), \
-                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, cursor, (int64_t) nlog)
+    hipLaunchKernelGGL((k_seg_scatter<TT, PER>),                                                        \
+                       dim3(grid_for(c, (const void *) k_seg_scatter<TT, PER>, TT, 0, ANY_CHUNKS)),      \
+                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, cursor)
     switch (per) {
     case 8:  PGA_SCAT(256, 8); break;
     case 32: PGA_SCAT(512, 16); break;
@@ -269,8 +274,8 @@ int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entri
     const PairDev *pairs = P<PairDev>(c->dpairs);
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
-                       dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, entries, ls, pre, skeys,         \
-                       (int64_t) nlog)
+                       dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, c->o.digit2_bits, entries, ls,   \
+                       pre, skeys, (int64_t) nlog)
     switch (T) {
     case 256: PGA_TSORT(256); break;
     case 1024: PGA_TSORT(1024); break;
@@ -323,6 +328,7 @@ void pga_default_opts(pga_opts *o)
     o->chunk_records = 4096;
     o->merge_tile = 256;
     o->build_cum = 1;
+    o->digit2_bits = 10;
 }
 
 const char *pga_last_error(const pga_ctx *c) { return c ? c->err : "no context"; }
@@ -332,7 +338,8 @@ static bool opts_valid(const pga_opts &q)
     return q.freq >= 1 && q.freq <= MAX_FREQ &&
            (q.tile_records == 2048 || q.tile_records == 4096 || q.tile_records == 8192) &&
            (q.chunk_records == 2048 || q.chunk_records == 4096 || q.chunk_records == 8192) &&
-           (q.merge_tile == 256 || q.merge_tile == 512 || q.merge_tile == 1024);
+           (q.merge_tile == 256 || q.merge_tile == 512 || q.merge_tile == 1024) &&
+           q.digit2_bits >= 8 && q.digit2_bits <= 10;
 }
 
 int pga_set_opts(pga_ctx *c, const pga_opts *o)
@@ -498,10 +505,11 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
 
 namespace {
 
-// MSD radix partition: the segments seg0 (records in buf0) are split on 8-bit
-// digits, level after level, until every bucket fits a sort tile; tiles go to
-// c->tiles after the tile0 ones (c->ntiles is set).  Level-1 digit counts come
-// from hist1 (one 256-row per pair) when given, else they are counted here.
+// MSD radix partition: the segments seg0 (records in buf0) are split on
+// digits (8 bits; opts.digit2_bits at the second level), level after level,
+// until every bucket fits a sort tile; tiles go to c->tiles after the tile0
+// ones (c->ntiles is set).  Level-1 digit counts come from hist1 (one 256-row
+// per pair) when given, else they are counted here.
 // The tiles' `buf` index refers to {buf0, buf1}.
 int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, const std::vector<Tile> &tile0,
               const unsigned long long *hist1, int64_t seg_cap, int64_t tile_cap, int *levels_out)
@@ -525,31 +533,33 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
                            nseg, CH, P<int64_t>(c->chk0));
         LAUNCHCHK(c);
-        int64_t ck[2] = {0, 0};
-        HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        const int64_t nchunks = ck[0];
         const bool counted = level == 1 && hist1;
+        const int64_t nd = level == 2 ? (int64_t) 1 << c->o.digit2_bits : 256;   // digit rows of this level
+        TRY(ensure(c, c->seghist, (size_t) nseg * nd * sizeof(uint64_t)));
+        TRY(ensure(c, c->cursor, (size_t) nseg * nd * sizeof(uint64_t)));
         if (!counted) {
-            TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * 256, 0));
-            TRY(launch_seg_hist(c, per, (unsigned) nchunks, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg,
-                                P<int64_t>(c->chk0), P<unsigned long long>(c->seghist)));
+            TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * nd, 0));
+            TRY(launch_seg_hist(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg, P<int64_t>(c->chk0), level,
+                                P<unsigned long long>(c->seghist)));
         }
         HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(int), c->st));
         HIPCHK(c, hipMemsetAsync(dcount + 2, 0, 2 * sizeof(int), c->st));
         hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, P<Seg>(c->segs[cur]),
                            counted ? hist1 : (const unsigned long long *) P<unsigned long long>(c->seghist),
-                           counted ? 1 : 0, CAP, P<unsigned long long>(c->cursor),
+                           counted ? 1 : 0, level, c->o.digit2_bits, CAP, P<unsigned long long>(c->cursor),
                            P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
                            (int) tile_cap, reinterpret_cast<unsigned long long *>(dcount + 2));
         LAUNCHCHK(c);
         hipEvent_t se[2];
         TRY(scatter_event(c, se));
         HIPCHK(c, hipEventRecord(se[0], c->st));
-        TRY(launch_seg_scatter(c, per, (unsigned) nchunks, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg,
-                               P<int64_t>(c->chk0), P<unsigned long long>(c->cursor)));
+        TRY(launch_seg_scatter(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg, P<int64_t>(c->chk0), level,
+                               P<unsigned long long>(c->cursor)));
         HIPCHK(c, hipEventRecord(se[1], c->st));
         c->nscatter++;
+        // chunk and record totals of the level: chk0[nseg], chk0[nseg + 1]
+        int64_t ck[2] = {0, 0};
+        HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipMemcpyAsync(hcount, dcount, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         uint64_t kept;

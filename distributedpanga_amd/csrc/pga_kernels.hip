@@ -526,10 +526,29 @@ __global__ void k_pair_ranges(const int64_t *__restrict__ blkoff, const int64_t 
 
 // ---------------------------------------------------------------- radix partition
 
-__device__ __forceinline__ uint32_t digit_at(uint64_t w0, uint64_t w1, int level)
+// Radix digits: 8 bits at every level except the second, which is w2 bits
+// (8..10) wide, so that one more level's worth of buckets fits the tiles.
+// Key bit 0 is the top bit of w0; bits past 127 read as zero.
+constexpr int ND_MAX = 1 << 10;      // largest digit count of a level
+
+__device__ __forceinline__ int digit_pos(int level, int w2)
 {
-    return level <= 8 ? (uint32_t) (w0 >> (64 - 8 * level)) & 0xFFu
-                      : (uint32_t) (w1 >> (128 - 8 * level)) & 0xFFu;
+    return level == 1 ? 0 : level == 2 ? 8 : w2 + 8 * (level - 2);
+}
+
+__device__ __forceinline__ int digit_bits(int level, int w2) { return level == 2 ? w2 : 8; }
+
+// the 64 key bits from key bit p on
+__device__ __forceinline__ uint64_t key_bits_from(uint64_t w0, uint64_t w1, int p)
+{
+    if (p == 0) return w0;
+    if (p < 64) return (w0 << p) | (w1 >> (64 - p));
+    return p < 128 ? w1 << (p - 64) : 0ull;
+}
+
+__device__ __forceinline__ uint32_t digit_at(uint64_t w0, uint64_t w1, int level, int w2)
+{
+    return (uint32_t) (key_bits_from(w0, w1, digit_pos(level, w2)) >> (64 - digit_bits(level, w2)));
 }
 
 __device__ __forceinline__ int seg_of_chunk(const int64_t *__restrict__ chk0, int nseg, int64_t b)
@@ -561,16 +580,21 @@ __global__ __launch_bounds__(1024) void k_chunk_prefix(const Seg *__restrict__ s
 
 constexpr int PT = 256;   // threads of the partition kernels
 
+// Digit counts of the segments at one level (all segments of a pass share
+// it): one row of nd = 2^digit_bits counters per segment.  The chunk count
+// comes from chk0[nseg], so the grid is sized without a round trip.
 template <int PER>
 __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, const Rec *__restrict__ buf1,
                                                  const Seg *__restrict__ segs,
-                                                 int nseg, const int64_t *__restrict__ chk0,
-                                                 unsigned long long *__restrict__ seghist, int64_t nlog)
+                                                 int nseg, const int64_t *__restrict__ chk0, int level, int w2,
+                                                 unsigned long long *__restrict__ seghist)
 {
-    __shared__ uint32_t hist[256];
+    __shared__ uint32_t hist[ND_MAX];
     const int CH = PT * PER;
+    const int64_t nlog = chk0[nseg];
+    const int nd = 1 << digit_bits(level, w2);
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
-        hist[threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < nd; d += PT) hist[d] = 0;
         __syncthreads();
         const int s = seg_of_chunk(chk0, nseg, LB);
         const Seg sg = segs[s];
@@ -582,12 +606,14 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
             const int i = k * PT + threadIdx.x;
             if (i < n) {
                 const Rec r = p[i];
-                atomicAdd(&hist[digit_at(r.w0, r.w1, sg.level)], 1u);
+                atomicAdd(&hist[digit_at(r.w0, r.w1, level, w2)], 1u);
             }
         }
         __syncthreads();
-        const uint32_t h = hist[threadIdx.x];
-        if (h) atomicAdd(&seghist[(size_t) s * 256 + threadIdx.x], (unsigned long long) h);
+        for (int d = threadIdx.x; d < nd; d += PT) {
+            const uint32_t h = hist[d];
+            if (h) atomicAdd(&seghist[(size_t) s * nd + d], (unsigned long long) h);
+        }
         __syncthreads();
     }
 }
@@ -600,27 +626,38 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
 // (segment or tile) stays where it is.
 __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
                                                   const unsigned long long *__restrict__ seghist,
-                                                  int hist_by_pair, int cap,
+                                                  int hist_by_pair, int level, int w2, int cap,
                                                   unsigned long long *__restrict__ cursor,
                                                   Seg *__restrict__ next, int *__restrict__ nnext, int next_cap,
                                                   Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap,
                                                   unsigned long long *__restrict__ kept)
 {
     __shared__ int64_t sh[256 / 64 + 1];
-    __shared__ int64_t cnt[256], off[256];
+    __shared__ int64_t cnt[ND_MAX], off[ND_MAX];
+    const int nd = 1 << digit_bits(level, w2);
+    const int pd = nd / 256;                        // digits per thread, contiguous
     const Seg sg = segs[blockIdx.x];
-    size_t row = hist_by_pair ? (size_t) sg.pair : (size_t) blockIdx.x;
-    int64_t c = (int64_t) seghist[row * 256 + threadIdx.x];
-    if (__syncthreads_count(c != 0) == 1) {
+    const size_t row = (hist_by_pair ? (size_t) sg.pair : (size_t) blockIdx.x) * (size_t) nd;
+    int64_t sum = 0;
+    int nz = 0;
+    for (int q = 0; q < pd; q++) {
+        const int d = threadIdx.x * pd + q;
+        const int64_t c = (int64_t) seghist[row + d];
+        cnt[d] = c;
+        sum += c;
+        nz += c != 0;
+    }
+    const int holders = __syncthreads_count(nz != 0);
+    if (holders == 1 && __syncthreads_count(nz > 1) == 0) {
         // a single bucket: no scatter; the child keeps the buffer and the range
         if (threadIdx.x == 0) {
             segs[blockIdx.x].skip = 1;
             if (sg.count > cap) {
                 const int q = atomicAdd(nnext, 1);
-                if (q < next_cap) next[q] = Seg{sg.start, sg.count, sg.pair, sg.level + 1, sg.buf, 0};
+                if (q < next_cap) next[q] = Seg{sg.start, sg.count, sg.pair, level + 1, sg.buf, 0};
             } else {
                 const int t = atomicAdd(ntiles, 1);
-                if (t < tile_cap) tiles[t] = Tile{sg.start, (int32_t) sg.count, sg.pair, sg.buf, sg.level};
+                if (t < tile_cap) tiles[t] = Tile{sg.start, (int32_t) sg.count, sg.pair, sg.buf, level};
             }
             atomicAdd(kept, (unsigned long long) sg.count);
         }
@@ -628,38 +665,48 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     }
     const int dstbuf = sg.buf ^ 1;
     int64_t tot;
-    int64_t o = block_excl_scan<256, int64_t>(c, sh, &tot);
-    cnt[threadIdx.x] = c;
-    off[threadIdx.x] = o;
-    cursor[(size_t) blockIdx.x * 256 + threadIdx.x] = (unsigned long long) (sg.start + o);
+    int64_t run = block_excl_scan<256, int64_t>(sum, sh, &tot);
+    for (int q = 0; q < pd; q++) {
+        const int d = threadIdx.x * pd + q;
+        off[d] = run;
+        cursor[(size_t) blockIdx.x * nd + d] = (unsigned long long) (sg.start + run);
+        run += cnt[d];
+    }
     __syncthreads();
-    // One lane groups the digits (a serial greedy over 256 counts) into LDS
-    // lists; the segment then reserves its slots with one atomic per list.
-    __shared__ Tile ltile[256];
-    __shared__ Seg lnext[256];
+    // One lane groups the digits (a serial greedy over the counts) into LDS
+    // lists of digit indices: a tile is a run of digits [tfirst, tlast], a
+    // next-level segment one digit.  The segment then reserves its slots with
+    // one atomic per list.
+    __shared__ uint16_t tfirst[ND_MAX], tlast[ND_MAX], lnext[ND_MAX];
     __shared__ int nt, nn, tbase, nbase;
     if (threadIdx.x == 0) {
-        int t = 0, q = 0;
-        int64_t ts = 0, tc = 0;
-        for (int d = 0; d < 256; d++) {
+        int t = 0, q = 0, tf = 0, tl = 0;
+        int64_t tc = 0;
+        for (int d = 0; d < nd; d++) {
             const int64_t k = cnt[d];
             if (k == 0) continue;
             if (k > cap) {
                 if (tc) {
-                    ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
+                    tfirst[t] = (uint16_t) tf;
+                    tlast[t++] = (uint16_t) tl;
                     tc = 0;
                 }
-                lnext[q++] = Seg{sg.start + off[d], k, sg.pair, sg.level + 1, dstbuf, 0};
+                lnext[q++] = (uint16_t) d;
             } else {
                 if (tc + k > cap) {
-                    ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
+                    tfirst[t] = (uint16_t) tf;
+                    tlast[t++] = (uint16_t) tl;
                     tc = 0;
                 }
-                if (tc == 0) ts = sg.start + off[d];
+                if (tc == 0) tf = d;
+                tl = d;
                 tc += k;
             }
         }
-        if (tc) ltile[t++] = Tile{ts, (int32_t) tc, sg.pair, dstbuf, sg.level};
+        if (tc) {
+            tfirst[t] = (uint16_t) tf;
+            tlast[t++] = (uint16_t) tl;
+        }
         nt = t;
         nn = q;
         tbase = t ? atomicAdd(ntiles, t) : 0;
@@ -668,8 +715,14 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     __syncthreads();
     // slots past the capacity are counted but not written (the host sees the
     // count and grows the lists)
-    if ((int) threadIdx.x < nt && tbase + (int) threadIdx.x < tile_cap) tiles[tbase + threadIdx.x] = ltile[threadIdx.x];
-    if ((int) threadIdx.x < nn && nbase + (int) threadIdx.x < next_cap) next[nbase + threadIdx.x] = lnext[threadIdx.x];
+    for (int i = threadIdx.x; i < nt && tbase + i < tile_cap; i += 256) {
+        const int f = tfirst[i], l = tlast[i];
+        tiles[tbase + i] = Tile{sg.start + off[f], (int32_t) (off[l] + cnt[l] - off[f]), sg.pair, dstbuf, level};
+    }
+    for (int i = threadIdx.x; i < nn && nbase + i < next_cap; i += 256) {
+        const int d = lnext[i];
+        next[nbase + i] = Seg{sg.start + off[d], cnt[d], sg.pair, level + 1, dstbuf, 0};
+    }
 }
 
 // Scatter one level: each chunk of CH records is ranked by its digit in LDS,
@@ -680,27 +733,28 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
 template <int T, int PER>
 __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *__restrict__ buf1,
                                                    const Seg *__restrict__ segs, int nseg,
-                                                   const int64_t *__restrict__ chk0,
-                                                   unsigned long long *__restrict__ cursor, int64_t nlog)
+                                                   const int64_t *__restrict__ chk0, int level, int w2,
+                                                   unsigned long long *__restrict__ cursor)
 {
     static_assert(T >= 256, "one thread per digit at least");
     constexpr int CH = T * PER;
+    const int64_t nlog = chk0[nseg];
     __shared__ Rec stage[CH];
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t loff[256];
-    __shared__ int64_t gbase[256];
+    __shared__ uint32_t hist[ND_MAX];               // digit counts, then the digits' runs in the stage
+    __shared__ int64_t gb[ND_MAX];                  // a digit's destination minus its stage offset
     __shared__ uint32_t sh[T / 64 + 1];
     const int tid = threadIdx.x;
+    const int nd = 1 << digit_bits(level, w2);
+    const int pd = nd > T ? nd / T : 1;             // digits per thread in the scan, contiguous
     Rec r[PER];
-    int s = 0, n = 0, level = 1, from = 0;
-    // records of chunk lb into r[] (segment s, n records, digit level, in
-    // buffer `from`); chunks of skipped segments have no records to move
+    int s = 0, n = 0, from = 0;
+    // records of chunk lb into r[] (segment s, n records, in buffer `from`);
+    // chunks of skipped segments have no records to move
     auto fetch = [&](int64_t lb) {
         s = seg_of_chunk(chk0, nseg, lb);
         const Seg sg = segs[s];
         const int64_t off = (lb - chk0[s]) * (int64_t) CH;
         n = sg.skip ? 0 : (int) min((int64_t) CH, sg.count - off);
-        level = sg.level;
         from = sg.buf;
         const Rec *p = (from ? buf1 : buf0) + sg.start + off;
 #pragma unroll
@@ -711,30 +765,38 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
     };
     if ((int64_t) blockIdx.x < nlog) fetch(blockIdx.x);
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
-    if (tid < 256) hist[tid] = 0;
+    for (int d = tid; d < nd; d += T) hist[d] = 0;
     __syncthreads();
     uint32_t rk[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         int i = k * T + tid;
-        if (i < n) rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, level)], 1u);
+        if (i < n) rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, level, w2)], 1u);
     }
     __syncthreads();
-    const uint32_t h = tid < 256 ? hist[tid] : 0u;
+    const int d0 = tid * pd;
+    uint32_t hs = 0;
+    for (int q = 0; q < pd; q++)
+        if (d0 + q < nd) hs += hist[d0 + q];
     uint32_t tot;
-    const uint32_t lo = block_excl_scan<T, uint32_t>(h, sh, &tot);
-    if (tid < 256) {
-        loff[tid] = lo;
-        if (h) gbase[tid] = (int64_t) atomicAdd(&cursor[(size_t) s * 256 + tid], (unsigned long long) h);
+    uint32_t lo = block_excl_scan<T, uint32_t>(hs, sh, &tot);
+    for (int q = 0; q < pd; q++) {
+        const int d = d0 + q;
+        if (d < nd) {
+            const uint32_t h = hist[d];
+            hist[d] = lo;
+            if (h) gb[d] = (int64_t) atomicAdd(&cursor[(size_t) s * nd + d], (unsigned long long) h) - lo;
+            lo += h;
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         int i = k * T + tid;
-        if (i < n) stage[loff[digit_at(r[k].w0, r[k].w1, level)] + rk[k]] = r[k];
+        if (i < n) stage[hist[digit_at(r[k].w0, r[k].w1, level, w2)] + rk[k]] = r[k];
     }
     __syncthreads();
-    const int ncur = n, lcur = level;
+    const int ncur = n;
     Rec *const dst = from ? buf0 : buf1;
     if (LB + gridDim.x < nlog) fetch(LB + gridDim.x);
 #pragma unroll
@@ -742,20 +804,19 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
         int i = k * T + tid;
         if (i < ncur) {
             Rec x = stage[i];
-            uint32_t d = digit_at(x.w0, x.w1, lcur);
-            dst[gbase[d] + (i - (int64_t) loff[d])] = x;
+            dst[gb[digit_at(x.w0, x.w1, level, w2)] + i] = x;
         }
     }
     __syncthreads();
     }
 }
 
-template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
-template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, unsigned long long *, int64_t);
+template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
+template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
+template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
+template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
+template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
+template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
 
 // ---------------------------------------------------------------- tile sort
 
@@ -790,7 +851,7 @@ constexpr int FB_BIN = 128;          // largest bin the binned tile sort ranks d
 template <int T>
 __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ bufs,
                                                  const Tile *__restrict__ tiles,
-                                                 const PairDev *__restrict__ pairs, int jbits,
+                                                 const PairDev *__restrict__ pairs, int jbits, int w2,
                                                  uint8_t *__restrict__ entries,
                                                  uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
                                                  Rec *__restrict__ skeys, int64_t nlog)
@@ -848,22 +909,22 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         if (mn0[w] < lo0 || (mn0[w] == lo0 && mn1[w] < lo1)) { lo0 = mn0[w]; lo1 = mn1[w]; }
         if (mx0[w] > hi0 || (mx0[w] == hi0 && mx1[w] > hi1)) { hi0 = mx0[w]; hi1 = mx1[w]; }
     }
-    // Bin of a record: (its digit at the tile's level - the smallest such
-    // digit, then the key bits right below that digit), with as many low bits
-    // as the digit span leaves.  A tile is a run of buckets of one parent, so
-    // buckets map to disjoint bin ranges and the bits inside a bucket are
-    // near-uniform: bins hold ~2 records unless the k-mers repeat.
+    // Bin of a record: the top BB bits of (its digit at the tile's level - the
+    // smallest such digit, then the key bits right below that digit).  A tile
+    // is a run of buckets of one parent, so buckets map to disjoint bin ranges
+    // (or, when the span is wider than BB, to ranges in key order) and the bits
+    // inside a bucket are near-uniform: bins hold ~2 records unless the k-mers
+    // repeat.
     constexpr int BB = __builtin_ctz(NB);
     const int L = tl.level;
-    const uint32_t dmin = digit_at(lo0, lo1, L), dmax = digit_at(hi0, hi1, L);
-    const int span = dmax > dmin ? 32 - __builtin_clz(dmax - dmin) : 0;
-    const int lowb = BB - span;                     // >= BB - 8
-    const int cp = 8 * L;                           // first key bit below the digit
+    const uint32_t dmin = digit_at(lo0, lo1, L, w2), dmax = digit_at(hi0, hi1, L, w2);
+    const int span = dmax > dmin ? 32 - __builtin_clz(dmax - dmin) : 0;     // <= 10 < 64 - BB
+    const int cp = digit_pos(L, w2) + digit_bits(L, w2);                    // first key bit below the digit
     auto bin_of = [&](uint64_t w0, uint64_t w1) -> uint32_t {
-        uint64_t top;
-        if (cp >= 64) top = cp >= 128 ? 0 : (w1 << (cp - 64));
-        else top = (w0 << cp) | (w1 >> (64 - cp));
-        return ((digit_at(w0, w1, L) - dmin) << lowb) | (uint32_t) (top >> (64 - lowb));
+        const uint64_t below = key_bits_from(w0, w1, cp);
+        const uint64_t v = span ? ((uint64_t) (digit_at(w0, w1, L, w2) - dmin) << (64 - span)) | (below >> span)
+                                : below;
+        return (uint32_t) (v >> (64 - BB));
     };
     uint32_t rk[E];
 #pragma unroll
@@ -1026,9 +1087,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
 
 // Pair of a global record index.
 __device__ __forceinline__ int pair_of(const PairDev *__restrict__ pairs, int npairs, int64_t g)

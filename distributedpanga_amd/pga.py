@@ -20,7 +20,8 @@ _ERR = {-1: "out of memory", -2: "HIP error", -3: "invalid argument", -4: "field
 
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("freq", C.c_int), ("tile_records", C.c_int),
-                ("chunk_records", C.c_int), ("merge_tile", C.c_int), ("build_cum", C.c_int)]
+                ("chunk_records", C.c_int), ("merge_tile", C.c_int), ("build_cum", C.c_int),
+                ("digit2_bits", C.c_int)]
 
 
 class IndexMeta(C.Structure):
@@ -83,12 +84,14 @@ class Context:
     """One device context: one HIP stream and a pool of device buffers."""
 
     def __init__(self, device=0, freq=10, tile_records=4096, chunk_records=4096, merge_tile=256,
-                 build_cum=True):
+                 build_cum=True, digit2_bits=None):
         self._L = _setup(hip_lib())
         o = Opts()
         self._L.pga_default_opts(C.byref(o))
         o.device, o.freq, o.tile_records = device, freq, tile_records
         o.chunk_records, o.merge_tile, o.build_cum = chunk_records, merge_tile, int(bool(build_cum))
+        if digit2_bits is not None:
+            o.digit2_bits = digit2_bits
         self.opts = o
         h = C.c_void_p()
         rc = self._L.pga_create(C.byref(o), C.byref(h))
@@ -121,7 +124,8 @@ class Context:
         self.close()
 
     def set_opts(self, **kw):
-        """Change tunables (tile_records, chunk_records, merge_tile, freq, build_cum) in place."""
+        """Change tunables (tile_records, chunk_records, merge_tile, digit2_bits, freq,
+        build_cum) in place."""
         o = Opts()
         C.memmove(C.byref(o), C.byref(self.opts), C.sizeof(Opts))
         for k, v in kw.items():

@@ -60,12 +60,13 @@ class Server:
     when driven from different threads -- ctypes releases the GIL)."""
 
     def __init__(self, target, device=0, freq=10, slots=1, tile_records=4096, chunk_records=4096,
-                 merge_tile=256):
+                 merge_tile=256, digit2_bits=10):
         self._L = _setup(hip_lib())
         o = Opts()
         self._L.pga_default_opts(C.byref(o))
         o.device, o.freq, o.tile_records = device, freq, tile_records
         o.chunk_records, o.merge_tile, o.build_cum = chunk_records, merge_tile, 1
+        o.digit2_bits = digit2_bits
         h = C.c_void_p()
         rc = self._L.pga_server_create(C.byref(o), int(slots), C.byref(h))
         if rc:

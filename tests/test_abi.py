@@ -41,3 +41,4 @@ def test_default_opts_without_a_device():
     o = pga.Opts()
     L.pga_default_opts(C.byref(o))
     assert o.freq == 10 and o.tile_records in (2048, 4096, 8192) and o.build_cum == 1
+    assert o.digit2_bits in (8, 9, 10)

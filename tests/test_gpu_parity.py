@@ -56,6 +56,15 @@ def test_tile_sizes(tile):
     _compare(synth.workload(1, 0.2), tile_records=tile)
 
 
+@pytest.mark.parametrize("tile,chunk,digit2", [(2048, 4096, 9), (2048, 2048, 10), (4096, 4096, 10),
+                                               (4096, 8192, 10), (2048, 8192, 9)])
+def test_second_digit_widths(tile, chunk, digit2):
+    """A 9- or 10-bit second digit: buckets of 2^(8+w) at the second level,
+    bins of tiles split there cut from a wider digit span."""
+    got, _ = _compare(synth.workload(1, 0.2), tile_records=tile, chunk_records=chunk, digit2_bits=digit2)
+    assert got["timings"]["partition_levels"] >= 2
+
+
 @pytest.mark.parametrize("chunk,mt", [(2048, 512), (8192, 1024)])
 def test_chunk_and_merge_tiles(chunk, mt):
     _compare(synth.workload(1, 0.2), chunk_records=chunk, merge_tile=mt)
@@ -90,13 +99,14 @@ def test_low_complexity_deep_partition():
     _compare(synth.from_codes(contigs), tile_records=2048)
 
 
-def test_identical_kmers_stay_in_place():
+@pytest.mark.parametrize("digit2", [8, 10])
+def test_identical_kmers_stay_in_place(digit2):
     """30 kb of poly-A: the forward records share one k-mer, the reverse ones
     another, and each group is beyond a tile.  Below the first digit every
     segment has one non-empty bucket until the payload digits, so those levels
     leave the records where they are: only levels that split move records."""
     g = synth.from_codes([np.zeros(30000, np.uint8)])
-    got, _ = _compare(g, tile_records=2048)
+    got, _ = _compare(g, tile_records=2048, digit2_bits=digit2)
     t, nrec = got["timings"], got["info"]["nrec"]
     assert t["partition_levels"] >= 12
     assert t["scatter_bytes"] <= 4 * 2 * 16 * nrec

@@ -32,6 +32,10 @@ def test_workload1_tiles(tile):
     _check(synth.workload(1, 0.3), tile_records=tile)
 
 
+def test_workload1_wide_second_digit():
+    _check(synth.workload(1, 0.3), tile_records=2048, digit2_bits=10)
+
+
 def test_many_contigs():
     rng = np.random.default_rng(21)
     contigs = [rng.integers(0, 4, int(n)).astype(np.uint8) for n in rng.integers(50, 3000, 400)]
