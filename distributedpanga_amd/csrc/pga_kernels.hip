@@ -877,8 +877,11 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     const PairDev pd = pairs[tl.pair];
     const int n = tl.count;
 
-    constexpr int NB = CAP / 2;                     // bins of the binned sort
+    // bins of the binned sort: one per record, 16-bit counters (later bin
+    // starts), two to a word; counts and starts are <= CAP < 2^16
+    constexpr int NB = CAP;
     uint32_t *bh = reinterpret_cast<uint32_t *>(smem_raw + (size_t) CAP * sizeof(Rec));
+    auto bin_start = [&](uint32_t b) -> uint32_t { return (bh[b >> 1] >> ((b & 1u) << 4)) & 0xFFFFu; };
     __shared__ uint64_t mn0[T / 64], mn1[T / 64], mx0[T / 64], mx1[T / 64];
     __shared__ uint32_t scan_sh[T / 64 + 1];
     __shared__ int fallback;
@@ -901,7 +904,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         if (y0 > hi0 || (y0 == hi0 && y1 > hi1)) { hi0 = y0; hi1 = y1; }
     }
     if ((tid & 63) == 0) { mn0[tid >> 6] = lo0; mn1[tid >> 6] = lo1; mx0[tid >> 6] = hi0; mx1[tid >> 6] = hi1; }
-    for (int c = tid; c < NB; c += T) bh[c] = 0;
+    for (int c = tid; c < NB / 2; c += T) bh[c] = 0;
     if (tid == 0) fallback = 0;
     __syncthreads();
     lo0 = mn0[0]; lo1 = mn1[0]; hi0 = mx0[0]; hi1 = mx1[0];
@@ -913,7 +916,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     // smallest such digit, then the key bits right below that digit).  A tile
     // is a run of buckets of one parent, so buckets map to disjoint bin ranges
     // (or, when the span is wider than BB, to ranges in key order) and the bits
-    // inside a bucket are near-uniform: bins hold ~2 records unless the k-mers
+    // inside a bucket are near-uniform: bins hold ~1 record unless the k-mers
     // repeat.
     constexpr int BB = __builtin_ctz(NB);
     const int L = tl.level;
@@ -929,17 +932,29 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     uint32_t rk[E];
 #pragma unroll
     for (int k = 0; k < E; k++)
-        if (k * T + tid < n) rk[k] = atomicAdd(&bh[bin_of(r[k].w0, r[k].w1)], 1u);
+        if (k * T + tid < n) {
+            const uint32_t b = bin_of(r[k].w0, r[k].w1), sh = (b & 1u) << 4;
+            rk[k] = (atomicAdd(&bh[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
+        }
     __syncthreads();
-    constexpr int BPT = NB / T;                     // bins per thread
-    uint32_t cnt[BPT], sum = 0, mx = 0;
+    constexpr int WPT = NB / 2 / T;                 // counter words per thread
+    uint32_t cw[WPT], sum = 0, mx = 0;
 #pragma unroll
-    for (int q = 0; q < BPT; q++) { cnt[q] = bh[tid * BPT + q]; sum += cnt[q]; mx = cnt[q] > mx ? cnt[q] : mx; }
+    for (int q = 0; q < WPT; q++) {
+        cw[q] = bh[tid * WPT + q];
+        const uint32_t lo = cw[q] & 0xFFFFu, hi = cw[q] >> 16;
+        sum += lo + hi;
+        mx = max(mx, max(lo, hi));
+    }
     if (mx > FB_BIN) fallback = 1;                  // a crowded bin (repeats): merge sort instead
     uint32_t tot;
     uint32_t run = block_excl_scan<T, uint32_t>(sum, scan_sh, &tot);
 #pragma unroll
-    for (int q = 0; q < BPT; q++) { bh[tid * BPT + q] = run; run += cnt[q]; }
+    for (int q = 0; q < WPT; q++) {
+        const uint32_t lo = cw[q] & 0xFFFFu;
+        bh[tid * WPT + q] = run | ((run + lo) << 16);
+        run += lo + (cw[q] >> 16);
+    }
     __syncthreads();
     const bool binned = !fallback;
     if (binned) {
@@ -950,7 +965,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         // lanes share bins and only the waves over a crowded bin loop long.
 #pragma unroll
         for (int k = 0; k < E; k++)
-            if (k * T + tid < n) sk[bh[bin_of(r[k].w0, r[k].w1)] + rk[k]] = r[k];
+            if (k * T + tid < n) sk[bin_start(bin_of(r[k].w0, r[k].w1)) + rk[k]] = r[k];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < E; k++) {
@@ -958,8 +973,8 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             if (me >= (uint32_t) n) continue;
             r[k] = sk[me];
             const uint32_t b = bin_of(r[k].w0, r[k].w1);
-            const uint32_t st = bh[b];
-            const uint32_t en = b + 1 < (uint32_t) NB ? bh[b + 1] : (uint32_t) n;
+            const uint32_t st = bin_start(b);
+            const uint32_t en = b + 1 < (uint32_t) NB ? bin_start(b + 1) : (uint32_t) n;
             uint32_t rank = 0;
             for (uint32_t i = st; i < en; i++) {
                 const Rec y = sk[i];
