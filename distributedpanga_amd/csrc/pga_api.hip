@@ -96,6 +96,11 @@ struct pga_ctx {
     DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax;
+    // extraction ranges (one workgroup's run of blocks inside a pair), their
+    // first-digit counts and bucket offsets, and each pair's bucket starts
+    DBuf xrange, prange, rangehist, rangeoff, bstart;
+    int nrange = 0;
+    int xbad = 0;                      // set by the emit pass when a digit overflows its run
 
     DBuf vout;                         // results of the small check kernels
     // seed sort: keys (ping-pong), their pointer pair, the packed records
@@ -392,7 +397,8 @@ void pga_destroy(pga_ctx *c)
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
-                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->skeys};
+                   &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->skeys,
+                   &c->xrange, &c->prange, &c->rangehist, &c->rangeoff, &c->bstart};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
@@ -498,6 +504,29 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
     pblk[npairs] = c->nblk;
     TRY(ensure(c, c->pblk, pblk.size() * sizeof(int64_t)));
     HIPCHK(c, hipMemcpyAsync(c->pblk.p, pblk.data(), pblk.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->st));
+    // extraction ranges: each pair's blocks cut into runs of about nblk / (one
+    // resident wave of count workgroups), a multiple of the emit's blocks per
+    // round; every range lies inside one pair
+    const int64_t xg = grid_for(c, (const void *) k_extract_count, 256, 0, std::max<int64_t>(c->nblk, 1));
+    int64_t per = std::max<int64_t>(1, (c->nblk + xg - 1) / xg);
+    per = (per + EMIT_QB - 1) / EMIT_QB * EMIT_QB;
+    std::vector<XRange> xr;
+    std::vector<int32_t> prange(npairs + 1);
+    for (int p = 0; p < npairs; p++) {
+        prange[p] = (int32_t) xr.size();
+        const int64_t b0 = c->hp[p].blk0, b1 = b0 + c->hp[p].nblk;
+        for (int64_t b = b0; b < b1; b += per) xr.push_back(XRange{b, std::min(b + per, b1), p, 0});
+    }
+    prange[npairs] = (int32_t) xr.size();
+    c->nrange = (int) xr.size();
+    TRY(ensure(c, c->xrange, std::max<size_t>(xr.size(), 1) * sizeof(XRange)));
+    if (!xr.empty())
+        HIPCHK(c, hipMemcpyAsync(c->xrange.p, xr.data(), xr.size() * sizeof(XRange), hipMemcpyHostToDevice, c->st));
+    TRY(ensure(c, c->prange, prange.size() * sizeof(int32_t)));
+    HIPCHK(c, hipMemcpyAsync(c->prange.p, prange.data(), prange.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+    TRY(ensure(c, c->rangehist, std::max<size_t>(xr.size(), 1) * 256 * sizeof(uint32_t)));
+    TRY(ensure(c, c->rangeoff, std::max<size_t>(xr.size(), 1) * 256 * sizeof(int64_t)));
+    TRY(ensure(c, c->bstart, (size_t) npairs * 256 * sizeof(int64_t)));
     HIPCHK(c, hipStreamSynchronize(c->st));     // host arrays above are stack/vector owned
     c->loaded = true;
     return PGA_OK;
@@ -509,10 +538,13 @@ namespace {
 // digits (8 bits; opts.digit2_bits at the second level), level after level,
 // until every bucket fits a sort tile; tiles go to c->tiles after the tile0
 // ones (c->ntiles is set).  Level-1 digit counts come from hist1 (one 256-row
-// per pair) when given, else they are counted here.
+// per pair) when given, else they are counted here.  level1_in_place: the
+// segments' records are already in level-1 bucket order (hist1's layout), so
+// the first level only plans and does not scatter.
 // The tiles' `buf` index refers to {buf0, buf1}.
 int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, const std::vector<Tile> &tile0,
-              const unsigned long long *hist1, int64_t seg_cap, int64_t tile_cap, int *levels_out)
+              const unsigned long long *hist1, bool level1_in_place, int64_t seg_cap, int64_t tile_cap,
+              int *levels_out)
 {
     const int CAP = cap_of(c);
     const int per = chunk_per(c);
@@ -534,6 +566,7 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
                            nseg, CH, P<int64_t>(c->chk0));
         LAUNCHCHK(c);
         const bool counted = level == 1 && hist1;
+        const bool in_place = level == 1 && level1_in_place;
         const int64_t nd = level == 2 ? (int64_t) 1 << c->o.digit2_bits : 256;   // digit rows of this level
         TRY(ensure(c, c->seghist, (size_t) nseg * nd * sizeof(uint64_t)));
         TRY(ensure(c, c->cursor, (size_t) nseg * nd * sizeof(uint64_t)));
@@ -546,17 +579,20 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         HIPCHK(c, hipMemsetAsync(dcount + 2, 0, 2 * sizeof(int), c->st));
         hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, P<Seg>(c->segs[cur]),
                            counted ? hist1 : (const unsigned long long *) P<unsigned long long>(c->seghist),
-                           counted ? 1 : 0, level, c->o.digit2_bits, CAP, P<unsigned long long>(c->cursor),
+                           counted ? 1 : 0, level, c->o.digit2_bits, in_place ? 1 : 0, CAP,
+                           P<unsigned long long>(c->cursor),
                            P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
                            (int) tile_cap, reinterpret_cast<unsigned long long *>(dcount + 2));
         LAUNCHCHK(c);
-        hipEvent_t se[2];
-        TRY(scatter_event(c, se));
-        HIPCHK(c, hipEventRecord(se[0], c->st));
-        TRY(launch_seg_scatter(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg, P<int64_t>(c->chk0), level,
-                               P<unsigned long long>(c->cursor)));
-        HIPCHK(c, hipEventRecord(se[1], c->st));
-        c->nscatter++;
+        if (!in_place) {
+            hipEvent_t se[2];
+            TRY(scatter_event(c, se));
+            HIPCHK(c, hipEventRecord(se[0], c->st));
+            TRY(launch_seg_scatter(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg, P<int64_t>(c->chk0), level,
+                                   P<unsigned long long>(c->cursor)));
+            HIPCHK(c, hipEventRecord(se[1], c->st));
+            c->nscatter++;
+        }
         // chunk and record totals of the level: chk0[nseg], chk0[nseg + 1]
         int64_t ck[2] = {0, 0};
         HIPCHK(c, hipMemcpyAsync(ck, P<int64_t>(c->chk0) + nseg, sizeof(ck), hipMemcpyDeviceToHost, c->st));
@@ -564,7 +600,8 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         HIPCHK(c, hipStreamSynchronize(c->st));
         uint64_t kept;
         memcpy(&kept, &hcount[2], sizeof(kept));
-        c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * (ck[1] - (int64_t) kept);   // moved records, read + write
+        if (!in_place)
+            c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * (ck[1] - (int64_t) kept);   // moved records, read + write
         nseg = hcount[0];
         cur ^= 1;
         level++;
@@ -614,10 +651,10 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->sel, (size_t) std::max<int64_t>(nblk, 1) * 256));    // 2 bits per position
     if (nblk > 0) {
-        hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, nblk)), dim3(256),
-                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), (const int32_t *) P<int32_t>(c->bctg),
-                           c->nctg, c->F, P<uint32_t>(c->blkcnt),
-                           P<uint8_t>(c->sel), nblk);
+        hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
+                           dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
+                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
+                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist));
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
     } else {
@@ -676,32 +713,48 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->tiles, (size_t) tile_cap * sizeof(Tile)));
     HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev),
                              hipMemcpyHostToDevice, c->st));
-    TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
 
-    // ---- extraction: emit records in extraction order + level-1 digit counts
-    if (nblk > 0 && R > 0) {
-        const unsigned xg = grid_for(c, (const void *) k_extract_emit, 256, 0, nblk);
-        const int64_t xper = (nblk + xg - 1) / xg;
-        hipLaunchKernelGGL(k_extract_emit, dim3(xg), dim3(256),
-                           0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg), (const int32_t *) P<int32_t>(c->bctg),
-                           c->nctg, c->F, c->jbits,
-                           (const uint8_t *) P<uint8_t>(c->sel), (const int64_t *) P<int64_t>(c->blkoff), P<Rec>(c->rec[0]),
-                           P<unsigned long long>(c->hist1), nblk, xper);
+    // ---- extraction: level-1 digit totals and bucket offsets per range, then
+    // every record straight into its level-1 bucket in record buffer 1
+    int *dbad = P<int>(c->counters) + 8;
+    HIPCHK(c, hipMemsetAsync(dbad, 0, sizeof(int), c->st));
+    if (c->nrange > 0) {
+        hipLaunchKernelGGL(k_xrange_scan, dim3(npairs, 16), dim3(256), 0, c->st, (const int *) P<int>(c->prange),
+                           (const uint32_t *) P<uint32_t>(c->rangehist), P<int64_t>(c->rangeoff),
+                           P<unsigned long long>(c->hist1));
+        LAUNCHCHK(c);
+    } else {
+        TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
+    }
+    hipLaunchKernelGGL(k_xrange_base, dim3(npairs), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr),
+                       (const unsigned long long *) P<unsigned long long>(c->hist1), P<int64_t>(c->bstart));
+    LAUNCHCHK(c);
+    if (c->nrange > 0 && R > 0) {
+        hipLaunchKernelGGL(k_extract_emit<EMIT_QB>,
+                           dim3(grid_for(c, (const void *) k_extract_emit<EMIT_QB>, 256 * EMIT_QB, 0, c->nrange)),
+                           dim3(256 * EMIT_QB), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
+                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
+                           (const XRange *) P<XRange>(c->xrange), c->nrange,
+                           (const uint32_t *) P<uint32_t>(c->rangehist), (const int64_t *) P<int64_t>(c->rangeoff),
+                           (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), dbad);
         LAUNCHCHK(c);
     }
+    c->xbad = 0;
+    HIPCHK(c, hipMemcpyAsync(&c->xbad, dbad, sizeof(int), hipMemcpyDeviceToHost, c->st));   // read after the partition's sync
     TRY(record(c, 1));
 
-    // ---- MSD radix partition
+    // ---- MSD radix partition, from level-1 buckets already in record buffer 1
     std::vector<Seg> seg0;
     std::vector<Tile> tile0;
     for (int p = 0; p < npairs; p++) {
         const PairDev &d = c->pairs[p];
-        if (d.nrec > CAP) seg0.push_back(Seg{d.rec0, d.nrec, p, 1});
-        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 0, 1});
+        if (d.nrec > CAP) seg0.push_back(Seg{d.rec0, d.nrec, p, 1, 1, 0});
+        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 1, 1});
     }
     int levels = 0;
-    TRY(partition(c, P<Rec>(c->rec[0]), P<Rec>(c->rec[1]), seg0, tile0, P<unsigned long long>(c->hist1), seg_cap,
-                  tile_cap, &levels));
+    TRY(partition(c, P<Rec>(c->rec[0]), P<Rec>(c->rec[1]), seg0, tile0, P<unsigned long long>(c->hist1), true,
+                  seg_cap, tile_cap, &levels));
+    if (c->xbad) return fail(c, PGA_EINVAL, "internal: extraction passes disagree on a record's first digit");
     c->tm.partition_levels = levels;
     c->tm.partition_launches = c->nscatter;
     TRY(record(c, 2));
@@ -1094,7 +1147,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         const int64_t sb = c->scatter_bytes;
         const int64_t nt = c->ntiles;
         int levels = 0;
-        const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0, nullptr, seg_cap, tile_cap,
+        const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0, nullptr, false, seg_cap, tile_cap,
                                  &levels);
         const int64_t sort_tiles = c->ntiles;
         c->nscatter = ns;

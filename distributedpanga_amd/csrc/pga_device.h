@@ -45,4 +45,9 @@ struct Tile {                    // a bucket run small enough for one workgroup 
     int32_t level;               // radix digit (1..16) its buckets differ in; all share the digits above
 };
 
+struct XRange {                  // a run of extraction blocks inside one pair, one workgroup's share
+    int64_t lb0, lb1;            // logical blocks [lb0, lb1)
+    int32_t pair, pad;
+};
+
 } // namespace pga

@@ -221,8 +221,20 @@ __device__ __forceinline__ void write_packed(uint8_t *__restrict__ g, const u128
 constexpr int XT = 256;
 constexpr int XU = 4;
 
+// First radix digit (the key's top byte) of the records at an offset whose
+// LSB-first window starts at W: the forward key begins with bases j..j+3,
+// MSB-first; the reverse-complement key with the complements of j+11..j+8.
+__device__ __forceinline__ uint32_t digit_fwd(uint64_t W)
+{
+    const uint32_t x = (uint32_t) W & 0xFFu;
+    return ((x & 3u) << 6) | ((x & 0xCu) << 2) | ((x >> 2) & 0xCu) | (x >> 6);
+}
+
+__device__ __forceinline__ uint32_t digit_rev(uint64_t W) { return ~(uint32_t) (W >> 16) & 0xFFu; }
+
 struct XEmit {
     uint64_t f0, f1, r0, r1;     // forward and reverse-complement record keys
+    uint32_t dg;                 // first radix digits: forward (bits 0-7), reverse complement (8-15)
     bool fwd, rev;
 };
 
@@ -237,6 +249,7 @@ __device__ __forceinline__ int scan_position(const uint64_t *__restrict__ bps64,
     out.fwd = out.rev = false;
     window80(bps64, c.bitoff + 2 * j, W0, W1);
     if (!syncmer(W0, tl, tm)) return 0;
+    out.dg = digit_fwd(W0) | (digit_rev(W0) << 8);
     out.fwd = j <= (int64_t) c.clen - 40;          // forward 40-mer fits
     out.rev = j >= 28;                             // reverse complement of [j-28, j+12) fits
     if (keys) {
@@ -278,19 +291,28 @@ __device__ __forceinline__ void record_keys(const uint64_t *__restrict__ bps64, 
     }
 }
 
-// bctg[LB] is the contig of logical block LB's first position; every thread
+// Per range of logical blocks (a workgroup's share, inside one pair): the
+// selection bits of every position (forward / reverse-complement record), the
+// record count of every block, and the range's count of each first radix
+// digit.  bctg[LB] is the contig of block LB's first position; every thread
 // steps forward from it (a thread of XU positions crosses few contig starts).
 __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
                                                       const ContigDev *__restrict__ ctg,
                                                       const int32_t *__restrict__ bctg, int nctg,
-                                                      int64_t F, uint32_t *__restrict__ blkcnt,
-                                                      uint8_t *__restrict__ sel, int64_t nlog)
+                                                      int64_t F, const XRange *__restrict__ ranges, int nrange,
+                                                      uint32_t *__restrict__ blkcnt, uint8_t *__restrict__ sel,
+                                                      uint32_t *__restrict__ rangehist)
 {
+    static_assert(XT == 256, "one thread per digit");
     __shared__ uint32_t tl[256], tm[256];
     __shared__ uint64_t red[XT / 64 + 1];
+    __shared__ uint32_t hist[256];
     load_tables(tl, tm);
+    for (int r = blockIdx.x; r < nrange; r += gridDim.x) {
+    hist[threadIdx.x] = 0;
     __syncthreads();
-    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
+    const XRange xr = ranges[r];
+    for (int64_t LB = xr.lb0; LB < xr.lb1; LB++) {
     const int64_t fb = LB * XT * XU;
     int c = bctg[LB];
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
@@ -315,6 +337,8 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
             const bool fwd = j <= (int64_t) c0.clen - 40, rev = j >= 28;
             cnt += (uint32_t) fwd + (uint32_t) rev;
             bits |= ((fwd ? 1u : 0u) | (rev ? 2u : 0u)) << (2 * u);
+            if (fwd) atomicAdd(&hist[digit_fwd(W0 >> (2 * u))], 1u);
+            if (rev) atomicAdd(&hist[digit_rev(W0 >> (2 * u))], 1u);
         }
     } else {
         XEmit x;
@@ -327,98 +351,155 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
             if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
             cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, x);
             bits |= ((x.fwd ? 1u : 0u) | (x.rev ? 2u : 0u)) << (2 * u);
+            if (x.fwd) atomicAdd(&hist[x.dg & 0xFFu], 1u);
+            if (x.rev) atomicAdd(&hist[x.dg >> 8], 1u);
         }
     }
     sel[fb / XU + threadIdx.x] = (uint8_t) bits;       // the emit pass reads the selection back
     uint64_t tot = block_sum_u64<XT>(cnt, red);
     if (threadIdx.x == 0) blkcnt[LB] = (uint32_t) tot;
     }
+    __syncthreads();
+    rangehist[(size_t) r * 256 + threadIdx.x] = hist[threadIdx.x];
+    __syncthreads();
+    }
 }
 
-// Emit records in extraction order at blkoff[block] (staged in LDS, written
-// with coalesced 16-byte stores) and count the first (level-1) radix digit
-// per pair.  Every logical block lies inside one pair; each workgroup takes a
-// contiguous run of `per` logical blocks and keeps its digit counts in LDS
-// until the pair changes, so the 256 per-pair counters see one atomic per
-// workgroup and pair rather than one per logical block.
-__global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict__ bps64,
-                                                     const ContigDev *__restrict__ ctg,
-                                                     const int32_t *__restrict__ bctg, int nctg,
-                                                     int64_t F, int jbits, const uint8_t *__restrict__ sel,
-                                                     const int64_t *__restrict__ blkoff,
-                                                     Rec *__restrict__ out,
-                                                     unsigned long long *__restrict__ hist1, int64_t nlog,
-                                                     int64_t per)
+// Per pair and 16 digits (blockIdx.y), one thread per (digit, lane of 16):
+// where each range's records of a digit start inside the pair's bucket of
+// that digit (records of the digit in the pair's earlier ranges), and the
+// pair's digit totals (hist1).  Each lane takes a contiguous share of the
+// pair's ranges.
+__global__ __launch_bounds__(256) void k_xrange_scan(const int *__restrict__ prange,
+                                                     const uint32_t *__restrict__ rangehist,
+                                                     int64_t *__restrict__ rangeoff,
+                                                     unsigned long long *__restrict__ hist1)
 {
-    __shared__ uint32_t tl[256], tm[256];
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t scan_sh[XT / 64 + 1];
-    __shared__ Rec stage[XT * XU];                  // one round's records (a block emits up to 2 per position)
-    load_tables(tl, tm);
-    for (int t = threadIdx.x; t < 256; t += XT) hist[t] = 0;
+    __shared__ int64_t part[16][17];
+    const int p = blockIdx.x;
+    const int dg = threadIdx.x & 15, lane = threadIdx.x >> 4;
+    const int d = blockIdx.y * 16 + dg;
+    const int r0 = prange[p], n = prange[p + 1] - r0, K = (n + 15) / 16;
+    const int a = r0 + min(n, lane * K), b = r0 + min(n, lane * K + K);
+    int64_t s = 0;
+#pragma unroll 8
+    for (int r = a; r < b; r++) s += rangehist[(size_t) r * 256 + d];
+    part[lane][dg] = s;
     __syncthreads();
-    const int64_t lb0 = (int64_t) blockIdx.x * per;
-    const int64_t lb1 = lb0 + per < nlog ? lb0 + per : nlog;
-    int cur = -1;
-    for (int64_t LB = lb0; LB < lb1; LB++) {
-    const int64_t fb = LB * XT * XU;
-    int c = bctg[LB];
-    const int pair0 = ctg[c].pair;
-    if (pair0 != cur) {                               // uniform: every thread sees the same pair0
-        if (cur >= 0)
-            for (int t = threadIdx.x; t < 256; t += XT)
-                if (hist[t]) { atomicAdd(&hist1[(size_t) cur * 256 + t], (unsigned long long) hist[t]); hist[t] = 0; }
-        cur = pair0;
-        __syncthreads();
+    int64_t run = 0;
+    for (int l = 0; l < lane; l++) run += part[l][dg];
+    if (lane == 15) hist1[(size_t) p * 256 + d] = (unsigned long long) (run + s);
+    for (int r = a; r < b; r++) {
+        rangeoff[(size_t) r * 256 + d] = run;
+        run += rangehist[(size_t) r * 256 + d];
     }
-    const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
-    const uint32_t bits = sel[fb / XU + threadIdx.x];  // selection found by the count pass
-    // places first (the selection bits give the counts), then each record's
-    // keys go straight to its slot: the block's first XT*XU records are
-    // staged for coalesced stores; the rest, when the block emits both
-    // strands at most of its positions, go out directly
-    uint32_t total;
-    uint32_t k = block_excl_scan<XT, uint32_t>((uint32_t) __builtin_popcount(bits), scan_sh, &total);
-    Rec *dst = out + blkoff[LB];
-    auto put = [&](uint64_t w0, uint64_t w1, int pair) {
-        Rec r;
-        r.w0 = w0;
-        r.w1 = w1;
-        if (k < (uint32_t) (XT * XU)) stage[k] = r;
-        else dst[k] = r;
-        k++;
-        const uint32_t d = (uint32_t) (w0 >> 56);
-        if (pair == pair0) atomicAdd(&hist[d], 1u);
-        else atomicAdd(&hist1[(size_t) pair * 256 + d], 1ull);
+}
+
+// Per pair: the start of each digit's bucket, pr[p] + the pair's records of
+// smaller digits.
+__global__ __launch_bounds__(256) void k_xrange_base(const int64_t *__restrict__ pr,
+                                                     const unsigned long long *__restrict__ hist1,
+                                                     int64_t *__restrict__ bstart)
+{
+    __shared__ int64_t sh[256 / 64 + 1];
+    const int p = blockIdx.x;
+    int64_t tot;
+    const int64_t o = block_excl_scan<256, int64_t>((int64_t) hist1[(size_t) p * 256 + threadIdx.x], sh, &tot);
+    bstart[(size_t) p * 256 + threadIdx.x] = pr[p] + o;
+}
+
+// Emit every selected record straight into its first-level bucket.  Range r's
+// records of digit d fill the run of rangehist[r][d] slots at bstart[pair][d]
+// + rangeoff[r][d].  A workgroup of 256*QB threads takes QB blocks per round:
+// each thread keeps its (at most 2*XU) records in registers and takes a rank
+// in its digit (an LDS counter); a scan of the round's digit counts lays the
+// records out in digit order in an LDS stage, and the stage goes out as one
+// contiguous run per digit at the digit's next free slots.  A record past its
+// digit's run (the two passes disagreeing on a digit) is not written and sets
+// *bad.
+constexpr int EMIT_QB = 4;           // blocks per emit round (1024 threads, 128 KB stage)
+
+template <int QB>
+__global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__restrict__ bps64,
+                                                           const ContigDev *__restrict__ ctg,
+                                                           const int32_t *__restrict__ bctg, int nctg,
+                                                           int jbits, const uint8_t *__restrict__ sel,
+                                                           const XRange *__restrict__ ranges, int nrange,
+                                                           const uint32_t *__restrict__ rangehist,
+                                                           const int64_t *__restrict__ rangeoff,
+                                                           const int64_t *__restrict__ bstart,
+                                                           Rec *__restrict__ out, int *__restrict__ bad)
+{
+    constexpr int NT = 256 * QB;
+    constexpr int NS = 2 * XU;                         // record slots per thread
+    static_assert(XT == 256, "one thread per digit");
+    __shared__ Rec stage[NS * NT];
+    __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; its slots this round
+    __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
+    __shared__ uint32_t stot;
+    const int tid = threadIdx.x, q = tid >> 8, t = tid & 255;
+    // inputs of a round, loaded one round ahead: selection bits, the contig
+    // of the thread's first position and, when its XU positions lie in that
+    // contig past base 28, the forward and reverse-complement 128-bit spans
+    uint32_t bits = 0;
+    int c = 0;
+    bool fast = false;
+    int64_t f0 = 0;
+    ContigDev c0{};
+    uint64_t A0 = 0, A1 = 0, B0 = 0, B1 = 0;
+    auto fetch = [&](int64_t LB, int64_t lb1) {
+        bits = LB < lb1 ? sel[LB * XT + t] : 0u;
+        fast = false;
+        if (!bits) return;
+        f0 = LB * XT * XU + (int64_t) t * XU;
+        c = bctg[LB];
+        while (c + 1 < nctg && ctg[c + 1].pos0 <= f0) c++;
+        c0 = ctg[c];
+        fast = f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand && f0 - c0.pos0 >= 28;
+        if (fast) {
+            window128(bps64, c0.bitoff + 2 * (f0 - c0.pos0), A0, A1);
+            window128(bps64, c0.bitoff + 2 * (f0 - c0.pos0 - 28), B0, B1);
+        }
     };
-    while (c + 1 < nctg && ctg[c + 1].pos0 <= f0) c++;
-    const ContigDev c0 = ctg[c];
-    if (bits && f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand && f0 - c0.pos0 >= 28) {
-        // the thread's XU positions lie in one contig, past its first 28
-        // bases: their forward windows come from one 128-bit span at j0 and
-        // their reverse-complement windows from one at j0-28
+    int par = 0;                                       // which digit counters this round uses
+    if (tid < 256) cnt[0][tid] = cnt[1][tid] = 0;
+    for (int r = blockIdx.x; r < nrange; r += gridDim.x) {
+    const XRange xr = ranges[r];
+    __syncthreads();                                   // the previous range is done with the runs
+    if (tid < 256) {
+        base[tid] = bstart[(size_t) xr.pair * 256 + tid] + rangeoff[(size_t) r * 256 + tid];
+        lim[tid] = rangehist[(size_t) r * 256 + tid];
+        done[tid] = 0;
+    }
+    fetch(xr.lb0 + q, xr.lb1);
+    for (int64_t lb = xr.lb0; lb < xr.lb1; lb += QB) {
+    Rec rr[NS];
+    uint32_t rk[NS];
+    uint32_t m = 0;                                    // slots holding a record
+    auto put = [&](int slot, uint64_t w0, uint64_t w1) {
+        rr[slot].w0 = w0;
+        rr[slot].w1 = w1;
+        rk[slot] = atomicAdd(&cnt[par][(uint32_t) (w0 >> 56)], 1u);
+        m |= 1u << slot;
+    };
+    if (bits && fast) {
         const int64_t j0 = f0 - c0.pos0;
-        uint64_t A0, A1, B0, B1;
-        window128(bps64, c0.bitoff + 2 * j0, A0, A1);
-        window128(bps64, c0.bitoff + 2 * (j0 - 28), B0, B1);
 #pragma unroll
         for (int u = 0; u < XU; u++) {
-            if (!((bits >> (2 * u)) & 3u)) continue;
             const uint64_t pay = ((uint64_t) c0.local << (jbits + 1)) | ((uint64_t) (j0 + u) << 1);
             const int sh = 2 * u;
             if ((bits >> (2 * u)) & 1u) {
                 const uint64_t W0 = sh ? (A0 >> sh) | (A1 << (64 - sh)) : A0;
                 const uint32_t W1 = (uint32_t) (A1 >> sh) & 0xFFFFu;
-                put(field_rev64(W0), ((uint64_t) field_rev16(W1) << 48) | pay, c0.pair);
+                put(2 * u, field_rev64(W0), ((uint64_t) field_rev16(W1) << 48) | pay);
             }
             if ((bits >> (2 * u + 1)) & 1u) {
                 const uint64_t V0 = sh ? (B0 >> sh) | (B1 << (64 - sh)) : B0;
                 const uint32_t V1 = (uint32_t) (B1 >> sh) & 0xFFFFu;
-                put(~(((uint64_t) V1 << 48) | (V0 >> 16)), ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull,
-                    c0.pair);
+                put(2 * u + 1, ~(((uint64_t) V1 << 48) | (V0 >> 16)), ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull);
             }
         }
-    } else {
+    } else if (bits) {
 #pragma unroll
         for (int u = 0; u < XU; u++) {
             const int64_t f = f0 + u;
@@ -429,19 +510,59 @@ __global__ __launch_bounds__(XT) void k_extract_emit(const uint64_t *__restrict_
             e.fwd = (bits >> (2 * u)) & 1u;
             e.rev = (bits >> (2 * u + 1)) & 1u;
             record_keys(bps64, cc, f - cc.pos0, jbits, e);
-            if (e.fwd) put(e.f0, e.f1, cc.pair);
-            if (e.rev) put(e.r0, e.r1, cc.pair);
+            if (e.fwd) put(2 * u, e.f0, e.f1);
+            if (e.rev) put(2 * u + 1, e.r0, e.r1);
         }
     }
+    if (lb + QB < xr.lb1) fetch(lb + QB + q, xr.lb1);  // the next round's inputs, in flight meanwhile
     __syncthreads();
-    const int nr = (int) min(total, (uint32_t) (XT * XU));
-    for (int i = threadIdx.x; i < nr; i += XT) dst[i] = stage[i];
-    __syncthreads();
+    // the round's digit runs (one wave: four digits a lane): offsets in the
+    // stage, slots in the buckets; the other counters are cleared for the
+    // next round
+    if (tid < 64) {
+        uint32_t cd[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) { cd[k] = cnt[par][4 * tid + k]; sum += cd[k]; }
+        uint32_t x = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (tid >= o) x += y;
+        }
+        uint32_t run = x - sum;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int d = 4 * tid + k;
+            loff[d] = run;
+            gpos[d] = base[d] + done[d];
+            done[d] += cd[k];
+            run += cd[k];
+        }
+        if (tid == 63) stot = x;
+    } else if (tid < 320) {
+        cnt[par ^ 1][tid - 64] = 0;
     }
-    if (cur >= 0)
-        for (int t = threadIdx.x; t < 256; t += XT)
-            if (hist[t]) atomicAdd(&hist1[(size_t) cur * 256 + t], (unsigned long long) hist[t]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NS; k++)
+        if ((m >> k) & 1u) stage[loff[(uint32_t) (rr[k].w0 >> 56)] + rk[k]] = rr[k];
+    __syncthreads();
+    const int tot = (int) stot;
+    for (int i = tid; i < tot; i += NT) {
+        const Rec x = stage[i];
+        const uint32_t d = (uint32_t) (x.w0 >> 56);
+        const int64_t dst = gpos[d] + (i - (int64_t) loff[d]);
+        if (dst < base[d] + lim[d]) out[dst] = x;
+        else *bad = 1;
+    }
+    par ^= 1;
+    }
+    }
 }
+
+template __global__ void k_extract_emit<EMIT_QB>(const uint64_t *, const ContigDev *, const int32_t *, int, int,
+                                                 const uint8_t *, const XRange *, int, const uint32_t *,
+                                                 const int64_t *, const int64_t *, Rec *, int *);
 
 // ---------------------------------------------------------------- scans
 
@@ -623,10 +744,11 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
 // buckets); larger ones become segments of the next level.  Children go to the
 // other record buffer, except when the segment has one non-empty bucket: then
 // its records are already in bucket order, the scatter skips it, and its child
-// (segment or tile) stays where it is.
+// (segment or tile) stays where it is.  With in_place the whole level is
+// already in bucket order (the extraction wrote it so) and is not scattered.
 __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
                                                   const unsigned long long *__restrict__ seghist,
-                                                  int hist_by_pair, int level, int w2, int cap,
+                                                  int hist_by_pair, int level, int w2, int in_place, int cap,
                                                   unsigned long long *__restrict__ cursor,
                                                   Seg *__restrict__ next, int *__restrict__ nnext, int next_cap,
                                                   Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap,
@@ -663,7 +785,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
         }
         return;
     }
-    const int dstbuf = sg.buf ^ 1;
+    const int dstbuf = in_place ? sg.buf : sg.buf ^ 1;   // in_place: already in bucket order, no scatter
     int64_t tot;
     int64_t run = block_excl_scan<256, int64_t>(sum, sh, &tot);
     for (int q = 0; q < pd; q++) {
