@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_xrange_base(const int64_t *__restrict__
 // contiguous run per digit at the digit's next free slots.  A record past its
 // digit's run (the two passes disagreeing on a digit) is not written and sets
 // *bad.
-constexpr int EMIT_QB = 4;           // blocks per emit round (1024 threads, 128 KB stage)
+constexpr int EMIT_QB = 2;           // blocks per emit round (512 threads, 64 KB stage, two workgroups a CU)
 
 template <int QB>
 __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__restrict__ bps64,
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
 {
     constexpr int NT = 256 * QB;
     constexpr int NS = 2 * XU;                         // record slots per thread
-    static_assert(XT == 256, "one thread per digit");
+    static_assert(XT == 256 && NT >= 256, "one thread per digit");
     __shared__ Rec stage[NS * NT];
     __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; its slots this round
     __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
@@ -539,8 +539,8 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             run += cd[k];
         }
         if (tid == 63) stot = x;
-    } else if (tid < 320) {
-        cnt[par ^ 1][tid - 64] = 0;
+    } else {
+        for (int d = tid - 64; d < 256; d += NT - 64) cnt[par ^ 1][d] = 0;
     }
     __syncthreads();
 #pragma unroll
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         const Rec x = stage[i];
         const uint32_t d = (uint32_t) (x.w0 >> 56);
         const int64_t dst = gpos[d] + (i - (int64_t) loff[d]);
-        if (dst < base[d] + lim[d]) out[dst] = x;
+        if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = x;
         else *bad = 1;
     }
     par ^= 1;
