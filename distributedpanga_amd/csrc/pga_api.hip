@@ -123,6 +123,9 @@ struct pga_ctx {
     int nscatter = 0;
     int64_t scatter_bytes = 0;
     pga_timings tm{};
+    // event times not read yet: the build and the merge return without
+    // waiting for the device; the times are read when asked for
+    bool tm_build = false, tm_merge = false;
 };
 
 namespace {
@@ -608,8 +611,8 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         if (nseg > seg_cap) return fail(c, PGA_EOVERFLOW, "segment list overflow");
         if (hcount[1] > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
     }
-    HIPCHK(c, hipMemcpyAsync(hcount, dcount, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    // hcount[1] is current: the last level read the counters back (or, with no
+    // segments, it still holds the tile0 count written above)
     c->ntiles = hcount[1];
     if (c->ntiles > tile_cap) return fail(c, PGA_EOVERFLOW, "tile list overflow");
     *levels_out = level - 1;
@@ -640,6 +643,7 @@ int pga_build_index(pga_ctx *c)
     c->nscatter = 0;
     c->scatter_bytes = 0;
     memset(&c->tm, 0, sizeof(c->tm));
+    c->tm_build = c->tm_merge = false;
     const int npairs = c->npairs;
     const int CAP = cap_of(c);
 
@@ -799,19 +803,13 @@ int pga_build_index(pga_ctx *c)
         }
     }
     TRY(record(c, 4));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-
-    c->tm.extract_ms = ev_ms(c->ev[0], c->ev[1]);
-    c->tm.partition_ms = ev_ms(c->ev[1], c->ev[2]);
-    c->tm.tile_sort_ms = ev_ms(c->ev[2], c->ev[3]);
-    c->tm.table_ms = ev_ms(c->ev[3], c->ev[4]);
-    c->tm.total_ms = ev_ms(c->ev[0], c->ev[4]);
+    // no wait here: the merge can queue behind the build; the stage times are
+    // read from the events when asked for (pga_timings_get)
     c->tm.tiles = c->ntiles;
-    double sc = 0;
-    for (int i = 0; i < c->nscatter; i++) sc += ev_ms(c->sev[2 * i], c->sev[2 * i + 1]);
-    c->tm.scatter_ms = sc;
     c->tm.scatter_launches = c->nscatter;
     c->tm.scatter_bytes = c->scatter_bytes;
+    c->tm_build = true;
+    c->tm_merge = false;
     c->built = true;
     return PGA_OK;
 }
@@ -1051,9 +1049,7 @@ int pga_merge(pga_ctx *c)
     }
 
     TRY(record(c, 6));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    c->tm.merge_ms = ev_ms(c->ev[5], c->ev[6]);
-    c->tm.merge_emit_ms = ev_ms(c->ev[7], c->ev[6]);
+    c->tm_merge = true;                  // times read when asked for
     c->merged = true;
     return PGA_OK;
 }
@@ -1181,6 +1177,24 @@ int pga_seed_sort_export(pga_ctx *c, uint8_t *records)
 int pga_timings_get(pga_ctx *c, pga_timings *t)
 {
     if (!c) return PGA_EINVAL;
+    if (c->tm_build || c->tm_merge) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (c->tm_build) {
+            c->tm.extract_ms = ev_ms(c->ev[0], c->ev[1]);
+            c->tm.partition_ms = ev_ms(c->ev[1], c->ev[2]);
+            c->tm.tile_sort_ms = ev_ms(c->ev[2], c->ev[3]);
+            c->tm.table_ms = ev_ms(c->ev[3], c->ev[4]);
+            c->tm.total_ms = ev_ms(c->ev[0], c->ev[4]);
+            double sc = 0;
+            for (int i = 0; i < (int) c->tm.scatter_launches; i++) sc += ev_ms(c->sev[2 * i], c->sev[2 * i + 1]);
+            c->tm.scatter_ms = sc;
+        }
+        if (c->tm_merge) {
+            c->tm.merge_ms = ev_ms(c->ev[5], c->ev[6]);
+            c->tm.merge_emit_ms = ev_ms(c->ev[7], c->ev[6]);
+        }
+        c->tm_build = c->tm_merge = false;
+    }
     *t = c->tm;
     return PGA_OK;
 }
