@@ -1043,13 +1043,15 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     constexpr int BB = __builtin_ctz(NB);
     const int L = tl.level;
     const uint32_t dmin = digit_at(lo0, lo1, L, w2), dmax = digit_at(hi0, hi1, L, w2);
-    const int span = dmax > dmin ? 32 - __builtin_clz(dmax - dmin) : 0;     // <= 10 < 64 - BB
-    const int cp = digit_pos(L, w2) + digit_bits(L, w2);                    // first key bit below the digit
+    const int span = dmax > dmin ? 32 - __builtin_clz(dmax - dmin) : 0;     // <= the digit's bits
+    // The 64 key bits from the digit on, minus dmin in the digit's field, are
+    // (digit - dmin) then the bits below it, with the top (bits - span) bits
+    // zero: the bin is their next BB bits.
+    const int dp = digit_pos(L, w2), dbits = digit_bits(L, w2);
+    const uint64_t dsub = (uint64_t) dmin << (64 - dbits);
+    const int bsh = 64 - BB - dbits + span;                                 // 41..53
     auto bin_of = [&](uint64_t w0, uint64_t w1) -> uint32_t {
-        const uint64_t below = key_bits_from(w0, w1, cp);
-        const uint64_t v = span ? ((uint64_t) (digit_at(w0, w1, L, w2) - dmin) << (64 - span)) | (below >> span)
-                                : below;
-        return (uint32_t) (v >> (64 - BB));
+        return (uint32_t) ((key_bits_from(w0, w1, dp) - dsub) >> bsh);
     };
     uint32_t rk[E];
 #pragma unroll
