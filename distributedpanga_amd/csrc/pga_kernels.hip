@@ -10,9 +10,11 @@
 // which is the canonical tie order for equal k-mers.  Keys are unique, so any
 // correct sort gives the same array; no stage needs to be stable.
 //
-// Pipeline: syncmer scan (count, scan, emit) -> MSD radix partition passes of
-// 8 bits until every bucket fits one workgroup -> per-bucket LDS merge sort
-// that also writes LCPs and GIX entries -> prefix table -> merge.
+// Pipeline: syncmer scan (count with first-digit counts per range of blocks,
+// scan, emit straight into the first-level buckets) -> MSD radix partition
+// (the first level only planned; a second digit of 8..10 bits, then 8-bit
+// digits, until every bucket fits one workgroup's tile) -> per-tile binned LDS
+// sort that also writes LCPs and GIX entries -> prefix table -> merge.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "pga_device.h"
@@ -90,16 +92,6 @@ __device__ __forceinline__ bool syncmer(uint64_t W, const uint32_t *tl, const ui
     for (int t = 0; t < 5; t++) h[t] = hash8(W, t, tl, tm);
     uint32_t m = min(min(min(h[0], h[1]), min(h[2], h[3])), h[4]);
     return h[0] == m || h[4] == m;
-}
-
-__device__ __forceinline__ int locate_contig(const ContigDev *__restrict__ ctg, int nctg, int64_t f)
-{
-    int lo = 0, hi = nctg;                      // ctg[nctg].pos0 is the end sentinel
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (ctg[mid].pos0 <= f) lo = mid; else hi = mid;
-    }
-    return lo;
 }
 
 __device__ __forceinline__ void load_tables(uint32_t *tl, uint32_t *tm)
@@ -1231,16 +1223,6 @@ template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const Pair
 template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
 
 // Pair of a global record index.
-__device__ __forceinline__ int pair_of(const PairDev *__restrict__ pairs, int npairs, int64_t g)
-{
-    int lo = 0, hi = npairs;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (pairs[mid].rec0 <= g) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 // LCP of each tile's first record with its predecessor (0 at a pair's start),
 // from the 24-bit prefixes and the 7 suffix bytes of the two entries.
 __global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const PairDev *__restrict__ pairs,
