@@ -197,3 +197,30 @@ def test_batch_mixed_table_paths():
             assert np.array_equal(cum, want["cum"]) and info["maxpre"] == want["maxpre"]
             assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"])
             assert np.array_equal(nb, want["nbuck"]) and np.array_equal(cb, want["cbuck"])
+
+
+def test_batch_with_empty_and_tiny_pairs():
+    """Pairs with blocks but no records (30 bases: candidate offsets, no
+    40-mer), with no blocks at all (10 bases), and with a handful of records,
+    batched between ordinary pairs: their extraction ranges, digit counts and
+    bucket starts are empty or tiny next to the others'."""
+    rng = np.random.default_rng(44)
+    gdbs = [synth.workload(1, 0.02),
+            synth.from_codes([rng.integers(0, 4, 30).astype(np.uint8)]),
+            synth.from_codes([rng.integers(0, 4, 10).astype(np.uint8)]),
+            synth.from_codes([rng.integers(0, 4, 90).astype(np.uint8), rng.integers(0, 4, 45).astype(np.uint8)]),
+            synth.workload(1, 0.01)]
+    with pga.Context(tile_records=2048) as ctx:
+        ctx.load_batch(gdbs)
+        ctx.build_index()
+        ctx.merge()
+        for p, g in enumerate(gdbs):
+            print(f"[empty-tiny] pair {p} bases={g.seqtot}", flush=True)
+            ent, cum, info = ctx.index_export(p)
+            N, Cs, nb, cb = ctx.seeds_export(p)
+            want = oracle.pipeline(g.bps, g.boff[:-1], g.clen)
+            assert info["nrec"] == want["nrec"]
+            assert np.array_equal(ent, want["entries"])
+            assert np.array_equal(cum, want["cum"]) and info["maxpre"] == want["maxpre"]
+            assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"])
+            assert np.array_equal(nb, want["nbuck"]) and np.array_equal(cb, want["cbuck"])
