@@ -1122,8 +1122,12 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         HIPCHK(c, hipMemcpyAsync(c->ssbufs.p, bp, sizeof(bp), hipMemcpyHostToDevice, c->st));
     }
     if (n > 0) {
+        // the keys' first-digit counts go to the pair's row of hist1 (the
+        // build's counts are no longer needed once it is done)
+        unsigned long long *hrow = P<unsigned long long>(c->hist1) + (size_t) pair * 256;
+        TRY(fill_u64(c, hrow, 256, 0));
         hipLaunchKernelGGL(k_seed_keys, dim3(grid_for(c, (const void *) k_seed_keys, 256, 0, (n + 255) / 256)), dim3(256),
-                           0, c->st, stream, n, d.P, d.C, maxctg, dbyte, comp, kf, P<Rec>(c->ssk[0]));
+                           0, c->st, stream, n, d.P, d.C, maxctg, dbyte, comp, kf, P<Rec>(c->ssk[0]), hrow);
         LAUNCHCHK(c);
         // partition + tile sort, the keys ending in ssk[0]; the build's statistics are kept
         const int CAP = cap_of(c);
@@ -1143,8 +1147,8 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         const int64_t sb = c->scatter_bytes;
         const int64_t nt = c->ntiles;
         int levels = 0;
-        const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0, nullptr, false, seg_cap, tile_cap,
-                                 &levels);
+        const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0,
+                                 P<unsigned long long>(c->hist1), false, seg_cap, tile_cap, &levels);
         const int64_t sort_tiles = c->ntiles;
         c->nscatter = ns;
         c->scatter_bytes = sb;

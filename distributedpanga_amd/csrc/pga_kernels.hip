@@ -1825,9 +1825,15 @@ __device__ __forceinline__ uint64_t le_bytes(const uint8_t *p, int n)
     return v;
 }
 
-__global__ void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P, int C, int64_t maxctg, int dbyte,
-                            int comp, SeedKeyFields kf, Rec *__restrict__ keys)
+// (also counts the keys' first radix digit, the top byte, into hrow: the
+// partition's first level needs no counting pass of its own)
+__global__ __launch_bounds__(256) void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P, int C,
+                                                   int64_t maxctg, int dbyte, int comp, SeedKeyFields kf,
+                                                   Rec *__restrict__ keys, unsigned long long *__restrict__ hrow)
 {
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
     const int s = 1 + 2 * (P + C);
     const int64_t amx = maxctg, bmx = maxctg, maxdag = amx + bmx;
     const uint64_t cmask = (1ull << (8 * C - 1)) - 1ull;
@@ -1850,7 +1856,10 @@ __global__ void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P,
         x.w0 = (uint64_t) (k >> 64);
         x.w1 = (uint64_t) k;
         keys[i] = x;
+        atomicAdd(&hist[(uint32_t) (x.w0 >> 56)], 1u);
     }
+    __syncthreads();
+    if (hist[threadIdx.x]) atomicAdd(&hrow[threadIdx.x], (unsigned long long) hist[threadIdx.x]);
 }
 
 // Sorted keys back to packed swide-byte records: the fields of each key are
