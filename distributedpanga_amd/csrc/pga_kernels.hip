@@ -996,35 +996,36 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     constexpr int NB = CAP;
     uint32_t *bh = reinterpret_cast<uint32_t *>(smem_raw + (size_t) CAP * sizeof(Rec));
     auto bin_start = [&](uint32_t b) -> uint32_t { return (bh[b >> 1] >> ((b & 1u) << 4)) & 0xFFFFu; };
-    __shared__ uint64_t mn0[T / 64], mn1[T / 64], mx0[T / 64], mx1[T / 64];
+    __shared__ uint32_t dmn[T / 64], dmx[T / 64];
     __shared__ uint32_t scan_sh[T / 64 + 1];
     __shared__ int fallback;
 
-    // min and max key of the tile
-    uint64_t lo0 = ~0ull, lo1 = ~0ull, hi0 = 0, hi1 = 0;
+    // smallest and largest digit at the tile's level (its records share the
+    // digits above it)
+    const int L = tl.level;
+    const int dp = digit_pos(L, w2), dbits = digit_bits(L, w2);
+    uint32_t dlo = ~0u, dhi = 0u;
 #pragma unroll
     for (int k = 0; k < E; k++) {
-        const int i = k * T + tid;
-        if (i < n) {
-            if (r[k].w0 < lo0 || (r[k].w0 == lo0 && r[k].w1 < lo1)) { lo0 = r[k].w0; lo1 = r[k].w1; }
-            if (r[k].w0 > hi0 || (r[k].w0 == hi0 && r[k].w1 > hi1)) { hi0 = r[k].w0; hi1 = r[k].w1; }
+        if (k * T + tid < n) {
+            const uint32_t d = (uint32_t) (key_bits_from(r[k].w0, r[k].w1, dp) >> (64 - dbits));
+            dlo = min(dlo, d);
+            dhi = max(dhi, d);
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        uint64_t x0 = __shfl_xor(lo0, o, 64), x1 = __shfl_xor(lo1, o, 64);
-        if (x0 < lo0 || (x0 == lo0 && x1 < lo1)) { lo0 = x0; lo1 = x1; }
-        uint64_t y0 = __shfl_xor(hi0, o, 64), y1 = __shfl_xor(hi1, o, 64);
-        if (y0 > hi0 || (y0 == hi0 && y1 > hi1)) { hi0 = y0; hi1 = y1; }
+        dlo = min(dlo, (uint32_t) __shfl_xor((int) dlo, o, 64));
+        dhi = max(dhi, (uint32_t) __shfl_xor((int) dhi, o, 64));
     }
-    if ((tid & 63) == 0) { mn0[tid >> 6] = lo0; mn1[tid >> 6] = lo1; mx0[tid >> 6] = hi0; mx1[tid >> 6] = hi1; }
+    if ((tid & 63) == 0) { dmn[tid >> 6] = dlo; dmx[tid >> 6] = dhi; }
     for (int c = tid; c < NB / 2; c += T) bh[c] = 0;
     if (tid == 0) fallback = 0;
     __syncthreads();
-    lo0 = mn0[0]; lo1 = mn1[0]; hi0 = mx0[0]; hi1 = mx1[0];
+    uint32_t dmin = dmn[0], dmax = dmx[0];
     for (int w = 1; w < T / 64; w++) {
-        if (mn0[w] < lo0 || (mn0[w] == lo0 && mn1[w] < lo1)) { lo0 = mn0[w]; lo1 = mn1[w]; }
-        if (mx0[w] > hi0 || (mx0[w] == hi0 && mx1[w] > hi1)) { hi0 = mx0[w]; hi1 = mx1[w]; }
+        dmin = min(dmin, dmn[w]);
+        dmax = max(dmax, dmx[w]);
     }
     // Bin of a record: the top BB bits of (its digit at the tile's level - the
     // smallest such digit, then the key bits right below that digit).  A tile
@@ -1033,13 +1034,10 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     // inside a bucket are near-uniform: bins hold ~1 record unless the k-mers
     // repeat.
     constexpr int BB = __builtin_ctz(NB);
-    const int L = tl.level;
-    const uint32_t dmin = digit_at(lo0, lo1, L, w2), dmax = digit_at(hi0, hi1, L, w2);
     const int span = dmax > dmin ? 32 - __builtin_clz(dmax - dmin) : 0;     // <= the digit's bits
     // The 64 key bits from the digit on, minus dmin in the digit's field, are
     // (digit - dmin) then the bits below it, with the top (bits - span) bits
     // zero: the bin is their next BB bits.
-    const int dp = digit_pos(L, w2), dbits = digit_bits(L, w2);
     const uint64_t dsub = (uint64_t) dmin << (64 - dbits);
     const int bsh = 64 - BB - dbits + span;                                 // 41..53
     auto bin_of = [&](uint64_t w0, uint64_t w1) -> uint32_t {
