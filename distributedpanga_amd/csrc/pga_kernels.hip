@@ -17,6 +17,7 @@
 // sort that also writes LCPs and GIX entries -> prefix table -> merge.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "pga_device.h"
 
 namespace pga {
@@ -1678,41 +1679,62 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         uint8_t *gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
         uint8_t *gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
         const uint64_t clear = ~(1ull << sbit);
+        // One anchor's seeds: partners qa..gi-1 then gi+1..qb-1, each to the N
+        // (same strand) or C run; a tile of several rounds keeps only the
+        // round's window [n0, n1) / [c0, c1).  PC4: payloads of >= 4 bytes (two
+        // 8-byte stores per seed), else two 4-byte stores.
+        auto walk = [&](auto pc4, auto windowed, int k, int &kn, int &kc, int n0, int n1, int c0, int c1,
+                        uint8_t *bN, uint8_t *bC) {
+            const int gi = H + threadIdx.x * A + k;
+            const uint64_t pi = pay[gi];
+            const uint64_t si = (pi >> sbit) & 1;
+            const uint64_t hd = (uint64_t) (v[k] & 63u) | ((pi & clear) << 8);
+            const int qa = gi - (int) ((v[k] >> 6) & 63u), qb = gi + (int) ((v[k] >> 12) & 63u);
+            auto one = [&](int q) {
+                const uint64_t pq = pay[q];
+                const bool same = ((pq >> sbit) & 1) == si;
+                int idx = same ? kn : kc;
+                kn += same;
+                kc += !same;
+                if (decltype(windowed)::value) {
+                    idx -= same ? n0 : c0;
+                    if (idx < 0 || idx >= (same ? n1 - n0 : c1 - c0)) return;
+                }
+                uint8_t *dst = (same ? bN : bC) + idx * s;
+                if (decltype(pc4)::value) {
+                    uint64_t w0, w1;
+                    seed_words(hd, pq, pc, w0, w1);
+                    __builtin_memcpy(dst, &w0, 8);
+                    __builtin_memcpy(dst + s - 8, &w1, 8);
+                } else {
+                    const uint64_t w = hd | (pq << (8 * (1 + pc)));
+                    const uint32_t x0 = (uint32_t) w, x1 = (uint32_t) (w >> (8 * (s - 4)));
+                    __builtin_memcpy(dst, &x0, 4);
+                    __builtin_memcpy(dst + s - 4, &x1, 4);
+                }
+            };
+            for (int q = qa; q < gi; q++) one(q);
+            for (int q = gi + 1; q < qb; q++) one(q);
+        };
         for (int r = 0; r < rounds; r++) {
             const int n0 = r * capN, n1 = min(n0 + capN, totN);
             const int c0 = r * capN, c1 = min(c0 + capN, totC);
             const int leadN = (int) ((uintptr_t) (gN + (int64_t) n0 * s) & 15);
             const int leadC = (int) ((uintptr_t) (gC + (int64_t) c0 * s) & 15);
+            // runs of the round, indexed by the seed's number in its stream
+            uint8_t *const bN = obuf[0] + leadN, *const bC = obuf[1] + leadC;
             int kn = (int) (off & 0xFFFFu), kc = (int) (off >> 16);
+            using T1 = std::integral_constant<bool, true>;
+            using F1 = std::integral_constant<bool, false>;
 #pragma unroll
             for (int k = 0; k < A; k++) {
                 if (!(v[k] >> 18)) continue;
-                const int gi = H + threadIdx.x * A + k;
-                const uint64_t pi = pay[gi];
-                const uint64_t si = (pi >> sbit) & 1;
-                const uint64_t hd = (uint64_t) (v[k] & 63u) | ((pi & clear) << 8);
-                const int qa = gi - (int) ((v[k] >> 6) & 63u), qb = gi + (int) ((v[k] >> 12) & 63u);
-                for (int q = qa; q < qb; q++) {
-                    if (q == gi) continue;
-                    const uint64_t pq = pay[q];
-                    const bool same = ((pq >> sbit) & 1) == si;
-                    const int idx = same ? kn - n0 : kc - c0;
-                    const int lim = same ? n1 - n0 : c1 - c0;
-                    kn += same;
-                    kc += !same;
-                    if (idx < 0 || idx >= lim) continue;
-                    uint8_t *dst = obuf[same ? 0 : 1] + (same ? leadN : leadC) + idx * s;
-                    if (pc >= 4) {
-                        uint64_t w0, w1;
-                        seed_words(hd, pq, pc, w0, w1);
-                        __builtin_memcpy(dst, &w0, 8);
-                        __builtin_memcpy(dst + s - 8, &w1, 8);
-                    } else {
-                        const uint64_t w = hd | (pq << (8 * (1 + pc)));
-                        const uint32_t x0 = (uint32_t) w, x1 = (uint32_t) (w >> (8 * (s - 4)));
-                        __builtin_memcpy(dst, &x0, 4);
-                        __builtin_memcpy(dst + s - 4, &x1, 4);
-                    }
+                if (rounds > 1) {
+                    if (pc >= 4) walk(T1(), T1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
+                    else walk(F1(), T1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
+                } else {
+                    if (pc >= 4) walk(T1(), F1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
+                    else walk(F1(), F1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
                 }
             }
             __syncthreads();
