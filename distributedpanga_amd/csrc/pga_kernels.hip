@@ -1524,30 +1524,36 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
     load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
     __syncthreads();
     uint32_t nN = 0, nC = 0;
+    // the thread's anchors, with the search chosen once per tile (loop-free
+    // for FREQ <= 16)
+    auto anchors = [&](auto swar_c) {
 #pragma unroll
-    for (int k = 0; k < A; k++) {
-        const int gi = H + threadIdx.x * A + k;
-        const int64_t g = lo + threadIdx.x * A + k;
-        uint32_t info = 0, xn = 0, xc = 0;
-        if (swar) {
-            info = anchor_info16(win, gi, freq, xn, xc);
-        } else {
-            int plen, a, b;
-            if (anchor_interval(win, gi, freq, plen, a, b)) {
-                const uint8_t si = win[gi] & 0x80;
-                for (int q = a; q < b; q++) {
-                    if (q == gi) continue;
-                    if ((win[q] & 0x80) == si) xn++; else xc++;
+        for (int k = 0; k < A; k++) {
+            const int gi = H + threadIdx.x * A + k;
+            const int64_t g = lo + threadIdx.x * A + k;
+            uint32_t info = 0, xn = 0, xc = 0;
+            if (decltype(swar_c)::value) {
+                info = anchor_info16(win, gi, freq, xn, xc);
+            } else {
+                int plen, a, b;
+                if (anchor_interval(win, gi, freq, plen, a, b)) {
+                    const uint8_t si = win[gi] & 0x80;
+                    for (int q = a; q < b; q++) {
+                        if (q == gi) continue;
+                        if ((win[q] & 0x80) == si) xn++; else xc++;
+                    }
+                    info = minfo_pack(plen, gi - a, b - gi, xn, xc);
                 }
-                info = minfo_pack(plen, gi - a, b - gi, xn, xc);
+            }
+            if (g < end) {
+                minfo[g] = info;
+                nN += xn;
+                nC += xc;
             }
         }
-        if (g < end) {
-            minfo[g] = info;
-            nN += xn;
-            nC += xc;
-        }
-    }
+    };
+    if (swar) anchors(std::integral_constant<bool, true>());
+    else anchors(std::integral_constant<bool, false>());
     const uint64_t sum = block_sum_u64<MTH>((uint64_t) nN | (uint64_t) nC << 32, red);
     if (threadIdx.x == 0) { cntN[LB] = (uint32_t) sum; cntC[LB] = (uint32_t) (sum >> 32); }
     __syncthreads();
