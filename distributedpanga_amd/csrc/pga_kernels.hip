@@ -75,18 +75,18 @@ __device__ __forceinline__ void window80(const uint64_t *__restrict__ b, int64_t
 // min(forward, reverse complement) of (TMap[4-mer] << 8 | TMap[4-mer]).
 // tl[x] = TMap[field-reversed x], tm[x] = TMap[x]; the reverse-complement
 // 4-mer of an LSB-first byte x, packed MSB-first, is ~x.
-__device__ __forceinline__ uint32_t hash8(uint64_t W, int t, const uint32_t *tl, const uint32_t *tm)
+__device__ __forceinline__ uint32_t hash8(uint64_t W, int t, const uint8_t *tl, const uint8_t *tm)
 {
     const uint32_t lo = (uint32_t) (W >> (2 * t)) & 0xFFu;
     const uint32_t hi = (uint32_t) (W >> (2 * t + 8)) & 0xFFu;
-    const uint32_t fw = (tl[lo] << 8) | tl[hi];
-    const uint32_t rv = (tm[hi ^ 0xFFu] << 8) | tm[lo ^ 0xFFu];
+    const uint32_t fw = ((uint32_t) tl[lo] << 8) | tl[hi];
+    const uint32_t rv = ((uint32_t) tm[hi ^ 0xFFu] << 8) | tm[lo ^ 0xFFu];
     return fw < rv ? fw : rv;
 }
 
 // Closed (12,8)-syncmer test on the 12 bases in bits 0..23 of W (LSB-first):
 // the earliest or the last of the five 8-mer hashes is the minimum.
-__device__ __forceinline__ bool syncmer(uint64_t W, const uint32_t *tl, const uint32_t *tm)
+__device__ __forceinline__ bool syncmer(uint64_t W, const uint8_t *tl, const uint8_t *tm)
 {
     uint32_t h[5];
 #pragma unroll
@@ -95,7 +95,9 @@ __device__ __forceinline__ bool syncmer(uint64_t W, const uint32_t *tl, const ui
     return h[0] == m || h[4] == m;
 }
 
-__device__ __forceinline__ void load_tables(uint32_t *tl, uint32_t *tm)
+// byte tables: 256 entries in 64 LDS words, so random lookups meet few bank
+// conflicts
+__device__ __forceinline__ void load_tables(uint8_t *tl, uint8_t *tm)
 {
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {
         uint32_t r = ((t & 3) << 6) | (((t >> 2) & 3) << 4) | (((t >> 4) & 3) << 2) | ((t >> 6) & 3);
@@ -234,7 +236,7 @@ struct XEmit {
 // Syncmer test at offset j of contig c; with keys, also the records' keys.
 __device__ __forceinline__ int scan_position(const uint64_t *__restrict__ bps64,
                                              const ContigDev &c, int64_t j, int jbits,
-                                             const uint32_t *tl, const uint32_t *tm,
+                                             const uint8_t *tl, const uint8_t *tm,
                                              bool keys, XEmit &out)
 {
     uint64_t W0;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
                                                       uint32_t *__restrict__ rangehist)
 {
     static_assert(XT == 256, "one thread per digit");
-    __shared__ uint32_t tl[256], tm[256];
+    __shared__ uint8_t tl[256], tm[256];
     __shared__ uint64_t red[XT / 64 + 1];
     __shared__ uint32_t hist[256];
     load_tables(tl, tm);
