@@ -197,8 +197,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0, help="workload scale (1.0 = full size)")
     ap.add_argument("--tile", type=int, default=4096)
-    ap.add_argument("--chunk", type=int, default=4096)
-    ap.add_argument("--merge-tile", type=int, default=256)
+    ap.add_argument("--chunk", type=int, default=8192)
+    ap.add_argument("--merge-tile", type=int, default=512)
     ap.add_argument("--digit2", type=int, default=10, help="bits of the partition's second digit (8..10)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

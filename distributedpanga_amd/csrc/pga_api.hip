@@ -333,8 +333,8 @@ void pga_default_opts(pga_opts *o)
     o->device = 0;
     o->freq = 10;
     o->tile_records = 4096;
-    o->chunk_records = 4096;
-    o->merge_tile = 256;
+    o->chunk_records = 8192;
+    o->merge_tile = 512;
     o->build_cum = 1;
     o->digit2_bits = 10;
 }

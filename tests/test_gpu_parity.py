@@ -65,7 +65,7 @@ def test_second_digit_widths(tile, chunk, digit2):
     assert got["timings"]["partition_levels"] >= 2
 
 
-@pytest.mark.parametrize("chunk,mt", [(2048, 512), (8192, 1024)])
+@pytest.mark.parametrize("chunk,mt", [(2048, 512), (8192, 1024), (4096, 256)])
 def test_chunk_and_merge_tiles(chunk, mt):
     _compare(synth.workload(1, 0.2), chunk_records=chunk, merge_tile=mt)
 
