@@ -1846,13 +1846,6 @@ struct SeedKeyFields {
     int shift;                 // 128 - key bits
 };
 
-__device__ __forceinline__ uint64_t le_bytes(const uint8_t *p, int n)
-{
-    uint64_t v = 0;
-    for (int k = n - 1; k >= 0; k--) v = (v << 8) | p[k];
-    return v;
-}
-
 // (also counts the keys' first radix digit, the top byte, into hrow: the
 // partition's first level needs no counting pass of its own)
 __global__ __launch_bounds__(256) void k_seed_keys(const uint8_t *__restrict__ seeds, int64_t n, int P, int C,
@@ -1867,12 +1860,21 @@ __global__ __launch_bounds__(256) void k_seed_keys(const uint8_t *__restrict__ s
     const uint64_t cmask = (1ull << (8 * C - 1)) - 1ull;
     const uint64_t dmask = dbyte >= 8 ? ~0ull : (1ull << (8 * dbyte)) - 1ull;
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        // the seed's s <= 15 bytes as one little-endian 128-bit value (two
+        // unaligned 8-byte loads; the streams carry 16 bytes of padding)
         const uint8_t *b = seeds + i * s;
-        const uint64_t lcp = b[0];
-        const int64_t ipost = (int64_t) le_bytes(b + 1, P);
-        const uint64_t icont = le_bytes(b + 1 + P, C);
-        const int64_t jpost = (int64_t) le_bytes(b + 1 + P + C, P);
-        const uint64_t jcont = le_bytes(b + 1 + 2 * P + C, C) & cmask;
+        uint64_t lo, hi;
+        __builtin_memcpy(&lo, b, 8);
+        __builtin_memcpy(&hi, b + 8, 8);
+        const u128 v = ((u128) hi << 64) | lo;
+        auto field = [&](int off, int w) -> uint64_t {
+            return (uint64_t) (v >> (8 * off)) & (w >= 8 ? ~0ull : (1ull << (8 * w)) - 1);
+        };
+        const uint64_t lcp = lo & 0xFFu;
+        const int64_t ipost = (int64_t) field(1, P);
+        const uint64_t icont = field(1 + P, C);
+        const int64_t jpost = (int64_t) field(1 + P + C, P);
+        const uint64_t jcont = field(1 + 2 * P + C, C) & cmask;
         int64_t diag, anti;
         if (comp) { diag = maxdag - (ipost + jpost); anti = amx - (ipost - jpost); }
         else      { diag = bmx + (ipost - jpost);    anti = ipost + jpost; }
