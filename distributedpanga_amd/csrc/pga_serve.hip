@@ -458,8 +458,19 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
                 const bool same = (L.win[x] & 0x80) == si;
                 uint8_t *dst = same ? seedN + L.bN[s] + (gN++) * sb : seedC + L.bC[s] + (gC++) * sb;
                 const uint64_t pq = key_payload(L.key[j], jbits, P, Cb);
-                dst[0] = (uint8_t) plen;
-                for (int t = 0; t < pc; t++) { dst[1 + t] = (uint8_t) (pi >> (8 * t)); dst[1 + pc + t] = (uint8_t) (pq >> (8 * t)); }
+                // the seed's exact sb = 1 + 2 pc bytes in two overlapping stores
+                const uint64_t hd = (uint64_t) plen | (pi << 8);
+                if (pc >= 4) {
+                    uint64_t w0, w1;
+                    seed_words(hd, pq, pc, w0, w1);
+                    __builtin_memcpy(dst, &w0, 8);
+                    __builtin_memcpy(dst + sb - 8, &w1, 8);
+                } else {
+                    const uint64_t w = hd | (pq << (8 * (1 + pc)));
+                    const uint32_t x0 = (uint32_t) w, x1 = (uint32_t) (w >> (8 * (sb - 4)));
+                    __builtin_memcpy(dst, &x0, 4);
+                    __builtin_memcpy(dst + sb - 4, &x1, 4);
+                }
                 if (same) ln++; else lc++;
             }
             const int icont = (int) ((pi >> (8 * P)) & ((1ull << (8 * Cb - 1)) - 1));
