@@ -1506,6 +1506,52 @@ __device__ __forceinline__ uint32_t anchor_info16(const uint8_t *__restrict__ wi
     return minfo_pack((int) plen, cb, cf + 1, xn, xc);
 }
 
+// anchor_info16 with the partners restricted to query entries (the serving
+// join: a target anchor pairs only with query entries).  qf points at the
+// query-flag bytes (bit 0) of the window, indexed like win: qf[x] for window
+// entry x; the 8-byte reads around the anchor may touch bytes outside the
+// interval, which the interval masks drop.
+__device__ __forceinline__ uint32_t anchor_info16_q(const uint8_t *__restrict__ win, const uint8_t *__restrict__ qf,
+                                                    int gi, int freq, uint32_t &xn, uint32_t &xc)
+{
+    const uint64_t b0 = win8(win, gi - 7), f0 = win8(win, gi + 1);
+    const uint32_t Li = (uint32_t) (b0 >> 56) & 63u, Ln = (uint32_t) f0 & 63u;
+    const uint32_t li = Li >= 12 ? Li : 0, ln = Ln >= 12 ? Ln : 0;
+    const uint32_t plen = li > ln ? li : ln;
+    xn = xc = 0;
+    if (plen < 12) return 0;
+    const uint64_t lb0 = lcp_below(b0, plen), lf0 = lcp_below(f0, plen);
+    uint64_t b1 = 0, f1 = 0;
+    int cb = lb0 ? __builtin_clzll(lb0) >> 3 : 8;
+    if (!lb0) {
+        b1 = win8(win, gi - 15);
+        const uint64_t lb1 = lcp_below(b1, plen);
+        cb = 8 + (lb1 ? __builtin_clzll(lb1) >> 3 : 8);
+    }
+    int cf = lf0 ? __builtin_ctzll(lf0) >> 3 : 8;
+    if (!lf0) {
+        f1 = win8(win, gi + 9);
+        const uint64_t lf1 = lcp_below(f1, plen);
+        cf = 8 + (lf1 ? __builtin_ctzll(lf1) >> 3 : 8);
+    }
+    const int m = cb + 1 + cf;
+    if (m >= freq) return 0;
+    const uint64_t HI = 0x8080808080808080ull;
+    const uint64_t mb0 = 0x00FFFFFFFFFFFFFFull & ~((1ull << (cb >= 7 ? 0 : 8 * (7 - cb))) - 1);
+    const uint64_t mb1 = cb >= 8 ? ~((1ull << (8 * (15 - cb))) - 1) : 0ull;
+    const uint64_t mf0 = cf >= 8 ? ~0ull : (1ull << (8 * cf)) - 1;
+    const uint64_t mf1 = cf > 8 ? (1ull << (8 * (cf - 8))) - 1 : 0ull;
+    // query flags moved to bit 7 of each byte, like the strand bits
+    auto q7 = [&](int i) -> uint64_t { return (win8(qf, i) & 0x0101010101010101ull) << 7; };
+    const uint64_t qb0 = mb0 & HI & q7(gi - 7), qb1 = mb1 & HI & (mb1 ? q7(gi - 15) : 0ull);
+    const uint64_t qf0 = mf0 & HI & q7(gi + 1), qf1 = mf1 & HI & (mf1 ? q7(gi + 9) : 0ull);
+    const uint32_t part = __popcll(qb0) + __popcll(qb1) + __popcll(qf0) + __popcll(qf1);
+    const uint32_t ones = __popcll(b0 & qb0) + __popcll(b1 & qb1) + __popcll(f0 & qf0) + __popcll(f1 & qf1);
+    xn = (b0 >> 63) ? ones : part - ones;
+    xc = part - xn;
+    return minfo_pack((int) plen, cb, cf + 1, xn, xc);
+}
+
 // Seed counts of the tiles of A*MTH anchors (A consecutive anchors per thread).
 template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,

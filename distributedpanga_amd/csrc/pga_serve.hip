@@ -390,7 +390,10 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
                                                     unsigned long long *__restrict__ cbuck)
 {
     __shared__ PackLds L;
-    const int H = freq + 1;
+    // FREQ <= 16: loop-free interval search, which reads 15 window bytes before
+    // an anchor and 16 after it, so the zeroed halo is 17 bytes
+    const bool swar = freq <= 16;
+    const int H = swar ? 17 : freq + 1;
     for (int64_t p = blockIdx.x; p < npack; p += gridDim.x) {
         pack_load(L, p, pstart, soff, js, segs, qd, tkeys, qkeys, tpos, ofsN, ofsC, H, EMIT);
         if (EMIT)
@@ -404,15 +407,22 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
             const int pos = pos0 + k;
             if (pos >= npos || !(L.flg[pos] & 2)) continue;
             const int gi = H + pos;
-            int plen, a, b;
-            if (!anchor_interval(L.win, gi, freq, plen, a, b)) continue;
             const bool iq = L.flg[pos] & 1;
-            const uint8_t si = L.win[gi] & 0x80;
             uint32_t ln = 0, lc = 0;
-            for (int x = a; x < b; x++) {
-                if (x == gi) continue;
-                if (!iq && !(L.flg[x - H] & 1)) continue;
-                if ((L.win[x] & 0x80) == si) ln++; else lc++;
+            if (swar) {
+                // a query anchor pairs with every entry of its interval, a
+                // target anchor only with query entries
+                if (iq) anchor_info16(L.win, gi, freq, ln, lc);
+                else anchor_info16_q(L.win, L.flg - H, gi, freq, ln, lc);
+            } else {
+                int plen, a, b;
+                if (!anchor_interval(L.win, gi, freq, plen, a, b)) continue;
+                const uint8_t si = L.win[gi] & 0x80;
+                for (int x = a; x < b; x++) {
+                    if (x == gi) continue;
+                    if (!iq && !(L.flg[x - H] & 1)) continue;
+                    if ((L.win[x] & 0x80) == si) ln++; else lc++;
+                }
             }
             tn += ln;
             tc += lc;
@@ -444,7 +454,15 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
             const int pos = pos0 + k;
             const int gi = H + pos;
             int plen, a, b;
-            anchor_interval(L.win, gi, freq, plen, a, b);
+            if (swar) {
+                uint32_t xn, xc;
+                const uint32_t info = anchor_info16(L.win, gi, freq, xn, xc);
+                plen = (int) (info & 63u);
+                a = gi - (int) ((info >> 6) & 63u);
+                b = gi + (int) ((info >> 12) & 63u);
+            } else {
+                anchor_interval(L.win, gi, freq, plen, a, b);
+            }
             const int s = L.sidx[pos];
             const int P = L.Pb[s], Cb = L.Cb[s], sb = L.sbytes[s], pc = P + Cb;
             const bool iq = L.flg[pos] & 1;
