@@ -891,19 +891,29 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
         if (i < n) rk[k] = atomicAdd(&hist[digit_at(r[k].w0, r[k].w1, level, w2)], 1u);
     }
     __syncthreads();
+    // a thread's digits: their slots are reserved before the block scan, so the
+    // atomics' round trips overlap the scan and end in one wait
+    constexpr int PDM = ND_MAX / T > 1 ? ND_MAX / T : 1;
     const int d0 = tid * pd;
-    uint32_t hs = 0;
-    for (int q = 0; q < pd; q++)
-        if (d0 + q < nd) hs += hist[d0 + q];
+    uint32_t hq[PDM], hs = 0;
+    unsigned long long gq[PDM];
+#pragma unroll
+    for (int q = 0; q < PDM; q++) {
+        hq[q] = q < pd && d0 + q < nd ? hist[d0 + q] : 0u;
+        hs += hq[q];
+        gq[q] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < PDM; q++)
+        if (hq[q]) gq[q] = atomicAdd(&cursor[(size_t) s * nd + d0 + q], (unsigned long long) hq[q]);
     uint32_t tot;
     uint32_t lo = block_excl_scan<T, uint32_t>(hs, sh, &tot);
-    for (int q = 0; q < pd; q++) {
-        const int d = d0 + q;
-        if (d < nd) {
-            const uint32_t h = hist[d];
-            hist[d] = lo;
-            if (h) gb[d] = (int64_t) atomicAdd(&cursor[(size_t) s * nd + d], (unsigned long long) h) - lo;
-            lo += h;
+#pragma unroll
+    for (int q = 0; q < PDM; q++) {
+        if (q < pd && d0 + q < nd) {
+            hist[d0 + q] = lo;
+            if (hq[q]) gb[d0 + q] = (int64_t) gq[q] - lo;
+            lo += hq[q];
         }
     }
     __syncthreads();
