@@ -1432,13 +1432,24 @@ __device__ __forceinline__ int merge_block_pair(const PairDev *__restrict__ pair
 constexpr int MTH = 256;             // threads of the merge kernels
 constexpr int BUCK_LDS = 1024;       // per-pair contigs counted in LDS by the merge
 
-// LCP/strand bytes of window [w0, w0+n) of the pair into win (0 outside it).
+// LCP/strand bytes of window [w0, w0+n) of the pair into win (0 outside it),
+// n <= NMAX.  Every load of the thread is issued before the first is used.
+template <int NMAX>
 __device__ __forceinline__ void load_win(uint8_t *__restrict__ win, const uint8_t *__restrict__ ls,
                                          int64_t w0, int n, int64_t rec0, int64_t end)
 {
-    for (int k = threadIdx.x; k < n; k += MTH) {
+    constexpr int WI = (NMAX + MTH - 1) / MTH;
+    uint8_t x[WI];
+#pragma unroll
+    for (int j = 0; j < WI; j++) {
+        const int k = (int) threadIdx.x + j * MTH;
         const int64_t g = w0 + k;
-        win[k] = (g >= rec0 && g < end) ? ls[g] : 0;
+        x[j] = k < n && g >= rec0 && g < end ? ls[g] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < WI; j++) {
+        const int k = (int) threadIdx.x + j * MTH;
+        if (k < n) win[k] = x[j];
     }
 }
 
@@ -1579,7 +1590,7 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
     const int64_t end = pd.rec0 + pd.nrec;
     const bool swar = freq <= 16;
     const int H = swar ? 17 : freq + 1;          // window halo on each side
-    load_win(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
+    load_win<TA + 2 * MAX_FREQ + 4>(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
     __syncthreads();
     uint32_t nN = 0, nC = 0;
     // the thread's anchors, with the search chosen once per tile (loop-free
@@ -1723,10 +1734,18 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
             const int64_t whi = lo + TA + H < end ? lo + TA + H : end;
             const uint8_t *ew = entries + pd.ent_off + (lo - H - pd.rec0) * (int64_t) e + 9;
             const int k0 = (int) (wlo - (lo - H)), k1 = (int) (whi - (lo - H));
-            for (int k = k0 + (int) threadIdx.x; k < k1; k += MTH) {
-                uint64_t x;
-                __builtin_memcpy(&x, ew + (int64_t) k * e, 8);
-                pay[k] = x & pmask;
+            // every load of the thread is issued before the first one is used
+            constexpr int WI = (W + MTH - 1) / MTH;
+            uint64_t x[WI];
+#pragma unroll
+            for (int j = 0; j < WI; j++) {
+                const int k = k0 + (int) threadIdx.x + j * MTH;
+                if (k < k1) __builtin_memcpy(&x[j], ew + (int64_t) k * e, 8);
+            }
+#pragma unroll
+            for (int j = 0; j < WI; j++) {
+                const int k = k0 + (int) threadIdx.x + j * MTH;
+                if (k < k1) pay[k] = x[j] & pmask;
             }
         }
         uint32_t sN = 0, sC = 0;
