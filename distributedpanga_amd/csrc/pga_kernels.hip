@@ -307,13 +307,22 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
     hist[threadIdx.x] = 0;
     __syncthreads();
     const XRange xr = ranges[r];
+    // the next block's first contig is loaded a block ahead, and a contig
+    // together with the next one's start
+    int cnext = xr.lb0 < xr.lb1 ? bctg[xr.lb0] : 0;
     for (int64_t LB = xr.lb0; LB < xr.lb1; LB++) {
     const int64_t fb = LB * XT * XU;
-    int c = bctg[LB];
+    int c = cnext;
+    if (LB + 1 < xr.lb1) cnext = bctg[LB + 1];
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint32_t cnt = 0, bits = 0;
-    while (c + 1 < nctg && ctg[c + 1].pos0 <= f0) c++;
-    const ContigDev c0 = ctg[c];
+    ContigDev c0 = ctg[c];
+    while (c + 1 < nctg) {
+        const ContigDev nx = ctg[c + 1];
+        if (nx.pos0 > f0) break;
+        c++;
+        c0 = nx;
+    }
     if (f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand) {
         // the thread's XU positions lie in one contig: the 8-mers at
         // j0 .. j0+XU+3 are hashed once and shared by the XU windows
@@ -442,14 +451,22 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     int64_t f0 = 0;
     ContigDev c0{};
     uint64_t A0 = 0, A1 = 0, B0 = 0, B1 = 0;
+    // (independent loads travel together: the bits with the block's first
+    // contig, a contig with the next one's start)
     auto fetch = [&](int64_t LB, int64_t lb1) {
-        bits = LB < lb1 ? sel[LB * XT + t] : 0u;
+        const bool in = LB < lb1;
+        bits = in ? sel[LB * XT + t] : 0u;
+        c = in ? bctg[LB] : 0;
         fast = false;
         if (!bits) return;
         f0 = LB * XT * XU + (int64_t) t * XU;
-        c = bctg[LB];
-        while (c + 1 < nctg && ctg[c + 1].pos0 <= f0) c++;
         c0 = ctg[c];
+        while (c + 1 < nctg) {
+            const ContigDev nx = ctg[c + 1];
+            if (nx.pos0 > f0) break;
+            c++;
+            c0 = nx;
+        }
         fast = f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand && f0 - c0.pos0 >= 28;
         if (fast) {
             window128(bps64, c0.bitoff + 2 * (f0 - c0.pos0), A0, A1);
