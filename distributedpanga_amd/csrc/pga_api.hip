@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "libpga_hip.h"
@@ -1204,6 +1205,174 @@ int pga_timings_get(pga_ctx *c, pga_timings *t)
 }
 
 } // extern "C"
+
+// ======================================================================== merge without allocation
+//
+// pga_merge_gix_noalloc: the self-merge of one GIX index that the caller holds
+// on the device, with every intermediate in one file-scope device array
+// (`scratch`) instead of pool buffers.  The kernels are the index merge's own
+// (k_merge_count, the scans, k_merge_emit); a small kernel first derives the
+// per-entry LCP/strand byte from the entries.
+
+namespace {
+
+constexpr int64_t NA_MAX = PGA_NOALLOC_MAX_ENTRIES;
+constexpr int NA_A = 2;                                   // anchors per thread: 512-entry merge tiles
+constexpr int64_t NA_TILE = (int64_t) MTH * NA_A;
+constexpr int64_t NA_MAXTILES = (NA_MAX + NA_TILE - 1) / NA_TILE;
+constexpr int64_t NA_SCANBLK = (NA_MAXTILES + SCAN_SPAN - 1) / SCAN_SPAN + 1;
+constexpr int64_t na_round(int64_t x) { return (x + 255) & ~(int64_t) 255; }
+// scratch layout (byte offsets, each 256-aligned)
+constexpr int64_t NA_OFF_PAIR = 0;
+constexpr int64_t NA_OFF_LS = NA_OFF_PAIR + na_round(sizeof(PairDev));
+constexpr int64_t NA_OFF_MINFO = NA_OFF_LS + na_round(NA_MAX + 64);
+constexpr int64_t NA_OFF_CNTN = NA_OFF_MINFO + na_round(NA_MAX * 4);
+constexpr int64_t NA_OFF_CNTC = NA_OFF_CNTN + na_round(NA_MAXTILES * 4);
+constexpr int64_t NA_OFF_OFSN = NA_OFF_CNTC + na_round(NA_MAXTILES * 4);
+constexpr int64_t NA_OFF_OFSC = NA_OFF_OFSN + na_round((NA_MAXTILES + 1) * 8);
+constexpr int64_t NA_OFF_BSUM = NA_OFF_OFSC + na_round((NA_MAXTILES + 1) * 8);
+constexpr int64_t NA_BYTES = NA_OFF_BSUM + na_round((NA_SCANBLK + 1) * 8);
+static_assert(NA_BYTES <= PGA_NOALLOC_SCRATCH_BYTES, "no-alloc scratch layout exceeds its stated size");
+
+} // namespace
+
+// One copy per GPU, alive until the process ends.  Host code never uses this
+// name as an address: the device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) uint8_t scratch[PGA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NA_MAXDEV = 64;
+std::mutex na_mu;
+uint8_t *na_ptr[NA_MAXDEV] = {};
+
+// The calling thread's current device's copy of `scratch` (looked up once per device).
+int na_scratch(uint8_t **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { (void) hipGetLastError(); return PGA_EHIP; }
+    if (dev < 0 || dev >= NA_MAXDEV) return PGA_EINVAL;
+    std::lock_guard<std::mutex> g(na_mu);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void) hipGetLastError();
+            return PGA_EHIP;
+        }
+        na_ptr[dev] = static_cast<uint8_t *>(p);
+    }
+    *out = na_ptr[dev];
+    return PGA_OK;
+}
+
+__global__ void k_na_pair(PairDev *dst, PairDev v) { *dst = v; }
+
+// LCP | strand << 7 of every entry (the merge count's input): the LCP byte at
+// offset 8, the strand as the top bit of the entry's last byte.
+__global__ void k_na_ls(const uint8_t *__restrict__ ent, int e, int64_t n, uint8_t *__restrict__ ls)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        ls[i] = (uint8_t) ((ent[i * e + 8] & 63) | (ent[i * e + e - 1] & 0x80));
+}
+
+unsigned na_grid(const void *fn, int threads, int64_t nlog)
+{
+    int dev = 0, ncu = 256, occ = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) {
+        (void) hipGetLastError();
+        ncu = 256;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0) != hipSuccess || occ < 1) {
+        (void) hipGetLastError();
+        occ = 1;
+    }
+    return (unsigned) std::max<int64_t>(1, std::min<int64_t>(nlog, (int64_t) ncu * occ));
+}
+
+} // namespace
+
+extern "C" int pga_merge_gix_noalloc(const uint8_t *entries, int64_t entries_bytes, int64_t nrec, int P, int C,
+                                     int ncontig, int freq, uint8_t *nseeds, int64_t ncap, uint8_t *cseeds,
+                                     int64_t ccap, int64_t *nbuck, int64_t *cbuck, int64_t *nlen, int64_t *clen,
+                                     void *stream)
+{
+    if (nlen) *nlen = 0;
+    if (clen) *clen = 0;
+    if (nrec < 0 || P < 1 || C < 1 || P + C > 7 || ncontig < 1 || freq < 1 || freq > MAX_FREQ ||
+        (nrec > 0 && !entries) || !nbuck || !cbuck || !nlen || !clen)
+        return PGA_EINVAL;
+    if (nrec > NA_MAX) return PGA_ELIMIT;
+    const int e = 9 + P + C, s = 1 + 2 * (P + C);
+    if (entries_bytes < nrec * e + 8) return PGA_EINVAL;      // the payload loads read 8 bytes at offset 9
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint8_t *sc = nullptr;
+    TRY(na_scratch(&sc));
+#define NA_CHK(call)                                                      \
+    do {                                                                  \
+        if ((call) != hipSuccess) { (void) hipGetLastError(); return PGA_EHIP; } \
+    } while (0)
+    PairDev pd{};
+    pd.rec0 = 0;
+    pd.nrec = nrec;
+    pd.P = P;
+    pd.C = C;
+    pd.e = e;
+    pd.s = s;
+    pd.ncontig = ncontig;
+    PairDev *dpair = reinterpret_cast<PairDev *>(sc + NA_OFF_PAIR);
+    uint8_t *ls = sc + NA_OFF_LS;
+    uint32_t *minfo = reinterpret_cast<uint32_t *>(sc + NA_OFF_MINFO);
+    uint32_t *cntN = reinterpret_cast<uint32_t *>(sc + NA_OFF_CNTN), *cntC = reinterpret_cast<uint32_t *>(sc + NA_OFF_CNTC);
+    int64_t *ofsN = reinterpret_cast<int64_t *>(sc + NA_OFF_OFSN), *ofsC = reinterpret_cast<int64_t *>(sc + NA_OFF_OFSC);
+    int64_t *bsum = reinterpret_cast<int64_t *>(sc + NA_OFF_BSUM);
+    const int64_t nmb = (nrec + NA_TILE - 1) / NA_TILE;
+
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) std::min<int64_t>((ncontig + 255) / 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long *>(nbuck), (int64_t) ncontig, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) std::min<int64_t>((ncontig + 255) / 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long *>(cbuck), (int64_t) ncontig, 0ull);
+    NA_CHK(hipGetLastError());
+    if (nmb == 0) {
+        NA_CHK(hipStreamSynchronize(st));
+        return PGA_OK;
+    }
+    hipLaunchKernelGGL(k_na_pair, dim3(1), dim3(1), 0, st, dpair, pd);
+    hipLaunchKernelGGL(k_na_ls, dim3(na_grid((const void *) k_na_ls, 256, (nrec + 255) / 256)), dim3(256), 0, st,
+                       entries, e, nrec, ls);
+    hipLaunchKernelGGL(k_merge_count<NA_A>, dim3(na_grid((const void *) k_merge_count<NA_A>, MTH, nmb)), dim3(MTH), 0,
+                       st, (const PairDev *) dpair, 1, (const uint8_t *) ls, freq, cntN, cntC, minfo, nmb);
+    NA_CHK(hipGetLastError());
+    const int64_t nb = (nmb + SCAN_SPAN - 1) / SCAN_SPAN;
+    for (int k = 0; k < 2; k++) {
+        const uint32_t *in = k ? cntC : cntN;
+        int64_t *out = k ? ofsC : ofsN;
+        hipLaunchKernelGGL(k_scan_reduce<uint32_t>, dim3((unsigned) nb), dim3(SCAN_T), 0, st, in, nmb, bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nb);
+        hipLaunchKernelGGL(k_scan_down<uint32_t>, dim3((unsigned) nb), dim3(SCAN_T), 0, st, in, nmb,
+                           (const int64_t *) bsum, out);
+        NA_CHK(hipGetLastError());
+    }
+    int64_t tot[2] = {0, 0};
+    NA_CHK(hipMemcpyAsync(&tot[0], ofsN + nmb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    NA_CHK(hipMemcpyAsync(&tot[1], ofsC + nmb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    NA_CHK(hipStreamSynchronize(st));
+    *nlen = tot[0] * s;
+    *clen = tot[1] * s;
+    if (*nlen > ncap || *clen > ccap) return PGA_EOVERFLOW;   // sizes of the streams reported, nothing written
+    if ((*nlen > 0 && !nseeds) || (*clen > 0 && !cseeds)) return PGA_EINVAL;
+    // a whole number of resident waves of blocks, four rounds deep (as pga_merge)
+    const int64_t egrid = std::min<int64_t>(nmb, (int64_t) na_grid((const void *) k_merge_emit<NA_A>, MTH, nmb) * 4);
+    const int64_t eper = (nmb + egrid - 1) / egrid;
+    hipLaunchKernelGGL(k_merge_emit<NA_A>, dim3((unsigned) egrid), dim3(MTH), 0, st, (const PairDev *) dpair, 1,
+                       entries, freq, (const int64_t *) ofsN, (const int64_t *) ofsC, nseeds, cseeds,
+                       reinterpret_cast<unsigned long long *>(nbuck), reinterpret_cast<unsigned long long *>(cbuck),
+                       (const uint32_t *) minfo, nmb, eper);
+    NA_CHK(hipGetLastError());
+    NA_CHK(hipStreamSynchronize(st));
+#undef NA_CHK
+    return PGA_OK;
+}
 
 // ======================================================================== serving
 

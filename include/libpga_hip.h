@@ -167,6 +167,36 @@ int pga_seed_sort_export(pga_ctx *ctx, uint8_t *records);
 
 int pga_timings_get(pga_ctx *ctx, pga_timings *t);
 
+/* ------------------------------------------------------------ no allocation
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the self-merge of pga_merge + pga_seeds_export for one GIX
+ * index that the caller already holds on the device, with no context and no
+ * allocation.  Its intermediates (LCP/strand bytes, interval records, tile
+ * counts and offsets) live in one static device array of
+ * PGA_NOALLOC_SCRATCH_BYTES that each GPU the library runs on keeps until the
+ * process ends, which limits an index to PGA_NOALLOC_MAX_ENTRIES entries
+ * (PGA_ELIMIT beyond).  All calls on one GPU share that array: calls that
+ * overlap on one GPU race (serialise them).
+ *
+ *   entries       device, nrec GIX entries of 9+P+C bytes (pga_index_export's
+ *                 layout), in a buffer of entries_bytes >= nrec*(9+P+C) + 8
+ *                 readable bytes (payloads are read 8 bytes at a time)
+ *   nseeds/ncap   device N stream and its capacity in bytes (cseeds/ccap: C)
+ *   nbuck, cbuck  device, ncontig int64 each: seeds per anchor contig
+ *   nlen, clen    host: the streams' lengths in bytes
+ *   stream        a hipStream_t (NULL: the null stream); returns once the
+ *                 outputs are complete on the device
+ * When a stream does not fit its capacity, PGA_EOVERFLOW is returned with
+ * *nlen and *clen set to the sizes needed and nothing written to the streams. */
+#define PGA_ELIMIT               -5        /* input beyond a fixed limit          */
+#define PGA_NOALLOC_MAX_ENTRIES  (1 << 23)
+#define PGA_NOALLOC_SCRATCH_BYTES (41u << 20)
+int pga_merge_gix_noalloc(const uint8_t *entries, int64_t entries_bytes, int64_t nrec, int P, int C,
+                          int ncontig, int freq, uint8_t *nseeds, int64_t ncap, uint8_t *cseeds, int64_t ccap,
+                          int64_t *nbuck, int64_t *cbuck, int64_t *nlen, int64_t *clen, void *stream);
+
 /* Wait for all work queued on the context's stream. */
 int pga_sync(pga_ctx *ctx);
 
