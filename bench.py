@@ -6,10 +6,12 @@ prefix table -> self-merge into N/C seed streams.  The `.bps` bytes are
 copied to the device once before timing (the PCIe-inclusive rate is reported
 separately).
 
-Printed (one JSON line): the plain single-pair rate on the 100 Mbp pair
-workload (`value`), the batched 64-pair workload in one batch and pair by pair,
-the dominant kernel's roofline, and the CPU baseline (the C oracle on a
-bounded sample of the same workload, one core).
+Printed (one JSON line): `value` is the rate of the human-scale pair (3.1 Gbp
+A + B, 4.84 G records, fixed launch shape) with the dominant kernel's roofline
+and the device check of its index; beside it the serving loop with genome A resident, the CPU baseline on a
+bounded sample of the same pair, the 100 Mbp pair (autotuned), and the
+64-pair workload batched and pair by pair, and the diagonal seed sort of the
+100 Mbp pair's streams.
 """
 import argparse
 import json
@@ -65,60 +67,87 @@ def algorithmic_bytes(seqtot, nrec, e, s, nseed):
     return seqtot / 4 + 16 * nrec + 64 * nrec + 16 * nrec + 2 * e * nrec + s * nseed
 
 
-TRAFFIC_PROFILE = "index_merge_workload2_hbm_traffic.md"
+TRAFFIC_PROFILE_HUMAN = "index_merge_human_scale_hbm_traffic.md"
+TRAFFIC_PROFILE_PLAIN = "index_merge_workload2_hbm_traffic.md"
 
 
-def measured_traffic(kernel, scale):
-    """HBM bytes per launch of one kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (valid for the full-size workload only)."""
-    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE)
-    if scale != 1.0 or not os.path.exists(path):
+def measured_traffic(kernel, profile):
+    """HBM bytes per launch of one kernel from a committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (of the full-size workload it names)."""
+    if profile is None:
+        return None, None
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
         return None, None
     key = f"traffic_bytes_per_launch[{kernel}]:"
     with open(path) as f:
         for line in f:
             if line.startswith(key):
-                return int(line.split(":")[1]), "profiles/" + TRAFFIC_PROFILE
+                return int(line.split(":")[1]), "profiles/" + profile
     return None, None
 
 
-def subsample(g, target_bases):
-    """The first A contigs and their B counterparts until target_bases."""
-    n_a = g.n_a
-    pick, tot = [], 0
-    for i in range(n_a):
-        if tot >= target_bases:
-            break
-        pick += [i, n_a + i]
-        tot += int(g.clen[i] + g.clen[n_a + i])
-    codes = [g.contig_codes(c) for c in pick]
-    return synth.from_codes(codes, n_a=len(pick) // 2)
+def subsample(g, per_contig):
+    """The first per_contig bases (a multiple of 4) of every contig of g, as one
+    GDB: a bounded sample that keeps the workload's contig count, so the
+    reference's split of whole contigs over workers stays as wide as on the
+    full pair."""
+    keep = np.minimum(g.clen, per_contig).astype(np.int64)
+    nbytes = (keep + 3) // 4
+    parts = [g.bps[int(g.boff[c]): int(g.boff[c]) + int(nbytes[c])] for c in range(g.ncontig)]
+    boff = np.zeros(g.ncontig + 1, np.int64)
+    boff[1:] = np.cumsum(nbytes)
+    return synth.Gdb(bps=np.concatenate(parts), boff=boff, clen=keep, n_a=g.n_a)
 
 
-def cpu_baseline(g, target_bases):
-    import oracle                                     # the checker, timed as the CPU baseline
-    s = subsample(g, target_bases)
+def cpu_threads(requested):
+    if requested > 0:
+        return requested
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env and env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(g, target_bases, threads):
+    """oracle/pga_cpu.c (the reference's pga-mpi decomposition on `threads`
+    host threads) on a bounded sample of the human-scale pair."""
+    import oracle                                     # the checker library, timed as the CPU baseline
+    T = cpu_threads(threads)
+    per = max(4096, (target_bases // max(g.ncontig, 1)) // 4 * 4)
+    s = subsample(g, per)
     t0 = time.perf_counter()
-    o = oracle.pipeline(s.bps, s.boff[:-1], s.clen)
+    o = oracle.pipeline(s.bps, s.boff[:-1], s.clen, threads=T)
     dt = time.perf_counter() - t0
-    return dict(value=round(s.seqtot / dt / 1e9, 6), unit="Gbases/s", cores=1, kind="port",
-                sample=f"first {len(s.clen) // 2} A contigs + their B counterparts of the 100 Mbp pair "
-                       f"workload: {s.seqtot} bases, {o['nrec']} records, single-threaded C oracle "
-                       f"(extract {o['t_extract']:.2f}s, sort+LCP {o['t_sort']:.2f}s, "
+    return dict(value=round(s.seqtot / dt / 1e9, 6), unit="Gbases/s", cores=T, kind="port",
+                host_cpus=os.cpu_count(), cpu_model=cpu_model(),
+                sample=f"the first {per} bases of each of the {s.ncontig} contigs of the human-scale pair: "
+                       f"{s.seqtot} bases, {o['nrec']} records; pga-mpi's decomposition on {T} threads "
+                       f"(extract {o['t_extract']:.2f}s, exchange+sort+LCP {o['t_sort']:.2f}s, "
                        f"adapter {o['t_adapter']:.2f}s, merge {o['t_merge']:.2f}s)",
                 seconds=round(dt, 3))
 
 
-def serving_leg(args):
-    """Serving: genome A of the human-scale pair workload (at --serve-scale)
-    resident as the target; query genomes (lognormal 0.5-50 Mbp, from random
+def serving_leg(args, target=None):
+    """Serving: genome A of the human-scale pair workload (at --serve-scale;
+    `target` when the caller already has it) resident as the target; query genomes (lognormal 0.5-50 Mbp, from random
     target segments at 1-20% divergence, segment order shuffled) arrive as a
     Poisson process at 70% of the measured capacity and are answered by the
     continuous-batching loop over --serve-slots slots."""
     t = time.perf_counter()
-    pair = synth.workload(4, args.serve_scale)
-    target = synth.take(pair, 0, pair.n_a)
-    del pair
+    if target is None:
+        pair = synth.workload(4, args.serve_scale)
+        target = synth.take(pair, 0, pair.n_a)
+        del pair
     rng = np.random.default_rng(0x5EED0005)
     sizes = np.clip(np.exp(rng.normal(np.log(5e6), 1.0, args.serve_pool)), 5e5, 5e7).astype(np.int64)
     divs = rng.uniform(0.01, 0.20, args.serve_pool)
@@ -152,130 +181,70 @@ def serving_leg(args):
                 gbases_s=round(sm["bases_per_s"] / 1e9, 4), p50_ms=round(sm["p50_ms"], 3),
                 p99_ms=round(sm["p99_ms"], 3), mean_ms=round(sm["mean_ms"], 3), batches=sm["batches"],
                 max_batch_seen=sm["max_batch"], max_inflight=sm["max_inflight"],
-                device_busy_frac=round(sm["device_ms"] / 1e3 / sm["wall_s"], 3), seeds=sm["seeds"],
+                slot_busy_sum=round(sm["device_ms"] / 1e3 / sm["wall_s"], 3),
+                slot_busy_mean=round(sm["device_ms"] / 1e3 / sm["wall_s"] / args.serve_slots, 3),
+                seeds=sm["seeds"],
                 pcie_inclusive=True)
 
 
-def human_leg(args, opts):
-    """The human-scale pair (3.1 Gbp A + B, 24 contigs of 50-250 Mbp, 45%
-    repeats, satellites; B with 2% SNP, 0.3% indel, 200 rearrangements) at
-    --human-scale: index build + merge with the input resident in HBM, the
-    device sortedness/LCP check, and the memory the context holds.  It runs
-    with the plain leg's shape options (tuned or given)."""
-    t = time.perf_counter()
-    log(f"[bench] generating the human-scale pair at scale {args.human_scale}")
-    g = synth.workload(4, args.human_scale)
-    log(f"[bench] human-scale pair at scale {args.human_scale}: {g.ncontig} contigs, {g.seqtot} bases "
-        f"generated ({time.perf_counter() - t:.1f}s)")
-    ctx = pga.Context(**opts)
-    ctx.load_gdb(g)
-    steps = max(1, args.human_steps)
-    dt, per, scat_bytes, _, _ = time_steps(ctx, steps, 1)
-    info = ctx.index_info(0)
-    sinfo = ctx.seeds_info(0)
-    nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
-    bad, first_bad, lcp_bad = ctx.verify(0)
-    ms = dt / steps * 1e3
-    scat_gbs = scat_bytes / (per["scatter_ms"] * 1e-3) / 1e9 if per["scatter_ms"] > 0 else 0.0
-    out = dict(workload=f"human-scale pair at scale {args.human_scale}", bases=g.seqtot, contigs=g.ncontig,
-               records=info["nrec"], seeds=int(nseed), ms_per_step=round(ms, 3),
-               gbases_s=round(g.seqtot / (ms * 1e-3) / 1e9, 4), steps=steps,
-               stages_ms={k: round(v, 3) for k, v in per.items()},
-               radix_scatter_gbs=round(scat_gbs, 1), radix_scatter_frac=round(scat_gbs / HBM_PEAK_GBS, 4),
-               verify=dict(kmer_order_violations=bad, first_bad=first_bad, lcp_mismatches=lcp_bad),
-               P=info["P"], C=info["C"], maxpre=info["maxpre"], opts=dict(opts))
-    ctx.close()
-    log(f"[bench] human-scale: {ms:.1f} ms/step, {out['gbases_s']} Gbases/s, {info['nrec']} records, "
-        f"{nseed} seeds, verify {out['verify']}")
-    return out
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=float, default=1.0, help="workload scale (1.0 = full size)")
-    ap.add_argument("--tile", type=int, default=4096)
-    ap.add_argument("--chunk", type=int, default=8192)
-    ap.add_argument("--merge-tile", type=int, default=512)
-    ap.add_argument("--digit2", type=int, default=10, help="bits of the partition's second digit (8..10)")
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-bases", type=int, default=150_000_000)
-    ap.add_argument("--batch-pairs", type=int, default=64)
-    ap.add_argument("--human-scale", type=float, default=0.0,
-                    help="also run the human-scale pair at this scale (1.0 = 3.1 Gbp A + B); 0 skips it")
-    ap.add_argument("--human-steps", type=int, default=3)
-    ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-seed-sort", action="store_true", help="skip the seed-sort leg (keeps profiles to the step)")
-    ap.add_argument("--serve-scale", type=float, default=0.1, help="scale of the resident target (1.0 = 3.1 Gbp)")
-    ap.add_argument("--serve-pool", type=int, default=48)
-    ap.add_argument("--serve-requests", type=int, default=240)
-    ap.add_argument("--serve-slots", type=int, default=2)
-    ap.add_argument("--serve-max-batch", type=int, default=8)
-    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--no-tune", action="store_true", help="use the fixed --tile/--chunk/--merge-tile/--digit2")
-    ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU exchange is a design, not a run)", file=sys.stderr)
-        sys.exit(2)
-
-    opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
-                digit2_bits=args.digit2)
-    t = time.perf_counter()
-    g = synth.workload(2, args.scale)
-    log(f"[bench] workload 2 generated: {g.ncontig} contigs, {g.seqtot} bases ({time.perf_counter() - t:.1f}s)")
-
-    ctx = pga.Context(**opts)
-    t = time.perf_counter()
-    ctx.load_gdb(g)
-    h2d_s = time.perf_counter() - t
-    tuned = None
-    if not args.no_tune:
-        t = time.perf_counter()
-        tuned, _ = autotune.tune(ctx, [g], use_cache=not args.retune, log=log)
-        opts.update(tuned)
-        log(f"[bench] tuned plain: {tuned} ({time.perf_counter() - t:.1f}s)")
-    dt, per, scat_bytes, scat_launches, tiles = time_steps(ctx, args.steps, args.warmup)
-    info = ctx.index_info(0)
-    sinfo = ctx.seeds_info(0)
-    nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
-    ms_step = dt / args.steps * 1e3
-    value = g.seqtot * args.steps / dt / 1e9
-    log(f"[bench] plain: {ms_step:.2f} ms/step, {value:.2f} Gbases/s, {info['nrec']} records, {nseed} seeds, "
-        f"stages {json.dumps({k: round(v, 3) for k, v in per.items()})}")
-
-    # The two longest launches of the step, each with its algorithmic bytes per
-    # launch.  Tile sort: every record read once (16 B); its GIX entry (e B),
-    # LCP/strand byte and 24-bit prefix (4 B) written once.  Merge emit: every
-    # GIX entry read once (e B), every seed (s B) and the two buck vectors
-    # written once.  The longer one is the roofline kernel, the other is
-    # reported beside it.
-    def kernel_line(name, nbytes, ms):
+def kernel_lines(info, nseed, per, traffic_profile):
+    """The step's two longest launches, each with its algorithmic bytes per
+    launch.  Tile sort: every record read once (16 B); its GIX entry (e B),
+    LCP/strand byte and 24-bit prefix (4 B) written once.  Merge emit: every
+    GIX entry read once (e B), every seed (s B) and the two buck vectors
+    written once.  Returns (dominant, other)."""
+    def line(name, nbytes, ms):
         gbs = nbytes / (ms * 1e-3) / 1e9
-        traffic, src = measured_traffic(name, args.scale)
+        traffic, src = measured_traffic(name, traffic_profile)
         return dict(kernel=name, achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
                     traffic_source=src, bytes_per_launch=int(nbytes), avg_launch_ms=round(ms, 4))
-    sort_k = kernel_line("k_tile_sort", info["nrec"] * (16 + info["e"] + 1 + 4), per["tile_sort_ms"])
-    emit_k = kernel_line("k_merge_emit", info["nrec"] * info["e"] + nseed * info["s"] + 16 * info["ncontig"],
-                         per["merge_emit_ms"])
-    dom, other = (emit_k, sort_k) if per["merge_emit_ms"] > per["tile_sort_ms"] else (sort_k, emit_k)
+    sort_k = line("k_tile_sort", info["nrec"] * (16 + info["e"] + 1 + 4), per["tile_sort_ms"])
+    emit_k = line("k_merge_emit", info["nrec"] * info["e"] + nseed * info["s"] + 16 * info["ncontig"],
+                  per["merge_emit_ms"])
+    return (emit_k, sort_k) if per["merge_emit_ms"] > per["tile_sort_ms"] else (sort_k, emit_k)
+
+
+def roofline_of(info, nseed, per, scat_bytes, seqtot, ms_step, traffic_profile):
+    dom, other = kernel_lines(info, nseed, per, traffic_profile)
     scat_gbs = scat_bytes / (per["scatter_ms"] * 1e-3) / 1e9 if per["scatter_ms"] > 0 else 0.0
-    balg = algorithmic_bytes(g.seqtot, info["nrec"], info["e"], info["s"], nseed)
+    balg = algorithmic_bytes(seqtot, info["nrec"], info["e"], info["s"], nseed)
     path_gbs = balg / (ms_step * 1e-3) / 1e9
     roofline = dict(bound="hbm", kernel=dom["kernel"], achieved=dom["achieved"], peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=dom["frac"], traffic=dom["traffic"], traffic_source=dom["traffic_source"],
                     bytes_per_launch=dom["bytes_per_launch"], avg_launch_ms=dom["avg_launch_ms"])
-    roofline[other["kernel"][2:]] = {k: v for k, v in other.items() if k not in ("kernel", "traffic_source")}
+    roofline[other["kernel"][2:]] = {k: v for k, v in other.items() if k != "kernel"}
     roofline["radix_scatter"] = dict(achieved=round(scat_gbs, 1), frac=round(scat_gbs / HBM_PEAK_GBS, 4),
-                                     bytes_per_step=int(scat_bytes), launches_per_step=scat_launches,
-                                     ms_per_step=round(per["scatter_ms"], 4))
+                                     bytes_per_step=int(scat_bytes), ms_per_step=round(per["scatter_ms"], 4))
     roofline["path"] = dict(algorithmic_bytes=int(balg), achieved=round(path_gbs, 1),
                             frac=round(path_gbs / HBM_PEAK_GBS, 4))
+    return roofline
+
+
+HUMAN_WORKLOAD = ("human-scale pair: A 3.1 Gbp in 24 contigs of 50-250 Mbp, GC 0.41, 45% repeats from 500 "
+                  "families (100-6000 bp, 5-25% divergence) + 2 Mbp of 171 bp satellites; B = A with 2% SNP, "
+                  "0.3% indel, 200 rearrangements of 10 kbp-5 Mbp; one GDB, k=40, FREQ=10")
+PLAIN_WORKLOAD = ("plain single pair: 100 Mbp A (500 lognormal contigs, GC 0.41, 2000x300 bp repeat at 10%, "
+                  "50 tandem arrays) + B (5% SNP, 0.5% indel, 10 inversions), one GDB, k=40, FREQ=10")
+
+
+def plain_leg(args):
+    """The 100 Mbp pair (autotuned, cached per shape): a side leg beside the
+    human-scale headline, with the diagonal seed sort of its two streams."""
+    opts = dict(tile_records=4096, chunk_records=8192, merge_tile=512, digit2_bits=10)
+    t = time.perf_counter()
+    g = synth.workload(2, 1.0)
+    log(f"[bench] 100 Mbp pair generated: {g.ncontig} contigs, {g.seqtot} bases ({time.perf_counter() - t:.1f}s)")
+    ctx = pga.Context(**opts)
+    ctx.load_gdb(g)
+    tuned = None
+    if not args.no_tune:
+        tuned, _ = autotune.tune(ctx, [g], use_cache=not args.retune, log=None)
+        opts.update(tuned)
+    steps = 10
+    dt, per, scat_bytes, _, _ = time_steps(ctx, steps, 2)
+    info = ctx.index_info(0)
+    sinfo = ctx.seeds_info(0)
+    nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
     # the next step downstream: the diagonal-sorted seed array of both streams
     ss = {}
     for comp, name in (() if args.no_seed_sort else ((0, "N"), (1, "C"))):
@@ -285,90 +254,158 @@ def main():
         for _ in range(reps):
             n = ctx._seed_sort_only(0, comp)
         ss[name] = dict(records=n, ms=round((time.perf_counter() - t0) / reps * 1e3, 3))
-    ss_seeds = sum(v["records"] for v in ss.values())
-    ss_ms = sum(v["ms"] for v in ss.values())
-    seed_sort = dict(streams=ss, records=ss_seeds, ms=round(ss_ms, 3),
-                     mrecords_s=round(ss_seeds / (ss_ms * 1e-3) / 1e6, 1) if ss_ms > 0 else None,
-                     note="reimport + per-panel sort of both seed streams, records left on the device")
-    if ss:
-        log(f"[bench] seed sort: {ss_seeds} records in {ss_ms:.2f} ms")
     ctx.close()
+    ms = dt / steps * 1e3
+    roof = roofline_of(info, nseed, per, scat_bytes, g.seqtot, ms, TRAFFIC_PROFILE_PLAIN)
+    log(f"[bench] 100 Mbp pair: {ms:.2f} ms/step, {g.seqtot / (ms * 1e-3) / 1e9:.2f} Gbases/s")
+    return dict(workload=PLAIN_WORKLOAD, bases=g.seqtot, records=info["nrec"], seeds=int(nseed),
+                ms_per_step=round(ms, 3), gbases_s=round(g.seqtot / (ms * 1e-3) / 1e9, 4), steps=steps,
+                opts=opts, autotuned=tuned is not None, stages_ms={k: round(v, 4) for k, v in per.items()},
+                roofline={k: roof[k] for k in ("kernel", "achieved", "frac", "radix_scatter", "path")},
+                seed_sort=dict(streams=ss, records=sum(v["records"] for v in ss.values()),
+                               ms=round(sum(v["ms"] for v in ss.values()), 3),
+                               note="reimport + per-panel sort of both seed streams, records left on the device"))
 
+
+def batch_leg(args):
+    t = time.perf_counter()
+    gdbs = synth.batch(args.batch_pairs, scale=1.0)
+    tot = sum(x.seqtot for x in gdbs)
+    log(f"[bench] batch workload generated: {len(gdbs)} pairs, {tot} bases ({time.perf_counter() - t:.1f}s)")
+    base = dict(tile_records=4096, chunk_records=8192, merge_tile=512, digit2_bits=10)
+    bopts = dict(base)
+    bctx = pga.Context(**bopts)
+    bctx.load_batch(gdbs)
+    if not args.no_tune:
+        btuned, _ = autotune.tune(bctx, gdbs, use_cache=not args.retune, log=None)
+        bopts.update(btuned)
+    bsteps = 5
+    bdt, bper, _, _, _ = time_steps(bctx, bsteps, 1)
+    bctx.close()
+    batched = tot * bsteps / bdt / 1e9
+    # plain form: the same pairs one at a time (input resident, one build + merge
+    # per pair), tuned on the median pair
+    popts = dict(base)
+    pctx = pga.Context(**popts)
+    if not args.no_tune:
+        med = sorted(gdbs, key=lambda x: x.seqtot)[len(gdbs) // 2]
+        pctx.load_gdb(med)
+        ptuned, _ = autotune.tune(pctx, [med], use_cache=not args.retune, log=None)
+        popts.update(ptuned)
+    plain_s = 0.0
+    for x in gdbs:
+        pctx.load_gdb(x)
+        pctx.build_index()
+        pctx.merge()
+        pctx.sync()
+        t0 = time.perf_counter()
+        pctx.build_index()
+        pctx.merge()
+        pctx.sync()
+        plain_s += time.perf_counter() - t0
+    pctx.close()
+    plain = tot / plain_s / 1e9
+    log(f"[bench] batched {batched:.2f} Gbases/s vs plain per pair {plain:.2f} Gbases/s")
+    return dict(workload=f"{len(gdbs)} independent pairs, A lognormal 1-12 Mbp, 1-50 contigs, "
+                         "GC 0.30-0.65, B at 1-15% SNP and 0.1-1% indel",
+                bases=tot, batched_gbases_s=round(batched, 4), batched_ms=round(bdt / bsteps * 1e3, 3),
+                plain_gbases_s=round(plain, 4), plain_ms=round(plain_s * 1e3, 3),
+                speedup=round(batched / plain, 3), batched_opts=bopts, plain_opts=popts,
+                batched_stages_ms={k: round(v, 3) for k, v in bper.items()})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="scale of the human-scale pair (1.0 = 3.1 Gbp A + B)")
+    ap.add_argument("--tile", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=8192)
+    ap.add_argument("--merge-tile", type=int, default=512)
+    ap.add_argument("--digit2", type=int, default=8, help="bits of the partition's second digit (8..10)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the 100 Mbp pair leg")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-seed-sort", action="store_true")
+    ap.add_argument("--only-headline", action="store_true", help="the human-scale pair only (profiling)")
+    ap.add_argument("--cpu-bases", type=int, default=500_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: OMP_NUM_THREADS, else the host's cores)")
+    ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--serve-scale", type=float, default=1.0,
+                    help="scale of the resident serving target (genome A of the human-scale pair)")
+    ap.add_argument("--serve-pool", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=240)
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-max-batch", type=int, default=8)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-tune", action="store_true", help="side legs use fixed shapes, not the autotuner")
+    ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU exchange is a design, not a run)", file=sys.stderr)
+        sys.exit(2)
+    if args.only_headline:
+        args.no_plain = args.no_batch = args.no_cpu = args.no_serve = args.no_seed_sort = True
+
+    # ---- headline: the human-scale pair, input resident in HBM
+    opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
+                digit2_bits=args.digit2)
+    t = time.perf_counter()
+    g = synth.workload(4, args.scale)
+    log(f"[bench] human-scale pair at scale {args.scale}: {g.ncontig} contigs, {g.seqtot} bases generated "
+        f"({time.perf_counter() - t:.1f}s)")
+    ctx = pga.Context(**opts)
+    t = time.perf_counter()
+    ctx.load_gdb(g)
+    h2d_s = time.perf_counter() - t
+    dt, per, scat_bytes, scat_launches, tiles = time_steps(ctx, args.steps, args.warmup)
+    info = ctx.index_info(0)
+    sinfo = ctx.seeds_info(0)
+    nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
+    ms_step = dt / args.steps * 1e3
+    value = g.seqtot * args.steps / dt / 1e9
+    log(f"[bench] human-scale pair: {ms_step:.2f} ms/step, {value:.2f} Gbases/s, {info['nrec']} records, "
+        f"{nseed} seeds, stages {json.dumps({k: round(v, 3) for k, v in per.items()})}")
+    bad, first_bad, lcp_bad = ctx.verify(0)
+    log(f"[bench] device check: {bad} order violations, {lcp_bad} LCP mismatches")
+    roofline = roofline_of(info, nseed, per, scat_bytes, g.seqtot, ms_step,
+                           TRAFFIC_PROFILE_HUMAN if args.scale == 1.0 else None)
+    roofline["radix_scatter"]["launches_per_step"] = scat_launches
+    ctx.close()
     out = {"metric": METRIC, "value": round(value, 4), "unit": "Gbases/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
            "data": "synthetic",
-           "config": {"workload": "plain single pair, 100 Mbp A (500 lognormal contigs, GC 0.41, "
-                                  "2000x300 bp repeat at 10%, 50 tandem arrays) + B (5% SNP, 0.5% indel, "
-                                  "10 inversions), one GDB, k=40, FREQ=10",
+           "config": {"workload": HUMAN_WORKLOAD if args.scale == 1.0 else
+                      f"{HUMAN_WORKLOAD} (at scale {args.scale})",
                       "bases": g.seqtot, "contigs": g.ncontig, "records": info["nrec"], "seeds": int(nseed),
-                      "scale": args.scale, "tile_records": opts["tile_records"],
+                      "P": info["P"], "C": info["C"], "scale": args.scale, "tile_records": opts["tile_records"],
                       "chunk_records": opts["chunk_records"], "merge_tile": opts["merge_tile"],
-                      "digit2_bits": opts["digit2_bits"],
-                      "autotuned": tuned is not None, "parallelism": "1 GPU"},
+                      "digit2_bits": opts["digit2_bits"], "parallelism": "1 GPU"},
            "stages_ms": {k: round(v, 4) for k, v in per.items()},
+           "tiles": tiles,
+           "device_check": dict(kmer_order_violations=bad, first_bad=first_bad, lcp_mismatches=lcp_bad),
            "h2d_load_s": round(h2d_s, 4),
            "pcie_inclusive_gbases_s": round(g.seqtot / (ms_step * 1e-3 + h2d_s) / 1e9, 4),
-           "roofline": roofline,
-           "seed_sort": seed_sort}
-
-    if not args.no_batch:
-        t = time.perf_counter()
-        gdbs = synth.batch(args.batch_pairs, scale=args.scale)
-        tot = sum(x.seqtot for x in gdbs)
-        log(f"[bench] batch workload generated: {len(gdbs)} pairs, {tot} bases ({time.perf_counter() - t:.1f}s)")
-        bopts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
-                     digit2_bits=args.digit2)
-        bctx = pga.Context(**bopts)
-        bctx.load_batch(gdbs)
-        if not args.no_tune:
-            btuned, _ = autotune.tune(bctx, gdbs, use_cache=not args.retune, log=log)
-            bopts.update(btuned)
-        bsteps = max(2, args.steps // 2)
-        bdt, bper, _, _, _ = time_steps(bctx, bsteps, 1)
-        bctx.close()
-        batched = tot * bsteps / bdt / 1e9
-        # plain form: the same pairs one at a time (input resident, one build + merge per pair),
-        # tuned on the median pair
-        popts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
-                     digit2_bits=args.digit2)
-        pctx = pga.Context(**popts)
-        if not args.no_tune:
-            med = sorted(gdbs, key=lambda x: x.seqtot)[len(gdbs) // 2]
-            pctx.load_gdb(med)
-            ptuned, _ = autotune.tune(pctx, [med], use_cache=not args.retune, log=log)
-            popts.update(ptuned)
-        plain_s = 0.0
-        for x in gdbs:
-            pctx.load_gdb(x)
-            pctx.build_index()
-            pctx.merge()
-            pctx.sync()
-            t0 = time.perf_counter()
-            pctx.build_index()
-            pctx.merge()
-            pctx.sync()
-            plain_s += time.perf_counter() - t0
-        pctx.close()
-        plain = tot / plain_s / 1e9
-        out["batched"] = dict(workload=f"{len(gdbs)} independent pairs, A lognormal 1-12 Mbp, 1-50 contigs, "
-                                       "GC 0.30-0.65, B at 1-15% SNP and 0.1-1% indel",
-                              bases=tot, batched_gbases_s=round(batched, 4), batched_ms=round(bdt / bsteps * 1e3, 3),
-                              plain_gbases_s=round(plain, 4), plain_ms=round(plain_s * 1e3, 3),
-                              speedup=round(batched / plain, 3), batched_opts=bopts, plain_opts=popts,
-                              batched_stages_ms={k: round(v, 3) for k, v in bper.items()})
-        log(f"[bench] batched {batched:.2f} Gbases/s vs plain per pair {plain:.2f} Gbases/s")
-
+           "roofline": roofline}
     if not args.no_serve:
-        out["serving"] = serving_leg(args)
-
-    if args.human_scale > 0:
-        out["human_scale"] = human_leg(args, opts)
-
+        target = synth.take(g, 0, g.n_a) if args.serve_scale == args.scale else None
+        out["serving"] = serving_leg(args, target)
     if not args.no_cpu:
         t = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(g, args.cpu_bases)
+        out["cpu_baseline"] = cpu_baseline(g, args.cpu_bases, args.cpu_threads)
         log(f"[bench] cpu baseline {out['cpu_baseline']['value']} Gbases/s ({time.perf_counter() - t:.1f}s)")
+    del g
+    if not args.no_plain:
+        out["plain_100mbp"] = plain_leg(args)
+    if not args.no_batch:
+        out["batched"] = batch_leg(args)
 
     print(json.dumps(out), flush=True)
 

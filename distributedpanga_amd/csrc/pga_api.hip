@@ -446,7 +446,10 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
         for (int i = 0; i < ncontig[p]; i++) {
             if (clen[p][i] < 0 || clen[p][i] > INT32_MAX)
                 return fail(c, PGA_EOVERFLOW, "pair %d contig %d: length %lld", p, i, (long long) clen[p][i]);
-            if (boff[p][i] < 0 || boff[p][i] + (clen[p][i] + 3) / 4 > bps_bytes[p])
+            // a negative offset marks a contig without sequence: it keeps its id
+            // and its length counts for the sizing, but it has no records
+            // (src/extract.c:97-98)
+            if (boff[p][i] >= 0 && boff[p][i] + (clen[p][i] + 3) / 4 > bps_bytes[p])
                 return fail(c, PGA_EINVAL, "pair %d contig %d: sequence outside the .bps image", p, i);
             h.seqtot += clen[p][i];
             h.maxctg = std::max<int64_t>(h.maxctg, clen[p][i]);
@@ -471,9 +474,10 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
         int64_t start = pos;
         for (int i = 0; i < h.ncontig; i++) {
             ContigDev &d = tab[h.ctg0 + i];
-            d.bitoff = 8 * (byte0[p] + boff[p][i]);
+            const bool seq = boff[p][i] >= 0;
+            d.bitoff = seq ? 8 * (byte0[p] + boff[p][i]) : 8 * byte0[p];
             d.pos0 = pos;
-            d.ncand = clen[p][i] >= 12 ? clen[p][i] - 11 : 0;
+            d.ncand = seq && clen[p][i] >= 12 ? clen[p][i] - 11 : 0;
             d.clen = (int32_t) clen[p][i];
             d.local = i + (contig_base ? contig_base[p] : 0);
             d.pair = p;
@@ -871,6 +875,20 @@ int pga_index_verify(pga_ctx *c, int pair, int64_t *bad, int64_t *first_bad, int
     return PGA_OK;
 }
 
+int pga_index_poke(pga_ctx *c, int pair, int64_t entry, int byte, int value)
+{
+    if (!c || !c->built || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no index for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    if (entry < 0 || entry >= d.nrec || byte < 0 || byte >= d.e || value < 0 || value > 255)
+        return fail(c, PGA_EINVAL, "entry %lld byte %d outside the index", (long long) entry, byte);
+    HIPCHK(c, hipSetDevice(c->o.device));
+    const uint8_t v = (uint8_t) value;
+    HIPCHK(c, hipMemcpyAsync(P<uint8_t>(c->entries) + d.ent_off + entry * d.e + byte, &v, 1, hipMemcpyHostToDevice,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
 int pga_bucket_counts(pga_ctx *c, int pair, int64_t *counts)
 {
     if (!c || !c->built || pair < 0 || pair >= c->npairs || !counts)
@@ -1083,6 +1101,71 @@ int pga_seeds_export(pga_ctx *c, int pair, uint8_t *nseeds, uint8_t *cseeds, int
     if (cbuck && d.ncontig > 0)
         HIPCHK(c, hipMemcpyAsync(cbuck, P<uint64_t>(c->cbuck) + row + d.buck_off, (size_t) d.ncontig * sizeof(int64_t),
                                  hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+int pga_index_export_range(pga_ctx *c, int pair, int64_t first, int64_t count, uint8_t *entries)
+{
+    if (!c || !c->built || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no index for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    if (first < 0 || count < 0 || first + count > d.nrec || (count > 0 && !entries))
+        return fail(c, PGA_EINVAL, "entries [%lld, +%lld) outside the index", (long long) first, (long long) count);
+    if (count > 0)
+        HIPCHK(c, hipMemcpyAsync(entries, P<uint8_t>(c->entries) + d.ent_off + first * d.e, (size_t) (count * d.e),
+                                 hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PGA_OK;
+}
+
+// The seeds of anchor a start, in each stream, at its merge tile's offset
+// plus the seeds of the tile's anchors before it (their interval records).
+int pga_seeds_locate(pga_ctx *c, int pair, int64_t anchor, int64_t *nfirst, int64_t *cfirst)
+{
+    if (!c || !c->merged || pair < 0 || pair >= c->npairs) return c ? fail(c, PGA_EINVAL, "no seeds for pair %d", pair) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    if (anchor < 0 || anchor > d.nrec || !nfirst || !cfirst)
+        return fail(c, PGA_EINVAL, "anchor %lld outside the index", (long long) anchor);
+    if (anchor == d.nrec) {
+        *nfirst = c->nseeds[pair];
+        *cfirst = c->cseeds[pair];
+        return PGA_OK;
+    }
+    HIPCHK(c, hipSetDevice(c->o.device));
+    const int64_t MT = c->o.merge_tile;
+    const int64_t tile = d.mblk0 + anchor / MT, t0 = anchor / MT * MT;
+    int64_t ofs[3] = {0, 0, 0};
+    std::vector<uint32_t> mi((size_t) (anchor - t0) + 1);
+    HIPCHK(c, hipMemcpyAsync(&ofs[0], P<int64_t>(c->mofsN) + tile, sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&ofs[1], P<int64_t>(c->mofsN) + d.mblk0, sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(&ofs[2], P<int64_t>(c->mofsC) + tile, sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    int64_t cofs0 = 0;
+    HIPCHK(c, hipMemcpyAsync(&cofs0, P<int64_t>(c->mofsC) + d.mblk0, sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    if (anchor > t0)
+        HIPCHK(c, hipMemcpyAsync(mi.data(), c->minfo_p + d.rec0 + t0, (size_t) (anchor - t0) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    int64_t n = ofs[0] - ofs[1], cc = ofs[2] - cofs0;
+    for (int64_t k = 0; k < anchor - t0; k++) {
+        n += (mi[k] >> 18) & 63u;
+        cc += (mi[k] >> 24) & 63u;
+    }
+    *nfirst = n;
+    *cfirst = cc;
+    return PGA_OK;
+}
+
+int pga_seeds_export_range(pga_ctx *c, int pair, int comp, int64_t first, int64_t count, uint8_t *out)
+{
+    if (!c || !c->merged || pair < 0 || pair >= c->npairs || (comp != 0 && comp != 1))
+        return c ? fail(c, PGA_EINVAL, "no seeds for pair %d / stream %d", pair, comp) : PGA_EINVAL;
+    const PairDev &d = c->pairs[pair];
+    const int64_t n = comp ? c->cseeds[pair] : c->nseeds[pair];
+    if (first < 0 || count < 0 || first + count > n || (count > 0 && !out))
+        return fail(c, PGA_EINVAL, "seeds [%lld, +%lld) outside stream %d", (long long) first, (long long) count, comp);
+    const uint8_t *src = comp ? c->seedC_p + d.cseed_off : c->seedN_p + d.nseed_off;
+    if (count > 0)
+        HIPCHK(c, hipMemcpyAsync(out, src + first * d.s, (size_t) (count * d.s), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return PGA_OK;
 }

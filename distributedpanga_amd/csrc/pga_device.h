@@ -6,6 +6,21 @@ namespace pga {
 
 constexpr int MAX_FREQ = 64;     // largest frequency cut the merge window supports
 
+// Digit counters cleared by the extraction emit between rounds: with NT threads,
+// threads 64..NT-1 clear digits tid-64, tid-64+(NT-64), ... below 256 while the
+// first wave scans.  True iff that covers all 256 digits (checked at compile
+// time for every instantiated NT, and on the host for every NT by the tests).
+constexpr bool emit_clear_covers(int nt)
+{
+    if (nt <= 64) return false;
+    bool seen[256] = {};
+    for (int tid = 64; tid < nt; tid++)
+        for (int d = tid - 64; d < 256; d += nt - 64) seen[d] = true;
+    for (int d = 0; d < 256; d++)
+        if (!seen[d]) return false;
+    return true;
+}
+
 struct Rec {                     // 16-byte device record (see pga_kernels.hip)
     uint64_t w0, w1;
 };

@@ -437,6 +437,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     constexpr int NT = 256 * QB;
     constexpr int NS = 2 * XU;                         // record slots per thread
     static_assert(XT == 256 && NT >= 256, "one thread per digit");
+    static_assert(emit_clear_covers(NT), "the counter clearing must reach every digit");
     __shared__ Rec stage[NS * NT];
     __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; its slots this round
     __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
@@ -552,7 +553,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         }
         if (tid == 63) stot = x;
     } else {
-        for (int d = tid - 64; d < 256; d += NT - 64) cnt[par ^ 1][d] = 0;
+        for (int d = tid - 64; d < 256; d += NT - 64) cnt[par ^ 1][d] = 0;   // emit_clear_covers(NT)
     }
     __syncthreads();
 #pragma unroll

@@ -15,7 +15,8 @@ import numpy as np
 from ._lib import hip_lib
 
 NUM_PREFIXES = 1 << 24
-_ERR = {-1: "out of memory", -2: "HIP error", -3: "invalid argument", -4: "field overflow"}
+_ERR = {-1: "out of memory", -2: "HIP error", -3: "invalid argument", -4: "field overflow",
+        -5: "beyond a fixed limit"}
 
 
 class Opts(C.Structure):
@@ -64,6 +65,10 @@ def _setup(L):
     L.pga_index_export.argtypes = [C.c_void_p, C.c_int, _u8p, _i64p]
     L.pga_index_verify.argtypes = [C.c_void_p, C.c_int, _i64p, _i64p, _i64p]
     L.pga_bucket_counts.argtypes = [C.c_void_p, C.c_int, _i64p]
+    L.pga_index_export_range.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int64, _u8p]
+    L.pga_seeds_locate.argtypes = [C.c_void_p, C.c_int, C.c_int64, _i64p, _i64p]
+    L.pga_seeds_export_range.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int64, C.c_int64, _u8p]
+    L.pga_index_poke.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_int]
     L.pga_destination_map.argtypes = [_i64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pga_merge.argtypes = [C.c_void_p]
     L.pga_seeds_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(SeedsMeta)]
@@ -172,11 +177,43 @@ class Context:
                     "pga_index_export")
         return ent[: info["nrec"] * info["e"]].reshape(info["nrec"], info["e"]), cp, info
 
+    def prefix_table(self, pair=0):
+        """The cumulative 2^24 prefix counts alone (no entries copied)."""
+        cp = np.zeros(NUM_PREFIXES, np.int64)
+        self._check(self._L.pga_index_export(self._h, pair, None, cp.ctypes.data_as(_i64p)), "pga_index_export")
+        return cp
+
     def verify(self, pair=0):
         """(bad, first_bad, lcp_bad) of the pair's index, checked on the device."""
         v = [C.c_int64(), C.c_int64(), C.c_int64()]
         self._check(self._L.pga_index_verify(self._h, pair, *[C.byref(x) for x in v]), "pga_index_verify")
         return tuple(x.value for x in v)
+
+    def index_export_range(self, pair, first, count):
+        """Entries [first, first+count) of pair: array [count, e]."""
+        e = self.index_info(pair)["e"]
+        out = np.zeros(max(count * e, 1), np.uint8)
+        self._check(self._L.pga_index_export_range(self._h, pair, int(first), int(count), out.ctypes.data_as(_u8p)),
+                    "pga_index_export_range")
+        return out[: count * e].reshape(count, e)
+
+    def seeds_locate(self, pair, anchor):
+        """(N, C) index of the first seed of anchor entry `anchor`."""
+        n, c = C.c_int64(), C.c_int64()
+        self._check(self._L.pga_seeds_locate(self._h, pair, int(anchor), C.byref(n), C.byref(c)), "pga_seeds_locate")
+        return n.value, c.value
+
+    def seeds_export_range(self, pair, comp, first, count):
+        """Seeds [first, first+count) of stream comp (0: N, 1: C) as bytes."""
+        s = self.seeds_info(pair)["s"]
+        out = np.zeros(max(count * s, 1), np.uint8)
+        self._check(self._L.pga_seeds_export_range(self._h, pair, comp, int(first), int(count),
+                                                   out.ctypes.data_as(_u8p)), "pga_seeds_export_range")
+        return out[: count * s]
+
+    def poke(self, pair, entry, byte, value):
+        """Overwrite one byte of a GIX entry on the device (tests of the check)."""
+        self._check(self._L.pga_index_poke(self._h, pair, int(entry), int(byte), int(value)), "pga_index_poke")
 
     def bucket_counts(self, pair=0):
         out = np.zeros(1024, np.int64)
@@ -203,6 +240,15 @@ class Context:
                     "pga_seeds_export")
         k = info["ncontig"]
         return N[: info["nlen"]], Cs[: info["clen"]], nb[:k], cb[:k]
+
+    def bucks(self, pair=0):
+        """(nbuck, cbuck): seeds per anchor contig, without copying the streams."""
+        k = self.seeds_info(pair)["ncontig"]
+        nb = np.zeros(max(k, 1), np.int64)
+        cb = np.zeros(max(k, 1), np.int64)
+        self._check(self._L.pga_seeds_export(self._h, pair, None, None, nb.ctypes.data_as(_i64p),
+                                             cb.ctypes.data_as(_i64p)), "pga_seeds_export")
+        return nb[:k], cb[:k]
 
     def seed_sort(self, pair=0, comp=0):
         """Diagonal-sorted seed records of one stream (0: N, 1: C): array [n, swide]."""

@@ -130,6 +130,11 @@ int pga_index_export(pga_ctx *ctx, int pair, uint8_t *entries, int64_t *cum);
  * from the LCP recomputed from the k-mers (0 for the first entry). */
 int pga_index_verify(pga_ctx *ctx, int pair, int64_t *bad, int64_t *first_bad, int64_t *lcp_bad);
 
+/* Test support, no counterpart in the reference: overwrite byte `byte` of GIX
+ * entry `entry` of pair on the device (e.g. to show that pga_index_verify
+ * flags a broken order or LCP).  Leaves the prefix table as it was. */
+int pga_index_poke(pga_ctx *ctx, int pair, int64_t entry, int byte, int value);
+
 /* Records of pair per 5-base bucket (the first 10 k-mer bits; record_bucket,
  * src/dsort.h:30-33), counts[1024]: bucket_local_counts, src/dsort.c:108-118,
  * read off the prefix table (needs opts.build_cum). */
@@ -150,6 +155,19 @@ int pga_seeds_info(pga_ctx *ctx, int pair, pga_seeds_meta *info);
  * anchors in index order, partners ascending, 1+2(P+C) bytes per seed. */
 int pga_seeds_export(pga_ctx *ctx, int pair, uint8_t *nseeds, uint8_t *cseeds,
                      int64_t *nbuck, int64_t *cbuck);
+
+/* Pieces of an index or a seed stream, for consumers that read a large index
+ * in parts (the GIX .ktab writer of one part, a check of sampled groups):
+ *   pga_index_export_range   entries [first, first+count) of pair
+ *   pga_seeds_locate         the index, in seeds, of the first N and C seed of
+ *                            anchor entry `anchor` (anchor == nrec: the
+ *                            streams' lengths); an anchor's seeds, and those
+ *                            of a run of anchors, are contiguous in each stream
+ *   pga_seeds_export_range   seeds [first, first+count) of stream comp
+ *                            (0: N, 1: C) of pair */
+int pga_index_export_range(pga_ctx *ctx, int pair, int64_t first, int64_t count, uint8_t *entries);
+int pga_seeds_locate(pga_ctx *ctx, int pair, int64_t anchor, int64_t *nfirst, int64_t *cfirst);
+int pga_seeds_export_range(pga_ctx *ctx, int pair, int comp, int64_t first, int64_t count, uint8_t *out);
 
 /* The seed array pair_sort_search sorts before its alignment search
  * (src/fastga_pipeline.c:1711-1822): every seed of one stream of pair (comp 0:

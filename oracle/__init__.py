@@ -75,6 +75,8 @@ def lib():
                                    C.POINTER(_u8p), _i64p, C.POINTER(_u8p), _i64p, _i64p, _i64p]
         L.oracle_pipeline.argtypes = [_u8p, _i64p, _i64p, C.c_int, C.c_int, C.POINTER(OracleResult)]
         L.oracle_result_free.argtypes = [C.POINTER(OracleResult)]
+        L.pga_cpu_bucket_counts.argtypes = [_u8p, _i64p, _i64p, C.c_int, C.c_int, _i64p]
+        L.pga_cpu_pipeline.argtypes = [_u8p, _i64p, _i64p, C.c_int, C.c_int, C.c_int, C.POINTER(OracleResult)]
         L.oracle_free.argtypes = [C.c_void_p]
         L.oracle_dbyte.argtypes = [C.c_int64]
         L.oracle_seed_sort.argtypes = [_u8p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int,
@@ -191,16 +193,21 @@ def merge(entries, P, Cb, ncontig, freq=10, mode=0):
     return N, Cs, nb[:ncontig], cb[:ncontig]
 
 
-def pipeline(bps, boff, clen, freq=10):
-    """The whole path on the CPU.  Returns a dict of numpy arrays and timings."""
+def pipeline(bps, boff, clen, freq=10, threads=0):
+    """The whole path on the CPU.  Returns a dict of numpy arrays and timings.
+    threads=0: the single-threaded oracle (oracle_pipeline); threads >= 1: the
+    CPU baseline of oracle/pga_cpu.c on that many threads (no sorted records)."""
     bps = np.ascontiguousarray(bps, dtype=np.uint8)
     boff = np.ascontiguousarray(boff, dtype=np.int64)
     clen = np.ascontiguousarray(clen, dtype=np.int64)
     r = OracleResult()
-    rc = lib().oracle_pipeline(_u8(bps), _i64(boff), _i64(clen), len(clen), freq, C.byref(r))
+    if threads:
+        rc = lib().pga_cpu_pipeline(_u8(bps), _i64(boff), _i64(clen), len(clen), freq, int(threads), C.byref(r))
+    else:
+        rc = lib().oracle_pipeline(_u8(bps), _i64(boff), _i64(clen), len(clen), freq, C.byref(r))
     if rc:
         lib().oracle_result_free(C.byref(r))
-        raise MemoryError("oracle_pipeline failed")
+        raise MemoryError("oracle pipeline failed")
     out = dict(P=r.P, C=r.C, rsize=r.rsize, e=r.e, s=r.s, nrec=r.nrec,
                records=_take(r.records, r.nrec * r.rsize).reshape(-1, r.rsize),
                entries=_take(r.entries, r.nrec * r.e).reshape(-1, r.e),
@@ -210,6 +217,18 @@ def pipeline(bps, boff, clen, freq=10):
                stats=dict(sel=r.xs.n_sel),
                t_extract=r.t_extract, t_sort=r.t_sort, t_adapter=r.t_adapter, t_merge=r.t_merge)
     lib().oracle_result_free(C.byref(r))
+    return out
+
+
+def bucket_counts(bps, boff, clen, threads=1):
+    """Records per 5-base bucket of a GDB (record_bucket, src/dsort.h:30-33),
+    counted by the rolling scan without storing the records."""
+    bps = np.ascontiguousarray(bps, dtype=np.uint8)
+    boff = np.ascontiguousarray(boff, dtype=np.int64)
+    clen = np.ascontiguousarray(clen, dtype=np.int64)
+    out = np.zeros(1024, np.int64)
+    if lib().pga_cpu_bucket_counts(_u8(bps), _i64(boff), _i64(clen), len(clen), int(threads), _i64(out)):
+        raise MemoryError("pga_cpu_bucket_counts failed")
     return out
 
 

@@ -378,3 +378,8 @@ def test_oracle_matches_compiled_reference(scale):
     # adapter entries of the canonicalised reference records
     ent, _, _ = oracle.build_adapter(oracle.recalc_lcps(canon), r["P"], r["C"])
     assert np.array_equal(ent, o["entries"])
+    # the oracle's adapter on the reference's own sorted records gives the
+    # reference adapter's entries, prefix table and largest group
+    ent_r, cum_r, maxpre_r = oracle.build_adapter(r["sorted"], r["P"], r["C"])
+    assert np.array_equal(ent_r.reshape(-1), r["entries"].reshape(-1))
+    assert np.array_equal(cum_r, r["cum"]) and maxpre_r == r["maxpre"]

@@ -65,3 +65,59 @@ def test_panel_sort_matches_reference_rmsd_sort(scale, comp):
     theirs = oracle.ref_rmsd_sort(unsorted, buck)
     assert len(mine) == buck.sum() > 100
     assert np.array_equal(mine, theirs)
+
+
+def _le(a):
+    v = np.zeros(a.shape[0], np.int64)
+    for k in range(a.shape[1]):
+        v |= a[:, k].astype(np.int64) << (8 * k)
+    return v
+
+
+@pytest.mark.parametrize("comp", [0, 1])
+def test_reimport_inverts_back_to_the_seeds(comp):
+    """An independent inverse of the reimport (src/fastga_pipeline.c:1436-1446):
+    from each record's (diag & 63, anti, band) recover diag = 64 band + (diag & 63),
+    then the anchor and partner positions -- comp 0: i = (anti + diag - M) / 2,
+    j = (anti - diag + M) / 2; comp 1: i = (3M - diag - anti) / 2,
+    j = (M - diag + anti) / 2 -- and compare them, the LCP and the partner
+    contig with the seeds themselves, panel by panel."""
+    rng = np.random.default_rng(3 + comp)
+    P, Cb, ncontig, maxctg = 3, 1, 5, 70_000            # DBYTE = 3 (2M = 140000)
+    n = 4000
+    plen = rng.integers(12, 41, n)
+    icont = np.sort(rng.integers(0, ncontig, n))        # stream order = anchor-contig order
+    ipos = rng.integers(0, maxctg + 1, n)
+    jpos = rng.integers(0, maxctg + 1, n)
+    jcont = rng.integers(0, ncontig, n)
+    jstr = rng.integers(0, 2, n)
+    raw = np.array([b for k in range(n) for b in seed(int(plen[k]), int(ipos[k]), int(icont[k]), int(jpos[k]),
+                                                      int(jcont[k]), int(jstr[k]), P, Cb)], np.uint8)
+    recs, sw = oracle.seed_sort(raw, P, Cb, ncontig, maxctg, comp, sort=False)
+    D = oracle.dbyte(maxctg)
+    assert D == 3 and sw == 2 * D + Cb + 2 and len(recs) == n
+    M = maxctg
+    lcp = recs[:, 0].astype(np.int64)
+    dlow = recs[:, 1].astype(np.int64)
+    anti = _le(recs[:, 2:2 + D])
+    band = _le(recs[:, 2 + D:2 + 2 * D])
+    jc = recs[:, 2 + 2 * D].astype(np.int64)
+    assert (dlow < 64).all()
+    diag = 64 * band + dlow
+    if comp == 0:
+        i2, j2 = anti + diag - M, anti - diag + M
+    else:
+        i2, j2 = 3 * M - diag - anti, M - diag + anti
+    assert (i2 % 2 == 0).all() and (j2 % 2 == 0).all()
+    got = sorted(zip((i2 // 2).tolist(), (j2 // 2).tolist(), lcp.tolist(), jc.tolist()))
+    # the records come panel by panel in anchor-contig order; the anchor
+    # contig is the panel, so compare within each panel
+    want = sorted(zip(ipos.tolist(), jpos.tolist(), plen.tolist(), jcont.tolist()))
+    assert got == want
+    counts = np.bincount(icont, minlength=ncontig)
+    off = 0
+    for c in range(ncontig):
+        sl = slice(off, off + counts[c])
+        w = sorted(zip(ipos[icont == c].tolist(), jpos[icont == c].tolist()))
+        assert sorted(zip((i2[sl] // 2).tolist(), (j2[sl] // 2).tolist())) == w
+        off += counts[c]
