@@ -43,7 +43,7 @@ def time_steps(ctx, steps, warmup):
                               "scatter_ms", "merge_emit_ms")}
     scat_bytes = 0
     scat_launches = 0
-    tiles = 0
+    last = {}
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.build_index()
@@ -53,11 +53,11 @@ def time_steps(ctx, steps, warmup):
             stage[k] += t[k]
         scat_bytes += t["scatter_bytes"]
         scat_launches += t["scatter_launches"]
-        tiles = t["tiles"]
+        last = t
     ctx.sync()
     dt = time.perf_counter() - t0
     per = {k: v / steps for k, v in stage.items()}
-    return dt, per, scat_bytes / steps, scat_launches / steps, tiles
+    return dt, per, scat_bytes / steps, scat_launches / steps, last
 
 
 def algorithmic_bytes(seqtot, nrec, e, s, nseed):
@@ -187,6 +187,50 @@ def serving_leg(args, target=None):
                 pcie_inclusive=True)
 
 
+def copy_calibration(gib=1, reps=5):
+    """Device-to-device copies of a known size through the HIP runtime (its
+    copy kernel): the FETCH_SIZE / WRITE_SIZE counters of these dispatches
+    against their bytes give the factors that turn the counters into HBM
+    bytes.  Prints one line per copy."""
+    import ctypes as C
+    from distributedpanga_amd._lib import hip_lib
+    hip_lib()                                              # the runtime the library links
+    L = C.CDLL("libamdhip64.so.7")
+    L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    L.hipFree.argtypes = [C.c_void_p]
+    n = gib << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    if L.hipMalloc(C.byref(a), n) or L.hipMalloc(C.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    L.hipMemset(a, 1, n)
+    L.hipDeviceSynchronize()
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        if L.hipMemcpy(b, a, n, 3):                        # device to device
+            raise RuntimeError("hipMemcpy failed")
+        L.hipDeviceSynchronize()
+        dt = time.perf_counter() - t0
+        print(f"copy {n} bytes: {dt * 1e3:.3f} ms host-timed, {2 * n / dt / 1e9:.1f} GB/s read+write", flush=True)
+    L.hipFree(a)
+    L.hipFree(b)
+
+
+def human_pair(scale, cache=None):
+    """The human-scale pair at `scale`; with `cache` (a .npz path) it is read
+    from there when present and written there otherwise, so repeated runs
+    (profiler passes) skip the generation."""
+    if cache and os.path.exists(cache):
+        z = np.load(cache)
+        if float(z["scale"]) == scale:
+            return synth.Gdb(bps=z["bps"], boff=z["boff"], clen=z["clen"], n_a=int(z["n_a"]))
+    g = synth.workload(4, scale)
+    if cache:
+        np.savez(cache, bps=g.bps, boff=g.boff, clen=g.clen, n_a=g.n_a, scale=scale)
+    return g
+
+
 def kernel_lines(info, nseed, per, traffic_profile):
     """The step's two longest launches, each with its algorithmic bytes per
     launch.  Tile sort: every record read once (16 B); its GIX entry (e B),
@@ -241,7 +285,7 @@ def plain_leg(args):
         tuned, _ = autotune.tune(ctx, [g], use_cache=not args.retune, log=None)
         opts.update(tuned)
     steps = 10
-    dt, per, scat_bytes, _, _ = time_steps(ctx, steps, 2)
+    dt, per, scat_bytes, _, last = time_steps(ctx, steps, 2)
     info = ctx.index_info(0)
     sinfo = ctx.seeds_info(0)
     nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
@@ -261,6 +305,8 @@ def plain_leg(args):
     return dict(workload=PLAIN_WORKLOAD, bases=g.seqtot, records=info["nrec"], seeds=int(nseed),
                 ms_per_step=round(ms, 3), gbases_s=round(g.seqtot / (ms * 1e-3) / 1e9, 4), steps=steps,
                 opts=opts, autotuned=tuned is not None, stages_ms={k: round(v, 4) for k, v in per.items()},
+                tiles=last.get("tiles"), crowded_tiles=last.get("crowded_tiles"),
+                fallback_tiles=last.get("fallback_tiles"),
                 roofline={k: roof[k] for k in ("kernel", "achieved", "frac", "radix_scatter", "path")},
                 seed_sort=dict(streams=ss, records=sum(v["records"] for v in ss.values()),
                                ms=round(sum(v["ms"] for v in ss.values()), 3),
@@ -331,6 +377,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-seed-sort", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the human-scale pair only (profiling)")
+    ap.add_argument("--gdb-cache", default=None,
+                    help="a .npz file holding the generated human-scale pair (read if present, else written)")
     ap.add_argument("--cpu-bases", type=int, default=500_000_000)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU baseline (0: OMP_NUM_THREADS, else the host's cores)")
@@ -344,7 +392,12 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-tune", action="store_true", help="side legs use fixed shapes, not the autotuner")
     ap.add_argument("--retune", action="store_true", help="ignore the autotuner cache")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 1 GiB device copies (the counter calibration for the traffic profiles)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -357,14 +410,14 @@ def main():
     opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
                 digit2_bits=args.digit2)
     t = time.perf_counter()
-    g = synth.workload(4, args.scale)
-    log(f"[bench] human-scale pair at scale {args.scale}: {g.ncontig} contigs, {g.seqtot} bases generated "
+    g = human_pair(args.scale, args.gdb_cache)
+    log(f"[bench] human-scale pair at scale {args.scale}: {g.ncontig} contigs, {g.seqtot} bases ready "
         f"({time.perf_counter() - t:.1f}s)")
     ctx = pga.Context(**opts)
     t = time.perf_counter()
     ctx.load_gdb(g)
     h2d_s = time.perf_counter() - t
-    dt, per, scat_bytes, scat_launches, tiles = time_steps(ctx, args.steps, args.warmup)
+    dt, per, scat_bytes, scat_launches, last = time_steps(ctx, args.steps, args.warmup)
     info = ctx.index_info(0)
     sinfo = ctx.seeds_info(0)
     nseed = (sinfo["nlen"] + sinfo["clen"]) // sinfo["s"]
@@ -389,7 +442,9 @@ def main():
                       "chunk_records": opts["chunk_records"], "merge_tile": opts["merge_tile"],
                       "digit2_bits": opts["digit2_bits"], "parallelism": "1 GPU"},
            "stages_ms": {k: round(v, 4) for k, v in per.items()},
-           "tiles": tiles,
+           "tiles": last.get("tiles"), "crowded_tiles": last.get("crowded_tiles"),
+           "fallback_tiles": last.get("fallback_tiles"),
+           "partition_levels": last.get("partition_levels"),
            "device_check": dict(kmer_order_violations=bad, first_bad=first_bad, lcp_mismatches=lcp_bad),
            "h2d_load_s": round(h2d_s, 4),
            "pcie_inclusive_gbases_s": round(g.seqtot / (ms_step * 1e-3 + h2d_s) / 1e9, 4),

@@ -275,7 +275,7 @@ int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *seg
 // bufs: device array {buffer 0, buffer 1} the tiles point into; entries ==
 // nullptr sorts keys only (into skeys).
 int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entries, uint8_t *ls, uint32_t *pre,
-                     Rec *skeys)
+                     Rec *skeys, int *nfall)
 {
     const int T = c->o.tile_records / 8;
     const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
@@ -284,7 +284,7 @@ int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entri
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
                        dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, c->o.digit2_bits, entries, ls,   \
-                       pre, skeys, (int64_t) nlog)
+                       pre, skeys, (int64_t) nlog, nfall)
     switch (T) {
     case 256: PGA_TSORT(256); break;
     case 1024: PGA_TSORT(1024); break;
@@ -726,7 +726,7 @@ int pga_build_index(pga_ctx *c)
     // ---- extraction: level-1 digit totals and bucket offsets per range, then
     // every record straight into its level-1 bucket in record buffer 1
     int *dbad = P<int>(c->counters) + 8;
-    HIPCHK(c, hipMemsetAsync(dbad, 0, sizeof(int), c->st));
+    HIPCHK(c, hipMemsetAsync(dbad, 0, 4 * sizeof(int), c->st));     // [8] digit mismatch, [10..11] tile sort paths
     if (c->nrange > 0) {
         hipLaunchKernelGGL(k_xrange_scan, dim3(npairs, 16), dim3(256), 0, c->st, (const int *) P<int>(c->prange),
                            (const uint32_t *) P<uint32_t>(c->rangehist), P<int64_t>(c->rangeoff),
@@ -771,7 +771,8 @@ int pga_build_index(pga_ctx *c)
     // ---- per-tile LDS sort, LCP, GIX entries
     if (c->ntiles > 0)
         TRY(launch_tile_sort(c, (unsigned) c->ntiles, (Rec *const *) c->bufs.p, P<uint8_t>(c->entries),
-                             P<uint8_t>(c->ls), P<uint32_t>(c->pre), c->keep_keys ? P<Rec>(c->skeys) : nullptr));
+                             P<uint8_t>(c->ls), P<uint32_t>(c->pre), c->keep_keys ? P<Rec>(c->skeys) : nullptr,
+                             P<int>(c->counters) + 10));
     TRY(record(c, 3));
     if (c->ntiles > 0) {
         hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->ntiles + 255) / 256)),
@@ -1240,7 +1241,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         TRY(rc);
         if (sort_tiles > 0)
             TRY(launch_tile_sort(c, (unsigned) sort_tiles, (Rec *const *) c->ssbufs.p, nullptr, nullptr, nullptr,
-                                 P<Rec>(c->ssk[0])));
+                                 P<Rec>(c->ssk[0]), nullptr));
         const int64_t nb = (n + SP_T * SP_PER - 1) / (SP_T * SP_PER);
         hipLaunchKernelGGL(k_seed_pack, dim3(grid_for(c, (const void *) k_seed_pack, SP_T, 0, nb)), dim3(SP_T), 0,
                            c->st, (const Rec *) P<Rec>(c->ssk[0]), n, kf, dbyte, swide, P<uint8_t>(c->ssout));
@@ -1276,6 +1277,11 @@ int pga_timings_get(pga_ctx *c, pga_timings *t)
             double sc = 0;
             for (int i = 0; i < (int) c->tm.scatter_launches; i++) sc += ev_ms(c->sev[2 * i], c->sev[2 * i + 1]);
             c->tm.scatter_ms = sc;
+            int nf[2] = {0, 0};
+            HIPCHK(c, hipMemcpyAsync(nf, P<int>(c->counters) + 10, sizeof(nf), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            c->tm.fallback_tiles = nf[0];
+            c->tm.crowded_tiles = nf[1];
         }
         if (c->tm_merge) {
             c->tm.merge_ms = ev_ms(c->ev[5], c->ev[6]);

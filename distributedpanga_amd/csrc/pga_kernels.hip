@@ -980,6 +980,8 @@ __device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, 
 }
 
 constexpr int FB_BIN = 128;          // largest bin the binned tile sort ranks directly
+constexpr int MAXC = 32;             // crowded ranges a tile sorts by sub-bins (more: merge sort)
+constexpr int CROWD_ROUNDS = 6;      // rounds of sub-binning before the merge sort takes over
 
 #define PGA_CE(i, j)                                                             \
     do {                                                                          \
@@ -999,7 +1001,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                                                  const PairDev *__restrict__ pairs, int jbits, int w2,
                                                  uint8_t *__restrict__ entries,
                                                  uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
-                                                 Rec *__restrict__ skeys, int64_t nlog)
+                                                 Rec *__restrict__ skeys, int64_t nlog, int *__restrict__ nfall)
 {
     constexpr int E = 8;
     constexpr int CAP = T * E;
@@ -1030,6 +1032,11 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     __shared__ uint32_t dmn[T / 64], dmx[T / 64];
     __shared__ uint32_t scan_sh[T / 64 + 1];
     __shared__ int fallback;
+    // crowded ranges (bins, then sub-bins, of more than FB_BIN records): start
+    // in the tile and size, two lists (this round's and the next's)
+    __shared__ int crst[2][MAXC], crm[2][MAXC], ncrow[2];
+    __shared__ int mo[MAXC + 1], cq[MAXC], csb[MAXC + 1], cp[MAXC];
+    __shared__ unsigned long long cx[MAXC][2];
 
     // smallest and largest digit at the tile's level (its records share the
     // digits above it)
@@ -1051,7 +1058,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
     if ((tid & 63) == 0) { dmn[tid >> 6] = dlo; dmx[tid >> 6] = dhi; }
     for (int c = tid; c < NB / 2; c += T) bh[c] = 0;
-    if (tid == 0) fallback = 0;
+    if (tid == 0) { fallback = 0; ncrow[0] = 0; }
     __syncthreads();
     uint32_t dmin = dmn[0], dmax = dmx[0];
     for (int w = 1; w < T / 64; w++) {
@@ -1082,24 +1089,34 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             rk[k] = (atomicAdd(&bh[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
         }
     __syncthreads();
+    // Bin counts -> bin starts (16-bit, two to a word).  A bin of more than
+    // FB_BIN records (repeated k-mers) is listed as a crowded range through
+    // `crowded(start, count)` instead of being ranked record by record.
     constexpr int WPT = NB / 2 / T;                 // counter words per thread
-    uint32_t cw[WPT], sum = 0, mx = 0;
+    auto scan_bins = [&](auto crowded) {
+        uint32_t cw[WPT], sum = 0;
 #pragma unroll
-    for (int q = 0; q < WPT; q++) {
-        cw[q] = bh[tid * WPT + q];
-        const uint32_t lo = cw[q] & 0xFFFFu, hi = cw[q] >> 16;
-        sum += lo + hi;
-        mx = max(mx, max(lo, hi));
-    }
-    if (mx > FB_BIN) fallback = 1;                  // a crowded bin (repeats): merge sort instead
-    uint32_t tot;
-    uint32_t run = block_excl_scan<T, uint32_t>(sum, scan_sh, &tot);
+        for (int q = 0; q < WPT; q++) {
+            cw[q] = bh[tid * WPT + q];
+            sum += (cw[q] & 0xFFFFu) + (cw[q] >> 16);
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan<T, uint32_t>(sum, scan_sh, &tot);
 #pragma unroll
-    for (int q = 0; q < WPT; q++) {
-        const uint32_t lo = cw[q] & 0xFFFFu;
-        bh[tid * WPT + q] = run | ((run + lo) << 16);
-        run += lo + (cw[q] >> 16);
-    }
+        for (int q = 0; q < WPT; q++) {
+            const uint32_t lo = cw[q] & 0xFFFFu, hi = cw[q] >> 16;
+            if (lo > FB_BIN) crowded(run, lo);
+            if (hi > FB_BIN) crowded(run + lo, hi);
+            bh[tid * WPT + q] = run | ((run + lo) << 16);
+            run += lo + hi;
+        }
+    };
+    auto add_crowd = [&](int list, int st, int m) {
+        const int x = atomicAdd(&ncrow[list], 1);
+        if (x < MAXC) { crst[list][x] = st; crm[list][x] = m; }
+        else fallback = 1;                          // too many: merge sort the tile
+    };
+    scan_bins([&](uint32_t st, uint32_t m) { add_crowd(0, (int) st, (int) m); });
     __syncthreads();
     const bool binned = !fallback;
     if (binned) {
@@ -1120,6 +1137,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             const uint32_t b = bin_of(r[k].w0, r[k].w1);
             const uint32_t st = bin_start(b);
             const uint32_t en = b + 1 < (uint32_t) NB ? bin_start(b + 1) : (uint32_t) n;
+            if (en - st > (uint32_t) FB_BIN) { rk[k] = me; continue; }   // crowded: sorted below
             uint32_t rank = 0;
             for (uint32_t i = st; i < en; i++) {
                 const Rec y = sk[i];
@@ -1132,6 +1150,124 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
 #pragma unroll
         for (int k = 0; k < E; k++)
             if (k * T + tid < n) sk[rk[k]] = r[k];
+        __syncthreads();
+        // Crowded ranges, sorted in place round by round: each range's records
+        // share the key bits above the first bit where any two of them differ
+        // (the OR of their XORs with the range's first record), so their
+        // sub-bin is the next q = floor(log2 m) bits from there (<= m sub-bins
+        // a range, <= NB in all).  Sub-bins of at most FB_BIN records are
+        // ranked as above; larger ones are the next round's ranges.  Ranges of
+        // identical keys (the seed sort's duplicates) are sorted already.
+        int cur = 0;
+        for (int round = 0; round < CROWD_ROUNDS; round++) {
+            const int nc = min(ncrow[cur], MAXC);
+            if (nc == 0 || fallback) break;
+            if (tid == 0) {
+                int acc = 0;
+                for (int j = 0; j < nc; j++) { mo[j] = acc; acc += crm[cur][j]; }
+                mo[nc] = acc;
+                ncrow[cur ^ 1] = 0;
+            }
+            if (tid < nc) { cx[tid][0] = 0; cx[tid][1] = 0; }
+            if (tid == 0 && round == 0 && nfall) atomicAdd(nfall + 1, 1);
+            __syncthreads();
+            const int M = mo[nc];
+            auto range_of = [&](int i) {                 // the range holding item i of the concatenation
+                int lo = 0, hi = nc;
+                while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (mo[mid] <= i) lo = mid; else hi = mid; }
+                return lo;
+            };
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                const int i = k * T + tid;
+                if (i >= M) continue;
+                const int j = range_of(i);
+                r[k] = sk[crst[cur][j] + i - mo[j]];
+                const Rec ref = sk[crst[cur][j]];
+                const unsigned long long d0 = r[k].w0 ^ ref.w0, d1 = r[k].w1 ^ ref.w1;
+                if (d0) atomicOr(&cx[j][0], d0);
+                if (d1) atomicOr(&cx[j][1], d1);
+            }
+            for (int c = tid; c < NB / 2; c += T) bh[c] = 0;
+            __syncthreads();
+            if (tid == 0) {
+                int acc = 0;
+                for (int j = 0; j < nc; j++) {
+                    const unsigned long long x0 = cx[j][0], x1 = cx[j][1];
+                    const int p = x0 ? __builtin_clzll(x0) : x1 ? 64 + __builtin_clzll(x1) : 128;
+                    const int lg = 31 - __builtin_clz((uint32_t) crm[cur][j]);
+                    const int q = p >= 128 ? 0 : min(min(lg, 12), 128 - p);
+                    cp[j] = p;
+                    cq[j] = q;
+                    csb[j] = acc;
+                    acc += 1 << q;
+                }
+                csb[nc] = acc;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                const int i = k * T + tid;
+                if (i >= M) continue;
+                const int j = range_of(i);
+                const int q = cq[j];
+                const uint32_t sb = (uint32_t) csb[j] +
+                                    (q ? (uint32_t) (key_bits_from(r[k].w0, r[k].w1, cp[j]) >> (64 - q)) : 0u);
+                const uint32_t sh = (sb & 1u) << 4;
+                rk[k] = (atomicAdd(&bh[sb >> 1], 1u << sh) >> sh) & 0xFFFFu;
+            }
+            __syncthreads();
+            // sub-bin starts in the concatenation; crowded sub-bins of ranges
+            // that can still split become the next round's ranges (in tile
+            // positions)
+            scan_bins([&](uint32_t st, uint32_t m) {
+                const int j = range_of((int) st);
+                if (cq[j] > 0) add_crowd(cur ^ 1, crst[cur][j] + (int) st - mo[j], (int) m);
+            });
+            __syncthreads();
+            if (fallback) break;
+            // lay each range out by sub-bin
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                const int i = k * T + tid;
+                if (i >= M) continue;
+                const int j = range_of(i);
+                if (!cq[j]) continue;
+                const uint32_t sb = (uint32_t) csb[j] + (uint32_t) (key_bits_from(r[k].w0, r[k].w1, cp[j]) >> (64 - cq[j]));
+                rk[k] += bin_start(sb) - (uint32_t) mo[j] + (uint32_t) crst[cur][j];   // tile position
+                sk[rk[k]] = r[k];
+            }
+            __syncthreads();
+            // rank inside sub-bins of at most FB_BIN records
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                const int i = k * T + tid;
+                if (i >= M) continue;
+                const int j = range_of(i);
+                if (!cq[j]) { rk[k] = ~0u; continue; }
+                const uint32_t sb = (uint32_t) csb[j] + (uint32_t) (key_bits_from(r[k].w0, r[k].w1, cp[j]) >> (64 - cq[j]));
+                const uint32_t base = (uint32_t) crst[cur][j] - (uint32_t) mo[j];
+                const uint32_t st = bin_start(sb) + base;
+                const uint32_t en = (sb + 1 < (uint32_t) csb[nc] ? bin_start(sb + 1) : (uint32_t) M) + base;
+                if (en - st > (uint32_t) FB_BIN) { rk[k] = ~0u; continue; }   // the next round's
+                const uint32_t me = rk[k];
+                uint32_t rank = 0;
+                for (uint32_t x = st; x < en; x++) {
+                    const Rec y = sk[x];
+                    rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && (y.w1 < r[k].w1 || (y.w1 == r[k].w1 && x < me))))
+                            ? 1u : 0u;
+                }
+                rk[k] = st + rank;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < E; k++)
+                if (k * T + tid < M && rk[k] != ~0u) sk[rk[k]] = r[k];
+            __syncthreads();
+            cur ^= 1;
+        }
+        if (tid == 0 && ncrow[cur] > 0) fallback = 1;   // clusters left after the last round
+        __syncthreads();
     } else {
 #pragma unroll
         for (int k = 0; k < E; k++) {
@@ -1141,7 +1277,14 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         }
     }
     __syncthreads();
-    if (!binned) {
+    const bool merge = fallback != 0;
+    if (merge && binned) {
+        // every record is in sk[0, n) already: pad the rest for the merge sort
+        for (int i = n + tid; i < CAP; i += T) { Rec x; x.w0 = ~0ull; x.w1 = ~0ull; sk[i] = x; }
+        __syncthreads();
+    }
+    if (merge && tid == 0 && nfall) atomicAdd(nfall, 1);
+    if (merge) {
     uint64_t a0[E], a1[E];
 #pragma unroll
     for (int k = 0; k < E; k++) { Rec x = sk[tid * E + k]; a0[k] = x.w0; a1[k] = x.w1; }
@@ -1247,9 +1390,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *);
 
 // Pair of a global record index.
 // LCP of each tile's first record with its predecessor (0 at a pair's start),

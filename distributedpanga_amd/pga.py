@@ -39,7 +39,8 @@ class Timings(C.Structure):
                 ("table_ms", C.c_double), ("merge_ms", C.c_double), ("total_ms", C.c_double),
                 ("partition_levels", C.c_int), ("partition_launches", C.c_int), ("tiles", C.c_int64),
                 ("scatter_ms", C.c_double), ("scatter_bytes", C.c_int64), ("scatter_launches", C.c_int),
-                ("merge_emit_ms", C.c_double)]
+                ("merge_emit_ms", C.c_double), ("fallback_tiles", C.c_int64),
+                ("crowded_tiles", C.c_int64)]
 
 
 _u8p = C.POINTER(C.c_uint8)

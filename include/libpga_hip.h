@@ -93,6 +93,11 @@ typedef struct {
     int    scatter_launches;
     /* the seed emission kernel of the merge alone (its own event pair) */
     double merge_emit_ms;
+    /* tiles with bins of repeated k-mers too crowded to rank record by
+       record: sorted by sub-binning those bins (crowded) or, when that does
+       not resolve them, by the merge-sort path (fallback) */
+    int64_t fallback_tiles;
+    int64_t crowded_tiles;
 } pga_timings;
 
 void        pga_default_opts(pga_opts *o);

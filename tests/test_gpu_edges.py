@@ -133,3 +133,41 @@ def test_load_failures_forced_from_the_host():
         ctx.load_gdb(g)
         ctx.build_index()
         assert ctx.verify(0) == (0, -1, 0)
+
+
+def _repeat_genome(copies, unit_len, div, seed, sat=0):
+    """Copies of one element at `div` per-base divergence between random
+    spacers (plus, optionally, a satellite of a 171 bp unit): repeated k-mers
+    that crowd the tile sort's bins."""
+    rng = np.random.default_rng(seed)
+    unit = rng.integers(0, 4, unit_len).astype(np.uint8)
+    parts = []
+    for _ in range(copies):
+        c = unit.copy()
+        m = rng.random(unit_len) < div
+        c[m] = (c[m] + rng.integers(1, 4, int(m.sum()))) % 4
+        parts += [rng.integers(0, 4, int(rng.integers(20, 400))).astype(np.uint8), c]
+    if sat:
+        s171 = rng.integers(0, 4, 171).astype(np.uint8)
+        parts.append(np.tile(s171, sat))
+    a = np.concatenate(parts)
+    b = a.copy()
+    m = rng.random(len(b)) < 0.02
+    b[m] = (b[m] + rng.integers(1, 4, int(m.sum()))) % 4
+    return synth.from_codes([a[: len(a) // 2], a[len(a) // 2:], b], n_a=2)
+
+
+@pytest.mark.parametrize("tile", [2048, 4096, 8192])
+def test_crowded_bins_of_repeats(tile):
+    """Hundreds of near-identical element copies and a satellite: bins of more
+    than 128 records, sorted by sub-binning (or the merge sort)."""
+    g = _repeat_genome(600, 500, 0.01, 21, sat=300)
+    got, _ = _compare(g, tile_records=tile, digit2_bits=8)
+    t = got["timings"]
+    assert t["crowded_tiles"] + t["fallback_tiles"] > 0
+
+
+@pytest.mark.parametrize("freq", [10, 64])
+def test_crowded_bins_divergent_copies(freq):
+    g = _repeat_genome(2000, 300, 0.08, 22)
+    _compare(g, freq=freq)
