@@ -327,8 +327,18 @@ def batch_leg(args):
         bopts.update(btuned)
     bsteps = 5
     bdt, bper, _, _, _ = time_steps(bctx, bsteps, 1)
+    # the same batch as one device-driven pipeline replayed from a HIP graph
+    bctx.run_graph()                               # capture
+    bctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(bsteps):
+        bctx.run_graph()
+    bctx.sync()
+    gdt = time.perf_counter() - t0
+    gfall = bctx.timings()["graph_fallbacks"]
     bctx.close()
     batched = tot * bsteps / bdt / 1e9
+    graphed = tot * bsteps / gdt / 1e9
     # plain form: the same pairs one at a time (input resident, one build + merge
     # per pair), tuned on the median pair
     popts = dict(base)
@@ -351,12 +361,14 @@ def batch_leg(args):
         plain_s += time.perf_counter() - t0
     pctx.close()
     plain = tot / plain_s / 1e9
-    log(f"[bench] batched {batched:.2f} Gbases/s vs plain per pair {plain:.2f} Gbases/s")
+    log(f"[bench] batched {batched:.2f} Gbases/s (graph {graphed:.2f}) vs plain per pair {plain:.2f} Gbases/s")
     return dict(workload=f"{len(gdbs)} independent pairs, A lognormal 1-12 Mbp, 1-50 contigs, "
                          "GC 0.30-0.65, B at 1-15% SNP and 0.1-1% indel",
                 bases=tot, batched_gbases_s=round(batched, 4), batched_ms=round(bdt / bsteps * 1e3, 3),
                 plain_gbases_s=round(plain, 4), plain_ms=round(plain_s * 1e3, 3),
                 speedup=round(batched / plain, 3), batched_opts=bopts, plain_opts=popts,
+                graph_gbases_s=round(graphed, 4), graph_ms=round(gdt / bsteps * 1e3, 3),
+                graph_speedup=round(graphed / plain, 3), graph_fallbacks=gfall,
                 batched_stages_ms={k: round(v, 3) for k, v in bper.items()})
 
 

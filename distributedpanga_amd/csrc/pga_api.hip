@@ -127,7 +127,21 @@ struct pga_ctx {
     // event times not read yet: the build and the merge return without
     // waiting for the device; the times are read when asked for
     bool tm_build = false, tm_merge = false;
+
+    // device-driven build + merge captured in a HIP graph (pga_run_graph):
+    // valid for the loaded batch while no pool buffer is reallocated
+    uint64_t pool_gen = 0, graph_gen = ~0ull;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    DBuf gstat, plist_all;
+    uint8_t *g_minfo = nullptr, *g_seedN = nullptr, *g_seedC = nullptr;
+    int64_t g_nmb_ub = 0, g_seg_cap = 0, g_tile_cap = 0;
+    int64_t graph_fallbacks = 0;
 };
+
+namespace {
+void graph_drop(pga_ctx *c);
+}
 
 namespace {
 
@@ -166,6 +180,7 @@ int ensure(pga_ctx *c, DBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return PGA_OK;
+    c->pool_gen++;                                  // a captured graph's pointers are stale now
     if (b.p) {
         hipError_t e = hipFree(b.p);
         b.p = nullptr;
@@ -239,11 +254,11 @@ int chunk_per(const pga_ctx *c) { return c->o.chunk_records / 256; }
 constexpr int64_t ANY_CHUNKS = (int64_t) 1 << 40;
 
 int launch_seg_hist(pga_ctx *c, int per, const Rec *buf0, const Rec *buf1, const Seg *segs, int nseg,
-                    const int64_t *chk0, int level, unsigned long long *hist)
+                    const int64_t *chk0, int level, unsigned long long *hist, const int *dn = nullptr)
 {
 #define PGA_HIST(PER)                                                                                  \
     hipLaunchKernelGGL(k_seg_hist<PER>, dim3(grid_for(c, (const void *) k_seg_hist<PER>, 256, 0, ANY_CHUNKS)), \
-                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, hist)
+                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, hist, dn)
     switch (per) {
     case 8:  PGA_HIST(8); break;
     case 32: PGA_HIST(32); break;
@@ -255,13 +270,13 @@ int launch_seg_hist(pga_ctx *c, int per, const Rec *buf0, const Rec *buf1, const
 }
 
 int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *segs,
-                       int nseg, const int64_t *chk0, int level, unsigned long long *cursor)
+                       int nseg, const int64_t *chk0, int level, unsigned long long *cursor, const int *dn = nullptr)
 {
     // chunks of 2048 / 4096 records: 256 threads; 8192: 512 threads of 16 records
 #define PGA_SCAT(TT, PER)                                                                              \
     hipLaunchKernelGGL((k_seg_scatter<TT, PER>),                                                        \
                        dim3(grid_for(c, (const void *) k_seg_scatter<TT, PER>, TT, 0, ANY_CHUNKS)),      \
-                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, cursor)
+                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, cursor, dn)
     switch (per) {
     case 8:  PGA_SCAT(256, 8); break;
     case 32: PGA_SCAT(512, 16); break;
@@ -275,7 +290,7 @@ int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *seg
 // bufs: device array {buffer 0, buffer 1} the tiles point into; entries ==
 // nullptr sorts keys only (into skeys).
 int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entries, uint8_t *ls, uint32_t *pre,
-                     Rec *skeys, int *nfall)
+                     Rec *skeys, int *nfall, const int *dn = nullptr)
 {
     const int T = c->o.tile_records / 8;
     const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
@@ -284,7 +299,7 @@ int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entri
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
                        dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, c->o.digit2_bits, entries, ls,   \
-                       pre, skeys, (int64_t) nlog, nfall)
+                       pre, skeys, (int64_t) nlog, nfall, dn)
     switch (T) {
     case 256: PGA_TSORT(256); break;
     case 1024: PGA_TSORT(1024); break;
@@ -357,6 +372,7 @@ int pga_set_opts(pga_ctx *c, const pga_opts *o)
     if (!opts_valid(*o) || o->device != c->o.device) return fail(c, PGA_EINVAL, "invalid options");
     c->o = *o;
     c->built = c->merged = false;
+    c->graph_gen = ~0ull;                           // the captured launch shapes changed
     return PGA_OK;
 }
 
@@ -404,6 +420,9 @@ void pga_destroy(pga_ctx *c)
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->skeys,
                    &c->xrange, &c->prange, &c->rangehist, &c->rangeoff, &c->bstart};
     for (DBuf *b : all)
+        if (b->p) (void) hipFree(b->p);
+    graph_drop(c);
+    for (DBuf *b : {&c->gstat, &c->plist_all})
         if (b->p) (void) hipFree(b->p);
     for (auto &e : c->ev) if (e) (void) hipEventDestroy(e);
     for (auto &e : c->sev) (void) hipEventDestroy(e);
@@ -537,6 +556,7 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
     TRY(ensure(c, c->bstart, (size_t) npairs * 256 * sizeof(int64_t)));
     HIPCHK(c, hipStreamSynchronize(c->st));     // host arrays above are stack/vector owned
     c->loaded = true;
+    c->graph_gen = ~0ull;                           // a new batch needs a new graph
     return PGA_OK;
 }
 
@@ -571,7 +591,7 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
     while (nseg > 0) {
         if (level > MAX_LEVELS) return fail(c, PGA_EINVAL, "radix partition did not converge");
         hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
-                           nseg, CH, P<int64_t>(c->chk0));
+                           nseg, CH, P<int64_t>(c->chk0), (const int *) nullptr);
         LAUNCHCHK(c);
         const bool counted = level == 1 && hist1;
         const bool in_place = level == 1 && level1_in_place;
@@ -590,7 +610,8 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
                            counted ? 1 : 0, level, c->o.digit2_bits, in_place ? 1 : 0, CAP,
                            P<unsigned long long>(c->cursor),
                            P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
-                           (int) tile_cap, reinterpret_cast<unsigned long long *>(dcount + 2));
+                           (int) tile_cap, reinterpret_cast<unsigned long long *>(dcount + 2), nseg,
+                           (const int *) nullptr);
         LAUNCHCHK(c);
         if (!in_place) {
             hipEvent_t se[2];
@@ -778,7 +799,8 @@ int pga_build_index(pga_ctx *c)
         hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->ntiles + 255) / 256)),
                            dim3(256), 0, c->st,
                            (const Tile *) P<Tile>(c->tiles), c->ntiles, (const PairDev *) P<PairDev>(c->dpairs),
-                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), (const uint32_t *) P<uint32_t>(c->pre));
+                           P<uint8_t>(c->entries), P<uint8_t>(c->ls), (const uint32_t *) P<uint32_t>(c->pre),
+                           (const int *) nullptr);
         LAUNCHCHK(c);
     }
 
@@ -990,7 +1012,8 @@ int pga_merge(pga_ctx *c)
     hipLaunchKernelGGL(k_merge_count<A_>, dim3(grid_for(c, (const void *) k_merge_count<A_>, MTH, 0, nmb)), \
                        dim3(MTH), 0, c->st,                                                             \
                        (const PairDev *) P<PairDev>(c->dpairs), npairs, (const uint8_t *) P<uint8_t>(c->ls), \
-                       c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC), minfo, nmb)
+                       c->o.freq, P<uint32_t>(c->mcntN), P<uint32_t>(c->mcntC), minfo, nmb,            \
+                       (const int64_t *) nullptr)
         switch (MT) {
         case 512: PGA_COUNT(2); break;
         case 1024: PGA_COUNT(4); break;
@@ -1058,7 +1081,8 @@ int pga_merge(pga_ctx *c)
                        (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
                        (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
                        c->seedN_p, c->seedC_p, P<unsigned long long>(c->nbuck),                         \
-                       P<unsigned long long>(c->cbuck), (const uint32_t *) minfo, nmb, eper)
+                       P<unsigned long long>(c->cbuck), (const uint32_t *) minfo, nmb, eper,            \
+                       (const int64_t *) nullptr, (const int *) nullptr)
         switch (MT) {
         case 512: PGA_EMIT(2); break;
         case 1024: PGA_EMIT(4); break;
@@ -1289,11 +1313,370 @@ int pga_timings_get(pga_ctx *c, pga_timings *t)
         }
         c->tm_build = c->tm_merge = false;
     }
+    c->tm.graph_fallbacks = c->graph_fallbacks;
     *t = c->tm;
     return PGA_OK;
 }
 
 } // extern "C"
+
+// ======================================================================== device-driven pipeline
+//
+// pga_run_graph: the build and the merge of the loaded batch with no host
+// round trip.  Every count a kernel needs -- records, segments of each
+// partition level, sort tiles, merge tiles, seed totals -- stays in device
+// memory (GStat, the per-pair layout) and is read there; buffers are sized
+// from bounds known at load time (at most two records a candidate position).
+// The launch sequence is captured once per loaded batch into a HIP graph and
+// replayed; the host reads one status block afterwards.  When the status
+// says the partition needed more than GRAPH_LEVELS levels, a list outgrew its
+// bound, or the seeds outgrew their buffers, the eager path redoes the work.
+
+namespace {
+
+constexpr int PLAN_GRID = 2048;      // persistent blocks of the plan kernel
+
+int graph_sizes(pga_ctx *c, int64_t &R_ub, int64_t &ent_ub, int64_t &nmb_ub)
+{
+    const int MT = c->o.merge_tile;
+    R_ub = 0;
+    ent_ub = 0;
+    nmb_ub = 0;
+    for (int p = 0; p < c->npairs; p++) {
+        const HostPair &h = c->hp[p];
+        const int64_t r = 2 * h.nblk * (int64_t) XBLOCK;        // two records a candidate position at most
+        const int e = 9 + bytes_for(h.maxctg) + bytes_for(2 * (int64_t) h.ncontig - 1);
+        R_ub += r;
+        ent_ub += r * e;
+        nmb_ub += (r + MT - 1) / MT;
+    }
+    return PGA_OK;
+}
+
+// Every buffer of the device-driven pipeline, at its bound; the load-time
+// fields of the pair layout uploaded.  No allocation happens after this.
+int graph_prepare(pga_ctx *c)
+{
+    const int npairs = c->npairs;
+    const int CAP = cap_of(c);
+    int64_t R_ub, ent_ub, nmb_ub;
+    TRY(graph_sizes(c, R_ub, ent_ub, nmb_ub));
+    const size_t r1 = (size_t) std::max<int64_t>(R_ub, 1);
+    const int64_t nblk = c->nblk;
+    TRY(ensure(c, c->blkcnt, (size_t) std::max<int64_t>(nblk, 1) * sizeof(uint32_t)));
+    TRY(ensure(c, c->blkoff, (size_t) (nblk + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->sel, (size_t) std::max<int64_t>(nblk, 1) * 256));
+    TRY(ensure(c, c->rec[0], r1 * sizeof(Rec)));
+    TRY(ensure(c, c->rec[1], r1 * sizeof(Rec)));
+    TRY(ensure(c, c->bufs, 2 * sizeof(Rec *)));
+    TRY(ensure(c, c->entries, (size_t) ent_ub + 32));
+    TRY(ensure(c, c->ls, r1 + 64));
+    TRY(ensure(c, c->pre, (r1 + 1) * sizeof(uint32_t)));
+    TRY(ensure(c, c->hist1, (size_t) npairs * 256 * sizeof(uint64_t)));
+    TRY(ensure(c, c->dpairs, (size_t) npairs * sizeof(PairDev)));
+    TRY(ensure(c, c->counters, 16 * sizeof(int)));
+    const int64_t seg_cap = R_ub / CAP + npairs + 16;
+    const int64_t tile_cap = 34 * (R_ub / CAP) + npairs + 1024;
+    const int64_t rows = std::max<int64_t>(std::min<int64_t>((int64_t) 256 * npairs, seg_cap) << c->o.digit2_bits,
+                                           seg_cap * 256);
+    TRY(ensure(c, c->segs[0], (size_t) seg_cap * sizeof(Seg)));
+    TRY(ensure(c, c->segs[1], (size_t) seg_cap * sizeof(Seg)));
+    TRY(ensure(c, c->seghist, (size_t) rows * sizeof(uint64_t)));
+    TRY(ensure(c, c->cursor, (size_t) rows * sizeof(uint64_t)));
+    TRY(ensure(c, c->chk0, (size_t) (seg_cap + 2) * sizeof(int64_t)));
+    TRY(ensure(c, c->tiles, (size_t) tile_cap * sizeof(Tile)));
+    TRY(ensure(c, c->maxpre, (size_t) npairs * sizeof(uint64_t)));
+    TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
+    TRY(ensure(c, c->plist_all, (size_t) npairs * sizeof(int)));
+    TRY(ensure(c, c->gmax, (size_t) npairs * (NPRE / CR_SPAN) * sizeof(uint64_t)));
+    TRY(ensure(c, c->mblk, (size_t) (npairs + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->mcntN, (size_t) (nmb_ub + 2) * sizeof(uint32_t)));
+    TRY(ensure(c, c->mcntC, (size_t) (nmb_ub + 2) * sizeof(uint32_t)));
+    TRY(ensure(c, c->mofsN, (size_t) (nmb_ub + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->mofsC, (size_t) (nmb_ub + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->pofs, (size_t) 2 * (npairs + 1) * sizeof(int64_t)));
+    TRY(ensure(c, c->nbuck, (size_t) std::max(c->nctg, 1) * sizeof(uint64_t)));
+    TRY(ensure(c, c->cbuck, (size_t) std::max(c->nctg, 1) * sizeof(uint64_t)));
+    TRY(ensure(c, c->gstat, sizeof(GStat)));
+    const int64_t nscan = std::max<int64_t>(std::max<int64_t>(nblk, nmb_ub), 1);
+    TRY(ensure(c, c->bsum, (size_t) ((nscan + 4095) / 4096 + 2) * sizeof(int64_t)));
+    // load-time fields of the pair layout (the rest is written on the device)
+    c->pairs.assign(npairs, PairDev());
+    for (int p = 0; p < npairs; p++) {
+        PairDev &d = c->pairs[p];
+        const HostPair &h = c->hp[p];
+        d.P = bytes_for(h.maxctg);
+        d.C = bytes_for(2 * (int64_t) h.ncontig - 1);
+        d.e = 9 + d.P + d.C;
+        d.s = 1 + 2 * (d.P + d.C);
+        d.ncontig = h.ncontig;
+        d.cum_off = (int64_t) p * NPRE;
+        d.buck_off = h.ctg0;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dpairs.p, c->pairs.data(), (size_t) npairs * sizeof(PairDev), hipMemcpyHostToDevice,
+                             c->st));
+    std::vector<int> all(npairs);
+    for (int p = 0; p < npairs; p++) all[p] = p;
+    HIPCHK(c, hipMemcpyAsync(c->plist_all.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+    Rec *bp[2] = {P<Rec>(c->rec[0]), P<Rec>(c->rec[1])};
+    HIPCHK(c, hipMemcpyAsync(c->bufs.p, bp, sizeof(bp), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));          // host arrays above are locals
+    // merge outputs inside the record buffers (dead after the tile sort):
+    // interval records then the N stream in buffer 0, the C stream in buffer 1
+    c->g_minfo = P<uint8_t>(c->rec[0]);
+    c->g_seedN = P<uint8_t>(c->rec[0]) + ((r1 * sizeof(uint32_t) + 255) & ~(size_t) 255);
+    c->g_seedC = P<uint8_t>(c->rec[1]);
+    c->g_nmb_ub = nmb_ub;
+    c->g_seg_cap = seg_cap;
+    c->g_tile_cap = tile_cap;
+    return PGA_OK;
+}
+
+// The whole launch sequence on the context's stream: no host read, no
+// allocation, no memset (buffers are cleared by kernels).
+int graph_enqueue(pga_ctx *c)
+{
+    const int npairs = c->npairs;
+    const int CAP = cap_of(c);
+    const int MT = c->o.merge_tile;
+    const int per = chunk_per(c);
+    const int CH = per * 256;
+    GStat *gs = P<GStat>(c->gstat);
+    const int64_t nblk = c->nblk;
+    // ---- extraction
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
+                           dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
+                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
+                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist));
+        LAUNCHCHK(c);
+        TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
+    } else {
+        TRY(fill_u64(c, c->blkoff.p, 1, 0));
+    }
+    hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
+                       (const int64_t *) P<int64_t>(c->blkoff), (const int64_t *) P<int64_t>(c->pblk), npairs,
+                       P<int64_t>(c->pr));
+    hipLaunchKernelGGL(k_graph_layout, dim3(1), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr), npairs, MT,
+                       CAP, P<PairDev>(c->dpairs), P<int64_t>(c->mblk), P<Seg>(c->segs[0]), P<Tile>(c->tiles), gs);
+    LAUNCHCHK(c);
+    TRY(fill_u64(c, P<int>(c->counters) + 10, 1, 0));                       // tile sort path counters
+    if (c->nrange > 0) {
+        hipLaunchKernelGGL(k_xrange_scan, dim3(npairs, 16), dim3(256), 0, c->st, (const int *) P<int>(c->prange),
+                           (const uint32_t *) P<uint32_t>(c->rangehist), P<int64_t>(c->rangeoff),
+                           P<unsigned long long>(c->hist1));
+        LAUNCHCHK(c);
+    } else {
+        TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
+    }
+    hipLaunchKernelGGL(k_xrange_base, dim3(npairs), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr),
+                       (const unsigned long long *) P<unsigned long long>(c->hist1), P<int64_t>(c->bstart));
+    LAUNCHCHK(c);
+    if (c->nrange > 0 && nblk > 0) {
+        hipLaunchKernelGGL(k_extract_emit<EMIT_QB>,
+                           dim3(grid_for(c, (const void *) k_extract_emit<EMIT_QB>, 256 * EMIT_QB, 0, c->nrange)),
+                           dim3(256 * EMIT_QB), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
+                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
+                           (const XRange *) P<XRange>(c->xrange), c->nrange,
+                           (const uint32_t *) P<uint32_t>(c->rangehist), (const int64_t *) P<int64_t>(c->rangeoff),
+                           (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), &gs->bad);
+        LAUNCHCHK(c);
+    }
+    // ---- MSD partition, GRAPH_LEVELS levels, counts on the device
+    Rec *buf[2] = {P<Rec>(c->rec[0]), P<Rec>(c->rec[1])};
+    int cur = 0;
+    for (int level = 1; level <= GRAPH_LEVELS; level++) {
+        const int *dn = &gs->nsegs[level];
+        hipLaunchKernelGGL(k_clamp_count, dim3(1), dim3(64), 0, c->st, &gs->nsegs[level], (int) c->g_seg_cap,
+                           &gs->capped);
+        hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]), 0, CH,
+                           P<int64_t>(c->chk0), dn);
+        LAUNCHCHK(c);
+        const bool first = level == 1;                 // counted and in place: the extraction wrote it
+        const int64_t nd = level == 2 ? (int64_t) 1 << c->o.digit2_bits : 256;
+        if (!first) {
+            hipLaunchKernelGGL(k_fill_u64_dev, dim3(1024), dim3(256), 0, c->st, P<unsigned long long>(c->seghist), dn,
+                               nd, 0ull);
+            TRY(launch_seg_hist(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), 0, P<int64_t>(c->chk0), level,
+                                P<unsigned long long>(c->seghist), dn));
+        }
+        hipLaunchKernelGGL(k_seg_plan, dim3(PLAN_GRID), dim3(256), 0, c->st, P<Seg>(c->segs[cur]),
+                           first ? (const unsigned long long *) P<unsigned long long>(c->hist1)
+                                 : (const unsigned long long *) P<unsigned long long>(c->seghist),
+                           first ? 1 : 0, level, c->o.digit2_bits, first ? 1 : 0, CAP,
+                           P<unsigned long long>(c->cursor), P<Seg>(c->segs[cur ^ 1]), &gs->nsegs[level + 1],
+                           (int) c->g_seg_cap, P<Tile>(c->tiles), &gs->ntiles, (int) c->g_tile_cap, &gs->kept, 0, dn);
+        LAUNCHCHK(c);
+        if (!first)
+            TRY(launch_seg_scatter(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), 0, P<int64_t>(c->chk0), level,
+                                   P<unsigned long long>(c->cursor), dn));
+        cur ^= 1;
+    }
+    // ---- tile sort, LCP fix-up, prefix tables
+    hipLaunchKernelGGL(k_clamp_count, dim3(1), dim3(64), 0, c->st, &gs->ntiles, (int) c->g_tile_cap, &gs->capped);
+    TRY(launch_tile_sort(c, (unsigned) c->g_tile_cap, (Rec *const *) c->bufs.p, P<uint8_t>(c->entries),
+                         P<uint8_t>(c->ls), P<uint32_t>(c->pre), nullptr, P<int>(c->counters) + 10, &gs->ntiles));
+    hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->g_tile_cap + 255) / 256)),
+                       dim3(256), 0, c->st, (const Tile *) P<Tile>(c->tiles), 0, (const PairDev *) P<PairDev>(c->dpairs),
+                       P<uint8_t>(c->entries), P<uint8_t>(c->ls), (const uint32_t *) P<uint32_t>(c->pre),
+                       (const int *) &gs->ntiles);
+    LAUNCHCHK(c);
+    TRY(fill_u64(c, c->maxpre.p, npairs, 0));
+    if (c->o.build_cum) {
+        const int nrange = NPRE / CR_SPAN;
+        hipLaunchKernelGGL(k_cum_ranges, dim3(nrange, (unsigned) npairs), dim3(256), 0, c->st,
+                           (const PairDev *) P<PairDev>(c->dpairs), (const int *) P<int>(c->plist_all),
+                           (const uint32_t *) P<uint32_t>(c->pre), P<int64_t>(c->cum), P<unsigned long long>(c->gmax));
+        hipLaunchKernelGGL(k_cum_max, dim3((unsigned) npairs), dim3(256), 0, c->st, (const int *) P<int>(c->plist_all),
+                           (const unsigned long long *) P<unsigned long long>(c->gmax), nrange,
+                           P<unsigned long long>(c->maxpre));
+        LAUNCHCHK(c);
+    }
+    // ---- merge
+    const int64_t nmb_ub = c->g_nmb_ub;
+    uint32_t *minfo = reinterpret_cast<uint32_t *>(c->g_minfo);
+    TRY(fill_u64(c, c->mcntN.p, (nmb_ub + 1) / 2, 0));
+    TRY(fill_u64(c, c->mcntC.p, (nmb_ub + 1) / 2, 0));
+    const int64_t *dnmb = &gs->nmb;
+#define PGA_GCOUNT(A_)                                                                                  \
+    hipLaunchKernelGGL(k_merge_count<A_>, dim3(grid_for(c, (const void *) k_merge_count<A_>, MTH, 0, nmb_ub)), \
+                       dim3(MTH), 0, c->st, (const PairDev *) P<PairDev>(c->dpairs), npairs,             \
+                       (const uint8_t *) P<uint8_t>(c->ls), c->o.freq, P<uint32_t>(c->mcntN),           \
+                       P<uint32_t>(c->mcntC), minfo, (int64_t) 0, dnmb)
+    switch (MT) {
+    case 512: PGA_GCOUNT(2); break;
+    case 1024: PGA_GCOUNT(4); break;
+    default: PGA_GCOUNT(1); break;
+    }
+#undef PGA_GCOUNT
+    LAUNCHCHK(c);
+    TRY(scan_u32(c, P<uint32_t>(c->mcntN), nmb_ub, P<int64_t>(c->mofsN)));
+    TRY(scan_u32(c, P<uint32_t>(c->mcntC), nmb_ub, P<int64_t>(c->mofsC)));
+    hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
+                       (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mblk), npairs,
+                       P<int64_t>(c->pofs));
+    hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
+                       (const int64_t *) P<int64_t>(c->mofsC), (const int64_t *) P<int64_t>(c->mblk), npairs,
+                       P<int64_t>(c->pofs) + (npairs + 1));
+    const int64_t capN = (int64_t) (P<uint8_t>(c->rec[0]) + c->rec[0].cap - c->g_seedN);
+    const int64_t capC = (int64_t) c->rec[1].cap;
+    hipLaunchKernelGGL(k_graph_seeds, dim3(1), dim3(64), 0, c->st, (const int64_t *) P<int64_t>(c->pofs), npairs,
+                       P<PairDev>(c->dpairs), capN, capC, gs);
+    LAUNCHCHK(c);
+    TRY(fill_u64(c, c->nbuck.p, std::max(c->nctg, 1), 0));
+    TRY(fill_u64(c, c->cbuck.p, std::max(c->nctg, 1), 0));
+    int occ = 1;
+    {
+        const void *fn = MT == 512 ? (const void *) k_merge_emit<2>
+                       : MT == 1024 ? (const void *) k_merge_emit<4> : (const void *) k_merge_emit<1>;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, MTH, 0) != hipSuccess || occ < 1) {
+            (void) hipGetLastError();
+            occ = 1;
+        }
+    }
+    const int64_t egrid = std::max<int64_t>(1, std::min<int64_t>(nmb_ub, (int64_t) c->ncu * occ * 4));
+#define PGA_GEMIT(A_)                                                                                   \
+    hipLaunchKernelGGL(k_merge_emit<A_>, dim3((unsigned) egrid), dim3(MTH), 0, c->st,                   \
+                       (const PairDev *) P<PairDev>(c->dpairs), npairs,                                 \
+                       (const uint8_t *) P<uint8_t>(c->entries), c->o.freq,                            \
+                       (const int64_t *) P<int64_t>(c->mofsN), (const int64_t *) P<int64_t>(c->mofsC),  \
+                       c->g_seedN, c->g_seedC, P<unsigned long long>(c->nbuck),                         \
+                       P<unsigned long long>(c->cbuck), (const uint32_t *) minfo, (int64_t) 0, (int64_t) 0, \
+                       dnmb, (const int *) &gs->overflow)
+    switch (MT) {
+    case 512: PGA_GEMIT(2); break;
+    case 1024: PGA_GEMIT(4); break;
+    default: PGA_GEMIT(1); break;
+    }
+#undef PGA_GEMIT
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+void graph_drop(pga_ctx *c)
+{
+    if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_gen = ~0ull;
+}
+
+int graph_capture(pga_ctx *c)
+{
+    graph_drop(c);
+    TRY(graph_prepare(c));
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    const int rc = graph_enqueue(c);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->st, &g);
+    if (rc) {
+        if (g) (void) hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess || !g) return fail(c, PGA_EHIP, "stream capture failed: %s", hipGetErrorString(ec));
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->graph_gen = c->pool_gen;
+    return PGA_OK;
+}
+
+} // namespace
+
+extern "C" int pga_run_graph(pga_ctx *c)
+{
+    if (!c || !c->loaded) return c ? fail(c, PGA_EINVAL, "no GDB loaded") : PGA_EINVAL;
+    HIPCHK(c, hipSetDevice(c->o.device));
+    c->built = c->merged = c->ss_ready = false;
+    if (!c->gexec || c->graph_gen != c->pool_gen) TRY(graph_capture(c));
+    memset(&c->tm, 0, sizeof(c->tm));
+    c->tm_build = c->tm_merge = false;
+    TRY(record(c, 0));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+    TRY(record(c, 4));
+    // the status, the seed ranges and the device-written pair layout
+    const int npairs = c->npairs;
+    GStat st;
+    std::vector<int64_t> po(2 * (npairs + 1));
+    std::vector<PairDev> pd(npairs);
+    HIPCHK(c, hipMemcpyAsync(&st, c->gstat.p, sizeof(st), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(po.data(), c->pofs.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(pd.data(), c->dpairs.p, pd.size() * sizeof(PairDev), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const bool redo = st.overflow || st.bad || st.capped || st.nsegs[GRAPH_LEVELS + 1] > 0;
+    if (redo) {
+        // beyond the captured form: the eager path, which sizes as it goes
+        c->graph_fallbacks++;
+        TRY(pga_build_index(c));
+        TRY(pga_merge(c));
+        return PGA_OK;
+    }
+    c->R = st.R;
+    c->pairs = pd;
+    c->ntiles = st.ntiles;
+    c->nseeds.assign(npairs, 0);
+    c->cseeds.assign(npairs, 0);
+    for (int p = 0; p < npairs; p++) {
+        c->nseeds[p] = po[p + 1] - po[p];
+        c->cseeds[p] = po[npairs + 1 + p + 1] - po[npairs + 1 + p];
+    }
+    c->minfo_p = reinterpret_cast<uint32_t *>(c->g_minfo);
+    c->seedN_p = c->g_seedN;
+    c->seedC_p = c->g_seedC;
+    c->built = c->merged = true;
+    int nf[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(nf, P<int>(c->counters) + 10, sizeof(nf), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->tm.total_ms = ev_ms(c->ev[0], c->ev[4]);
+    c->tm.tiles = st.ntiles;
+    c->tm.fallback_tiles = nf[0];
+    c->tm.crowded_tiles = nf[1];
+    int levels = 0;
+    for (int l = 1; l <= GRAPH_LEVELS; l++) if (st.nsegs[l] > 0) levels = l;
+    c->tm.partition_levels = levels;
+    return PGA_OK;
+}
 
 // ======================================================================== merge without allocation
 //
@@ -1430,7 +1813,8 @@ extern "C" int pga_merge_gix_noalloc(const uint8_t *entries, int64_t entries_byt
     hipLaunchKernelGGL(k_na_ls, dim3(na_grid((const void *) k_na_ls, 256, (nrec + 255) / 256)), dim3(256), 0, st,
                        entries, e, nrec, ls);
     hipLaunchKernelGGL(k_merge_count<NA_A>, dim3(na_grid((const void *) k_merge_count<NA_A>, MTH, nmb)), dim3(MTH), 0,
-                       st, (const PairDev *) dpair, 1, (const uint8_t *) ls, freq, cntN, cntC, minfo, nmb);
+                       st, (const PairDev *) dpair, 1, (const uint8_t *) ls, freq, cntN, cntC, minfo, nmb,
+                       (const int64_t *) nullptr);
     NA_CHK(hipGetLastError());
     const int64_t nb = (nmb + SCAN_SPAN - 1) / SCAN_SPAN;
     for (int k = 0; k < 2; k++) {
@@ -1456,7 +1840,7 @@ extern "C" int pga_merge_gix_noalloc(const uint8_t *entries, int64_t entries_byt
     hipLaunchKernelGGL(k_merge_emit<NA_A>, dim3((unsigned) egrid), dim3(MTH), 0, st, (const PairDev *) dpair, 1,
                        entries, freq, (const int64_t *) ofsN, (const int64_t *) ofsC, nseeds, cseeds,
                        reinterpret_cast<unsigned long long *>(nbuck), reinterpret_cast<unsigned long long *>(cbuck),
-                       (const uint32_t *) minfo, nmb, eper);
+                       (const uint32_t *) minfo, nmb, eper, (const int64_t *) nullptr, (const int *) nullptr);
     NA_CHK(hipGetLastError());
     NA_CHK(hipStreamSynchronize(st));
 #undef NA_CHK

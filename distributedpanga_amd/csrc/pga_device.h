@@ -65,4 +65,19 @@ struct XRange {                  // a run of extraction blocks inside one pair, 
     int32_t pair, pad;
 };
 
+constexpr int GRAPH_LEVELS = 8;  // partition levels the device-driven pipeline runs
+
+// Status of a device-driven build + merge, written on the device and read by
+// the host after it (pga_run_graph).
+struct GStat {
+    int nsegs[GRAPH_LEVELS + 2]; // segments split at level L (nsegs[GRAPH_LEVELS+1] > 0: not converged)
+    int ntiles;                  // sort tiles
+    int overflow;                // seed streams larger than their buffers: nothing emitted
+    int bad;                     // the extraction passes disagreed on a first digit
+    int capped;                  // a segment or tile count reached its list's bound
+    int pad[2];
+    unsigned long long kept;     // records the partition left in place (unused here)
+    int64_t R, nmb;              // records, merge tiles
+};
+
 } // namespace pga

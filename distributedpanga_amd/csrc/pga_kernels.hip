@@ -650,6 +650,15 @@ __global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n, unsign
         p[i] = v;
 }
 
+// p[0, *cnt * mult) = v: a fill whose length is a device-side count.
+__global__ void k_fill_u64_dev(unsigned long long *__restrict__ p, const int *__restrict__ cnt, int64_t mult,
+                               unsigned long long v)
+{
+    const int64_t n = (int64_t) *cnt * mult;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // Record ranges of the pairs from the extraction block offsets.
 __global__ void k_pair_ranges(const int64_t *__restrict__ blkoff, const int64_t *__restrict__ pblk,
                               int npairs, int64_t *__restrict__ out)
@@ -697,10 +706,13 @@ __device__ __forceinline__ int seg_of_chunk(const int64_t *__restrict__ chk0, in
 
 // Exclusive prefix of per-segment chunk counts (one block); chk0[nseg] = total
 // chunks, chk0[nseg+1] = total records of the segments.
+// dn (may be null): the segment count in device memory, read instead of nseg
+// (the device-driven pipeline; the same holds for the kernels below).
 __global__ __launch_bounds__(1024) void k_chunk_prefix(const Seg *__restrict__ segs, int nseg, int chunk,
-                                                       int64_t *__restrict__ chk0)
+                                                       int64_t *__restrict__ chk0, const int *__restrict__ dn)
 {
     __shared__ int64_t sh[1024 / 64 + 1];
+    if (dn) nseg = *dn;
     int per = (nseg + 1023) / 1024;
     int b0 = threadIdx.x * per, b1 = b0 + per < nseg ? b0 + per : nseg;
     int64_t s = 0, r = 0;
@@ -721,9 +733,10 @@ template <int PER>
 __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, const Rec *__restrict__ buf1,
                                                  const Seg *__restrict__ segs,
                                                  int nseg, const int64_t *__restrict__ chk0, int level, int w2,
-                                                 unsigned long long *__restrict__ seghist)
+                                                 unsigned long long *__restrict__ seghist, const int *__restrict__ dn)
 {
     __shared__ uint32_t hist[ND_MAX];
+    if (dn) nseg = *dn;
     const int CH = PT * PER;
     const int64_t nlog = chk0[nseg];
     const int nd = 1 << digit_bits(level, w2);
@@ -765,14 +778,20 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
                                                   unsigned long long *__restrict__ cursor,
                                                   Seg *__restrict__ next, int *__restrict__ nnext, int next_cap,
                                                   Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap,
-                                                  unsigned long long *__restrict__ kept)
+                                                  unsigned long long *__restrict__ kept, int nseg,
+                                                  const int *__restrict__ dn)
 {
     __shared__ int64_t sh[256 / 64 + 1];
     __shared__ int64_t cnt[ND_MAX], off[ND_MAX];
+    __shared__ uint16_t tfirst[ND_MAX], tlast[ND_MAX], lnext[ND_MAX];
+    __shared__ int nt, nn, tbase, nbase;
     const int nd = 1 << digit_bits(level, w2);
     const int pd = nd / 256;                        // digits per thread, contiguous
-    const Seg sg = segs[blockIdx.x];
-    const size_t row = (hist_by_pair ? (size_t) sg.pair : (size_t) blockIdx.x) * (size_t) nd;
+    if (dn) nseg = *dn;
+    for (int sb = blockIdx.x; sb < nseg; sb += gridDim.x) {
+    __syncthreads();                                // the previous segment is done with the LDS lists
+    const Seg sg = segs[sb];
+    const size_t row = (hist_by_pair ? (size_t) sg.pair : (size_t) sb) * (size_t) nd;
     int64_t sum = 0;
     int nz = 0;
     for (int q = 0; q < pd; q++) {
@@ -786,7 +805,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     if (holders == 1 && __syncthreads_count(nz > 1) == 0) {
         // a single bucket: no scatter; the child keeps the buffer and the range
         if (threadIdx.x == 0) {
-            segs[blockIdx.x].skip = 1;
+            segs[sb].skip = 1;
             if (sg.count > cap) {
                 const int q = atomicAdd(nnext, 1);
                 if (q < next_cap) next[q] = Seg{sg.start, sg.count, sg.pair, level + 1, sg.buf, 0};
@@ -796,7 +815,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
             }
             atomicAdd(kept, (unsigned long long) sg.count);
         }
-        return;
+        continue;
     }
     const int dstbuf = in_place ? sg.buf : sg.buf ^ 1;   // in_place: already in bucket order, no scatter
     int64_t tot;
@@ -804,7 +823,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     for (int q = 0; q < pd; q++) {
         const int d = threadIdx.x * pd + q;
         off[d] = run;
-        cursor[(size_t) blockIdx.x * nd + d] = (unsigned long long) (sg.start + run);
+        cursor[(size_t) sb * nd + d] = (unsigned long long) (sg.start + run);
         run += cnt[d];
     }
     __syncthreads();
@@ -812,8 +831,6 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     // lists of digit indices: a tile is a run of digits [tfirst, tlast], a
     // next-level segment one digit.  The segment then reserves its slots with
     // one atomic per list.
-    __shared__ uint16_t tfirst[ND_MAX], tlast[ND_MAX], lnext[ND_MAX];
-    __shared__ int nt, nn, tbase, nbase;
     if (threadIdx.x == 0) {
         int t = 0, q = 0, tf = 0, tl = 0;
         int64_t tc = 0;
@@ -848,8 +865,9 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
         nbase = q ? atomicAdd(nnext, q) : 0;
     }
     __syncthreads();
-    // slots past the capacity are counted but not written (the host sees the
-    // count and grows the lists)
+    // slots past the capacity are counted but not written: the caller sees the
+    // count above the capacity and reports an internal error (the lists are
+    // sized from a bound that a correct count never exceeds)
     for (int i = threadIdx.x; i < nt && tbase + i < tile_cap; i += 256) {
         const int f = tfirst[i], l = tlast[i];
         tiles[tbase + i] = Tile{sg.start + off[f], (int32_t) (off[l] + cnt[l] - off[f]), sg.pair, dstbuf, level};
@@ -857,6 +875,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     for (int i = threadIdx.x; i < nn && nbase + i < next_cap; i += 256) {
         const int d = lnext[i];
         next[nbase + i] = Seg{sg.start + off[d], cnt[d], sg.pair, level + 1, dstbuf, 0};
+    }
     }
 }
 
@@ -869,9 +888,10 @@ template <int T, int PER>
 __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *__restrict__ buf1,
                                                    const Seg *__restrict__ segs, int nseg,
                                                    const int64_t *__restrict__ chk0, int level, int w2,
-                                                   unsigned long long *__restrict__ cursor)
+                                                   unsigned long long *__restrict__ cursor, const int *__restrict__ dn)
 {
     static_assert(T >= 256, "one thread per digit at least");
+    if (dn) nseg = *dn;
     constexpr int CH = T * PER;
     const int64_t nlog = chk0[nseg];
     __shared__ Rec stage[CH];
@@ -956,12 +976,12 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
     }
 }
 
-template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
-template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
-template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
-template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
-template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
-template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *);
+template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
+template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
+template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
+template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
+template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
+template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
 
 // ---------------------------------------------------------------- tile sort
 
@@ -1001,9 +1021,11 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                                                  const PairDev *__restrict__ pairs, int jbits, int w2,
                                                  uint8_t *__restrict__ entries,
                                                  uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
-                                                 Rec *__restrict__ skeys, int64_t nlog, int *__restrict__ nfall)
+                                                 Rec *__restrict__ skeys, int64_t nlog, int *__restrict__ nfall,
+                                                 const int *__restrict__ dn)
 {
     constexpr int E = 8;
+    if (dn) nlog = *dn;
     constexpr int CAP = T * E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Rec *sk = reinterpret_cast<Rec *>(smem_raw);
@@ -1390,17 +1412,18 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *, const int *);
 
 // Pair of a global record index.
 // LCP of each tile's first record with its predecessor (0 at a pair's start),
 // from the 24-bit prefixes and the 7 suffix bytes of the two entries.
 __global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const PairDev *__restrict__ pairs,
                            uint8_t *__restrict__ entries, uint8_t *__restrict__ ls,
-                           const uint32_t *__restrict__ pre)
+                           const uint32_t *__restrict__ pre, const int *__restrict__ dn)
 {
+    if (dn) ntiles = *dn;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
     const Tile tl = tiles[t];
     const PairDev pd = pairs[tl.pair];
@@ -1739,9 +1762,11 @@ template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__ pairs, int npairs,
                                                      const uint8_t *__restrict__ ls, int freq,
                                                      uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC,
-                                                     uint32_t *__restrict__ minfo, int64_t nlog)
+                                                     uint32_t *__restrict__ minfo, int64_t nlog,
+                                                     const int64_t *__restrict__ dn)
 {
     constexpr int TA = MTH * A;
+    if (dn) nlog = *dn;
     __shared__ uint8_t win[TA + 2 * MAX_FREQ + 4];
     __shared__ uint64_t red[MTH / 64 + 1];
     for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
@@ -1851,8 +1876,16 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
                                                     unsigned long long *__restrict__ nbuck,
                                                     unsigned long long *__restrict__ cbuck,
                                                     const uint32_t *__restrict__ minfo,
-                                                    int64_t nmb, int64_t per_block)
+                                                    int64_t nmb, int64_t per_block,
+                                                    const int64_t *__restrict__ dn, const int *__restrict__ skip)
 {
+    // device-driven form: the tile count from dn, an even share per block,
+    // and nothing written when the streams overflow their buffers (*skip)
+    if (skip && *skip) return;
+    if (dn) {
+        nmb = *dn;
+        per_block = (nmb + gridDim.x - 1) / gridDim.x;
+    }
     constexpr int TA = MTH * A;
     constexpr int W = TA + 2 * MAX_FREQ + 4;
     __shared__ uint64_t pay[W];
@@ -2007,6 +2040,79 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         flush_buck(lb, pairs[cur_pair].ncontig, pairs[cur_pair].buck_off, nbuck, cbuck);
 }
 
+// ---------------------------------------------------------------- device-driven pipeline
+
+// Per-pair layout from the record ranges pr[0..npairs]: records, GIX entry
+// offsets, merge tiles, and the first partition level's segments (pairs of
+// more than cap records) and tiles (the others); also clears the status.
+// The load-time fields of pairs[] (P, C, e, s, ncontig, cum_off, buck_off)
+// are already set.
+__global__ __launch_bounds__(256) void k_graph_layout(const int64_t *__restrict__ pr, int npairs, int mt, int cap,
+                                                      PairDev *__restrict__ pairs, int64_t *__restrict__ mblk,
+                                                      Seg *__restrict__ seg0, Tile *__restrict__ tile0,
+                                                      GStat *__restrict__ st)
+{
+    if (threadIdx.x == 0) {
+        int64_t ent = 0, mb = 0;
+        int nseg = 0, nt = 0;
+        for (int p = 0; p < npairs; p++) {
+            PairDev d = pairs[p];
+            d.rec0 = pr[p];
+            d.nrec = pr[p + 1] - pr[p];
+            d.ent_off = ent;
+            ent += d.nrec * d.e;
+            d.mblk0 = mb;
+            mblk[p] = mb;
+            mb += (d.nrec + mt - 1) / mt;
+            pairs[p] = d;
+            if (d.nrec > cap) seg0[nseg++] = Seg{d.rec0, d.nrec, p, 1, 1, 0};
+            else if (d.nrec > 0) tile0[nt++] = Tile{d.rec0, (int32_t) d.nrec, p, 1, 1};
+        }
+        mblk[npairs] = mb;
+        st->R = pr[npairs];
+        st->nmb = mb;
+        st->nsegs[0] = 0;
+        st->nsegs[1] = nseg;
+        for (int l = 2; l < GRAPH_LEVELS + 2; l++) st->nsegs[l] = 0;
+        st->ntiles = nt;
+        st->overflow = 0;
+        st->bad = 0;
+        st->capped = 0;
+        st->kept = 0;
+    }
+}
+
+// A device-side count held to its list's capacity before kernels read it
+// (the lists are sized from a bound no correct count exceeds; *capped tells
+// the host to redo the work).
+__global__ void k_clamp_count(int *__restrict__ cnt, int cap, int *__restrict__ capped)
+{
+    if (threadIdx.x == 0 && *cnt > cap) {
+        *cnt = cap;
+        *capped = 1;
+    }
+}
+
+// Seed stream offsets of the pairs from the per-pair seed ranges (N in
+// pofs[0..npairs], C in pofs[npairs+1..]); flags an overflow of either
+// stream's buffer.
+__global__ __launch_bounds__(64) void k_graph_seeds(const int64_t *__restrict__ pofs, int npairs,
+                                                    PairDev *__restrict__ pairs, int64_t capN, int64_t capC,
+                                                    GStat *__restrict__ st)
+{
+    if (threadIdx.x == 0) {
+        int64_t nb = 0, cb = 0;
+        for (int p = 0; p < npairs; p++) {
+            const int s = pairs[p].s;
+            pairs[p].nseed_off = nb;
+            pairs[p].cseed_off = cb;
+            nb += (pofs[p + 1] - pofs[p]) * s;
+            cb += (pofs[npairs + 1 + p + 1] - pofs[npairs + 1 + p]) * s;
+        }
+        st->overflow = nb + 16 > capN || cb + 16 > capC;
+    }
+}
+
 // ---------------------------------------------------------------- checks
 
 // 80-bit k-mer of GIX entry i: 24-bit prefix, then the entry's 7 suffix bytes.
@@ -2157,11 +2263,12 @@ __global__ __launch_bounds__(SP_T) void k_seed_pack(const Rec *__restrict__ keys
 
 #define PGA_MERGE_INST(A_)                                                                              \
     template __global__ void k_merge_count<A_>(const PairDev *, int, const uint8_t *, int, uint32_t *, \
-                                               uint32_t *, uint32_t *, int64_t);                        \
+                                               uint32_t *, uint32_t *, int64_t, const int64_t *);       \
     template __global__ void k_merge_emit<A_>(const PairDev *, int, const uint8_t *,                    \
                                               int, const int64_t *, const int64_t *, uint8_t *,         \
                                               uint8_t *, unsigned long long *, unsigned long long *,    \
-                                              const uint32_t *, int64_t, int64_t);
+                                              const uint32_t *, int64_t, int64_t, const int64_t *,      \
+                                              const int *);
 PGA_MERGE_INST(1)
 PGA_MERGE_INST(2)
 PGA_MERGE_INST(4)

@@ -40,7 +40,7 @@ class Timings(C.Structure):
                 ("partition_levels", C.c_int), ("partition_launches", C.c_int), ("tiles", C.c_int64),
                 ("scatter_ms", C.c_double), ("scatter_bytes", C.c_int64), ("scatter_launches", C.c_int),
                 ("merge_emit_ms", C.c_double), ("fallback_tiles", C.c_int64),
-                ("crowded_tiles", C.c_int64)]
+                ("crowded_tiles", C.c_int64), ("graph_fallbacks", C.c_int64)]
 
 
 _u8p = C.POINTER(C.c_uint8)
@@ -72,6 +72,7 @@ def _setup(L):
     L.pga_index_poke.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_int]
     L.pga_destination_map.argtypes = [_i64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pga_merge.argtypes = [C.c_void_p]
+    L.pga_run_graph.argtypes = [C.c_void_p]
     L.pga_seeds_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(SeedsMeta)]
     L.pga_seeds_export.argtypes = [C.c_void_p, C.c_int, _u8p, _u8p, _i64p, _i64p]
     L.pga_timings_get.argtypes = [C.c_void_p, C.POINTER(Timings)]
@@ -142,12 +143,10 @@ class Context:
     # -- input
     def load_batch(self, gdbs):
         n = len(gdbs)
-        bps = [np.ascontiguousarray(g.bps, np.uint8) for g in gdbs]
+        # an empty .bps image still needs a valid host pointer
+        bps = [np.ascontiguousarray(g.bps, np.uint8) if g.bps.size else np.zeros(1, np.uint8) for g in gdbs]
         boff = [np.ascontiguousarray(g.boff[:g.ncontig], np.int64) for g in gdbs]
         clen = [np.ascontiguousarray(g.clen, np.int64) for g in gdbs]
-        for b in bps:
-            if b.size == 0:
-                b.resize(1, refcheck=False)
         self._keep = (bps, boff, clen)
         pb = (_u8p * n)(*[b.ctypes.data_as(_u8p) for b in bps])
         nb = (C.c_int64 * n)(*[int(g.bps.size) for g in gdbs])
@@ -220,6 +219,11 @@ class Context:
         out = np.zeros(1024, np.int64)
         self._check(self._L.pga_bucket_counts(self._h, pair, out.ctypes.data_as(_i64p)), "pga_bucket_counts")
         return out
+
+    def run_graph(self):
+        """build_index + merge as one device-driven pipeline, captured in a HIP
+        graph on the first call after a load and replayed afterwards."""
+        self._check(self._L.pga_run_graph(self._h), "pga_run_graph")
 
     # -- merge
     def merge(self):

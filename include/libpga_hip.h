@@ -98,6 +98,9 @@ typedef struct {
        not resolve them, by the merge-sort path (fallback) */
     int64_t fallback_tiles;
     int64_t crowded_tiles;
+    /* pga_run_graph calls that had to redo the work on the eager path (a
+       partition deeper than the captured levels, or seeds past the buffers) */
+    int64_t graph_fallbacks;
 } pga_timings;
 
 void        pga_default_opts(pga_opts *o);
@@ -219,6 +222,17 @@ int pga_timings_get(pga_ctx *ctx, pga_timings *t);
 int pga_merge_gix_noalloc(const uint8_t *entries, int64_t entries_bytes, int64_t nrec, int P, int C,
                           int ncontig, int freq, uint8_t *nseeds, int64_t ncap, uint8_t *cseeds, int64_t ccap,
                           int64_t *nbuck, int64_t *cbuck, int64_t *nlen, int64_t *clen, void *stream);
+
+/* pga_build_index + pga_merge of the loaded batch as one device-driven
+ * pipeline (no host round trip between kernels; every count stays on the
+ * device) captured in a HIP graph on the first call after a load and replayed
+ * on later calls.  Outputs and exports are those of the eager calls.  Buffers
+ * are sized from load-time bounds (two records a candidate position), so it
+ * takes more device memory than the eager path.  When a batch needs more than
+ * 8 partition levels (long low-complexity runs) or its seeds outgrow the
+ * buffers, the call redoes the work on the eager path (counted in
+ * pga_timings.graph_fallbacks).  Only the total time is timed. */
+int pga_run_graph(pga_ctx *ctx);
 
 /* Wait for all work queued on the context's stream. */
 int pga_sync(pga_ctx *ctx);

@@ -20,8 +20,9 @@ pytestmark = pytest.mark.gpu
 THREADS = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
 
 
-def _check_pair(ctx, p, g, freq=10):
-    want = oracle.pipeline(g.bps, g.boff[:-1], g.clen, freq=freq, threads=THREADS)
+def _check_pair(ctx, p, g, freq=10, want=None):
+    if want is None:
+        want = oracle.pipeline(g.bps, g.boff[:-1], g.clen, freq=freq, threads=THREADS)
     ent, cum, info = ctx.index_export(p)
     assert info["nrec"] == want["nrec"] and (info["P"], info["C"]) == (want["P"], want["C"])
     assert np.array_equal(ent.reshape(-1), want["entries"].reshape(-1)), f"pair {p}: entries"
@@ -29,6 +30,7 @@ def _check_pair(ctx, p, g, freq=10):
     N, Cs, nb, cb = ctx.seeds_export(p)
     assert np.array_equal(N, want["nseeds"]) and np.array_equal(Cs, want["cseeds"]), f"pair {p}: seeds"
     assert np.array_equal(nb, want["nbuck"]) and np.array_equal(cb, want["cbuck"]), f"pair {p}: bucks"
+    return want
 
 
 def test_100mbp_pair_full_size():
@@ -48,8 +50,13 @@ def test_64_pair_batch_full_size():
         ctx.load_batch(gs)
         ctx.build_index()
         ctx.merge()
+        wants = [_check_pair(ctx, p, g) for p, g in enumerate(gs)]
+        # the same batch through the captured device-driven pipeline
+        ctx.run_graph()
+        ctx.run_graph()
+        assert ctx.timings()["graph_fallbacks"] == 0
         for p, g in enumerate(gs):
-            _check_pair(ctx, p, g)
+            _check_pair(ctx, p, g, want=wants[p])
 
 
 def _groups(cum, rng, n):
