@@ -6,11 +6,11 @@ Tunables (all bit-exact-neutral: every setting gives the same output bytes):
   chunk_records  records per radix-scatter workgroup (2K / 4K / 8K): the run
                  length per digit, i.e. write coalescing, against occupancy
   merge_tile     anchors per merge workgroup (256 / 512 / 1024)
-  digit2_bits    width of the partition's second digit (8 / 9 / 10): wider
+  digit2_bits    width of the partition's second digit (8 / 9 / 10 / 11): wider
                  digits split the second level finer, so with smaller tiles
                  fewer records need a third pass
 
-The search is exhaustive (81 points, a couple of steps each, on the loaded
+The search is exhaustive (108 points, a couple of steps each, on the loaded
 batch) and the winner is cached in a JSON file keyed by the batch shape.
 """
 import itertools
@@ -20,7 +20,7 @@ import os
 import time
 
 SPACE = dict(tile_records=(2048, 4096, 8192), chunk_records=(2048, 4096, 8192), merge_tile=(256, 512, 1024),
-             digit2_bits=(8, 9, 10))
+             digit2_bits=(8, 9, 10, 11))
 
 
 def cache_path():

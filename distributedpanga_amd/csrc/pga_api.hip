@@ -363,7 +363,7 @@ static bool opts_valid(const pga_opts &q)
            (q.tile_records == 2048 || q.tile_records == 4096 || q.tile_records == 8192) &&
            (q.chunk_records == 2048 || q.chunk_records == 4096 || q.chunk_records == 8192) &&
            (q.merge_tile == 256 || q.merge_tile == 512 || q.merge_tile == 1024) &&
-           q.digit2_bits >= 8 && q.digit2_bits <= 10;
+           q.digit2_bits >= 8 && q.digit2_bits <= 11;
 }
 
 int pga_set_opts(pga_ctx *c, const pga_opts *o)

@@ -672,7 +672,7 @@ __global__ void k_pair_ranges(const int64_t *__restrict__ blkoff, const int64_t 
 // Radix digits: 8 bits at every level except the second, which is w2 bits
 // (8..10) wide, so that one more level's worth of buckets fits the tiles.
 // Key bit 0 is the top bit of w0; bits past 127 read as zero.
-constexpr int ND_MAX = 1 << 10;      // largest digit count of a level
+constexpr int ND_MAX = 1 << 11;      // largest digit count of a level
 
 __device__ __forceinline__ int digit_pos(int level, int w2)
 {

@@ -53,8 +53,8 @@ typedef struct {
     int chunk_records;   /* records per radix-scatter workgroup: 2048..8192     */
     int merge_tile;      /* entries per merge workgroup: 256, 512 or 1024       */
     int build_cum;       /* 1: build the 2^24 prefix table (GIX layout)         */
-    int digit2_bits;     /* width of the radix partition's second digit: 8, 9 or
-                            10 (the others are 8); wider digits give smaller
+    int digit2_bits;     /* width of the radix partition's second digit: 8 to 11
+                            (the others are 8); wider digits give smaller
                             buckets, so fewer records need a third level      */
 } pga_opts;
 

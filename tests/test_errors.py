@@ -28,7 +28,7 @@ def _opts(**kw):
 @pytest.mark.parametrize("field,value", [("freq", 0), ("freq", 65), ("tile_records", 1000),
                                          ("tile_records", 16384), ("chunk_records", 1024),
                                          ("merge_tile", 128), ("merge_tile", 2048),
-                                         ("digit2_bits", 7), ("digit2_bits", 11)])
+                                         ("digit2_bits", 7), ("digit2_bits", 12)])
 def test_create_rejects_bad_options(field, value):
     L = _lib()
     h = C.c_void_p()

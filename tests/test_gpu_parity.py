@@ -57,7 +57,8 @@ def test_tile_sizes(tile):
 
 
 @pytest.mark.parametrize("tile,chunk,digit2", [(2048, 4096, 9), (2048, 2048, 10), (4096, 4096, 10),
-                                               (4096, 8192, 10), (2048, 8192, 9)])
+                                               (4096, 8192, 10), (2048, 8192, 9), (2048, 8192, 11),
+                                               (4096, 4096, 11), (2048, 2048, 11)])
 def test_second_digit_widths(tile, chunk, digit2):
     """A 9- or 10-bit second digit: buckets of 2^(8+w) at the second level,
     bins of tiles split there cut from a wider digit span."""
