@@ -137,6 +137,8 @@ struct pga_ctx {
     uint8_t *g_minfo = nullptr, *g_seedN = nullptr, *g_seedC = nullptr;
     int64_t g_nmb_ub = 0, g_seg_cap = 0, g_tile_cap = 0;
     int64_t graph_fallbacks = 0;
+    // 5-base bucket range of the records this context indexes (all by default)
+    int shard_lo = 0, shard_hi = 1024;
 };
 
 namespace {
@@ -430,6 +432,18 @@ void pga_destroy(pga_ctx *c)
     delete c;
 }
 
+int pga_set_shard(pga_ctx *c, int bucket_lo, int bucket_hi)
+{
+    if (!c) return PGA_EINVAL;
+    if (bucket_lo < 0 || bucket_hi > 1024 || bucket_lo > bucket_hi)
+        return fail(c, PGA_EINVAL, "bucket range [%d, %d) outside [0, 1024)", bucket_lo, bucket_hi);
+    c->shard_lo = bucket_lo;
+    c->shard_hi = bucket_hi;
+    c->built = c->merged = false;
+    c->graph_gen = ~0ull;                           // the captured extraction filters differently
+    return PGA_OK;
+}
+
 int pga_sync(pga_ctx *c)
 {
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -684,7 +698,8 @@ int pga_build_index(pga_ctx *c)
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
                            dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
                            (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
-                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist));
+                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
+                           (uint32_t) c->shard_lo, (uint32_t) c->shard_hi);
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
     } else {
@@ -733,6 +748,14 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->hist1, (size_t) npairs * 256 * sizeof(uint64_t)));
     TRY(ensure(c, c->dpairs, (size_t) npairs * sizeof(PairDev)));
     TRY(ensure(c, c->counters, 16 * sizeof(int)));
+    // List bounds (a count past them is an internal error, reported as
+    // PGA_EOVERFLOW).  Segments: a segment holds more than CAP records and the
+    // segments of one level are disjoint, so a level has at most R/CAP of them
+    // (plus the pairs at level 1).  Tiles: the plan closes a tile only when the
+    // next bucket would overflow it, so two consecutive tiles of a segment hold
+    // more than CAP records together -- at most 2R/CAP such tiles -- plus, over
+    // at most 16 levels, one partial tile per segment and one tile break per
+    // bucket that becomes a segment (each at most R/CAP a level): 34 R/CAP.
     const int64_t seg_cap = R / CAP + npairs + 16;
     const int64_t tile_cap = 34 * (R / CAP) + npairs + 1024;
     TRY(ensure(c, c->segs[0], (size_t) seg_cap * sizeof(Seg)));
@@ -1449,7 +1472,8 @@ int graph_enqueue(pga_ctx *c)
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
                            dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
                            (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
-                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist));
+                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
+                           (uint32_t) c->shard_lo, (uint32_t) c->shard_hi);
         LAUNCHCHK(c);
         TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
     } else {

@@ -129,7 +129,8 @@ int pga_gdbfile_read(const char *path, pga_gdbfile *g)
             goto done;
         } else if (t == 'S') {
             int64_t n;
-            if (get_int(&q, eol, &n) || n < 0 || q + 1 + n > eol || *q != ' ') goto done;
+            /* the bound is checked before any pointer arithmetic with n */
+            if (get_int(&q, eol, &n) || n < 0 || n > (int64_t) (eol - q) - 1 || *q != ' ') goto done;
             q++;
             if (g->nscaff > 0) g->slen[g->nscaff - 1] = spos;
             if (grow_i64(&g->slen, &cs, (size_t) g->nscaff + 1) || grow_int(&g->fctg, &cf, (size_t) g->nscaff + 2) ||
@@ -144,7 +145,8 @@ int pga_gdbfile_read(const char *path, pga_gdbfile *g)
             spos = 0;
         } else if (t == 'G' || t == 'C') {
             int64_t v;
-            if (g->nscaff == 0 || get_int(&q, eol, &v) || v < 0) goto done;
+            /* lengths past INT32_MAX are refused (the device loader takes int32 contigs) */
+            if (g->nscaff == 0 || get_int(&q, eol, &v) || v < 0 || v > INT32_MAX) goto done;
             if (t == 'C') {
                 if (grow_i64(&g->sbeg, &cb, (size_t) g->ncontig + 1) || grow_i64(&g->clen, &cc, (size_t) g->ncontig + 1) ||
                     grow_i64(&g->boff, &co, (size_t) g->ncontig + 2)) { rc = -1; goto done; }

@@ -227,9 +227,16 @@ __device__ __forceinline__ uint32_t digit_fwd(uint64_t W)
 
 __device__ __forceinline__ uint32_t digit_rev(uint64_t W) { return ~(uint32_t) (W >> 16) & 0xFFu; }
 
+// 5-base bucket (record_bucket, src/dsort.h:30-33: the key's top 10 bits) of
+// the same records: the first digit and the next base (j+4 forward; the
+// complement of j+7 for the reverse complement).
+__device__ __forceinline__ uint32_t bucket_fwd(uint64_t W) { return (digit_fwd(W) << 2) | ((uint32_t) (W >> 8) & 3u); }
+__device__ __forceinline__ uint32_t bucket_rev(uint64_t W) { return (digit_rev(W) << 2) | (~(uint32_t) (W >> 14) & 3u); }
+
 struct XEmit {
     uint64_t f0, f1, r0, r1;     // forward and reverse-complement record keys
     uint32_t dg;                 // first radix digits: forward (bits 0-7), reverse complement (8-15)
+    uint32_t bk;                 // 5-base buckets: forward (bits 0-9), reverse complement (10-19)
     bool fwd, rev;
 };
 
@@ -245,6 +252,7 @@ __device__ __forceinline__ int scan_position(const uint64_t *__restrict__ bps64,
     window80(bps64, c.bitoff + 2 * j, W0, W1);
     if (!syncmer(W0, tl, tm)) return 0;
     out.dg = digit_fwd(W0) | (digit_rev(W0) << 8);
+    out.bk = bucket_fwd(W0) | (bucket_rev(W0) << 10);
     out.fwd = j <= (int64_t) c.clen - 40;          // forward 40-mer fits
     out.rev = j >= 28;                             // reverse complement of [j-28, j+12) fits
     if (keys) {
@@ -291,13 +299,16 @@ __device__ __forceinline__ void record_keys(const uint64_t *__restrict__ bps64, 
 // record count of every block, and the range's count of each first radix
 // digit.  bctg[LB] is the contig of block LB's first position; every thread
 // steps forward from it (a thread of XU positions crosses few contig starts).
+// Only records whose 5-base bucket lies in [bk_lo, bk_hi) are selected (a
+// shard of the index; [0, 1024) keeps all).
 __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
                                                       const ContigDev *__restrict__ ctg,
                                                       const int32_t *__restrict__ bctg, int nctg,
                                                       int64_t F, const XRange *__restrict__ ranges, int nrange,
                                                       uint32_t *__restrict__ blkcnt, uint8_t *__restrict__ sel,
-                                                      uint32_t *__restrict__ rangehist)
+                                                      uint32_t *__restrict__ rangehist, uint32_t bk_lo, uint32_t bk_hi)
 {
+    const bool shard = bk_lo > 0 || bk_hi < 1024;
     static_assert(XT == 256, "one thread per digit");
     __shared__ uint8_t tl[256], tm[256];
     __shared__ uint64_t red[XT / 64 + 1];
@@ -338,7 +349,12 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
             const uint32_t m = min(min(min(h[u], h[u + 1]), min(h[u + 2], h[u + 3])), h[u + 4]);
             if (h[u] != m && h[u + 4] != m) continue;
             const int64_t j = j0 + u;
-            const bool fwd = j <= (int64_t) c0.clen - 40, rev = j >= 28;
+            bool fwd = j <= (int64_t) c0.clen - 40, rev = j >= 28;
+            if (shard) {
+                const uint32_t bf = bucket_fwd(W0 >> (2 * u)), br = bucket_rev(W0 >> (2 * u));
+                fwd = fwd && bf >= bk_lo && bf < bk_hi;
+                rev = rev && br >= bk_lo && br < bk_hi;
+            }
             cnt += (uint32_t) fwd + (uint32_t) rev;
             bits |= ((fwd ? 1u : 0u) | (rev ? 2u : 0u)) << (2 * u);
             if (fwd) atomicAdd(&hist[digit_fwd(W0 >> (2 * u))], 1u);
@@ -353,7 +369,13 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
             while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
             const ContigDev cc = ctg[c];
             if (f >= cc.pos0 + cc.ncand) continue;         // padding before the next pair
-            cnt += scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, x);
+            scan_position(bps64, cc, f - cc.pos0, 0, tl, tm, false, x);
+            if (shard) {
+                const uint32_t bf = x.bk & 1023u, br = x.bk >> 10;
+                x.fwd = x.fwd && bf >= bk_lo && bf < bk_hi;
+                x.rev = x.rev && br >= bk_lo && br < bk_hi;
+            }
+            cnt += (uint32_t) x.fwd + (uint32_t) x.rev;
             bits |= ((x.fwd ? 1u : 0u) | (x.rev ? 2u : 0u)) << (2 * u);
             if (x.fwd) atomicAdd(&hist[x.dg & 0xFFu], 1u);
             if (x.rev) atomicAdd(&hist[x.dg >> 8], 1u);

@@ -73,6 +73,7 @@ def _setup(L):
     L.pga_destination_map.argtypes = [_i64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pga_merge.argtypes = [C.c_void_p]
     L.pga_run_graph.argtypes = [C.c_void_p]
+    L.pga_set_shard.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.pga_seeds_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(SeedsMeta)]
     L.pga_seeds_export.argtypes = [C.c_void_p, C.c_int, _u8p, _u8p, _i64p, _i64p]
     L.pga_timings_get.argtypes = [C.c_void_p, C.POINTER(Timings)]
@@ -139,6 +140,10 @@ class Context:
             setattr(o, k, int(v))
         self._check(self._L.pga_set_opts(self._h, C.byref(o)), "pga_set_opts")
         self.opts = o
+
+    def set_shard(self, bucket_lo=0, bucket_hi=1024):
+        """Index only records whose 5-base bucket is in [bucket_lo, bucket_hi)."""
+        self._check(self._L.pga_set_shard(self._h, int(bucket_lo), int(bucket_hi)), "pga_set_shard")
 
     # -- input
     def load_batch(self, gdbs):

@@ -132,6 +132,17 @@ int pga_index_info(pga_ctx *ctx, int pair, pga_index_meta *info);
  * (contig, offset, forward-before-reverse) order). */
 int pga_index_export(pga_ctx *ctx, int pair, uint8_t *entries, int64_t *cum);
 
+/* Index only the records whose 5-base bucket (record_bucket, src/dsort.h:30-33:
+ * the first 10 k-mer bits) lies in [bucket_lo, bucket_hi), from the next build
+ * on; [0, 1024) (the default) keeps all.  This is one shard of a multi-GPU
+ * index built from the whole (replicated) `.bps` image: with the ranges of
+ * pga_destination_map, the shards' entries, prefix counts and seed streams,
+ * concatenated in bucket order, are the whole index's (prefix groups never
+ * cross a bucket), except the LCP byte of each shard's first entry, which a
+ * shard computes without its predecessor (0) -- the seam the reference fixes
+ * across ranks (src/dsort.c:507-516). */
+int pga_set_shard(pga_ctx *ctx, int bucket_lo, int bucket_hi);
+
 /* Check pair's index on the device (verify_per_rank_sorted, src/dsort.c:229-246):
  * *bad = entries whose k-mer is smaller than the previous entry's, *first_bad =
  * the first such entry (-1 if none), *lcp_bad = entries whose LCP byte differs

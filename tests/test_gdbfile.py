@@ -114,3 +114,21 @@ def test_gpu_index_and_merge_of_a_gdb_from_disk(tmp_path):
     assert np.array_equal(got["cseeds"], want["cseeds"])
     assert np.array_equal(got["nbuck"], want["nbuck"])
     assert np.array_equal(got["cbuck"], want["cbuck"])
+
+
+@pytest.mark.parametrize("old,new", [("S 18 chr1 test scaffold", "S 9223372036854775000 chr1 test scaffold"),
+                                     ("S 18 chr1 test scaffold", "S 40 chr1 test scaffold"),
+                                     ("S 2 s3", "S 3 s3"),
+                                     ("C 1000", "C 2147483648"),
+                                     ("C 1000", "C 99999999999999999"),
+                                     ("G 50", "G -5")])
+def test_malformed_lines_are_refused(tmp_path, old, new):
+    """Lengths read from the file are bounded before they are used: a scaffold
+    id longer than its line, a contig past INT32_MAX, a negative gap."""
+    g = _toy()
+    gdbfile.write(tmp_path / "toy.gdb", g)
+    text = (tmp_path / "toy.gdb").read_text()
+    assert old in text
+    (tmp_path / "toy.gdb").write_text(text.replace(old, new, 1))
+    with pytest.raises(OSError):
+        gdbfile.read(tmp_path / "toy.gdb")
