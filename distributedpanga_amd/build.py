@@ -61,7 +61,7 @@ def build_driver():
     out = os.path.join(PKG, "pga_drive")
     tmp = out + ".tmp"
     _run(["gcc"] + CC_FLAGS + ["-o", tmp, src, "-L" + PKG, "-lpga_hip", "-lpga_host", "-lm",
-                               "-Wl,-rpath,$ORIGIN"])
+                               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"])
     os.replace(tmp, out)
     return out
 
