@@ -2188,6 +2188,79 @@ int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes, 
     return PGA_OK;
 }
 
+int pga_server_set_target_gix(pga_server *s, const uint8_t *entries, int64_t nrec, int Pb, int Cb, const int64_t *cum,
+                              int64_t maxpre, const int64_t *clen, int ncontig)
+{
+    if (!s) return PGA_EINVAL;
+    s->err[0] = 0;
+    s->have_target = false;
+    for (pga_slot *sl : s->slots) sl->have_result = false;
+    if (nrec < 0 || (nrec > 0 && !entries) || !cum || !clen || ncontig < 1 || maxpre < 0)
+        return sfail(s->err, PGA_EINVAL, "bad GIX target arguments");
+    int64_t maxctg = 0, seqtot = 0;
+    for (int i = 0; i < ncontig; i++) {
+        if (clen[i] < 0) return sfail(s->err, PGA_EINVAL, "contig %d: negative length", i);
+        maxctg = std::max(maxctg, clen[i]);
+        seqtot += clen[i];
+    }
+    if (bytes_for(maxctg) != Pb || bytes_for(2 * (int64_t) ncontig - 1) != Cb)
+        return sfail(s->err, PGA_EINVAL, "the index's P=%d C=%d do not fit %d contigs of at most %lld bases", Pb, Cb,
+                     ncontig, (long long) maxctg);
+    if (cum[NPRE - 1] != nrec) return sfail(s->err, PGA_EINVAL, "prefix counts end at %lld, not %lld entries",
+                                            (long long) cum[NPRE - 1], (long long) nrec);
+    if (bits_for(maxctg) > SERVE_JBITS) return sfail(s->err, PGA_EOVERFLOW, "a target contig is longer than 2^%d bases", SERVE_JBITS);
+    if (ncontig > 32767) return sfail(s->err, PGA_EOVERFLOW, "too many target contigs");
+    pga_ctx *T = s->T;
+    T->err[0] = 0;
+    if (hipSetDevice(s->o.device) != hipSuccess) return sfail(s->err, PGA_EHIP, "hipSetDevice failed");
+    T->loaded = T->built = T->merged = T->ss_ready = false;
+    T->graph_gen = ~0ull;
+    T->npairs = 1;
+    T->hp.assign(1, HostPair());
+    T->hp[0].ncontig = ncontig;
+    T->hp[0].seqtot = seqtot;
+    T->hp[0].maxctg = maxctg;
+    T->nctg = ncontig;
+    T->jbits = SERVE_JBITS;
+    T->R = nrec;
+    T->pairs.assign(1, PairDev());
+    PairDev &d = T->pairs[0];
+    d.rec0 = 0;
+    d.nrec = nrec;
+    d.P = Pb;
+    d.C = Cb;
+    d.e = 9 + Pb + Cb;
+    d.s = 1 + 2 * (Pb + Cb);
+    d.ncontig = ncontig;
+    const size_t n1 = (size_t) std::max<int64_t>(nrec, 1);
+    TRY(from_ctx(s->err, T, ensure(T, T->entries, n1 * d.e + 32)));
+    TRY(from_ctx(s->err, T, ensure(T, T->skeys, n1 * sizeof(Rec))));
+    TRY(from_ctx(s->err, T, ensure(T, T->pre, (n1 + 1) * sizeof(uint32_t))));
+    TRY(from_ctx(s->err, T, ensure(T, T->ls, n1 + 64)));
+    TRY(from_ctx(s->err, T, ensure(T, T->cum, (size_t) NPRE * sizeof(int64_t))));
+    TRY(from_ctx(s->err, T, ensure(T, T->maxpre, sizeof(uint64_t))));
+    TRY(from_ctx(s->err, T, ensure(T, T->dpairs, sizeof(PairDev))));
+    const uint64_t mp = (uint64_t) maxpre;
+    HIPCHK(T, hipMemcpyAsync(T->cum.p, cum, (size_t) NPRE * sizeof(int64_t), hipMemcpyHostToDevice, T->st));
+    HIPCHK(T, hipMemcpyAsync(T->maxpre.p, &mp, sizeof(mp), hipMemcpyHostToDevice, T->st));
+    HIPCHK(T, hipMemcpyAsync(T->dpairs.p, &d, sizeof(PairDev), hipMemcpyHostToDevice, T->st));
+    if (nrec > 0) {
+        HIPCHK(T, hipMemcpyAsync(T->entries.p, entries, (size_t) (nrec * d.e), hipMemcpyHostToDevice, T->st));
+        hipLaunchKernelGGL(k_gix_prefix, dim3(grid_for(T, (const void *) k_gix_prefix, 256, 0, NPRE / 256)), dim3(256), 0,
+                           T->st, (const int64_t *) P<int64_t>(T->cum), P<uint32_t>(T->pre));
+        hipLaunchKernelGGL(k_gix_keys, dim3(grid_for(T, (const void *) k_gix_keys, 256, 0, (nrec + 255) / 256)), dim3(256),
+                           0, T->st, (const uint8_t *) P<uint8_t>(T->entries), d.e, Pb, Cb,
+                           (const uint32_t *) P<uint32_t>(T->pre), nrec, T->jbits, P<Rec>(T->skeys), P<uint8_t>(T->ls));
+        if (hipGetLastError() != hipSuccess) return sfail(s->err, PGA_EHIP, "kernel launch failed");
+    }
+    if (hipStreamSynchronize(T->st) != hipSuccess) return sfail(s->err, PGA_EHIP, "target load failed");
+    T->built = true;
+    s->nT = ncontig;
+    s->maxctgT = maxctg;
+    s->have_target = true;
+    return PGA_OK;
+}
+
 int pga_server_target_info(pga_server *s, pga_index_meta *info)
 {
     if (!s) return PGA_EINVAL;

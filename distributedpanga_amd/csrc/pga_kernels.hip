@@ -2135,6 +2135,43 @@ __global__ __launch_bounds__(64) void k_graph_seeds(const int64_t *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------- an index from GIX entries
+
+// pre[i] = the 24-bit prefix of entry i, from the cumulative prefix counts
+// (entries [cum[p-1], cum[p]) have prefix p).
+__global__ void k_gix_prefix(const int64_t *__restrict__ cum, uint32_t *__restrict__ pre)
+{
+    for (int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; p < (1 << 24);
+         p += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t lo = p ? cum[p - 1] : 0, hi = cum[p];
+        for (int64_t i = lo; i < hi; i++) pre[i] = (uint32_t) p;
+    }
+}
+
+// The sorted 16-byte keys of GIX entries (the layout k_tile_sort writes): the
+// k-mer from the prefix and the 7 suffix bytes, the offset j = pos - 12 strand,
+// the contig and the strand from the payload; and each entry's LCP/strand byte.
+__global__ void k_gix_keys(const uint8_t *__restrict__ ent, int e, int P, int C, const uint32_t *__restrict__ pre,
+                           int64_t n, int jbits, Rec *__restrict__ keys, uint8_t *__restrict__ ls)
+{
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const uint8_t *d = ent + i * e;
+        uint64_t suf = 0;                          // bases 12..39, MSB-first (56 bits)
+        for (int k = 0; k < 7; k++) suf = (suf << 8) | d[k];
+        uint64_t pos = 0, cs = 0;
+        for (int k = 0; k < P; k++) pos |= (uint64_t) d[9 + k] << (8 * k);
+        for (int k = 0; k < C; k++) cs |= (uint64_t) d[9 + P + k] << (8 * k);
+        const uint64_t strand = cs >> (8 * C - 1);
+        const uint64_t contig = cs & ((1ull << (8 * C - 1)) - 1);
+        const uint64_t j = pos - 12 * strand;
+        Rec r;
+        r.w0 = ((uint64_t) pre[i] << 40) | (suf >> 16);
+        r.w1 = ((suf & 0xFFFFull) << 48) | (contig << (jbits + 1)) | (j << 1) | strand;
+        keys[i] = r;
+        ls[i] = (uint8_t) ((d[8] & 63) | (strand << 7));
+    }
+}
+
 // ---------------------------------------------------------------- checks
 
 // 80-bit k-mer of GIX entry i: 24-bit prefix, then the entry's 7 suffix bytes.

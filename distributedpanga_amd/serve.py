@@ -32,6 +32,8 @@ def _setup(L):
     L.pga_server_slots.argtypes = [C.c_void_p]
     L.pga_server_error.argtypes = [C.c_void_p, C.c_int]
     L.pga_server_error.restype = C.c_char_p
+    L.pga_server_set_target_gix.argtypes = [C.c_void_p, _u8p, C.c_int64, C.c_int, C.c_int, _i64p, C.c_int64, _i64p,
+                                            C.c_int]
     L.pga_server_set_target.argtypes = [C.c_void_p, _u8p, C.c_int64, _i64p, _i64p, C.c_int]
     L.pga_server_target_info.argtypes = [C.c_void_p, C.POINTER(IndexMeta)]
     L.pga_server_footprint.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
@@ -74,11 +76,45 @@ class Server:
         self._h = h
         self.freq = freq
         self.slots = int(slots)
+        self.n_target = 0
+        if target is not None:
+            self.set_target(target)
+
+    def set_target(self, target):
+        """Index `target` (a GDB) on the device; it stays resident."""
         self.n_target = target.ncontig
         bps, boff, clen = _gdb_arrays([target])
         self._check(self._L.pga_server_set_target(self._h, bps[0].ctypes.data_as(_u8p), int(target.bps.size),
                                                   boff[0].ctypes.data_as(_i64p), clen[0].ctypes.data_as(_i64p),
                                                   target.ncontig), "pga_server_set_target", -1)
+
+    def set_target_gix(self, index, clen):
+        """The target from a persisted GIX index (gix.read's GixIndex: entries,
+        prefix counts, P, C, maxpre) and its contig lengths, without its
+        sequence: a restart of the serving loop without re-indexing."""
+        ent = np.ascontiguousarray(index.entries, np.uint8).reshape(-1)
+        if ent.size == 0:
+            ent = np.zeros(1, np.uint8)
+        cum = np.ascontiguousarray(index.cum, np.int64)
+        cl = np.ascontiguousarray(clen, np.int64)
+        nrec = int(index.entries.shape[0])
+        self.n_target = int(cl.size)
+        self._check(self._L.pga_server_set_target_gix(self._h, ent.ctypes.data_as(_u8p), nrec, int(index.P),
+                                                      int(index.C), cum.ctypes.data_as(_i64p), int(index.maxpre),
+                                                      cl.ctypes.data_as(_i64p), int(cl.size)),
+                    "pga_server_set_target_gix", -1)
+
+    @classmethod
+    def from_gix(cls, directory, root, clen, **kw):
+        """A server whose target is the GIX index <directory>/<root>.gix (+ .ktab parts)."""
+        from . import gix
+        srv = cls(None, **kw)
+        try:
+            srv.set_target_gix(gix.read(directory, root), clen)
+        except Exception:
+            srv.close()
+            raise
+        return srv
 
     def _check(self, rc, what, slot):
         if rc:

@@ -276,6 +276,15 @@ const char *pga_server_error(const pga_server *s, int slot);
 /* Index the target (same arguments as pga_load_gdb); it stays resident. */
 int pga_server_set_target(pga_server *s, const uint8_t *bps, int64_t bps_bytes,
                           const int64_t *boff, const int64_t *clen, int ncontig);
+/* The target from a persisted GIX index instead of its sequence (a restart of
+ * the serving loop without re-indexing): nrec entries of 9+P+C bytes in the
+ * adapter layout, the cumulative 2^24 prefix counts and the largest group (the
+ * .ktab / .gix contents, notes/GIXmake_dump.c:1632-1709, as pga_gix_read_*
+ * return them), and the target's contig lengths (P and C must be the widths
+ * those lengths give).  Answers equal those of a target set from its
+ * sequence. */
+int pga_server_set_target_gix(pga_server *s, const uint8_t *entries, int64_t nrec, int P, int C,
+                              const int64_t *cum, int64_t maxpre, const int64_t *clen, int ncontig);
 int pga_server_target_info(pga_server *s, pga_index_meta *info);
 
 /* Device bytes a query of query_bases bases may take while it is answered

@@ -214,3 +214,43 @@ def test_serve_loop_overlapping_slots_match_alone():
             want = _expected(target, q, 10)
             _same(answers[i], want, f"request {i} in the loop")
             _same(srv.answer([q], slot=i % 3)[0], want, f"request {i} alone")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("freq", [10, 24])
+def test_server_target_from_persisted_gix(tmp_path, freq):
+    """A serving target restarted from its GIX files (pga_server_set_target_gix):
+    every query's answer equals the one from the target indexed from its
+    sequence, and the oracle's."""
+    from distributedpanga_amd import gix, pga
+    target = synth.workload(1, 0.1)
+    qs = _queries(target)
+    with pga.Context() as ctx:
+        ctx.load_gdb(target)
+        ctx.build_index()
+        gix.write_from_context(ctx, tmp_path, "target", nparts=2)
+    with serve.Server(target, freq=freq) as srv:
+        want = srv.answer(qs)
+        tinfo = srv.target_info()
+    with serve.Server.from_gix(tmp_path, "target", target.clen, freq=freq) as srv2:
+        info2 = srv2.target_info()
+        assert info2["nrec"] == tinfo["nrec"] and info2["maxpre"] == tinfo["maxpre"]
+        assert (info2["P"], info2["C"]) == (tinfo["P"], tinfo["C"])
+        got = srv2.answer(qs)
+        for i, q in enumerate(qs):
+            _same(got[i], want[i], f"query {i} from the GIX target")
+            _same(got[i], _expected(target, q, freq), f"query {i} against the oracle")
+
+
+@pytest.mark.gpu
+def test_server_gix_target_rejects_mismatched_contigs(tmp_path):
+    from distributedpanga_amd import gix, pga
+    target = synth.workload(1, 0.05)
+    with pga.Context() as ctx:
+        ctx.load_gdb(target)
+        ctx.build_index()
+        gix.write_from_context(ctx, tmp_path, "t")
+    idx = gix.read(tmp_path, "t")
+    with serve.Server(None) as srv:
+        with pytest.raises(pga.PgaError, match="do not fit"):
+            srv.set_target_gix(idx, np.full(target.ncontig, 1 << 26, np.int64))   # P would be 4, not 3
