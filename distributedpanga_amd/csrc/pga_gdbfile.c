@@ -21,14 +21,19 @@ static const char *const SCHEMA_LINES[] = {
     "~ D M 1 8 INT_LIST                    mask pair list for a contig",
 };
 
-/* ".../<root>.gdb" -> ".../.<root>.bps" (NULL unless path ends in ".gdb"). */
+/* ".../<root>.gdb" or ".../<root>.1gdb" -> ".../.<root>.bps" (NULL for any
+ * other name). */
 static char *bps_path(const char *path)
 {
     const size_t n = strlen(path);
-    if (n < 5 || strcmp(path + n - 4, ".gdb") != 0) return NULL;
+    size_t ext;
+    if (n >= 6 && strcmp(path + n - 5, ".1gdb") == 0) ext = 5;
+    else if (n >= 5 && strcmp(path + n - 4, ".gdb") == 0) ext = 4;
+    else return NULL;
     const char *slash = strrchr(path, '/');
     const size_t dir = slash ? (size_t) (slash - path) + 1 : 0;
-    const size_t root = n - 4 - dir;
+    if (n - ext <= dir) return NULL;
+    const size_t root = n - ext - dir;
     char *out = (char *) malloc(dir + root + 6);
     if (!out) return NULL;
     memcpy(out, path, dir);
@@ -102,6 +107,165 @@ void pga_gdbfile_free(pga_gdbfile *g)
     memset(g, 0, sizeof(*g));
 }
 
+/* Reader state: the arrays' capacities and the running scaffold position and
+ * .bps offset. */
+typedef struct {
+    pga_gdbfile *g;
+    size_t cs, cf, ch, cb, cc, co, ct;
+    int64_t spos, boff;
+} rd_state;
+
+/* A scaffold line with id[0..n). */
+static int rd_scaffold(rd_state *r, const char *id, int64_t n)
+{
+    pga_gdbfile *g = r->g;
+    if (g->nscaff > 0) g->slen[g->nscaff - 1] = r->spos;
+    if (grow_i64(&g->slen, &r->cs, (size_t) g->nscaff + 1) || grow_int(&g->fctg, &r->cf, (size_t) g->nscaff + 2) ||
+        grow_i64(&g->hoff, &r->ch, (size_t) g->nscaff + 1) ||
+        grow_chr(&g->headers, &r->ct, (size_t) (g->hdrtot + n + 1)))
+        return -1;
+    g->fctg[g->nscaff] = g->ncontig;
+    g->hoff[g->nscaff] = g->hdrtot;
+    memcpy(g->headers + g->hdrtot, id, (size_t) n);
+    g->hdrtot += n;
+    g->headers[g->hdrtot++] = '\0';
+    g->nscaff++;
+    r->spos = 0;
+    return 0;
+}
+
+/* A gap ('G') or contig ('C') line of length v inside the current scaffold.
+ * Lengths past INT32_MAX are refused (the device loader takes int32 contigs). */
+static int rd_piece(rd_state *r, char t, int64_t v)
+{
+    pga_gdbfile *g = r->g;
+    if (g->nscaff == 0 || v < 0 || v > INT32_MAX) return -3;
+    if (t == 'C') {
+        if (grow_i64(&g->sbeg, &r->cb, (size_t) g->ncontig + 1) || grow_i64(&g->clen, &r->cc, (size_t) g->ncontig + 1) ||
+            grow_i64(&g->boff, &r->co, (size_t) g->ncontig + 2))
+            return -1;
+        g->sbeg[g->ncontig] = r->spos;
+        g->clen[g->ncontig] = v;
+        g->boff[g->ncontig] = r->boff;
+        g->ncontig++;
+        g->seqtot += v;
+        if (v > g->maxctg) g->maxctg = v;
+        r->boff += (v + 3) >> 2;
+    }
+    r->spos += v;
+    return 0;
+}
+
+/* One integer in ONE's binary encoding (lib/ONElib.c:3730-3830): a first byte
+ * 01xxxxxx holds 0..63 (11xxxxxx: -64..-1), 001xxxxx and one more byte hold
+ * 0..8191, 0000 0kkk (1000 0kkk: negative) is followed by the value's k+1
+ * low-order bytes, little-endian. */
+static int one_int(const unsigned char **pp, const unsigned char *end, int64_t *v)
+{
+    const unsigned char *p = *pp;
+    if (p >= end) return -3;
+    const unsigned u0 = p[0];
+    if (u0 & 0x40) {
+        *v = (u0 & 0x80) ? (int64_t) (int8_t) u0 : (int64_t) (u0 & 0x3f);
+        *pp = p + 1;
+        return 0;
+    }
+    if (u0 & 0x20) {
+        if (u0 & 0x80 || end - p < 2) return -3;
+        *v = ((int64_t) (u0 & 0x1f) << 8) | p[1];
+        *pp = p + 2;
+        return 0;
+    }
+    const int k = (int) (u0 & 0x07);
+    if (k == 0 || (u0 & 0x78) || end - p < k + 2) return -3;
+    uint64_t x = 0;
+    for (int b = 0; b <= k; b++) x |= (uint64_t) p[1 + b] << (8 * b);
+    if ((u0 & 0x80) && k < 7) x |= ~(uint64_t) 0 << (8 * (k + 1));
+    *v = (int64_t) x;
+    *pp = p + 2 + k;
+    return 0;
+}
+
+/* The binary records of a .1gdb file (after its "$ 0" header line) up to the
+ * blank record that ends them: each starts with a byte 1ccccccz (c: the line
+ * type, A-Z as 0-25 and a-z as 26-51; z: its list is codec-compressed), then
+ * the fields (a REAL as 8 bytes, an INT or a list length as one_int), then the
+ * list (a STRING's bytes; an INT_LIST's first element, element width and
+ * packed rest).  Compressed mask lists are skipped; compressed scaffold ids
+ * need ONE's codecs and are refused (-2). */
+static int rd_binary(rd_state *r, const unsigned char *p, const unsigned char *end)
+{
+    pga_gdbfile *g = r->g;
+    while (p < end) {
+        const unsigned x = *p++;
+        if (x == '\n') return 0;                  /* end of the records; the footer follows */
+        if (!(x & 0x80)) return -3;
+        const int code = (int) ((x & 0x7f) >> 1), comp = (int) (x & 1);
+        const char t = code < 26 ? (char) ('A' + code) : code < 52 ? (char) ('a' + code - 26) : code == 54 ? '/' : 0;
+        int64_t v, n;
+        int rc;
+        switch (t) {
+        case 'f':
+            if (end - p < 32) return -3;
+            for (int i = 0; i < 4; i++) {
+                double d;
+                memcpy(&d, p + 8 * i, 8);
+                g->freq[i] = (float) d;
+            }
+            p += 32;
+            break;
+        case 'u':
+            break;
+        case 'S':
+            if (one_int(&p, end, &n) || n < 0) return -3;
+            if (comp) return -2;
+            if (n > end - p) return -3;
+            if ((rc = rd_scaffold(r, (const char *) p, n))) return rc;
+            p += n;
+            break;
+        case 'G':
+        case 'C':
+            if (one_int(&p, end, &v)) return -3;
+            if ((rc = rd_piece(r, t, v))) return rc;
+            break;
+        case 'M':                                  /* mask intervals: not used, skipped */
+            if (one_int(&p, end, &n) || n < 0) return -3;
+            if (n == 0) break;
+            if (one_int(&p, end, &v)) return -3;
+            if (n == 1) break;
+            if (p >= end) return -3;
+            {
+                const int64_t width = *p++;
+                int64_t skip;
+                if (comp) {
+                    int64_t bits;
+                    if (one_int(&p, end, &bits) || bits < 0) return -3;
+                    skip = (bits + 7) >> 3;
+                } else {
+                    if (width < 1 || width > 8 || n - 1 > (end - p) / width) return -3;
+                    skip = (n - 1) * width;
+                }
+                if (skip > end - p) return -3;
+                p += skip;
+            }
+            break;
+        case '/':                                  /* a comment */
+            if (one_int(&p, end, &n) || n < 0) return -3;
+            if (comp) {
+                int64_t bits;
+                if (one_int(&p, end, &bits) || bits < 0) return -3;
+                n = (bits + 7) >> 3;
+            }
+            if (n > end - p) return -3;
+            p += n;
+            break;
+        default:
+            return -3;
+        }
+    }
+    return 0;
+}
+
 int pga_gdbfile_read(const char *path, pga_gdbfile *g)
 {
     if (!path || !g) return -3;
@@ -111,8 +275,9 @@ int pga_gdbfile_read(const char *path, pga_gdbfile *g)
     if (slurp(path, &buf, &len)) return -1;
     const char *p = buf, *end = buf + len;
     int rc = -3;
-    size_t cs = 0, cf = 0, ch = 0, cb = 0, cc = 0, co = 0, ct = 0;   /* capacities */
-    int64_t spos = 0, boff = 0;
+    rd_state st;
+    memset(&st, 0, sizeof(st));
+    st.g = g;
     int first = 1;
     while (p < end) {
         const char *eol = memchr(p, '\n', (size_t) (end - p));
@@ -125,40 +290,24 @@ int pga_gdbfile_read(const char *path, pga_gdbfile *g)
             if (t != '1' || get_int(&q, eol, &n) || n != 3 || q + 4 > eol || memcmp(q, " gdb", 4) != 0) goto done;
             first = 0;
         } else if (t == '$') {
-            rc = -2;                               /* the binary lines follow */
-            goto done;
+            /* "$ <big-endian>": the binary records follow (little-endian only) */
+            int64_t big;
+            if (get_int(&q, eol, &big) || big != 0 || eol >= end) goto done;
+            const int b = rd_binary(&st, (const unsigned char *) eol + 1, (const unsigned char *) end);
+            if (b) { rc = b; goto done; }
+            break;                                 /* the footer needs no reading */
         } else if (t == 'S') {
             int64_t n;
             /* the bound is checked before any pointer arithmetic with n */
             if (get_int(&q, eol, &n) || n < 0 || n > (int64_t) (eol - q) - 1 || *q != ' ') goto done;
             q++;
-            if (g->nscaff > 0) g->slen[g->nscaff - 1] = spos;
-            if (grow_i64(&g->slen, &cs, (size_t) g->nscaff + 1) || grow_int(&g->fctg, &cf, (size_t) g->nscaff + 2) ||
-                grow_i64(&g->hoff, &ch, (size_t) g->nscaff + 1) ||
-                grow_chr(&g->headers, &ct, (size_t) (g->hdrtot + n + 1))) { rc = -1; goto done; }
-            g->fctg[g->nscaff] = g->ncontig;
-            g->hoff[g->nscaff] = g->hdrtot;
-            memcpy(g->headers + g->hdrtot, q, (size_t) n);
-            g->hdrtot += n;
-            g->headers[g->hdrtot++] = '\0';
-            g->nscaff++;
-            spos = 0;
+            const int r2 = rd_scaffold(&st, q, n);
+            if (r2) { rc = r2; goto done; }
         } else if (t == 'G' || t == 'C') {
             int64_t v;
-            /* lengths past INT32_MAX are refused (the device loader takes int32 contigs) */
-            if (g->nscaff == 0 || get_int(&q, eol, &v) || v < 0 || v > INT32_MAX) goto done;
-            if (t == 'C') {
-                if (grow_i64(&g->sbeg, &cb, (size_t) g->ncontig + 1) || grow_i64(&g->clen, &cc, (size_t) g->ncontig + 1) ||
-                    grow_i64(&g->boff, &co, (size_t) g->ncontig + 2)) { rc = -1; goto done; }
-                g->sbeg[g->ncontig] = spos;
-                g->clen[g->ncontig] = v;
-                g->boff[g->ncontig] = boff;
-                g->ncontig++;
-                g->seqtot += v;
-                if (v > g->maxctg) g->maxctg = v;
-                boff += (v + 3) >> 2;
-            }
-            spos += v;
+            if (get_int(&q, eol, &v)) goto done;
+            const int r2 = rd_piece(&st, t, v);
+            if (r2) { rc = r2; goto done; }
         } else if (t == 'f') {
             for (int i = 0; i < 4; i++) {
                 while (q < eol && *q == ' ') q++;
@@ -177,13 +326,13 @@ int pga_gdbfile_read(const char *path, pga_gdbfile *g)
         p = eol + 1;
     }
     if (first) goto done;
-    if (g->nscaff > 0) g->slen[g->nscaff - 1] = spos;
-    if (grow_int(&g->fctg, &cf, (size_t) g->nscaff + 1) || grow_i64(&g->boff, &co, (size_t) g->ncontig + 1)) {
+    if (g->nscaff > 0) g->slen[g->nscaff - 1] = st.spos;
+    if (grow_int(&g->fctg, &st.cf, (size_t) g->nscaff + 1) || grow_i64(&g->boff, &st.co, (size_t) g->ncontig + 1)) {
         rc = -1;
         goto done;
     }
     g->fctg[g->nscaff] = g->ncontig;
-    g->boff[g->ncontig] = boff;
+    g->boff[g->ncontig] = st.boff;
     rc = 0;
 done:
     free(buf);

@@ -22,7 +22,7 @@ class _File(C.Structure):
                 ("sbeg", C.POINTER(C.c_int64)), ("clen", C.POINTER(C.c_int64)), ("boff", C.POINTER(C.c_int64))]
 
 
-_ERR = {-1: "I/O error", -2: "binary ONE file (only the .gdb text form is supported)",
+_ERR = {-1: "I/O error", -2: "binary ONE file with codec-compressed scaffold ids (not decoded)",
         -3: "bad argument or file contents"}
 
 

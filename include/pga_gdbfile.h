@@ -16,11 +16,15 @@
  *                   starting on a byte: contig c occupies (clen[c]+3)/4 bytes
  *                   from boff[c]
  *
- * The same skeleton saved as `<root>.1gdb` is a binary ONE file; reading one
- * returns -2 (only the text form is supported).  Host C; every call returns 0,
- * or -1 (I/O or allocation, errno set), -2 (binary ONE file) or -3 (bad
- * argument or file contents).  The tables feed pga_load_gdb (libpga_hip.h)
- * directly: the .bps bytes, boff and clen.
+ * The same skeleton saved as `<root>.1gdb` is a binary ONE file (the same
+ * header, a "$ 0" line, then one binary record per data line; lib/ONElib.c):
+ * pga_gdbfile_read reads it too, with its `.<root>.bps`.  Binary mask lists
+ * are skipped as their text form is; scaffold ids that ONE compressed with a
+ * trained codec (it starts after 100 KB of ids) are not decoded and the read
+ * returns -2.  Host C; every call returns 0, or -1 (I/O or allocation, errno
+ * set), -2 (codec-compressed ids) or -3 (bad argument or file contents).  The
+ * tables feed pga_load_gdb (libpga_hip.h) directly: the .bps bytes, boff and
+ * clen.
  */
 #ifndef PGA_GDBFILE_H
 #define PGA_GDBFILE_H
@@ -46,7 +50,8 @@ typedef struct {
     int64_t *boff;       /* [ncontig + 1] byte offset in the .bps file    */
 } pga_gdbfile;
 
-/* Read the skeleton of path (".../<root>.gdb") into g (arrays malloc'd). */
+/* Read the skeleton of path (".../<root>.gdb" or ".../<root>.1gdb") into g
+ * (arrays malloc'd). */
 int pga_gdbfile_read(const char *path, pga_gdbfile *g);
 
 /* Read the .bps file next to path (".../.<root>.bps") into bps, which holds

@@ -72,13 +72,69 @@ def test_we_read_the_references(tmp_path):
 
 
 @needs_ref
-def test_binary_one_file_is_refused(tmp_path):
+def test_we_read_the_references_binary_file(tmp_path):
+    """The reference's Write_GDB saving `<root>.1gdb` (ONE binary); the package
+    reads the skeleton and the .bps next to it."""
     g = _toy()
     (tmp_path / ".bin.bps").write_bytes(g.bps.tobytes())
     oracle.ref_gdb_write(tmp_path / "bin.1gdb", g.ids, g.slen, g.fctg, g.sbeg, g.clen, g.freq)
-    (tmp_path / "bin.gdb").write_bytes((tmp_path / "bin.1gdb").read_bytes())
-    with pytest.raises(OSError, match="binary ONE"):
-        gdbfile.read(tmp_path / "bin.gdb")
+    _same(gdbfile.read(tmp_path / "bin.1gdb"), g)
+
+
+def _big(nscaff, ids_len, seed):
+    """Scaffolds with many contigs of lengths and gaps over every integer width
+    of the binary encoding (1, 2, 3, 4 bytes)."""
+    rng = np.random.default_rng(seed)
+    ids, slen, fctg, sbeg, clen = [], [], [0], [], []
+    for s in range(nscaff):
+        ids.append(f"scaffold_{s}_" + "x" * int(rng.integers(0, ids_len)))
+        pos = int(rng.choice([0, 5, 100, 9000, 70000]))
+        for c in range(int(rng.integers(1, 6))):
+            n = int(rng.choice([1, 50, 63, 64, 8191, 8192, 65535, 65536, 300000, 2 ** 24 + 3]))
+            sbeg.append(pos)
+            clen.append(n)
+            pos += n + int(rng.choice([1, 63, 64, 8192, 100000]))
+        slen.append(pos + int(rng.integers(0, 500)))
+        fctg.append(len(clen))
+    clen = np.array(clen, np.int64)
+    boff = np.concatenate([[0], np.cumsum((clen + 3) // 4)]).astype(np.int64)
+    return gdbfile.GdbGenome(ids=ids, slen=np.array(slen, np.int64), fctg=np.array(fctg, np.int32),
+                             sbeg=np.array(sbeg, np.int64), clen=clen, boff=boff,
+                             bps=np.zeros(int(boff[-1]), np.uint8), freq=(0.31, 0.19, 0.21, 0.29))
+
+
+@needs_ref
+def test_binary_file_with_wide_integers(tmp_path):
+    g = _big(40, 30, 8)
+    (tmp_path / ".big.bps").write_bytes(g.bps.tobytes())
+    oracle.ref_gdb_write(tmp_path / "big.1gdb", g.ids, g.slen, g.fctg, g.sbeg, g.clen, g.freq)
+    got = gdbfile.read(tmp_path / "big.1gdb", with_bases=False)
+    _same(got, g, bases=False)
+    # and the reference's own reader agrees on the same binary file
+    r = oracle.ref_gdb_read(tmp_path / "big.1gdb", cap=1 << 12)
+    assert r["ids"] == got.ids and np.array_equal(r["clen"], got.clen) and np.array_equal(r["sbeg"], got.sbeg)
+
+
+@needs_ref
+def test_binary_file_with_codec_compressed_ids_is_refused(tmp_path):
+    """Past 100 KB of scaffold ids ONE compresses them with a trained codec,
+    which this reader does not decode: a clear error, not wrong ids."""
+    g = _big(1500, 120, 9)
+    (tmp_path / ".many.bps").write_bytes(g.bps.tobytes())
+    oracle.ref_gdb_write(tmp_path / "many.1gdb", g.ids, g.slen, g.fctg, g.sbeg, g.clen, g.freq)
+    with pytest.raises(OSError, match="codec-compressed"):
+        gdbfile.read(tmp_path / "many.1gdb", with_bases=False)
+
+
+def test_binary_fixture_written_by_the_reference():
+    """tests/golden/ref_pair.1gdb: a 2 Mbp-pair-shaped genome at 1/50 saved by
+    the reference's Write_GDB as ONE binary (tests/golden/make_gdb1.py)."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = gdbfile.read(os.path.join(here, "ref_pair.1gdb"))
+    z = np.load(os.path.join(here, "ref_pair_1gdb.npz"))
+    assert np.array_equal(g.clen, z["clen"]) and np.array_equal(g.boff, z["boff"])
+    assert np.array_equal(g.bps, z["bps"])
 
 
 @pytest.mark.parametrize("text", [
@@ -114,6 +170,20 @@ def test_gpu_index_and_merge_of_a_gdb_from_disk(tmp_path):
     assert np.array_equal(got["cseeds"], want["cseeds"])
     assert np.array_equal(got["nbuck"], want["nbuck"])
     assert np.array_equal(got["cbuck"], want["cbuck"])
+
+
+@pytest.mark.gpu
+def test_gpu_index_and_merge_of_a_binary_gdb_written_by_the_reference():
+    """The reference's own binary `.1gdb` (tests/golden/ref_pair.1gdb) read from
+    disk, indexed and merged on the device, against the oracle."""
+    import os
+    h = gdbfile.read(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_pair.1gdb"))
+    print(f"[gdbfile] binary: ncontig={h.ncontig} bases={h.seqtot}", flush=True)
+    got = pga.run_pair(h)
+    want = oracle.pipeline(h.bps, h.boff[:-1], h.clen, freq=10)
+    assert want["nrec"] > 0
+    for k in ("entries", "cum", "nseeds", "cseeds", "nbuck", "cbuck"):
+        assert np.array_equal(got[k], want[k]), k
 
 
 @pytest.mark.parametrize("old,new", [("S 18 chr1 test scaffold", "S 9223372036854775000 chr1 test scaffold"),
