@@ -1989,8 +1989,7 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
             const uint64_t si = (pi >> sbit) & 1;
             const uint64_t hd = (uint64_t) (v[k] & 63u) | ((pi & clear) << 8);
             const int qa = gi - (int) ((v[k] >> 6) & 63u), qb = gi + (int) ((v[k] >> 12) & 63u);
-            auto one = [&](int q) {
-                const uint64_t pq = pay[q];
+            auto one = [&](uint64_t pq) {
                 const bool same = ((pq >> sbit) & 1) == si;
                 int idx = same ? kn : kc;
                 kn += same;
@@ -2012,8 +2011,19 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
                     __builtin_memcpy(dst + s - 4, &x1, 4);
                 }
             };
-            for (int q = qa; q < gi; q++) one(q);
-            for (int q = gi + 1; q < qb; q++) one(q);
+            // partners qa..qb-1 but the anchor, each payload read one partner
+            // ahead so the read's latency overlaps the previous seed's stores
+            int q = qa == gi ? gi + 1 : qa;
+            if (q >= qb) return;
+            uint64_t nxt = pay[q];
+            while (true) {
+                const uint64_t pq = nxt;
+                q += q + 1 == gi ? 2 : 1;
+                const bool more = q < qb;
+                if (more) nxt = pay[q];
+                one(pq);
+                if (!more) break;
+            }
         };
         for (int r = 0; r < rounds; r++) {
             const int n0 = r * capN, n1 = min(n0 + capN, totN);
