@@ -180,6 +180,37 @@ def test_server_low_complexity_groups():
 
 
 @pytest.mark.gpu
+def test_server_many_contigs():
+    """Target plus query past 1024 contigs: per-contig seed counts of contigs
+    beyond the join's LDS counters go through the global-atomic branch."""
+    rng = np.random.default_rng(17)
+    tcont = [rng.integers(0, 4, int(n)).astype(np.uint8) for n in rng.integers(300, 1500, 900)]
+    target = synth.from_codes(tcont)
+
+    def pieces(n, seed):
+        r = np.random.default_rng(seed)
+        out = []
+        for _ in range(n):
+            c = tcont[int(r.integers(0, len(tcont)))]
+            a = int(r.integers(0, c.size // 2))
+            p = c[a:a + int(r.integers(150, c.size - a + 1))].copy()
+            flip = r.random(p.size) < 0.02
+            p[flip] = r.integers(0, 4, int(flip.sum()))
+            out.append(p if r.random() < 0.5 else (3 - p[::-1]).astype(np.uint8))
+        return synth.from_codes(out)
+
+    qs = [pieces(700, 1), pieces(300, 2)]
+    assert target.ncontig + qs[0].ncontig > 1024
+    with serve.Server(target) as srv:
+        got = srv.answer(qs)
+        for i, q in enumerate(qs):
+            print(f"[serve-many] query {i} contigs={q.ncontig}", flush=True)
+            want = _expected(target, q, 10)
+            assert want["nbuck"][1024:].sum() > 0
+            _same(got[i], want, f"query {i}")
+
+
+@pytest.mark.gpu
 def test_server_many_batches_reuse():
     target = synth.workload(1, 0.05)
     qs = [synth.query(target, 200 + i, 10_000 + 5_000 * i, 0.02 * i, seg_min=1_000, seg_max=6_000)
