@@ -1788,18 +1788,39 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
                                                      const int64_t *__restrict__ dn)
 {
     constexpr int TA = MTH * A;
+    constexpr int NW = TA + 2 * MAX_FREQ + 4;
+    constexpr int WI = (NW + MTH - 1) / MTH;
     if (dn) nlog = *dn;
-    __shared__ uint8_t win[TA + 2 * MAX_FREQ + 4];
+    __shared__ uint8_t win[NW];
     __shared__ uint64_t red[MTH / 64 + 1];
-    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
-    const int p = merge_block_pair(pairs, npairs, LB);
-    const PairDev pd = pairs[p];
-    const int64_t lo = pd.rec0 + (LB - pd.mblk0) * (int64_t) TA;
-    const int64_t end = pd.rec0 + pd.nrec;
     const bool swar = freq <= 16;
     const int H = swar ? 17 : freq + 1;          // window halo on each side
-    load_win<TA + 2 * MAX_FREQ + 4>(win, ls, lo - H, TA + 2 * H, pd.rec0, end);
+    // A tile's window bytes (its LCP/strand bytes [lo - H, lo + TA + H), 0
+    // outside the pair) are loaded into registers one tile ahead, so their
+    // latency overlaps the previous tile's evaluation.
+    uint8_t x[WI];
+    auto fetch = [&](int64_t lb, int64_t &lo, int64_t &end) {
+        const PairDev &pd = pairs[merge_block_pair(pairs, npairs, lb)];
+        lo = pd.rec0 + (lb - pd.mblk0) * (int64_t) TA;
+        end = pd.rec0 + pd.nrec;
+#pragma unroll
+        for (int j = 0; j < WI; j++) {
+            const int k = (int) threadIdx.x + j * MTH;
+            const int64_t g = lo - H + k;
+            x[j] = k < TA + 2 * H && g >= pd.rec0 && g < end ? ls[g] : 0;
+        }
+    };
+    int64_t nlo = 0, nend = 0;
+    if ((int64_t) blockIdx.x < nlog) fetch(blockIdx.x, nlo, nend);
+    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
+    const int64_t lo = nlo, end = nend;
+#pragma unroll
+    for (int j = 0; j < WI; j++) {
+        const int k = (int) threadIdx.x + j * MTH;
+        if (k < TA + 2 * H) win[k] = x[j];
+    }
     __syncthreads();
+    if (LB + gridDim.x < nlog) fetch(LB + gridDim.x, nlo, nend);
     uint32_t nN = 0, nC = 0;
     // the thread's anchors, with the search chosen once per tile (loop-free
     // for FREQ <= 16)
@@ -1870,12 +1891,15 @@ __device__ __forceinline__ void copy_run(uint8_t *__restrict__ g, const uint8_t 
     const int nchunks = (end + 15) >> 4;
     for (int c = threadIdx.x; c < nchunks; c += MTH) {
         const int o = 16 * c;
+        const uint4 w = *reinterpret_cast<const uint4 *>(ob + o);
         if (o >= lead && o + 16 <= end) {
-            *reinterpret_cast<uint4 *>(a0 + o) = *reinterpret_cast<const uint4 *>(ob + o);
+            *reinterpret_cast<uint4 *>(a0 + o) = w;
         } else {
+            // an end chunk: its bytes inside the run, from the one LDS read
+            const uint32_t q[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
             for (int b = 0; b < 16; b++)
-                if (o + b >= lead && o + b < end) a0[o + b] = ob[o + b];
+                if (o + b >= lead && o + b < end) a0[o + b] = (uint8_t) (q[b >> 2] >> (8 * (b & 3)));
         }
     }
 }
