@@ -1402,8 +1402,8 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         }
     }
     __syncthreads();
-    // Each record's GIX entry (src/kmer_adapter.c:55-62 layout) replaces it in
-    // its own 16-byte slot: k-mer bytes 3..9, mask 0, LCP, pos LE, contig|strand LE.
+    // Each record's GIX entry (src/kmer_adapter.c:55-62 layout): k-mer bytes
+    // 3..9, mask 0, LCP, pos LE, contig|strand LE, stored at its place.
     const int e = pd.e, P = pd.P, C = pd.C;
     const uint64_t jmask = (1ull << jbits) - 1ull;
 #pragma unroll
@@ -1425,11 +1425,14 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             ls[tl.start + idx] = (uint8_t) (lcp | (strand << 7));
             pre[tl.start + idx] = (uint32_t) (x.w0 >> 40);
             if (skeys) skeys[tl.start + idx] = x;
-            sk[idx] = ent;
+            // the entry's e bytes as two overlapping 8-byte stores (a wave's
+            // lanes write one contiguous run of entries)
+            uint8_t *dst = entries + pd.ent_off + (tl.start - pd.rec0 + idx) * (int64_t) e;
+            const uint64_t tail = e == 16 ? ent.w1 : (ent.w0 >> (8 * (e - 8))) | (ent.w1 << (8 * (16 - e)));
+            __builtin_memcpy(dst, &ent.w0, 8);
+            __builtin_memcpy(dst + e - 8, &tail, 8);
         }
     }
-    __syncthreads();
-    write_packed<T>(entries + pd.ent_off + (tl.start - pd.rec0) * (int64_t) e, reinterpret_cast<const u128 *>(sk), n, e);
     __syncthreads();
     }
 }
