@@ -1387,23 +1387,10 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         __syncthreads();
         continue;
     }
-    // Output.  LCPs with the predecessor first (striped, record k*T + tid), into
-    // the bins' LDS, which is free now; the LCP of the tile's first record is
-    // set by k_tile_fix.
-    uint8_t *lcb = reinterpret_cast<uint8_t *>(bh);
-#pragma unroll
-    for (int k = 0; k < E; k++) {
-        const int idx = k * T + tid;
-        if (idx < n) {
-            const Rec x = sk[idx];
-            uint32_t l = 0;
-            if (idx > 0) { const Rec y = sk[idx - 1]; l = (uint32_t) lcp_bases(y.w0, y.w1, x.w0, x.w1); }
-            lcb[idx] = (uint8_t) l;
-        }
-    }
-    __syncthreads();
-    // Each record's GIX entry (src/kmer_adapter.c:55-62 layout): k-mer bytes
-    // 3..9, mask 0, LCP, pos LE, contig|strand LE, stored at its place.
+    // Output, record k*T + tid of the sorted tile: its LCP with its predecessor
+    // (the tile's first record's is set by k_tile_fix) and its GIX entry
+    // (src/kmer_adapter.c:55-62 layout): k-mer bytes 3..9, mask 0, LCP, pos LE,
+    // contig|strand LE, stored at its place.
     const int e = pd.e, P = pd.P, C = pd.C;
     const uint64_t jmask = (1ull << jbits) - 1ull;
 #pragma unroll
@@ -1411,7 +1398,8 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         const int idx = k * T + tid;
         if (idx < n) {
             const Rec x = sk[idx];
-            const uint64_t lcp = lcb[idx];
+            uint64_t lcp = 0;
+            if (idx > 0) { const Rec y = sk[idx - 1]; lcp = (uint64_t) lcp_bases(y.w0, y.w1, x.w0, x.w1); }
             const uint64_t strand = x.w1 & 1ull;
             const uint64_t j = (x.w1 >> 1) & jmask;
             const uint64_t contig = (x.w1 & 0xFFFFFFFFFFFFull) >> (jbits + 1);
