@@ -1012,6 +1012,16 @@ __device__ __forceinline__ bool key_le(uint64_t a0, uint64_t a1, uint64_t b0, ui
     return a0 < b0 || (a0 == b0 && a1 <= b1);
 }
 
+// 1 if (y, tie) comes before x: y < x as 128-bit keys, or y == x and tie --
+// the borrow out of y - x - tie, with no branches.
+__device__ __forceinline__ uint32_t comes_before(const Rec &y, const Rec &x, bool tie)
+{
+    unsigned long long b0, b1;
+    (void) __builtin_subcll(y.w1, x.w1, tie ? 1ull : 0ull, &b0);
+    (void) __builtin_subcll(y.w0, x.w0, b0, &b1);
+    return (uint32_t) b1;
+}
+
 __device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1)
 {
     uint64_t x = a0 ^ b0;
@@ -1185,8 +1195,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             uint32_t rank = 0;
             for (uint32_t i = st; i < en; i++) {
                 const Rec y = sk[i];
-                rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && (y.w1 < r[k].w1 || (y.w1 == r[k].w1 && i < me))))
-                        ? 1u : 0u;
+                rank += comes_before(y, r[k], i < me);
             }
             rk[k] = st + rank;                          // the record's place in the tile
         }
@@ -1298,8 +1307,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                 uint32_t rank = 0;
                 for (uint32_t x = st; x < en; x++) {
                     const Rec y = sk[x];
-                    rank += (y.w0 < r[k].w0 || (y.w0 == r[k].w0 && (y.w1 < r[k].w1 || (y.w1 == r[k].w1 && x < me))))
-                            ? 1u : 0u;
+                    rank += comes_before(y, r[k], x < me);
                 }
                 rk[k] = st + rank;
             }
