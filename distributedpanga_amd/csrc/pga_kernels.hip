@@ -1022,13 +1022,16 @@ __device__ __forceinline__ uint32_t comes_before(const Rec &y, const Rec &x, boo
     return (uint32_t) b1;
 }
 
+// Leading equal bases of two keys (40 if the k-mers are equal), with no
+// branches: the first differing bit in w0, else in the 16 k-mer bits of w1
+// (bit 15 set below them stands for "none", giving 32 + 8).
 __device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1)
 {
-    uint64_t x = a0 ^ b0;
-    if (x) return __builtin_clzll(x) >> 1;
-    uint32_t y = (uint32_t) ((a1 ^ b1) >> 48);
-    if (y) return 32 + ((__builtin_clz(y) - 16) >> 1);
-    return 40;
+    const uint64_t x = a0 ^ b0;
+    const uint32_t y = (uint32_t) ((a1 ^ b1) >> 48);
+    const int lx = __builtin_clzll(x | 1ull) >> 1;
+    const int ly = 32 + (__builtin_clz((y << 16) | 0x8000u) >> 1);
+    return x ? lx : ly;
 }
 
 constexpr int FB_BIN = 128;          // largest bin the binned tile sort ranks directly
@@ -2043,7 +2046,7 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
                 const uint64_t pq = nxt;
                 q += q + 1 == gi ? 2 : 1;
                 const bool more = q < qb;
-                if (more) nxt = pay[q];
+                nxt = pay[q];                            // q <= qb: inside the window
                 one(pq);
                 if (!more) break;
             }
