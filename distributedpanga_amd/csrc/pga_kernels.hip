@@ -465,24 +465,29 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
     __shared__ uint32_t stot;
     const int tid = threadIdx.x, q = tid >> 8, t = tid & 255;
-    // inputs of a round, loaded one round ahead: selection bits, the contig
-    // of the thread's first position and, when its XU positions lie in that
-    // contig past base 28, the forward and reverse-complement 128-bit spans
-    uint32_t bits = 0;
-    int c = 0;
+    // Inputs of a round, in two stages so that no load waits on one issued
+    // in the same round: two rounds ahead, the selection bits and the block's
+    // first contig (fetch1); one round ahead, from those, the contig of the
+    // thread's first position and, when its XU positions lie in that contig
+    // past base 28, the forward and reverse-complement 128-bit spans (fetch2).
+    uint32_t bits = 0, bits_n = 0;
+    int c = 0, c_n = 0;
     bool fast = false;
-    int64_t f0 = 0;
+    int64_t f0 = 0, lb_n = 0;
     ContigDev c0{};
     uint64_t A0 = 0, A1 = 0, B0 = 0, B1 = 0;
-    // (independent loads travel together: the bits with the block's first
-    // contig, a contig with the next one's start)
-    auto fetch = [&](int64_t LB, int64_t lb1) {
+    auto fetch1 = [&](int64_t LB, int64_t lb1) {
         const bool in = LB < lb1;
-        bits = in ? sel[LB * XT + t] : 0u;
-        c = in ? bctg[LB] : 0;
+        lb_n = LB;
+        bits_n = in ? sel[LB * XT + t] : 0u;
+        c_n = in ? bctg[LB] : 0;
+    };
+    auto fetch2 = [&]() {
+        bits = bits_n;
+        c = c_n;
         fast = false;
         if (!bits) return;
-        f0 = LB * XT * XU + (int64_t) t * XU;
+        f0 = lb_n * XT * XU + (int64_t) t * XU;
         c0 = ctg[c];
         while (c + 1 < nctg) {
             const ContigDev nx = ctg[c + 1];
@@ -506,7 +511,9 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         lim[tid] = rangehist[(size_t) r * 256 + tid];
         done[tid] = 0;
     }
-    fetch(xr.lb0 + q, xr.lb1);
+    fetch1(xr.lb0 + q, xr.lb1);
+    fetch2();
+    fetch1(xr.lb0 + QB + q, xr.lb1);
     for (int64_t lb = xr.lb0; lb < xr.lb1; lb += QB) {
     Rec rr[NS];
     uint32_t rk[NS];
@@ -549,7 +556,10 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             if (e.rev) put(2 * u + 1, e.r0, e.r1);
         }
     }
-    if (lb + QB < xr.lb1) fetch(lb + QB + q, xr.lb1);  // the next round's inputs, in flight meanwhile
+    if (lb + QB < xr.lb1) {                            // the next rounds' inputs, in flight meanwhile
+        fetch2();
+        fetch1(lb + 2 * QB + q, xr.lb1);
+    }
     __syncthreads();
     // the round's digit runs (one wave: four digits a lane): offsets in the
     // stage, slots in the buckets; the other counters are cleared for the
