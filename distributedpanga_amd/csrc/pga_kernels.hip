@@ -1044,7 +1044,7 @@ __device__ __forceinline__ int lcp_bases(uint64_t a0, uint64_t a1, uint64_t b0, 
     return x ? lx : ly;
 }
 
-constexpr int FB_BIN = 128;          // largest bin the binned tile sort ranks directly
+constexpr int FB_BIN = 256;          // largest bin the binned tile sort ranks directly
 constexpr int MAXC = 32;             // crowded ranges a tile sorts by sub-bins (more: merge sort)
 constexpr int CROWD_ROUNDS = 6;      // rounds of sub-binning before the merge sort takes over
 
@@ -1205,11 +1205,13 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             const uint32_t st = bin_start(b);
             const uint32_t en = b + 1 < (uint32_t) NB ? bin_start(b + 1) : (uint32_t) n;
             if (en - st > (uint32_t) FB_BIN) { rk[k] = me; continue; }   // crowded: sorted below
-            uint32_t rank = 0;
-            for (uint32_t i = st; i < en; i++) {
-                const Rec y = sk[i];
-                rank += comes_before(y, r[k], i < me);
+            // two reads in flight a step
+            uint32_t rank = 0, i = st;
+            for (; i + 2 <= en; i += 2) {
+                const Rec y0 = sk[i], y1 = sk[i + 1];
+                rank += comes_before(y0, r[k], i < me) + comes_before(y1, r[k], i + 1 < me);
             }
+            if (i < en) rank += comes_before(sk[i], r[k], i < me);
             rk[k] = st + rank;                          // the record's place in the tile
         }
         __syncthreads();

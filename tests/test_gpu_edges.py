@@ -160,7 +160,7 @@ def _repeat_genome(copies, unit_len, div, seed, sat=0):
 @pytest.mark.parametrize("tile", [2048, 4096, 8192])
 def test_crowded_bins_of_repeats(tile):
     """Hundreds of near-identical element copies and a satellite: bins of more
-    than 128 records, sorted by sub-binning (or the merge sort)."""
+    than 256 records, sorted by sub-binning (or the merge sort)."""
     g = _repeat_genome(600, 500, 0.01, 21, sat=300)
     got, _ = _compare(g, tile_records=tile, digit2_bits=8)
     t = got["timings"]
