@@ -1882,8 +1882,6 @@ __device__ __forceinline__ void flush_buck(uint32_t *__restrict__ lb, int n, int
     }
 }
 
-constexpr int OB = 8192;             // bytes of each LDS output run (N and C streams)
-
 // Bytes [s-8, s) and [0, 8) of a seed whose first 1+pc bytes are hd (plen,
 // then the anchor payload) and whose last pc bytes are pq (the partner
 // payload), for pc >= 4.  Both 8-byte words lie inside the seed, so two
@@ -1894,38 +1892,22 @@ __device__ __forceinline__ void seed_words(uint64_t hd, uint64_t pq, int pc, uin
     w1 = (hd >> (8 * (2 * pc - 7))) | (pq << (8 * (8 - pc)));
 }
 
-// Copy the packed run of nb bytes staged at ob + lead (lead = g & 15) to g:
-// aligned 16-byte chunks go out as vector stores, the partial chunks at the
-// two ends byte by byte (their other bytes belong to neighbouring runs).
-__device__ __forceinline__ void copy_run(uint8_t *__restrict__ g, const uint8_t *__restrict__ ob, int lead, int nb)
-{
-    if (nb <= 0) return;
-    uint8_t *const a0 = g - lead;
-    const int end = lead + nb;
-    const int nchunks = (end + 15) >> 4;
-    for (int c = threadIdx.x; c < nchunks; c += MTH) {
-        const int o = 16 * c;
-        const uint4 w = *reinterpret_cast<const uint4 *>(ob + o);
-        if (o >= lead && o + 16 <= end) {
-            *reinterpret_cast<uint4 *>(a0 + o) = w;
-        } else {
-            // an end chunk: its bytes inside the run, from the one LDS read
-            const uint32_t q[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-            for (int b = 0; b < 16; b++)
-                if (o + b >= lead && o + b < end) a0[o + b] = (uint8_t) (q[b >> 2] >> (8 * (b & 3)));
-        }
-    }
-}
+constexpr int SMAP = 2048;           // seeds of each stream a merge-emit round maps in LDS
 
 // Seeds of a run of merge tiles (A*MTH anchors each, A consecutive per
-// thread), from the interval records of the count pass.  Per tile: one 64-bit
-// payload per entry of the tile plus a FREQ halo on each side goes to LDS
-// (the strand is the payload's top bit); a block scan places each thread's
-// seeds in the tile's slices of the N and C streams (tile offsets from the
-// count pass).  Each round, every anchor's thread writes its seeds' exact
-// bytes into two LDS runs laid out with the same 16-byte phase as their
-// destinations, and all lanes copy the runs out in aligned 16-byte stores.
+// thread), from the interval records of the count pass.  Per tile, LDS holds
+// one payload per window entry (the tile's anchors plus a FREQ halo each side;
+// the strand is the payload's top bit) and the window's strand bits; a block
+// scan places each anchor's seeds in the tile's slices of the N and C streams.
+// Each round of up to SMAP seeds a stream:
+//  - every anchor's thread lists its seeds in an LDS map, one aligned 32-bit
+//    word a seed (anchor index | partner window index << 11 | plen << 22),
+//    walking the set bits of its same-strand and other-strand partner masks
+//    (a short loop per anchor: the walk is the only lane-divergent part);
+//  - then one lane takes one seed: its map word and the two payloads give its
+//    bytes, stored straight to HBM as two overlapping 8-byte stores (4-byte
+//    for payloads under 4 bytes).  Consecutive lanes hold consecutive seeds,
+//    so a wave's stores cover one contiguous run of the stream.
 // Per-anchor-contig counts accumulate in LDS while the pair stays the same.
 template <int A>
 __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ pairs, int npairs,
@@ -1948,13 +1930,18 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
     }
     constexpr int TA = MTH * A;
     constexpr int W = TA + 2 * MAX_FREQ + 4;
+    constexpr int NSW = (W + 63) / 64;
+    static_assert(W <= 2048, "anchor and partner window indexes fit 11 bits of a map word");
     __shared__ uint64_t pay[W];
-    __shared__ __align__(16) uint8_t obuf[2][OB + 16];
+    __shared__ uint64_t sbits[NSW + 1];             // strand bit of window entry k: bit k & 63 of word k >> 6
+    __shared__ uint32_t smap[2][SMAP];              // the round's N and C seeds: anchor | partner << 11 | plen << 22
     __shared__ uint32_t lb[2 * BUCK_LDS];
     __shared__ uint32_t sh[MTH / 64 + 1];
     for (int c = threadIdx.x; c < 2 * BUCK_LDS; c += MTH) lb[c] = 0;
+    if (threadIdx.x == 0) sbits[NSW] = 0;
     const int64_t t0 = (int64_t) blockIdx.x * per_block;
     const int64_t t1 = t0 + per_block < nmb ? t0 + per_block : nmb;
+    const int lane = (int) threadIdx.x & 63, wv = (int) threadIdx.x >> 6;
     int cur_pair = -1;
     for (int64_t tile = t0; tile < t1; tile++) {
         const int p = merge_block_pair(pairs, npairs, tile);
@@ -1979,7 +1966,7 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
             const int64_t g = lo + threadIdx.x * A + k;
             v[k] = g < end ? minfo[g] : 0u;
         }
-        __syncthreads();                                 // the previous tile is done with pay[]
+        __syncthreads();                                 // the previous tile is done with the LDS arrays
         {
             // window entry k is global entry lo-H+k; only those inside the pair
             // are read (intervals never leave the pair).  The 8-byte load at
@@ -2011,33 +1998,84 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
         uint32_t tot;
         const uint32_t off = block_excl_scan<MTH, uint32_t>(sN | sC << 16, sh, &tot);   // also orders pay[]
         const int totN = (int) (tot & 0xFFFFu), totC = (int) (tot >> 16);
-        const int capN = OB / s;
-        const int rounds = max((totN + capN - 1) / capN, (totC + capN - 1) / capN);
-        uint8_t *gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
-        uint8_t *gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
+        // strand bits of the window (entries outside the pair read as garbage;
+        // no interval reaches them)
+        for (int w = wv; w < NSW; w += MTH / 64) {
+            const int k = 64 * w + lane;
+            const uint64_t b = __ballot(k < W && ((pay[k < W ? k : 0] >> sbit) & 1ull));
+            if (lane == 0) sbits[w] = b;
+        }
+        // per-anchor-contig counts (src/fastga_pipeline.c:403)
+#pragma unroll
+        for (int k = 0; k < A; k++) {
+            if (!(v[k] >> 18)) continue;
+            const uint32_t nn = (v[k] >> 18) & 63u, nc = (v[k] >> 24) & 63u;
+            const int gi = H + (int) threadIdx.x * A + k;
+            const uint64_t icont = (pay[gi] >> (8 * pd.P)) & ((1ull << (8 * pd.C - 1)) - 1);
+            if (lds_buck) {
+                if (nn) atomicAdd(&lb[icont], nn);
+                if (nc) atomicAdd(&lb[BUCK_LDS + icont], nc);
+            } else {
+                if (nn) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nn);
+                if (nc) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nc);
+            }
+        }
+        __syncthreads();
+        // each anchor's partners over its interval [qa, qb): same strand (N)
+        // and other strand (C), bit j = window entry qa + j
+        uint64_t mn[A], mc[A];
+#pragma unroll
+        for (int k = 0; k < A; k++) {
+            mn[k] = mc[k] = 0;
+            if (!(v[k] >> 18)) continue;
+            const int aoff = (int) ((v[k] >> 6) & 63u), len = aoff + (int) ((v[k] >> 12) & 63u);
+            const int qa = H + (int) threadIdx.x * A + k - aoff;
+            const int w = qa >> 6, o = qa & 63;
+            const uint64_t bits = o ? (sbits[w] >> o) | (sbits[w + 1] << (64 - o)) : sbits[w];
+            const uint64_t range = len >= 64 ? ~0ull : (1ull << len) - 1;
+            const uint64_t me = 1ull << aoff;
+            mn[k] = ((bits & me) ? bits : ~bits) & range & ~me;
+            mc[k] = range & ~mn[k] & ~me;
+        }
+        uint8_t *const gN = seedN + pd.nseed_off + (ofsN[tile] - ofsN[pd.mblk0]) * (int64_t) s;
+        uint8_t *const gC = seedC + pd.cseed_off + (ofsC[tile] - ofsC[pd.mblk0]) * (int64_t) s;
         const uint64_t clear = ~(1ull << sbit);
-        // One anchor's seeds: partners qa..gi-1 then gi+1..qb-1, each to the N
-        // (same strand) or C run; a tile of several rounds keeps only the
-        // round's window [n0, n1) / [c0, c1).  PC4: payloads of >= 4 bytes (two
-        // 8-byte stores per seed), else two 4-byte stores.
-        auto walk = [&](auto pc4, auto windowed, int k, int &kn, int &kc, int n0, int n1, int c0, int c1,
-                        uint8_t *bN, uint8_t *bC) {
-            const int gi = H + threadIdx.x * A + k;
-            const uint64_t pi = pay[gi];
-            const uint64_t si = (pi >> sbit) & 1;
-            const uint64_t hd = (uint64_t) (v[k] & 63u) | ((pi & clear) << 8);
-            const int qa = gi - (int) ((v[k] >> 6) & 63u), qb = gi + (int) ((v[k] >> 12) & 63u);
-            auto one = [&](uint64_t pq) {
-                const bool same = ((pq >> sbit) & 1) == si;
-                int idx = same ? kn : kc;
-                kn += same;
-                kc += !same;
-                if (decltype(windowed)::value) {
-                    idx -= same ? n0 : c0;
-                    if (idx < 0 || idx >= (same ? n1 - n0 : c1 - c0)) return;
+        const int rounds = (max(totN, totC) + SMAP - 1) / SMAP;
+        for (int r = 0; r < rounds; r++) {
+            // (a stream with fewer rounds than the other has none left: n1 = n0)
+            const int n0 = r * SMAP, n1 = max(n0, min(n0 + SMAP, totN));
+            const int c0 = r * SMAP, c1 = max(c0, min(c0 + SMAP, totC));
+            // map the round's seeds: anchors in order, partners ascending
+            {
+                int kn = (int) (off & 0xFFFFu), kc = (int) (off >> 16);
+#pragma unroll
+                for (int k = 0; k < A; k++) {
+                    const int i = (int) threadIdx.x * A + k;
+                    const uint32_t qa = (uint32_t) (H + i) - ((v[k] >> 6) & 63u);
+                    const uint32_t tag = (uint32_t) i | qa << 11 | (v[k] & 63u) << 22;
+                    if (rounds == 1) {
+                        for (uint64_t m = mn[k]; m; m &= m - 1) smap[0][kn++] = tag + ((uint32_t) __builtin_ctzll(m) << 11);
+                        for (uint64_t m = mc[k]; m; m &= m - 1) smap[1][kc++] = tag + ((uint32_t) __builtin_ctzll(m) << 11);
+                    } else {
+                        for (uint64_t m = mn[k]; m; m &= m - 1, kn++)
+                            if (kn >= n0 && kn < n1) smap[0][kn - n0] = tag + ((uint32_t) __builtin_ctzll(m) << 11);
+                        for (uint64_t m = mc[k]; m; m &= m - 1, kc++)
+                            if (kc >= c0 && kc < c1) smap[1][kc - c0] = tag + ((uint32_t) __builtin_ctzll(m) << 11);
+                    }
                 }
-                uint8_t *dst = (same ? bN : bC) + idx * s;
-                if (decltype(pc4)::value) {
+            }
+            __syncthreads();
+            // the round's seeds, one lane a seed: consecutive lanes write
+            // consecutive seeds, so a wave's stores cover one contiguous run
+            const int nN = n1 - n0, nR = nN + c1 - c0;
+            for (int u = (int) threadIdx.x; u < nR; u += MTH) {
+                const int st = u >= nN;                  // 0: N stream, 1: C stream
+                const int t = st ? u - nN : u;
+                const uint32_t m = smap[st][t];
+                const uint64_t pa = pay[H + (int) (m & 0x7FFu)], pq = pay[(m >> 11) & 0x7FFu];
+                const uint64_t hd = (uint64_t) (m >> 22) | ((pa & clear) << 8);
+                uint8_t *const dst = st ? gC + (int64_t) (c0 + t) * s : gN + (int64_t) (n0 + t) * s;
+                if (pc >= 4) {
                     uint64_t w0, w1;
                     seed_words(hd, pq, pc, w0, w1);
                     __builtin_memcpy(dst, &w0, 8);
@@ -2048,61 +2086,8 @@ __global__ __launch_bounds__(MTH) void k_merge_emit(const PairDev *__restrict__ 
                     __builtin_memcpy(dst, &x0, 4);
                     __builtin_memcpy(dst + s - 4, &x1, 4);
                 }
-            };
-            // partners qa..qb-1 but the anchor, each payload read one partner
-            // ahead so the read's latency overlaps the previous seed's stores
-            int q = qa == gi ? gi + 1 : qa;
-            if (q >= qb) return;
-            uint64_t nxt = pay[q];
-            while (true) {
-                const uint64_t pq = nxt;
-                q += q + 1 == gi ? 2 : 1;
-                const bool more = q < qb;
-                nxt = pay[q];                            // q <= qb: inside the window
-                one(pq);
-                if (!more) break;
-            }
-        };
-        for (int r = 0; r < rounds; r++) {
-            const int n0 = r * capN, n1 = min(n0 + capN, totN);
-            const int c0 = r * capN, c1 = min(c0 + capN, totC);
-            const int leadN = (int) ((uintptr_t) (gN + (int64_t) n0 * s) & 15);
-            const int leadC = (int) ((uintptr_t) (gC + (int64_t) c0 * s) & 15);
-            // runs of the round, indexed by the seed's number in its stream
-            uint8_t *const bN = obuf[0] + leadN, *const bC = obuf[1] + leadC;
-            int kn = (int) (off & 0xFFFFu), kc = (int) (off >> 16);
-            using T1 = std::integral_constant<bool, true>;
-            using F1 = std::integral_constant<bool, false>;
-#pragma unroll
-            for (int k = 0; k < A; k++) {
-                if (!(v[k] >> 18)) continue;
-                if (rounds > 1) {
-                    if (pc >= 4) walk(T1(), T1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
-                    else walk(F1(), T1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
-                } else {
-                    if (pc >= 4) walk(T1(), F1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
-                    else walk(F1(), F1(), k, kn, kc, n0, n1, c0, c1, bN, bC);
-                }
             }
             __syncthreads();
-            copy_run(gN + (int64_t) n0 * s, obuf[0], leadN, (n1 - n0) * s);
-            copy_run(gC + (int64_t) c0 * s, obuf[1], leadC, (c1 - c0) * s);
-            __syncthreads();
-        }
-        // per-anchor-contig counts (src/fastga_pipeline.c:403)
-#pragma unroll
-        for (int k = 0; k < A; k++) {
-            if (!(v[k] >> 18)) continue;
-            const uint32_t nn = (v[k] >> 18) & 63u, nc = (v[k] >> 24) & 63u;
-            const int gi = H + threadIdx.x * A + k;
-            const uint64_t icont = (pay[gi] >> (8 * pd.P)) & ((1ull << (8 * pd.C - 1)) - 1);
-            if (lds_buck) {
-                if (nn) atomicAdd(&lb[icont], nn);
-                if (nc) atomicAdd(&lb[BUCK_LDS + icont], nc);
-            } else {
-                if (nn) atomicAdd(&nbuck[pd.buck_off + icont], (unsigned long long) nn);
-                if (nc) atomicAdd(&cbuck[pd.buck_off + icont], (unsigned long long) nc);
-            }
         }
     }
     __syncthreads();
