@@ -256,11 +256,12 @@ int chunk_per(const pga_ctx *c) { return c->o.chunk_records / 256; }
 constexpr int64_t ANY_CHUNKS = (int64_t) 1 << 40;
 
 int launch_seg_hist(pga_ctx *c, int per, const Rec *buf0, const Rec *buf1, const Seg *segs, int nseg,
-                    const int64_t *chk0, int level, unsigned long long *hist, const int *dn = nullptr)
+                    const int64_t *chk0, int level, unsigned long long *hist, const uint8_t *dig,
+                    const int *dn = nullptr)
 {
 #define PGA_HIST(PER)                                                                                  \
     hipLaunchKernelGGL(k_seg_hist<PER>, dim3(grid_for(c, (const void *) k_seg_hist<PER>, 256, 0, ANY_CHUNKS)), \
-                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, hist, dn)
+                       dim3(256), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, hist, dig, dn)
     switch (per) {
     case 8:  PGA_HIST(8); break;
     case 32: PGA_HIST(32); break;
@@ -272,13 +273,14 @@ int launch_seg_hist(pga_ctx *c, int per, const Rec *buf0, const Rec *buf1, const
 }
 
 int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *segs,
-                       int nseg, const int64_t *chk0, int level, unsigned long long *cursor, const int *dn = nullptr)
+                       int nseg, const int64_t *chk0, int level, unsigned long long *cursor, uint8_t *dig,
+                       const int *dn = nullptr)
 {
     // chunks of 2048 / 4096 records: 256 threads; 8192: 512 threads of 16 records
 #define PGA_SCAT(TT, PER)                                                                              \
     hipLaunchKernelGGL((k_seg_scatter<TT, PER>),                                                        \
                        dim3(grid_for(c, (const void *) k_seg_scatter<TT, PER>, TT, 0, ANY_CHUNKS)),      \
-                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, cursor, dn)
+                       dim3(TT), 0, c->st, buf0, buf1, segs, nseg, chk0, level, c->o.digit2_bits, cursor, dig, dn)
     switch (per) {
     case 8:  PGA_SCAT(256, 8); break;
     case 32: PGA_SCAT(512, 16); break;
@@ -583,10 +585,13 @@ namespace {
 // per pair) when given, else they are counted here.  level1_in_place: the
 // segments' records are already in level-1 bucket order (hist1's layout), so
 // the first level only plans and does not scatter.
-// The tiles' `buf` index refers to {buf0, buf1}.
+// The tiles' `buf` index refers to {buf0, buf1}.  dig (may be null): a byte
+// per record position, free during the partition; each scatter from the
+// second level on writes the moved records' next digits there, and the next
+// level counts those segments from it.
 int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, const std::vector<Tile> &tile0,
               const unsigned long long *hist1, bool level1_in_place, int64_t seg_cap, int64_t tile_cap,
-              int *levels_out)
+              uint8_t *dig, int *levels_out)
 {
     const int CAP = cap_of(c);
     const int per = chunk_per(c);
@@ -609,19 +614,20 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         LAUNCHCHK(c);
         const bool counted = level == 1 && hist1;
         const bool in_place = level == 1 && level1_in_place;
+        const bool dig_next = dig && !in_place && level >= 2;        // the next level's digits are 8-bit
         const int64_t nd = level == 2 ? (int64_t) 1 << c->o.digit2_bits : 256;   // digit rows of this level
         TRY(ensure(c, c->seghist, (size_t) nseg * nd * sizeof(uint64_t)));
         TRY(ensure(c, c->cursor, (size_t) nseg * nd * sizeof(uint64_t)));
         if (!counted) {
             TRY(fill_u64(c, c->seghist.p, (int64_t) nseg * nd, 0));
             TRY(launch_seg_hist(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg, P<int64_t>(c->chk0), level,
-                                P<unsigned long long>(c->seghist)));
+                                P<unsigned long long>(c->seghist), dig));
         }
         HIPCHK(c, hipMemsetAsync(dcount, 0, sizeof(int), c->st));
         HIPCHK(c, hipMemsetAsync(dcount + 2, 0, 2 * sizeof(int), c->st));
         hipLaunchKernelGGL(k_seg_plan, dim3(nseg), dim3(256), 0, c->st, P<Seg>(c->segs[cur]),
                            counted ? hist1 : (const unsigned long long *) P<unsigned long long>(c->seghist),
-                           counted ? 1 : 0, level, c->o.digit2_bits, in_place ? 1 : 0, CAP,
+                           counted ? 1 : 0, level, c->o.digit2_bits, in_place ? 1 : 0, dig_next ? 1 : 0, CAP,
                            P<unsigned long long>(c->cursor),
                            P<Seg>(c->segs[cur ^ 1]), dcount, (int) seg_cap, P<Tile>(c->tiles), dcount + 1,
                            (int) tile_cap, reinterpret_cast<unsigned long long *>(dcount + 2), nseg,
@@ -632,7 +638,7 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
             TRY(scatter_event(c, se));
             HIPCHK(c, hipEventRecord(se[0], c->st));
             TRY(launch_seg_scatter(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), nseg, P<int64_t>(c->chk0), level,
-                                   P<unsigned long long>(c->cursor)));
+                                   P<unsigned long long>(c->cursor), dig_next ? dig : nullptr));
             HIPCHK(c, hipEventRecord(se[1], c->st));
             c->nscatter++;
         }
@@ -805,8 +811,10 @@ int pga_build_index(pga_ctx *c)
         else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 1, 1});
     }
     int levels = 0;
+    // the LCP/strand bytes are written by the tile sort: until then their
+    // buffer holds the partition's digit bytes
     TRY(partition(c, P<Rec>(c->rec[0]), P<Rec>(c->rec[1]), seg0, tile0, P<unsigned long long>(c->hist1), true,
-                  seg_cap, tile_cap, &levels));
+                  seg_cap, tile_cap, P<uint8_t>(c->ls), &levels));
     if (c->xbad) return fail(c, PGA_EINVAL, "internal: extraction passes disagree on a record's first digit");
     c->tm.partition_levels = levels;
     c->tm.partition_launches = c->nscatter;
@@ -1280,7 +1288,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         const int64_t nt = c->ntiles;
         int levels = 0;
         const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0,
-                                 P<unsigned long long>(c->hist1), false, seg_cap, tile_cap, &levels);
+                                 P<unsigned long long>(c->hist1), false, seg_cap, tile_cap, nullptr, &levels);
         const int64_t sort_tiles = c->ntiles;
         c->nscatter = ns;
         c->scatter_bytes = sb;
@@ -1523,18 +1531,18 @@ int graph_enqueue(pga_ctx *c)
             hipLaunchKernelGGL(k_fill_u64_dev, dim3(1024), dim3(256), 0, c->st, P<unsigned long long>(c->seghist), dn,
                                nd, 0ull);
             TRY(launch_seg_hist(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), 0, P<int64_t>(c->chk0), level,
-                                P<unsigned long long>(c->seghist), dn));
+                                P<unsigned long long>(c->seghist), P<uint8_t>(c->ls), dn));
         }
         hipLaunchKernelGGL(k_seg_plan, dim3(PLAN_GRID), dim3(256), 0, c->st, P<Seg>(c->segs[cur]),
                            first ? (const unsigned long long *) P<unsigned long long>(c->hist1)
                                  : (const unsigned long long *) P<unsigned long long>(c->seghist),
-                           first ? 1 : 0, level, c->o.digit2_bits, first ? 1 : 0, CAP,
+                           first ? 1 : 0, level, c->o.digit2_bits, first ? 1 : 0, first ? 0 : 1, CAP,
                            P<unsigned long long>(c->cursor), P<Seg>(c->segs[cur ^ 1]), &gs->nsegs[level + 1],
                            (int) c->g_seg_cap, P<Tile>(c->tiles), &gs->ntiles, (int) c->g_tile_cap, &gs->kept, 0, dn);
         LAUNCHCHK(c);
         if (!first)
             TRY(launch_seg_scatter(c, per, buf[0], buf[1], P<Seg>(c->segs[cur]), 0, P<int64_t>(c->chk0), level,
-                                   P<unsigned long long>(c->cursor), dn));
+                                   P<unsigned long long>(c->cursor), P<uint8_t>(c->ls), dn));
         cur ^= 1;
     }
     // ---- tile sort, LCP fix-up, prefix tables

@@ -51,7 +51,8 @@ struct Seg {                     // a key range still to be partitioned
     int64_t start, count;
     int32_t pair, level;         // level = digit (1..16) to split on next
     int32_t buf;                 // record buffer (0/1) holding its records
-    int32_t skip;                // set by the plan: one digit only, left in place
+    int32_t skip;                // bit 0, set by the plan: one digit only, left in place;
+                                 // bit 1: the previous level's scatter moved it and wrote its digits
 };
 
 struct Tile {                    // a bucket run small enough for one workgroup sort

@@ -757,15 +757,20 @@ __global__ __launch_bounds__(1024) void k_chunk_prefix(const Seg *__restrict__ s
 }
 
 constexpr int PT = 256;   // threads of the partition kernels
+constexpr unsigned long long CURSOR_DIG = 1ull << 62;   // digit cursor flag: its bucket is a next-level segment
 
 // Digit counts of the segments at one level (all segments of a pass share
 // it): one row of nd = 2^digit_bits counters per segment.  The chunk count
-// comes from chk0[nseg], so the grid is sized without a round trip.
+// comes from chk0[nseg], so the grid is sized without a round trip.  A
+// segment whose records the previous level's scatter moved (skip bit 1) has
+// its digits in the byte array dig, which that scatter wrote beside the
+// records: its counts read one byte a record instead of the 16-byte record.
 template <int PER>
 __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, const Rec *__restrict__ buf1,
                                                  const Seg *__restrict__ segs,
                                                  int nseg, const int64_t *__restrict__ chk0, int level, int w2,
-                                                 unsigned long long *__restrict__ seghist, const int *__restrict__ dn)
+                                                 unsigned long long *__restrict__ seghist,
+                                                 const uint8_t *__restrict__ dig, const int *__restrict__ dn)
 {
     __shared__ uint32_t hist[ND_MAX];
     if (dn) nseg = *dn;
@@ -779,13 +784,22 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
         const Seg sg = segs[s];
         const int64_t off = (LB - chk0[s]) * (int64_t) CH;
         const int n = (int) min((int64_t) CH, sg.count - off);
-        const Rec *p = (sg.buf ? buf1 : buf0) + sg.start + off;
+        if (dig && (sg.skip & 2)) {
+            const uint8_t *q = dig + sg.start + off;
 #pragma unroll 4
-        for (int k = 0; k < PER; k++) {
-            const int i = k * PT + threadIdx.x;
-            if (i < n) {
-                const Rec r = p[i];
-                atomicAdd(&hist[digit_at(r.w0, r.w1, level, w2)], 1u);
+            for (int k = 0; k < PER; k++) {
+                const int i = k * PT + threadIdx.x;
+                if (i < n) atomicAdd(&hist[q[i]], 1u);
+            }
+        } else {
+            const Rec *p = (sg.buf ? buf1 : buf0) + sg.start + off;
+#pragma unroll 4
+            for (int k = 0; k < PER; k++) {
+                const int i = k * PT + threadIdx.x;
+                if (i < n) {
+                    const Rec r = p[i];
+                    atomicAdd(&hist[digit_at(r.w0, r.w1, level, w2)], 1u);
+                }
             }
         }
         __syncthreads();
@@ -804,9 +818,14 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
 // its records are already in bucket order, the scatter skips it, and its child
 // (segment or tile) stays where it is.  With in_place the whole level is
 // already in bucket order (the extraction wrote it so) and is not scattered.
+// dig_next: this level's scatter writes the next level's digits into the
+// digit byte array for the children that become next-level segments (their
+// cursors carry CURSOR_DIG), and those children are marked (skip bit 1) so
+// that the next level counts them from there.
 __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
                                                   const unsigned long long *__restrict__ seghist,
-                                                  int hist_by_pair, int level, int w2, int in_place, int cap,
+                                                  int hist_by_pair, int level, int w2, int in_place, int dig_next,
+                                                  int cap,
                                                   unsigned long long *__restrict__ cursor,
                                                   Seg *__restrict__ next, int *__restrict__ nnext, int next_cap,
                                                   Tile *__restrict__ tiles, int *__restrict__ ntiles, int tile_cap,
@@ -854,8 +873,10 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     int64_t run = block_excl_scan<256, int64_t>(sum, sh, &tot);
     for (int q = 0; q < pd; q++) {
         const int d = threadIdx.x * pd + q;
+        const int64_t c = cnt[d];
         off[d] = run;
-        cursor[(size_t) sb * nd + d] = (unsigned long long) (sg.start + run);
+        cursor[(size_t) sb * nd + d] = (unsigned long long) (sg.start + run) |
+                                       (dig_next && c > cap ? CURSOR_DIG : 0ull);
         run += cnt[d];
     }
     __syncthreads();
@@ -906,7 +927,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     }
     for (int i = threadIdx.x; i < nn && nbase + i < next_cap; i += 256) {
         const int d = lnext[i];
-        next[nbase + i] = Seg{sg.start + off[d], cnt[d], sg.pair, level + 1, dstbuf, 0};
+        next[nbase + i] = Seg{sg.start + off[d], cnt[d], sg.pair, level + 1, dstbuf, dig_next ? 2 : 0};
     }
     }
 }
@@ -915,12 +936,15 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
 // staged in digit order, and written as one run per digit at the digit's
 // cursor (reserved with one atomic per digit and chunk).  The next chunk's
 // records are loaded into registers before the current chunk is written out,
-// so its reads overlap the writes.
+// so its reads overlap the writes.  With dig, a moved record whose bucket is a
+// next-level segment (its cursor carries CURSOR_DIG) also writes its digit at
+// the next level (an 8-bit one) to dig at its new position.
 template <int T, int PER>
 __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *__restrict__ buf1,
                                                    const Seg *__restrict__ segs, int nseg,
                                                    const int64_t *__restrict__ chk0, int level, int w2,
-                                                   unsigned long long *__restrict__ cursor, const int *__restrict__ dn)
+                                                   unsigned long long *__restrict__ cursor,
+                                                   uint8_t *__restrict__ dig, const int *__restrict__ dn)
 {
     static_assert(T >= 256, "one thread per digit at least");
     if (dn) nseg = *dn;
@@ -929,6 +953,7 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
     __shared__ Rec stage[CH];
     __shared__ uint32_t hist[ND_MAX];               // digit counts, then the digits' runs in the stage
     __shared__ int64_t gb[ND_MAX];                  // a digit's destination minus its stage offset
+    __shared__ uint8_t gdig[ND_MAX];                // the digit's bucket is a next-level segment: write dig
     __shared__ uint32_t sh[T / 64 + 1];
     const int tid = threadIdx.x;
     const int nd = 1 << digit_bits(level, w2);
@@ -941,7 +966,7 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
         s = seg_of_chunk(chk0, nseg, lb);
         const Seg sg = segs[s];
         const int64_t off = (lb - chk0[s]) * (int64_t) CH;
-        n = sg.skip ? 0 : (int) min((int64_t) CH, sg.count - off);
+        n = (sg.skip & 1) ? 0 : (int) min((int64_t) CH, sg.count - off);
         from = sg.buf;
         const Rec *p = (from ? buf1 : buf0) + sg.start + off;
 #pragma unroll
@@ -982,7 +1007,10 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
     for (int q = 0; q < PDM; q++) {
         if (q < pd && d0 + q < nd) {
             hist[d0 + q] = lo;
-            if (hq[q]) gb[d0 + q] = (int64_t) gq[q] - lo;
+            if (hq[q]) {
+                gb[d0 + q] = (int64_t) (gq[q] & ~CURSOR_DIG) - lo;
+                gdig[d0 + q] = (gq[q] & CURSOR_DIG) ? 1 : 0;
+            }
             lo += hq[q];
         }
     }
@@ -1001,19 +1029,22 @@ __global__ __launch_bounds__(T) void k_seg_scatter(Rec *__restrict__ buf0, Rec *
         int i = k * T + tid;
         if (i < ncur) {
             Rec x = stage[i];
-            dst[gb[digit_at(x.w0, x.w1, level, w2)] + i] = x;
+            const uint32_t d = digit_at(x.w0, x.w1, level, w2);
+            const int64_t to = gb[d] + i;
+            dst[to] = x;
+            if (dig && gdig[d]) dig[to] = (uint8_t) digit_at(x.w0, x.w1, level + 1, w2);
         }
     }
     __syncthreads();
     }
 }
 
-template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
-template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
-template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
-template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
-template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
-template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const int *);
+template __global__ void k_seg_hist<16>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const uint8_t *, const int *);
+template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const uint8_t *, const int *);
+template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const uint8_t *, const int *);
+template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
+template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
+template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
 
 // ---------------------------------------------------------------- tile sort
 
