@@ -151,6 +151,8 @@ constexpr int PK_SEG = PK_CAP / PK_MINC;
 constexpr int PK_T = 512;              // threads of the pack kernels
 constexpr int PK_PER = PK_CAP / PK_T;  // anchor slots per thread
 constexpr int PK_BUCK = 1024;          // anchor contigs counted in LDS (the pack's first query)
+constexpr int PK_MAP = 1024;           // seeds of each stream an emit round maps in LDS
+static_assert(PK_CAP <= 2048, "slot indexes fit 11 bits of a map word");
 
 // Largest segment in combined entries for a FREQ: 33W core target entries,
 // FREQ+2 of context each side, SEG_Q + 2(FREQ+2) query records.
@@ -245,6 +247,7 @@ struct PackLds {
     uint32_t cN[PK_SEG], cC[PK_SEG];   // seeds per segment (count pass)
     uint8_t sbytes[PK_SEG], Pb[PK_SEG], Cb[PK_SEG];
     uint32_t bk[2][PK_BUCK];           // buck counters of the pack's first query (emit pass)
+    uint32_t smap[2][PK_MAP];          // the emit round's N and C seeds: anchor slot | partner slot << 11 | plen << 22
     uint32_t scan[PK_T / 64 + 1];
     int ns, npos, qa;
 };
@@ -402,8 +405,13 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
         const int pos0 = threadIdx.x * PK_PER;
         uint32_t tn = 0, tc = 0;
         uint32_t mN = 0, mC = 0;             // anchors of this thread: bit k set when slot k emits
+        // the anchors' intervals [ia, ib) and plen, kept for the emit
+        int ia[PK_PER], ib[PK_PER], ipl[PK_PER];
+        uint32_t kn_[PK_PER], kc_[PK_PER];
 #pragma unroll
         for (int k = 0; k < PK_PER; k++) {
+            ia[k] = ib[k] = ipl[k] = 0;
+            kn_[k] = kc_[k] = 0;
             const int pos = pos0 + k;
             if (pos >= npos || !(L.flg[pos] & 2)) continue;
             const int gi = H + pos;
@@ -412,8 +420,11 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
             if (swar) {
                 // a query anchor pairs with every entry of its interval, a
                 // target anchor only with query entries
-                if (iq) anchor_info16(L.win, gi, freq, ln, lc);
-                else anchor_info16_q(L.win, L.flg - H, gi, freq, ln, lc);
+                const uint32_t info = iq ? anchor_info16(L.win, gi, freq, ln, lc)
+                                         : anchor_info16_q(L.win, L.flg - H, gi, freq, ln, lc);
+                ipl[k] = (int) (info & 63u);
+                ia[k] = gi - (int) ((info >> 6) & 63u);
+                ib[k] = gi + (int) ((info >> 12) & 63u);
             } else {
                 int plen, a, b;
                 if (!anchor_interval(L.win, gi, freq, plen, a, b)) continue;
@@ -423,7 +434,12 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
                     if (!iq && !(L.flg[x - H] & 1)) continue;
                     if ((L.win[x] & 0x80) == si) ln++; else lc++;
                 }
+                ipl[k] = plen;
+                ia[k] = a;
+                ib[k] = b;
             }
+            kn_[k] = ln;
+            kc_[k] = lc;
             tn += ln;
             tc += lc;
             if (ln) mN |= 1u << k;
@@ -441,43 +457,77 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
             __syncthreads();
             continue;
         }
-        // emit: this thread's first seed index in the batch-wide N / C order
+        // emit: pack-local seed numbers of this thread's first N / C seed; the
+        // pack's seeds are numbered in stream order (segments, anchors,
+        // partners ascending), starting at the first segment's stream offset
         uint32_t totN, totC;
         const uint32_t rn = block_excl_scan<PK_T, uint32_t>(tn, L.scan, &totN);
         const uint32_t rc = block_excl_scan<PK_T, uint32_t>(tc, L.scan, &totC);
         const int64_t s0 = pstart[p];
-        int64_t gN = (L.ns > 0 ? ofsN[s0] : 0) + rn, gC = (L.ns > 0 ? ofsC[s0] : 0) + rc;
+        const int64_t gN0 = L.ns > 0 ? ofsN[s0] : 0, gC0 = L.ns > 0 ? ofsC[s0] : 0;
         const int qa = L.qa;
-#pragma unroll 1
+        // per-anchor-contig counts
+#pragma unroll
         for (int k = 0; k < PK_PER; k++) {
             if (!(((mN | mC) >> k) & 1u)) continue;
             const int pos = pos0 + k;
-            const int gi = H + pos;
-            int plen, a, b;
-            if (swar) {
-                uint32_t xn, xc;
-                const uint32_t info = anchor_info16(L.win, gi, freq, xn, xc);
-                plen = (int) (info & 63u);
-                a = gi - (int) ((info >> 6) & 63u);
-                b = gi + (int) ((info >> 12) & 63u);
-            } else {
-                anchor_interval(L.win, gi, freq, plen, a, b);
-            }
+            const uint32_t ln = kn_[k], lc = kc_[k];
             const int s = L.sidx[pos];
-            const int P = L.Pb[s], Cb = L.Cb[s], sb = L.sbytes[s], pc = P + Cb;
-            const bool iq = L.flg[pos] & 1;
-            const uint8_t si = L.win[gi] & 0x80;
-            const uint64_t pi = key_payload(L.key[pos], jbits, P, Cb) & ~(0x80ull << (8 * (pc - 1)));
-            uint32_t ln = 0, lc = 0;
-            for (int x = a; x < b; x++) {
-                if (x == gi) continue;
-                const int j = x - H;
-                if (!iq && !(L.flg[j] & 1)) continue;
-                const bool same = (L.win[x] & 0x80) == si;
-                uint8_t *dst = same ? seedN + L.bN[s] + (gN++) * sb : seedC + L.bC[s] + (gC++) * sb;
+            const int P = L.Pb[s], Cb = L.Cb[s];
+            const uint64_t pi = key_payload(L.key[pos], jbits, P, Cb);
+            const int icont = (int) ((pi >> (8 * P)) & ((1ull << (8 * Cb - 1)) - 1));
+            if (L.q[s] == qa && icont < PK_BUCK) {
+                if (ln) atomicAdd(&L.bk[0][icont], ln);
+                if (lc) atomicAdd(&L.bk[1][icont], lc);
+            } else {
+                const int64_t bo = qd[L.q[s]].buck_off + icont;
+                if (ln) atomicAdd(&nbuck[bo], (unsigned long long) ln);
+                if (lc) atomicAdd(&cbuck[bo], (unsigned long long) lc);
+            }
+        }
+        // Rounds of up to PK_MAP seeds a stream: every anchor lists its seeds
+        // in an LDS map (anchor slot | partner slot << 11 | plen << 22), then
+        // one lane takes one seed, so a wave's stores cover a contiguous run.
+        const int rounds = (int) ((max(totN, totC) + PK_MAP - 1) / PK_MAP);
+        for (int r = 0; r < rounds; r++) {
+            const int n0 = r * PK_MAP, n1 = max(n0, min(n0 + PK_MAP, (int) totN));
+            const int c0 = r * PK_MAP, c1 = max(c0, min(c0 + PK_MAP, (int) totC));
+            int kn = (int) rn, kc = (int) rc;
+#pragma unroll 1
+            for (int k = 0; k < PK_PER; k++) {
+                if (!(((mN | mC) >> k) & 1u)) continue;
+                const int pos = pos0 + k;
+                const int gi = H + pos;
+                const bool iq = L.flg[pos] & 1;
+                const uint8_t si = L.win[gi] & 0x80;
+                const uint32_t tag = (uint32_t) pos | (uint32_t) ipl[k] << 22;
+                for (int x = ia[k]; x < ib[k]; x++) {
+                    const int j = x - H;
+                    if (x == gi || (!iq && !(L.flg[j] & 1))) continue;
+                    const uint32_t m = tag | (uint32_t) j << 11;
+                    if ((L.win[x] & 0x80) == si) {
+                        if (kn >= n0 && kn < n1) L.smap[0][kn - n0] = m;
+                        kn++;
+                    } else {
+                        if (kc >= c0 && kc < c1) L.smap[1][kc - c0] = m;
+                        kc++;
+                    }
+                }
+            }
+            __syncthreads();
+            const int nN = n1 - n0, nR = nN + c1 - c0;
+            for (int u = threadIdx.x; u < nR; u += PK_T) {
+                const int st = u >= nN;
+                const int t = st ? u - nN : u;
+                const uint32_t m = L.smap[st][t];
+                const int pos = (int) (m & 0x7FFu), j = (int) ((m >> 11) & 0x7FFu);
+                const int s = L.sidx[pos];
+                const int P = L.Pb[s], Cb = L.Cb[s], sb = L.sbytes[s], pc = P + Cb;
+                const uint64_t pi = key_payload(L.key[pos], jbits, P, Cb) & ~(0x80ull << (8 * (pc - 1)));
                 const uint64_t pq = key_payload(L.key[j], jbits, P, Cb);
+                uint8_t *dst = st ? seedC + L.bC[s] + (gC0 + c0 + t) * sb : seedN + L.bN[s] + (gN0 + n0 + t) * sb;
                 // the seed's exact sb = 1 + 2 pc bytes in two overlapping stores
-                const uint64_t hd = (uint64_t) plen | (pi << 8);
+                const uint64_t hd = (uint64_t) (m >> 22) | (pi << 8);
                 if (pc >= 4) {
                     uint64_t w0, w1;
                     seed_words(hd, pq, pc, w0, w1);
@@ -489,17 +539,8 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
                     __builtin_memcpy(dst, &x0, 4);
                     __builtin_memcpy(dst + sb - 4, &x1, 4);
                 }
-                if (same) ln++; else lc++;
             }
-            const int icont = (int) ((pi >> (8 * P)) & ((1ull << (8 * Cb - 1)) - 1));
-            if (L.q[s] == qa && icont < PK_BUCK) {
-                if (ln) atomicAdd(&L.bk[0][icont], ln);
-                if (lc) atomicAdd(&L.bk[1][icont], lc);
-            } else {
-                const int64_t bo = qd[L.q[s]].buck_off + icont;
-                if (ln) atomicAdd(&nbuck[bo], (unsigned long long) ln);
-                if (lc) atomicAdd(&cbuck[bo], (unsigned long long) lc);
-            }
+            __syncthreads();
         }
         __syncthreads();
         if (qa >= 0) {
