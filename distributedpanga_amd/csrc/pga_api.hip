@@ -1895,6 +1895,8 @@ struct pga_slot {
     std::vector<int64_t> nseeds, cseeds;
     DBuf tpos, flag, segidx, segstart, segs, qdev, cntN, cntC, ofsN, ofsC, gath, qrec, seedN, seedC, nbuck, cbuck;
     DBuf jseg, cost, soff, pstart, err;
+    DBuf tofN, tofC, tmpN, tmpC, used;    // the join's staging streams and each segment's place in them
+    unsigned long long capN = 0, capC = 0;
     hipEvent_t ev[2] = {};
     double last_ms = 0.0;
 };
@@ -1937,7 +1939,8 @@ void slot_free(pga_slot *q, int device)
     if (q->Q && q->Q->st) (void) hipStreamSynchronize(q->Q->st);
     DBuf *all[] = {&q->tpos, &q->flag, &q->segidx, &q->segstart, &q->segs, &q->qdev, &q->cntN, &q->cntC,
                    &q->ofsN, &q->ofsC, &q->gath, &q->qrec, &q->seedN, &q->seedC, &q->nbuck, &q->cbuck,
-                   &q->jseg, &q->cost, &q->soff, &q->pstart, &q->err};
+                   &q->jseg, &q->cost, &q->soff, &q->pstart, &q->err, &q->tofN, &q->tofC, &q->tmpN,
+                   &q->tmpC, &q->used};
     for (DBuf *b : all)
         if (b->p) (void) hipFree(b->p);
     for (auto &e : q->ev) if (e) (void) hipEventDestroy(e);
@@ -2053,14 +2056,40 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
                            dim3(256), 0, Q->st, (const int64_t *) P<int64_t>(sl->soff), nseg, hp, npack,
                            P<int64_t>(sl->pstart));
         LAUNCHCHK(Q);
-        const unsigned gj = grid_for(Q, (const void *) k_join_pack<false>, PK_T, 0, npack);
-        hipLaunchKernelGGL(k_join_pack<false>, dim3(gj), dim3(PK_T), 0, Q->st, (const int64_t *) P<int64_t>(sl->pstart),
-                           npack, (const int64_t *) P<int64_t>(sl->soff), (const JSeg *) P<JSeg>(sl->jseg),
-                           (const QSeg *) P<QSeg>(sl->segs), (const QDev *) P<QDev>(sl->qdev),
-                           (const Rec *) P<Rec>(T->skeys), (const Rec *) P<Rec>(Q->skeys),
-                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq, P<uint32_t>(sl->cntN),
-                           P<uint32_t>(sl->cntC), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-        LAUNCHCHK(Q);
+        // one pass per pack: seed counts, bucks and the seeds themselves, in
+        // staging streams; a pass that outgrew them is run again with room
+        TRY(ensure(Q, sl->nbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+        TRY(ensure(Q, sl->cbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+        TRY(ensure(Q, sl->tofN, (size_t) nseg * sizeof(int64_t)));
+        TRY(ensure(Q, sl->tofC, (size_t) nseg * sizeof(int64_t)));
+        TRY(ensure(Q, sl->used, 2 * sizeof(uint64_t)));
+        sl->capN = std::max<unsigned long long>(sl->capN, (unsigned long long) RQ * 64);
+        sl->capC = std::max<unsigned long long>(sl->capC, (unsigned long long) RQ * 32);
+        const unsigned gj = grid_for(Q, (const void *) k_join_pack, PK_T, 0, npack);
+        for (int attempt = 0;; attempt++) {
+            TRY(ensure(Q, sl->tmpN, (size_t) sl->capN + 16));
+            TRY(ensure(Q, sl->tmpC, (size_t) sl->capC + 16));
+            TRY(fill_u64(Q, sl->nbuck.p, nb, 0));
+            TRY(fill_u64(Q, sl->cbuck.p, nb, 0));
+            TRY(fill_u64(Q, sl->used.p, 2, 0));
+            hipLaunchKernelGGL(k_join_pack, dim3(gj), dim3(PK_T), 0, Q->st, (const int64_t *) P<int64_t>(sl->pstart),
+                               npack, (const int64_t *) P<int64_t>(sl->soff), (const JSeg *) P<JSeg>(sl->jseg),
+                               (const QSeg *) P<QSeg>(sl->segs), (const QDev *) P<QDev>(sl->qdev),
+                               (const Rec *) P<Rec>(T->skeys), (const Rec *) P<Rec>(Q->skeys),
+                               (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq, P<uint32_t>(sl->cntN),
+                               P<uint32_t>(sl->cntC), P<int64_t>(sl->tofN), P<int64_t>(sl->tofC),
+                               P<uint8_t>(sl->tmpN), P<uint8_t>(sl->tmpC), sl->capN, sl->capC,
+                               P<unsigned long long>(sl->used), P<unsigned long long>(sl->nbuck),
+                               P<unsigned long long>(sl->cbuck));
+            LAUNCHCHK(Q);
+            unsigned long long u[2] = {0, 0};
+            HIPCHK(Q, hipMemcpyAsync(u, sl->used.p, sizeof(u), hipMemcpyDeviceToHost, Q->st));
+            HIPCHK(Q, hipStreamSynchronize(Q->st));
+            if (u[0] <= sl->capN && u[1] <= sl->capC) break;
+            if (attempt > 0) return sfail(sl->err_msg, PGA_EINVAL, "internal: join staging outgrew its exact size");
+            sl->capN = std::max(sl->capN, u[0] + (u[0] >> 3));
+            sl->capC = std::max(sl->capC, u[1] + (u[1] >> 3));
+        }
         TRY(scan_u32(Q, P<uint32_t>(sl->cntN), nseg, P<int64_t>(sl->ofsN)));
         TRY(scan_u32(Q, P<uint32_t>(sl->cntC), nseg, P<int64_t>(sl->ofsC)));
         // per-query seed totals: the scanned offsets at each query's first segment
@@ -2090,21 +2119,21 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
     }
     TRY(ensure(Q, sl->seedN, (size_t) nbytes + 16));
     TRY(ensure(Q, sl->seedC, (size_t) cbytes + 16));
-    TRY(ensure(Q, sl->nbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
-    TRY(ensure(Q, sl->cbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
-    TRY(fill_u64(Q, sl->nbuck.p, nb, 0));
-    TRY(fill_u64(Q, sl->cbuck.p, nb, 0));
+    if (nseg == 0) {                                 // no pass ran: no seeds, zero bucks
+        TRY(ensure(Q, sl->nbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+        TRY(ensure(Q, sl->cbuck, (size_t) std::max<int64_t>(nb, 1) * sizeof(uint64_t)));
+        TRY(fill_u64(Q, sl->nbuck.p, nb, 0));
+        TRY(fill_u64(Q, sl->cbuck.p, nb, 0));
+    }
     HIPCHK(Q, hipMemcpyAsync(sl->qdev.p, sl->qd.data(), (size_t) nq * sizeof(QDev), hipMemcpyHostToDevice, Q->st));
-    if (nseg > 0 && npack > 0) {
-        const unsigned gj = grid_for(Q, (const void *) k_join_pack<true>, PK_T, 0, npack);
-        hipLaunchKernelGGL(k_join_pack<true>, dim3(gj), dim3(PK_T), 0, Q->st, (const int64_t *) P<int64_t>(sl->pstart),
-                           npack, (const int64_t *) P<int64_t>(sl->soff), (const JSeg *) P<JSeg>(sl->jseg),
-                           (const QSeg *) P<QSeg>(sl->segs), (const QDev *) P<QDev>(sl->qdev),
-                           (const Rec *) P<Rec>(T->skeys), (const Rec *) P<Rec>(Q->skeys),
-                           (const int64_t *) P<int64_t>(sl->tpos), T->jbits, Q->o.freq, nullptr, nullptr,
-                           (const int64_t *) P<int64_t>(sl->ofsN), (const int64_t *) P<int64_t>(sl->ofsC),
-                           P<uint8_t>(sl->seedN), P<uint8_t>(sl->seedC), P<unsigned long long>(sl->nbuck),
-                           P<unsigned long long>(sl->cbuck));
+    if (nseg > 0) {
+        hipLaunchKernelGGL(k_join_copy, dim3(grid_for(Q, (const void *) k_join_copy, 256, 0, (nseg + 255) / 256)),
+                           dim3(256), 0, Q->st, (const QSeg *) P<QSeg>(sl->segs), nseg,
+                           (const QDev *) P<QDev>(sl->qdev), (const uint32_t *) P<uint32_t>(sl->cntN),
+                           (const uint32_t *) P<uint32_t>(sl->cntC), (const int64_t *) P<int64_t>(sl->tofN),
+                           (const int64_t *) P<int64_t>(sl->tofC), (const int64_t *) P<int64_t>(sl->ofsN),
+                           (const int64_t *) P<int64_t>(sl->ofsC), (const uint8_t *) P<uint8_t>(sl->tmpN),
+                           (const uint8_t *) P<uint8_t>(sl->tmpC), P<uint8_t>(sl->seedN), P<uint8_t>(sl->seedC));
         LAUNCHCHK(Q);
     }
     HIPCHK(Q, hipEventRecord(sl->ev[1], Q->st));
@@ -2281,8 +2310,10 @@ int pga_server_footprint(pga_server *s, int64_t query_bases, int64_t *bytes)
     if (!s || query_bases < 0 || !bytes) return PGA_EINVAL;
     // Per base: at most 2 records (one per strand) in practice 0.8; per record two
     // 16 B partition buffers, 16 B kept keys, e <= 18 B entry, LCP, prefix, join
-    // position and flags (~70 B), plus up to ~2 seeds of s <= 19 B.
-    *bytes = query_bases * 120 + ((int64_t) 64 << 20);
+    // position and flags (~70 B); seeds (target anchors pair with query
+    // entries: ~6 seeds a query base at FREQ 10, s = 11 B) twice, in the
+    // join's staging streams and in the answer.
+    *bytes = query_bases * 200 + ((int64_t) 64 << 20);
     return PGA_OK;
 }
 

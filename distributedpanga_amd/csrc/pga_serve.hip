@@ -153,6 +153,7 @@ constexpr int PK_PER = PK_CAP / PK_T;  // anchor slots per thread
 constexpr int PK_BUCK = 1024;          // anchor contigs counted in LDS (the pack's first query)
 constexpr int PK_MAP = 1024;           // seeds of each stream an emit round maps in LDS
 static_assert(PK_CAP <= 2048, "slot indexes fit 11 bits of a map word");
+static_assert(PK_CAP / PK_MINC <= PK_T, "one thread per segment of a pack");
 
 // Largest segment in combined entries for a FREQ: 33W core target entries,
 // FREQ+2 of context each side, SEG_Q + 2(FREQ+2) query records.
@@ -240,7 +241,8 @@ struct PackLds {
     uint16_t qtp[PK_CAP];              // query slots: insertion point relative to the segment's tlo
     // per segment of the pack
     int64_t tlo[PK_SEG], qlo[PK_SEG];
-    int64_t bN[PK_SEG], bC[PK_SEG];    // seed byte base: byte = b + (global seed index) * s
+    uint32_t sN0[PK_SEG], sC0[PK_SEG]; // pack-local number of the segment's first N / C seed
+    uint32_t bN0[PK_SEG], bC0[PK_SEG]; // pack-local byte offset of the segment's N / C seeds
     uint16_t so[PK_SEG + 1], nt[PK_SEG], nq[PK_SEG], lo[PK_SEG], hi[PK_SEG];
     uint16_t ct0[PK_SEG], ct1[PK_SEG], cq0[PK_SEG], cq1[PK_SEG];
     int32_t q[PK_SEG];
@@ -249,7 +251,9 @@ struct PackLds {
     uint32_t bk[2][PK_BUCK];           // buck counters of the pack's first query (emit pass)
     uint32_t smap[2][PK_MAP];          // the emit round's N and C seeds: anchor slot | partner slot << 11 | plen << 22
     uint32_t scan[PK_T / 64 + 1];
-    int ns, npos, qa;
+    uint64_t scanw[PK_T / 64 + 1];
+    int ns, npos, qa, ovf;
+    unsigned long long baseN, baseC;   // the pack's byte ranges in the staging streams
 };
 
 __device__ __forceinline__ int pack_seg_of(const PackLds &L, int pos)
@@ -267,8 +271,7 @@ __device__ void pack_load(PackLds &L, int64_t p, const int64_t *__restrict__ pst
                           const int64_t *__restrict__ soff, const JSeg *__restrict__ js,
                           const QSeg *__restrict__ segs, const QDev *__restrict__ qd,
                           const Rec *__restrict__ tkeys, const Rec *__restrict__ qkeys,
-                          const int64_t *__restrict__ tpos, const int64_t *__restrict__ ofsN,
-                          const int64_t *__restrict__ ofsC, int H, bool emit)
+                          const int64_t *__restrict__ tpos, int H)
 {
     const int64_t s0 = pstart[p], s1 = pstart[p + 1];
     const int ns = (int) (s1 - s0);
@@ -288,14 +291,10 @@ __device__ void pack_load(PackLds &L, int64_t p, const int64_t *__restrict__ pst
         L.q[t] = q;
         L.cN[t] = 0;
         L.cC[t] = 0;
-        if (emit) {
-            const QDev d = qd[q];
-            L.sbytes[t] = (uint8_t) d.s;
-            L.Pb[t] = (uint8_t) d.P;
-            L.Cb[t] = (uint8_t) d.C;
-            L.bN[t] = d.nseed_off - ofsN[d.seg0] * (int64_t) d.s;
-            L.bC[t] = d.cseed_off - ofsC[d.seg0] * (int64_t) d.s;
-        }
+        const QDev d = qd[q];
+        L.sbytes[t] = (uint8_t) d.s;
+        L.Pb[t] = (uint8_t) d.P;
+        L.Cb[t] = (uint8_t) d.C;
         // valid window needs the query insertion points: filled below
         L.lo[t] = (uint16_t) j.trunc;            // trunc bits parked here until then
     }
@@ -379,16 +378,22 @@ __device__ void pack_load(PackLds &L, int64_t p, const int64_t *__restrict__ pst
     __syncthreads();
 }
 
-// S3/S4. Count (EMIT = false: seeds per segment) or emit the seeds of every pack.
-template <bool EMIT>
-__global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ pstart, int64_t npack,
+// S3. One pass per pack: seeds counted per segment and written, in stream
+// order, to the pack's own byte ranges of two staging streams (reserved with
+// one atomic each), with per-anchor-contig counts.  A pack whose range would
+// pass a staging stream's capacity writes no seeds; the host sees the
+// reserved total above the capacity, grows the streams and runs the pass
+// again.  k_join_copy then moves every segment's seeds to their place.
+__global__ __launch_bounds__(PK_T, 4) void k_join_pack(const int64_t *__restrict__ pstart, int64_t npack,
                                                     const int64_t *__restrict__ soff, const JSeg *__restrict__ js,
                                                     const QSeg *__restrict__ segs, const QDev *__restrict__ qd,
                                                     const Rec *__restrict__ tkeys, const Rec *__restrict__ qkeys,
                                                     const int64_t *__restrict__ tpos, int jbits, int freq,
                                                     uint32_t *__restrict__ cntN, uint32_t *__restrict__ cntC,
-                                                    const int64_t *__restrict__ ofsN, const int64_t *__restrict__ ofsC,
-                                                    uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC,
+                                                    int64_t *__restrict__ tofN, int64_t *__restrict__ tofC,
+                                                    uint8_t *__restrict__ tmpN, uint8_t *__restrict__ tmpC,
+                                                    unsigned long long capN, unsigned long long capC,
+                                                    unsigned long long *__restrict__ used,
                                                     unsigned long long *__restrict__ nbuck,
                                                     unsigned long long *__restrict__ cbuck)
 {
@@ -398,20 +403,19 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
     const bool swar = freq <= 16;
     const int H = swar ? 17 : freq + 1;
     for (int64_t p = blockIdx.x; p < npack; p += gridDim.x) {
-        pack_load(L, p, pstart, soff, js, segs, qd, tkeys, qkeys, tpos, ofsN, ofsC, H, EMIT);
-        if (EMIT)
-            for (int c = threadIdx.x; c < PK_BUCK; c += PK_T) { L.bk[0][c] = 0; L.bk[1][c] = 0; }
+        pack_load(L, p, pstart, soff, js, segs, qd, tkeys, qkeys, tpos, H);
+        for (int c = threadIdx.x; c < PK_BUCK; c += PK_T) { L.bk[0][c] = 0; L.bk[1][c] = 0; }
         const int npos = L.npos;
         const int pos0 = threadIdx.x * PK_PER;
+        const int qa = L.qa;
         uint32_t tn = 0, tc = 0;
-        uint32_t mN = 0, mC = 0;             // anchors of this thread: bit k set when slot k emits
+        uint32_t mE = 0;                     // anchors of this thread: bit k set when slot k emits
         // the anchors' intervals [ia, ib) and plen, kept for the emit
         int ia[PK_PER], ib[PK_PER], ipl[PK_PER];
-        uint32_t kn_[PK_PER], kc_[PK_PER];
+        __syncthreads();                     // bk cleared
 #pragma unroll
         for (int k = 0; k < PK_PER; k++) {
             ia[k] = ib[k] = ipl[k] = 0;
-            kn_[k] = kc_[k] = 0;
             const int pos = pos0 + k;
             if (pos >= npos || !(L.flg[pos] & 2)) continue;
             const int gi = H + pos;
@@ -438,41 +442,14 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
                 ia[k] = a;
                 ib[k] = b;
             }
-            kn_[k] = ln;
-            kc_[k] = lc;
+            if (!(ln | lc)) continue;
             tn += ln;
             tc += lc;
-            if (ln) mN |= 1u << k;
-            if (lc) mC |= 1u << k;
-            if (!EMIT && (ln | lc)) {
-                const int s = L.sidx[pos];
-                if (ln) atomicAdd(&L.cN[s], ln);
-                if (lc) atomicAdd(&L.cC[s], lc);
-            }
-        }
-        if (!EMIT) {
-            __syncthreads();
-            const int64_t s0 = pstart[p];
-            for (int t = threadIdx.x; t < L.ns; t += PK_T) { cntN[s0 + t] = L.cN[t]; cntC[s0 + t] = L.cC[t]; }
-            __syncthreads();
-            continue;
-        }
-        // emit: pack-local seed numbers of this thread's first N / C seed; the
-        // pack's seeds are numbered in stream order (segments, anchors,
-        // partners ascending), starting at the first segment's stream offset
-        uint32_t totN, totC;
-        const uint32_t rn = block_excl_scan<PK_T, uint32_t>(tn, L.scan, &totN);
-        const uint32_t rc = block_excl_scan<PK_T, uint32_t>(tc, L.scan, &totC);
-        const int64_t s0 = pstart[p];
-        const int64_t gN0 = L.ns > 0 ? ofsN[s0] : 0, gC0 = L.ns > 0 ? ofsC[s0] : 0;
-        const int qa = L.qa;
-        // per-anchor-contig counts
-#pragma unroll
-        for (int k = 0; k < PK_PER; k++) {
-            if (!(((mN | mC) >> k) & 1u)) continue;
-            const int pos = pos0 + k;
-            const uint32_t ln = kn_[k], lc = kc_[k];
+            mE |= 1u << k;
             const int s = L.sidx[pos];
+            if (ln) atomicAdd(&L.cN[s], ln);
+            if (lc) atomicAdd(&L.cC[s], lc);
+            // per-anchor-contig counts (src/fastga_pipeline.c:403)
             const int P = L.Pb[s], Cb = L.Cb[s];
             const uint64_t pi = key_payload(L.key[pos], jbits, P, Cb);
             const int icont = (int) ((pi >> (8 * P)) & ((1ull << (8 * Cb - 1)) - 1));
@@ -485,17 +462,54 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
                 if (lc) atomicAdd(&cbuck[bo], (unsigned long long) lc);
             }
         }
+        // pack-local seed numbers of this thread's first N / C seed (seeds are
+        // numbered in stream order: segments, anchors, partners ascending)
+        uint32_t totN, totC;
+        const uint32_t rn = block_excl_scan<PK_T, uint32_t>(tn, L.scan, &totN);
+        const uint32_t rc = block_excl_scan<PK_T, uint32_t>(tc, L.scan, &totC);
+        // the segments' first seeds and byte offsets inside the pack (thread t:
+        // segment t; a pack holds at most PK_SEG <= PK_T segments)
+        const int ns = L.ns;
+        const int t = threadIdx.x;
+        const uint32_t cn = t < ns ? L.cN[t] : 0u, cc = t < ns ? L.cC[t] : 0u;
+        const uint32_t sb = t < ns ? L.sbytes[t] : 0u;
+        uint64_t totb;
+        const uint64_t on = block_excl_scan<PK_T, uint64_t>((uint64_t) cn | (uint64_t) (cn * sb) << 32, L.scanw, &totb);
+        const uint64_t totNb = totb >> 32;
+        const uint64_t oc = block_excl_scan<PK_T, uint64_t>((uint64_t) cc | (uint64_t) (cc * sb) << 32, L.scanw, &totb);
+        const uint64_t totCb = totb >> 32;
+        if (t < ns) {
+            L.sN0[t] = (uint32_t) on;
+            L.bN0[t] = (uint32_t) (on >> 32);
+            L.sC0[t] = (uint32_t) oc;
+            L.bC0[t] = (uint32_t) (oc >> 32);
+        }
+        if (t == 0) {
+            const unsigned long long bn = totNb ? atomicAdd(&used[0], (unsigned long long) totNb) : 0ull;
+            const unsigned long long bc = totCb ? atomicAdd(&used[1], (unsigned long long) totCb) : 0ull;
+            L.baseN = bn;
+            L.baseC = bc;
+            L.ovf = bn + totNb > capN || bc + totCb > capC;
+        }
+        __syncthreads();
+        const int64_t s0 = pstart[p];
+        if (t < ns) {
+            cntN[s0 + t] = cn;
+            cntC[s0 + t] = cc;
+            tofN[s0 + t] = (int64_t) (L.baseN + L.bN0[t]);
+            tofC[s0 + t] = (int64_t) (L.baseC + L.bC0[t]);
+        }
         // Rounds of up to PK_MAP seeds a stream: every anchor lists its seeds
         // in an LDS map (anchor slot | partner slot << 11 | plen << 22), then
         // one lane takes one seed, so a wave's stores cover a contiguous run.
-        const int rounds = (int) ((max(totN, totC) + PK_MAP - 1) / PK_MAP);
+        const int rounds = L.ovf ? 0 : (int) ((max(totN, totC) + PK_MAP - 1) / PK_MAP);
         for (int r = 0; r < rounds; r++) {
             const int n0 = r * PK_MAP, n1 = max(n0, min(n0 + PK_MAP, (int) totN));
             const int c0 = r * PK_MAP, c1 = max(c0, min(c0 + PK_MAP, (int) totC));
             int kn = (int) rn, kc = (int) rc;
 #pragma unroll 1
             for (int k = 0; k < PK_PER; k++) {
-                if (!(((mN | mC) >> k) & 1u)) continue;
+                if (!((mE >> k) & 1u)) continue;
                 const int pos = pos0 + k;
                 const int gi = H + pos;
                 const bool iq = L.flg[pos] & 1;
@@ -518,26 +532,27 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
             const int nN = n1 - n0, nR = nN + c1 - c0;
             for (int u = threadIdx.x; u < nR; u += PK_T) {
                 const int st = u >= nN;
-                const int t = st ? u - nN : u;
-                const uint32_t m = L.smap[st][t];
+                const int i = st ? u - nN + c0 : u + n0;          // pack-local seed number
+                const uint32_t m = L.smap[st][st ? i - c0 : i - n0];
                 const int pos = (int) (m & 0x7FFu), j = (int) ((m >> 11) & 0x7FFu);
                 const int s = L.sidx[pos];
-                const int P = L.Pb[s], Cb = L.Cb[s], sb = L.sbytes[s], pc = P + Cb;
+                const int P = L.Pb[s], Cb = L.Cb[s], sbs = L.sbytes[s], pc = P + Cb;
                 const uint64_t pi = key_payload(L.key[pos], jbits, P, Cb) & ~(0x80ull << (8 * (pc - 1)));
                 const uint64_t pq = key_payload(L.key[j], jbits, P, Cb);
-                uint8_t *dst = st ? seedC + L.bC[s] + (gC0 + c0 + t) * sb : seedN + L.bN[s] + (gN0 + n0 + t) * sb;
-                // the seed's exact sb = 1 + 2 pc bytes in two overlapping stores
+                uint8_t *dst = st ? tmpC + L.baseC + L.bC0[s] + (uint64_t) (i - (int) L.sC0[s]) * sbs
+                                  : tmpN + L.baseN + L.bN0[s] + (uint64_t) (i - (int) L.sN0[s]) * sbs;
+                // the seed's exact sbs = 1 + 2 pc bytes in two overlapping stores
                 const uint64_t hd = (uint64_t) (m >> 22) | (pi << 8);
                 if (pc >= 4) {
                     uint64_t w0, w1;
                     seed_words(hd, pq, pc, w0, w1);
                     __builtin_memcpy(dst, &w0, 8);
-                    __builtin_memcpy(dst + sb - 8, &w1, 8);
+                    __builtin_memcpy(dst + sbs - 8, &w1, 8);
                 } else {
                     const uint64_t w = hd | (pq << (8 * (1 + pc)));
-                    const uint32_t x0 = (uint32_t) w, x1 = (uint32_t) (w >> (8 * (sb - 4)));
+                    const uint32_t x0 = (uint32_t) w, x1 = (uint32_t) (w >> (8 * (sbs - 4)));
                     __builtin_memcpy(dst, &x0, 4);
-                    __builtin_memcpy(dst + sb - 4, &x1, 4);
+                    __builtin_memcpy(dst + sbs - 4, &x1, 4);
                 }
             }
             __syncthreads();
@@ -552,6 +567,40 @@ __global__ __launch_bounds__(PK_T) void k_join_pack(const int64_t *__restrict__ 
             }
         }
         __syncthreads();
+    }
+}
+
+// S4. Every segment's seeds from the staging streams to their place in their
+// query's streams (global seed index ofs[s] onward); one thread a segment,
+// 8 bytes at a time (the last 8 bytes overlapping).
+__device__ __forceinline__ void copy_bytes(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src, int64_t n)
+{
+    if (n < 8) {
+        for (int64_t b = 0; b < n; b++) dst[b] = src[b];
+        return;
+    }
+    for (int64_t b = 0; b < n; b += 8) {
+        const int64_t o = b + 8 <= n ? b : n - 8;
+        uint64_t w;
+        __builtin_memcpy(&w, src + o, 8);
+        __builtin_memcpy(dst + o, &w, 8);
+    }
+}
+
+__global__ void k_join_copy(const QSeg *__restrict__ segs, int64_t nseg, const QDev *__restrict__ qd,
+                            const uint32_t *__restrict__ cntN, const uint32_t *__restrict__ cntC,
+                            const int64_t *__restrict__ tofN, const int64_t *__restrict__ tofC,
+                            const int64_t *__restrict__ ofsN, const int64_t *__restrict__ ofsC,
+                            const uint8_t *__restrict__ tmpN, const uint8_t *__restrict__ tmpC,
+                            uint8_t *__restrict__ seedN, uint8_t *__restrict__ seedC)
+{
+    for (int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
+         s += (int64_t) gridDim.x * blockDim.x) {
+        const QDev d = qd[segs[s].q];
+        if (cntN[s])
+            copy_bytes(seedN + d.nseed_off + (ofsN[s] - ofsN[d.seg0]) * d.s, tmpN + tofN[s], (int64_t) cntN[s] * d.s);
+        if (cntC[s])
+            copy_bytes(seedC + d.cseed_off + (ofsC[s] - ofsC[d.seg0]) * d.s, tmpC + tofC[s], (int64_t) cntC[s] * d.s);
     }
 }
 
