@@ -1897,6 +1897,7 @@ struct pga_slot {
     DBuf jseg, cost, soff, pstart, err;
     DBuf tofN, tofC, tmpN, tmpC, used;    // the join's staging streams and each segment's place in them
     unsigned long long capN = 0, capC = 0;
+    double rateN = 0, rateC = 0;          // largest staging bytes per query record seen so far
     hipEvent_t ev[2] = {};
     double last_ms = 0.0;
 };
@@ -2063,8 +2064,10 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
         TRY(ensure(Q, sl->tofN, (size_t) nseg * sizeof(int64_t)));
         TRY(ensure(Q, sl->tofC, (size_t) nseg * sizeof(int64_t)));
         TRY(ensure(Q, sl->used, 2 * sizeof(uint64_t)));
-        sl->capN = std::max<unsigned long long>(sl->capN, (unsigned long long) RQ * 64);
-        sl->capC = std::max<unsigned long long>(sl->capC, (unsigned long long) RQ * 32);
+        // sized from the largest rate seen (a slot's first batch with seeds
+        // runs the pass twice: once to learn it)
+        sl->capN = std::max<unsigned long long>(sl->capN, (unsigned long long) (sl->rateN * 1.125 * (double) RQ) + 256);
+        sl->capC = std::max<unsigned long long>(sl->capC, (unsigned long long) (sl->rateC * 1.125 * (double) RQ) + 256);
         const unsigned gj = grid_for(Q, (const void *) k_join_pack, PK_T, 0, npack);
         for (int attempt = 0;; attempt++) {
             TRY(ensure(Q, sl->tmpN, (size_t) sl->capN + 16));
@@ -2085,6 +2088,10 @@ int slot_run(pga_server *s, pga_slot *sl, int nq, const uint8_t *const *bps, con
             unsigned long long u[2] = {0, 0};
             HIPCHK(Q, hipMemcpyAsync(u, sl->used.p, sizeof(u), hipMemcpyDeviceToHost, Q->st));
             HIPCHK(Q, hipStreamSynchronize(Q->st));
+            if (RQ > 0) {
+                sl->rateN = std::max(sl->rateN, (double) u[0] / (double) RQ);
+                sl->rateC = std::max(sl->rateC, (double) u[1] / (double) RQ);
+            }
             if (u[0] <= sl->capN && u[1] <= sl->capC) break;
             if (attempt > 0) return sfail(sl->err_msg, PGA_EINVAL, "internal: join staging outgrew its exact size");
             sl->capN = std::max(sl->capN, u[0] + (u[0] >> 3));
