@@ -96,7 +96,7 @@ struct pga_ctx {
     // device buffers
     DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
-        mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax;
+        mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax, cbmax;
     // extraction ranges (one workgroup's run of blocks inside a pair), their
     // first-digit counts and bucket offsets, and each pair's bucket starts
     DBuf xrange, prange, rangehist, rangeoff, bstart;
@@ -294,7 +294,7 @@ int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *seg
 // bufs: device array {buffer 0, buffer 1} the tiles point into; entries ==
 // nullptr sorts keys only (into skeys).
 int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entries, uint8_t *ls, uint32_t *pre,
-                     Rec *skeys, int *nfall, const int *dn = nullptr)
+                     unsigned long long *cum, Rec *skeys, int *nfall, const int *dn = nullptr)
 {
     const int T = c->o.tile_records / 8;
     const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
@@ -303,13 +303,35 @@ int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entri
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
                        dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, c->o.digit2_bits, entries, ls,   \
-                       pre, skeys, (int64_t) nlog, nfall, dn)
+                       pre, cum, skeys, (int64_t) nlog, nfall, dn)
     switch (T) {
     case 256: PGA_TSORT(256); break;
     case 1024: PGA_TSORT(1024); break;
     default: PGA_TSORT(512); break;
     }
 #undef PGA_TSORT
+    LAUNCHCHK(c);
+    return PGA_OK;
+}
+
+// The prefix tables of all pairs from the tile sort's group-end marks (the
+// tables were cleared before it): range maxima, carries, fill; the largest
+// prefix group per pair to maxpre.
+int fill_prefix_tables(pga_ctx *c, int npairs, const int *plist_all)
+{
+    const int nrange = NPRE / CR_SPAN;
+    TRY(ensure(c, c->cbmax, (size_t) npairs * nrange * sizeof(uint64_t)));
+    TRY(ensure(c, c->gmax, (size_t) npairs * nrange * sizeof(uint64_t)));
+    hipLaunchKernelGGL(k_cum_bmax, dim3(nrange, (unsigned) npairs), dim3(256), 0, c->st,
+                       (const PairDev *) P<PairDev>(c->dpairs), (const int64_t *) P<int64_t>(c->cum),
+                       P<unsigned long long>(c->cbmax));
+    hipLaunchKernelGGL(k_cum_carry, dim3((unsigned) npairs), dim3(256), 0, c->st, P<unsigned long long>(c->cbmax), nrange);
+    hipLaunchKernelGGL(k_cum_fill, dim3(nrange, (unsigned) npairs), dim3(256), 0, c->st,
+                       (const PairDev *) P<PairDev>(c->dpairs), P<int64_t>(c->cum),
+                       (const unsigned long long *) P<unsigned long long>(c->cbmax), P<unsigned long long>(c->gmax));
+    hipLaunchKernelGGL(k_cum_max, dim3((unsigned) npairs), dim3(256), 0, c->st, plist_all,
+                       (const unsigned long long *) P<unsigned long long>(c->gmax), nrange,
+                       P<unsigned long long>(c->maxpre));
     LAUNCHCHK(c);
     return PGA_OK;
 }
@@ -420,7 +442,7 @@ void pga_destroy(pga_ctx *c)
     DBuf *all[] = {&c->bps, &c->ctg, &c->bctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
-                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
+                   &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->cbmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
                    &c->seedN, &c->seedC, &c->nbuck, &c->cbuck, &c->skeys,
                    &c->xrange, &c->prange, &c->rangehist, &c->rangeoff, &c->bstart};
     for (DBuf *b : all)
@@ -820,11 +842,17 @@ int pga_build_index(pga_ctx *c)
     c->tm.partition_launches = c->nscatter;
     TRY(record(c, 2));
 
-    // ---- per-tile LDS sort, LCP, GIX entries
+    // ---- per-tile LDS sort, LCP, GIX entries; group ends marked in the
+    // cleared prefix tables
+    if (c->o.build_cum) {
+        TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
+        TRY(fill_u64(c, c->cum.p, (int64_t) npairs * NPRE, 0));
+    }
+    unsigned long long *cum_marks = c->o.build_cum ? P<unsigned long long>(c->cum) : nullptr;
     if (c->ntiles > 0)
         TRY(launch_tile_sort(c, (unsigned) c->ntiles, (Rec *const *) c->bufs.p, P<uint8_t>(c->entries),
-                             P<uint8_t>(c->ls), P<uint32_t>(c->pre), c->keep_keys ? P<Rec>(c->skeys) : nullptr,
-                             P<int>(c->counters) + 10));
+                             P<uint8_t>(c->ls), P<uint32_t>(c->pre), cum_marks,
+                             c->keep_keys ? P<Rec>(c->skeys) : nullptr, P<int>(c->counters) + 10));
     TRY(record(c, 3));
     if (c->ntiles > 0) {
         hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->ntiles + 255) / 256)),
@@ -839,27 +867,12 @@ int pga_build_index(pga_ctx *c)
     TRY(ensure(c, c->maxpre, (size_t) npairs * sizeof(uint64_t)));
     TRY(fill_u64(c, c->maxpre.p, npairs, 0));
     if (c->o.build_cum) {
-        TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
-        std::vector<int> live;
-        for (int p = 0; p < npairs; p++) {
-            if (c->pairs[p].nrec == 0) TRY(fill_u64(c, P<int64_t>(c->cum) + (int64_t) p * NPRE, NPRE, 0));
-            else live.push_back(p);
-        }
-        if (!live.empty()) {
-            const int nrange = NPRE / CR_SPAN;
-            TRY(ensure(c, c->plist, live.size() * sizeof(int)));
-            TRY(ensure(c, c->gmax, live.size() * nrange * sizeof(uint64_t)));
-            HIPCHK(c, hipMemcpyAsync(c->plist.p, live.data(), live.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
-            hipLaunchKernelGGL(k_cum_ranges, dim3(nrange, (unsigned) live.size()), dim3(256), 0, c->st,
-                               (const PairDev *) P<PairDev>(c->dpairs), (const int *) P<int>(c->plist),
-                               (const uint32_t *) P<uint32_t>(c->pre), P<int64_t>(c->cum),
-                               P<unsigned long long>(c->gmax));
-            LAUNCHCHK(c);
-            hipLaunchKernelGGL(k_cum_max, dim3((unsigned) live.size()), dim3(256), 0, c->st,
-                               (const int *) P<int>(c->plist), (const unsigned long long *) P<unsigned long long>(c->gmax),
-                               nrange, P<unsigned long long>(c->maxpre));
-            LAUNCHCHK(c);
-        }
+        std::vector<int> all(npairs);
+        for (int p = 0; p < npairs; p++) all[p] = p;
+        TRY(ensure(c, c->plist, (size_t) npairs * sizeof(int)));
+        HIPCHK(c, hipMemcpyAsync(c->plist.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+        TRY(fill_prefix_tables(c, npairs, (const int *) P<int>(c->plist)));
+        HIPCHK(c, hipStreamSynchronize(c->st));       // the pair list is a local
     }
     TRY(record(c, 4));
     // no wait here: the merge can queue behind the build; the stage times are
@@ -1296,7 +1309,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         TRY(rc);
         if (sort_tiles > 0)
             TRY(launch_tile_sort(c, (unsigned) sort_tiles, (Rec *const *) c->ssbufs.p, nullptr, nullptr, nullptr,
-                                 P<Rec>(c->ssk[0]), nullptr));
+                                 nullptr, P<Rec>(c->ssk[0]), nullptr));
         const int64_t nb = (n + SP_T * SP_PER - 1) / (SP_T * SP_PER);
         hipLaunchKernelGGL(k_seed_pack, dim3(grid_for(c, (const void *) k_seed_pack, SP_T, 0, nb)), dim3(SP_T), 0,
                            c->st, (const Rec *) P<Rec>(c->ssk[0]), n, kf, dbyte, swide, P<uint8_t>(c->ssout));
@@ -1421,6 +1434,7 @@ int graph_prepare(pga_ctx *c)
     TRY(ensure(c, c->cum, (size_t) npairs * NPRE * sizeof(int64_t)));
     TRY(ensure(c, c->plist_all, (size_t) npairs * sizeof(int)));
     TRY(ensure(c, c->gmax, (size_t) npairs * (NPRE / CR_SPAN) * sizeof(uint64_t)));
+    TRY(ensure(c, c->cbmax, (size_t) npairs * (NPRE / CR_SPAN) * sizeof(uint64_t)));
     TRY(ensure(c, c->mblk, (size_t) (npairs + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->mcntN, (size_t) (nmb_ub + 2) * sizeof(uint32_t)));
     TRY(ensure(c, c->mcntC, (size_t) (nmb_ub + 2) * sizeof(uint32_t)));
@@ -1545,26 +1559,21 @@ int graph_enqueue(pga_ctx *c)
                                    P<unsigned long long>(c->cursor), P<uint8_t>(c->ls), dn));
         cur ^= 1;
     }
-    // ---- tile sort, LCP fix-up, prefix tables
+    // ---- tile sort (group ends marked in the cleared prefix tables), LCP
+    // fix-up, prefix tables
     hipLaunchKernelGGL(k_clamp_count, dim3(1), dim3(64), 0, c->st, &gs->ntiles, (int) c->g_tile_cap, &gs->capped);
+    if (c->o.build_cum) TRY(fill_u64(c, c->cum.p, (int64_t) npairs * NPRE, 0));
     TRY(launch_tile_sort(c, (unsigned) c->g_tile_cap, (Rec *const *) c->bufs.p, P<uint8_t>(c->entries),
-                         P<uint8_t>(c->ls), P<uint32_t>(c->pre), nullptr, P<int>(c->counters) + 10, &gs->ntiles));
+                         P<uint8_t>(c->ls), P<uint32_t>(c->pre),
+                         c->o.build_cum ? P<unsigned long long>(c->cum) : nullptr, nullptr,
+                         P<int>(c->counters) + 10, &gs->ntiles));
     hipLaunchKernelGGL(k_tile_fix, dim3(grid_for(c, (const void *) k_tile_fix, 256, 0, (c->g_tile_cap + 255) / 256)),
                        dim3(256), 0, c->st, (const Tile *) P<Tile>(c->tiles), 0, (const PairDev *) P<PairDev>(c->dpairs),
                        P<uint8_t>(c->entries), P<uint8_t>(c->ls), (const uint32_t *) P<uint32_t>(c->pre),
                        (const int *) &gs->ntiles);
     LAUNCHCHK(c);
     TRY(fill_u64(c, c->maxpre.p, npairs, 0));
-    if (c->o.build_cum) {
-        const int nrange = NPRE / CR_SPAN;
-        hipLaunchKernelGGL(k_cum_ranges, dim3(nrange, (unsigned) npairs), dim3(256), 0, c->st,
-                           (const PairDev *) P<PairDev>(c->dpairs), (const int *) P<int>(c->plist_all),
-                           (const uint32_t *) P<uint32_t>(c->pre), P<int64_t>(c->cum), P<unsigned long long>(c->gmax));
-        hipLaunchKernelGGL(k_cum_max, dim3((unsigned) npairs), dim3(256), 0, c->st, (const int *) P<int>(c->plist_all),
-                           (const unsigned long long *) P<unsigned long long>(c->gmax), nrange,
-                           P<unsigned long long>(c->maxpre));
-        LAUNCHCHK(c);
-    }
+    if (c->o.build_cum) TRY(fill_prefix_tables(c, npairs, (const int *) P<int>(c->plist_all)));
     // ---- merge
     const int64_t nmb_ub = c->g_nmb_ub;
     uint32_t *minfo = reinterpret_cast<uint32_t *>(c->g_minfo);

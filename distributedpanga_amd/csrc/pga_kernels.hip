@@ -1097,6 +1097,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                                                  const PairDev *__restrict__ pairs, int jbits, int w2,
                                                  uint8_t *__restrict__ entries,
                                                  uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
+                                                 unsigned long long *__restrict__ cum,
                                                  Rec *__restrict__ skeys, int64_t nlog, int *__restrict__ nfall,
                                                  const int *__restrict__ dn)
 {
@@ -1452,8 +1453,16 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         const int idx = k * T + tid;
         if (idx < n) {
             const Rec x = sk[idx];
+            const int64_t li = tl.start - pd.rec0 + idx;      // the record's place in its pair
             uint64_t lcp = 0;
-            if (idx > 0) { const Rec y = sk[idx - 1]; lcp = (uint64_t) lcp_bases(y.w0, y.w1, x.w0, x.w1); }
+            if (idx > 0) {
+                const Rec y = sk[idx - 1];
+                lcp = (uint64_t) lcp_bases(y.w0, y.w1, x.w0, x.w1);
+                // the record before ends its prefix group: mark the table
+                if (cum && (y.w0 >> 40) != (x.w0 >> 40))
+                    atomicMax(&cum[pd.cum_off + (y.w0 >> 40)], (unsigned long long) li);
+            }
+            if (cum && idx == n - 1) atomicMax(&cum[pd.cum_off + (x.w0 >> 40)], (unsigned long long) (li + 1));
             const uint64_t strand = x.w1 & 1ull;
             const uint64_t j = (x.w1 >> 1) & jmask;
             const uint64_t contig = (x.w1 & 0xFFFFFFFFFFFFull) >> (jbits + 1);
@@ -1469,7 +1478,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             if (skeys) skeys[tl.start + idx] = x;
             // the entry's e bytes as two overlapping 8-byte stores (a wave's
             // lanes write one contiguous run of entries)
-            uint8_t *dst = entries + pd.ent_off + (tl.start - pd.rec0 + idx) * (int64_t) e;
+            uint8_t *dst = entries + pd.ent_off + li * (int64_t) e;
             const uint64_t tail = e == 16 ? ent.w1 : (ent.w0 >> (8 * (e - 8))) | (ent.w1 << (8 * (16 - e)));
             __builtin_memcpy(dst, &ent.w0, 8);
             __builtin_memcpy(dst + e - 8, &tail, 8);
@@ -1479,9 +1488,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *, const int *);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *, const int *);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, unsigned long long *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, unsigned long long *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, unsigned long long *, Rec *, int64_t, int *, const int *);
 
 // Pair of a global record index.
 // LCP of each tile's first record with its predecessor (0 at a pair's start),
@@ -1516,86 +1525,48 @@ __global__ void k_tile_fix(const Tile *__restrict__ tiles, int ntiles, const Pai
 // ---------------------------------------------------------------- prefix table
 
 // Prefix table of a pair (src/kmer_adapter.c:64-80: cum[p] = entries with
-// prefix <= p), one block per 4096 prefixes: block (x, y) covers prefixes
-// [x*CR_SPAN, (x+1)*CR_SPAN) of pair plist[y].  It finds the pair's records in
-// that range; the last record of each prefix group writes its running count
-// at its prefix in LDS (one writer per prefix), a block max-scan carries the
-// counts over the empty prefixes, and the range goes out in coalesced 8-byte
-// stores.  The range's largest prefix group (the largest step of the table)
-// goes to gmax.
+// prefix <= p) from the marks the tile sort leaves in it: the last record of
+// every prefix group inside a tile, and a tile's last record, raise the table
+// at their prefix to their count (a group split over tiles is marked from
+// each, so the largest mark wins); every other prefix holds 0.  Prefix ranges
+// of CR_SPAN: k_cum_bmax takes each range's largest mark, k_cum_carry the
+// largest mark before each range, and k_cum_fill carries that over the
+// range's empty prefixes (a block max-scan) and finds the range's largest
+// group (the largest step of the table), which k_cum_max reduces per pair.
 constexpr int CR_SPAN = 4096;
+constexpr int CR_PER = CR_SPAN / 256;
 
-__global__ __launch_bounds__(256) void k_cum_ranges(const PairDev *__restrict__ pairs, const int *__restrict__ plist,
-                                                    const uint32_t *__restrict__ pre, int64_t *__restrict__ cum,
-                                                    unsigned long long *__restrict__ gmax)
+// block (x, y): range x of pair y
+__global__ __launch_bounds__(256) void k_cum_bmax(const PairDev *__restrict__ pairs, const int64_t *__restrict__ cum,
+                                                  unsigned long long *__restrict__ bmax)
 {
-    constexpr int PER = CR_SPAN / 256;
-    __shared__ uint64_t val[CR_SPAN];
-    __shared__ uint64_t sh[256 / 64 + 1];
-    __shared__ unsigned long long gm;
-    __shared__ int64_t rng[2];
-    const PairDev pd = pairs[plist[blockIdx.y]];
-    const uint32_t p0 = blockIdx.x * (uint32_t) CR_SPAN;
-    const uint32_t *pp = pre + pd.rec0;
-    for (int k = threadIdx.x; k < CR_SPAN; k += 256) val[k] = 0;
-    if (threadIdx.x == 0) gm = 0;
-    // first record with prefix >= p0 (threads 0..127) and >= p0 + CR_SPAN
-    // (threads 128..255), both at once: 128-way searches, each round every
-    // thread tests one sample of its half's interval
-    {
-        __shared__ int hcount[2];
-        const int w = threadIdx.x >> 7, t = threadIdx.x & 127;
-        const uint32_t key = p0 + w * (uint32_t) CR_SPAN;
-        int64_t lo = 0, hi = pd.nrec;                  // answer in [lo, hi]
-        bool done = false;
-        while (true) {
-            const bool act = !done && hi > lo;
-            if (threadIdx.x < 2) hcount[threadIdx.x] = 0;
-            const int any = __syncthreads_or(act ? 1 : 0);
-            if (!any) break;
-            const int64_t step = act ? (hi - lo + 127) / 128 : 1;
-            const int64_t idx = lo + (int64_t) t * step;
-            if (act && idx < hi && pp[idx] < key) atomicAdd(&hcount[w], 1);
-            __syncthreads();
-            if (act) {
-                const int k = hcount[w];              // samples below key: a prefix of the samples
-                if (k == 0) {
-                    hi = lo;                          // pp[lo] >= key (lo is sample 0)
-                } else {
-                    const int64_t nlo = lo + (int64_t) (k - 1) * step + 1;
-                    const int64_t nhi = lo + (int64_t) k * step < hi ? lo + (int64_t) k * step : hi;
-                    lo = nlo;
-                    hi = nhi;
-                }
-                done = hi <= lo;
-            }
-            __syncthreads();
-        }
-        if (t == 0) rng[w] = lo;
+    __shared__ uint64_t red[256 / 64];
+    const PairDev pd = pairs[blockIdx.y];
+    const int64_t *cp = cum + pd.cum_off + (int64_t) blockIdx.x * CR_SPAN;
+    uint64_t m = 0;
+#pragma unroll
+    for (int k = 0; k < CR_PER; k++) {
+        const uint64_t x = (uint64_t) cp[k * 256 + threadIdx.x];
+        m = x > m ? x : m;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { const uint64_t x = __shfl_xor(m, o, 64); m = x > m ? x : m; }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    const int64_t lo = rng[0], cnt = rng[1] - rng[0];
-    // group ends, 8 records per thread per round with the loads issued first
-    constexpr int U = 8;
-    for (int64_t base = 0; base < cnt; base += 256 * U) {
-        uint32_t cur[U], nxt[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t i = base + u * 256 + threadIdx.x;
-            cur[u] = i < cnt ? pp[lo + i] : 0u;
-            nxt[u] = i + 1 < cnt ? pp[lo + i + 1] : 0xFFFFFFFFu;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t i = base + u * 256 + threadIdx.x;
-            if (i < cnt && nxt[u] != cur[u]) val[cur[u] - p0] = (uint64_t) (i + 1);
-        }
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) m = red[w] > m ? red[w] : m;
+        bmax[(size_t) blockIdx.y * gridDim.x + blockIdx.x] = m;
     }
-    __syncthreads();
-    // inclusive max-scan over the range, PER consecutive prefixes per thread
-    uint64_t v[PER], m = 0;
-#pragma unroll
-    for (int k = 0; k < PER; k++) { const uint64_t x = val[threadIdx.x * PER + k]; m = x > m ? x : m; v[k] = m; }
+}
+
+// block y: exclusive max-scan of pair y's nrange range maxima (in place)
+__global__ __launch_bounds__(256) void k_cum_carry(unsigned long long *__restrict__ bmax, int nrange)
+{
+    __shared__ uint64_t sh[256 / 64];
+    unsigned long long *b = bmax + (size_t) blockIdx.x * nrange;
+    const int per = (nrange + 255) / 256, k0 = threadIdx.x * per;
+    uint64_t m = 0;
+    for (int k = k0; k < k0 + per && k < nrange; k++) m = b[k] > m ? b[k] : m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint64_t inc = m;
 #pragma unroll
@@ -1605,23 +1576,54 @@ __global__ __launch_bounds__(256) void k_cum_ranges(const PairDev *__restrict__ 
     }
     if (lane == 63) sh[wv] = inc;
     __syncthreads();
-    uint64_t carry = __shfl_up(inc, 1, 64);            // the running count before this thread's prefixes
-    if (lane == 0) carry = 0;
-    for (int w = 0; w < wv; w++) carry = sh[w] > carry ? sh[w] : carry;
-    uint64_t prev = carry, big = 0;
-    int64_t *cp = cum + pd.cum_off + p0;
+    uint64_t run = __shfl_up(inc, 1, 64);
+    if (lane == 0) run = 0;
+    for (int w = 0; w < wv; w++) run = sh[w] > run ? sh[w] : run;
+    for (int k = k0; k < k0 + per && k < nrange; k++) {
+        const uint64_t x = b[k];
+        b[k] = run;
+        run = x > run ? x : run;
+    }
+}
+
+// block (x, y): range x of pair y filled; its largest step to gmax
+__global__ __launch_bounds__(256) void k_cum_fill(const PairDev *__restrict__ pairs, int64_t *__restrict__ cum,
+                                                  const unsigned long long *__restrict__ carry,
+                                                  unsigned long long *__restrict__ gmax)
+{
+    __shared__ uint64_t sh[256 / 64];
+    __shared__ unsigned long long gm;
+    const PairDev pd = pairs[blockIdx.y];
+    int64_t *cp = cum + pd.cum_off + (int64_t) blockIdx.x * CR_SPAN + threadIdx.x * CR_PER;
+    const uint64_t cin = carry[(size_t) blockIdx.y * gridDim.x + blockIdx.x];
+    if (threadIdx.x == 0) gm = 0;
+    uint64_t v[CR_PER], m = cin;
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const uint64_t x = v[k] > carry ? v[k] : carry;
+    for (int k = 0; k < CR_PER; k++) { const uint64_t x = (uint64_t) cp[k]; m = x > m ? x : m; v[k] = m; }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t inc = m;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc = y > inc ? y : inc;
+    }
+    if (lane == 63) sh[wv] = inc;
+    __syncthreads();
+    uint64_t before = __shfl_up(inc, 1, 64);          // the count before this thread's prefixes
+    if (lane == 0) before = cin;
+    for (int w = 0; w < wv; w++) before = sh[w] > before ? sh[w] : before;
+    uint64_t prev = before, big = 0;
+#pragma unroll
+    for (int k = 0; k < CR_PER; k++) {
+        const uint64_t x = v[k] > before ? v[k] : before;
         big = x - prev > big ? x - prev : big;
         prev = x;
-        val[threadIdx.x * PER + k] = x;
+        cp[k] = (int64_t) x;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) { const uint64_t x = __shfl_xor(big, o, 64); big = x > big ? x : big; }
     if (lane == 0) atomicMax(&gm, (unsigned long long) big);
     __syncthreads();
-    for (int k = threadIdx.x; k < CR_SPAN; k += 256) cp[k] = lo + (int64_t) val[k];
     if (threadIdx.x == 0) gmax[(size_t) blockIdx.y * gridDim.x + blockIdx.x] = gm;
 }
 
