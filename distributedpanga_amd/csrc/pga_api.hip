@@ -97,6 +97,7 @@ struct pga_ctx {
     DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax, cbmax;
+    bool pre_full = true;             // pre holds every record's prefix (else the tile ends' only)
     // extraction ranges (one workgroup's run of blocks inside a pair), their
     // first-digit counts and bucket offsets, and each pair's bucket starts
     DBuf xrange, prange, rangehist, rangeoff, bstart;
@@ -296,6 +297,10 @@ int launch_seg_scatter(pga_ctx *c, int per, Rec *buf0, Rec *buf1, const Seg *seg
 int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entries, uint8_t *ls, uint32_t *pre,
                      unsigned long long *cum, Rec *skeys, int *nfall, const int *dn = nullptr)
 {
+    // every record's prefix only where something reads it (kept keys, no
+    // prefix table); otherwise the tile ends' for the LCP fix-up
+    const int pre_all = (!cum || skeys) ? 1 : 0;
+    if (entries) c->pre_full = pre_all != 0;
     const int T = c->o.tile_records / 8;
     const size_t lds = (size_t) c->o.tile_records * sizeof(Rec) + (size_t) c->o.tile_records / 2 * 4 + 16;
     const Tile *tiles = P<Tile>(c->tiles);
@@ -303,7 +308,7 @@ int launch_tile_sort(pga_ctx *c, unsigned nlog, Rec *const *bufs, uint8_t *entri
 #define PGA_TSORT(TV)                                                                                   \
     hipLaunchKernelGGL(k_tile_sort<TV>, dim3(grid_for(c, (const void *) k_tile_sort<TV>, TV, lds, nlog)), \
                        dim3(TV), lds, c->st, bufs, tiles, pairs, c->jbits, c->o.digit2_bits, entries, ls,   \
-                       pre, cum, skeys, (int64_t) nlog, nfall, dn)
+                       pre, pre_all, cum, skeys, (int64_t) nlog, nfall, dn)
     switch (T) {
     case 256: PGA_TSORT(256); break;
     case 1024: PGA_TSORT(1024); break;
@@ -931,7 +936,9 @@ int pga_index_verify(pga_ctx *c, int pair, int64_t *bad, int64_t *first_bad, int
     if (d.nrec > 0) {
         hipLaunchKernelGGL(k_verify, dim3(grid_for(c, (const void *) k_verify, 256, 0, (d.nrec + 255) / 256)), dim3(256),
                            0, c->st, (const uint8_t *) P<uint8_t>(c->entries) + d.ent_off, d.e,
-                           (const uint32_t *) P<uint32_t>(c->pre), d.rec0, d.nrec, P<unsigned long long>(c->vout));
+                           (const uint32_t *) P<uint32_t>(c->pre),
+                           c->pre_full ? (const int64_t *) nullptr : (const int64_t *) P<int64_t>(c->cum) + d.cum_off,
+                           d.rec0, d.nrec, P<unsigned long long>(c->vout));
         LAUNCHCHK(c);
     }
     HIPCHK(c, hipMemcpyAsync(res, c->vout.p, sizeof(res), hipMemcpyDeviceToHost, c->st));

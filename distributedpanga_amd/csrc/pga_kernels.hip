@@ -1096,7 +1096,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                                                  const Tile *__restrict__ tiles,
                                                  const PairDev *__restrict__ pairs, int jbits, int w2,
                                                  uint8_t *__restrict__ entries,
-                                                 uint8_t *__restrict__ ls, uint32_t *__restrict__ pre,
+                                                 uint8_t *__restrict__ ls, uint32_t *__restrict__ pre, int pre_all,
                                                  unsigned long long *__restrict__ cum,
                                                  Rec *__restrict__ skeys, int64_t nlog, int *__restrict__ nfall,
                                                  const int *__restrict__ dn)
@@ -1474,7 +1474,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             ent.w0 = __builtin_bswap64(low56 << 8);
             ent.w1 = lcp | (pos << 8) | (cs << (8 + 8 * P));
             ls[tl.start + idx] = (uint8_t) (lcp | (strand << 7));
-            pre[tl.start + idx] = (uint32_t) (x.w0 >> 40);
+            // the 24-bit prefix: every record's when asked for (kept keys),
+            // else only the tile's ends (the LCP fix-up across tiles reads them)
+            if (pre_all || idx == 0 || idx == n - 1) pre[tl.start + idx] = (uint32_t) (x.w0 >> 40);
             if (skeys) skeys[tl.start + idx] = x;
             // the entry's e bytes as two overlapping 8-byte stores (a wave's
             // lanes write one contiguous run of entries)
@@ -1488,9 +1490,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     }
 }
 
-template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, unsigned long long *, Rec *, int64_t, int *, const int *);
-template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, unsigned long long *, Rec *, int64_t, int *, const int *);
-template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, unsigned long long *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<256>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, int, unsigned long long *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<512>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, int, unsigned long long *, Rec *, int64_t, int *, const int *);
+template __global__ void k_tile_sort<1024>(Rec *const *, const Tile *, const PairDev *, int, int, uint8_t *, uint8_t *, uint32_t *, int, unsigned long long *, Rec *, int64_t, int *, const int *);
 
 // Pair of a global record index.
 // LCP of each tile's first record with its predecessor (0 at a pair's start),
@@ -2248,27 +2250,63 @@ __device__ __forceinline__ void entry_kmer(const uint8_t *__restrict__ ent, int 
     uint64_t suf = 0;
 #pragma unroll
     for (int b = 0; b < 7; b++) suf = (suf << 8) | x[b];
-    k0 = ((uint64_t) pre[g] << 40) | (suf >> 16);
+    k0 = (pre ? (uint64_t) pre[g] << 40 : 0ull) | (suf >> 16);
     k1 = (suf & 0xFFFFull) << 48;
+}
+
+// Prefix of pair entry i from the pair's prefix table: the first p in
+// [lo, hi] with cum[p] > i (cum[hi] > i).
+__device__ __forceinline__ uint32_t prefix_of(const int64_t *__restrict__ cum, int64_t i, uint32_t lo, uint32_t hi)
+{
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cum[mid] > i) hi = mid; else lo = mid + 1;
+    }
+    return lo;
 }
 
 // Sortedness and LCP check of one pair's index (src/dsort.c:229-246): bad
 // counts entries whose k-mer is smaller than its predecessor's; lcp_bad
 // counts entries whose LCP byte differs from the recomputed LCP with the
 // predecessor (0 for the first entry).  out = {bad, first_bad, lcp_bad}.
-__global__ void k_verify(const uint8_t *__restrict__ ent, int e, const uint32_t *__restrict__ pre, int64_t rec0,
-                         int64_t nrec, unsigned long long *__restrict__ out)
+// Prefixes come from pre (every record's) or, when cum is given, from the
+// prefix table: a block's 256 consecutive entries search only between the
+// prefixes of the entries before and after them.
+__global__ void k_verify(const uint8_t *__restrict__ ent, int e, const uint32_t *__restrict__ pre,
+                         const int64_t *__restrict__ cum, int64_t rec0, int64_t nrec,
+                         unsigned long long *__restrict__ out)
 {
     __shared__ uint64_t red[256 / 64 + 1];
+    __shared__ uint32_t span[2];
     uint64_t bad = 0, lbad = 0, first = ~0ull;
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nrec;
-         i += (int64_t) gridDim.x * blockDim.x) {
+    // grid-stride over blocks of 256 consecutive entries (uniform trip count)
+    for (int64_t b = (int64_t) blockIdx.x * blockDim.x; b < nrec; b += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t i = b + threadIdx.x;
+        uint32_t plo = 0, phi = 0;
+        if (cum) {
+            __syncthreads();                            // the previous round is done with span
+            if (threadIdx.x == 0) span[0] = prefix_of(cum, b > 0 ? b - 1 : 0, 0, (1u << 24) - 1);
+            if (threadIdx.x == 1) span[1] = prefix_of(cum, min(b + 255, nrec - 1), 0, (1u << 24) - 1);
+            __syncthreads();
+            plo = span[0];
+            phi = span[1];
+        }
+        if (i >= nrec) continue;
+        auto kmer = [&](int64_t x, uint64_t &k0, uint64_t &k1) {
+            if (cum) {
+                const uint64_t p = prefix_of(cum, x, plo, phi);
+                entry_kmer(ent, e, nullptr, 0, x, k0, k1);
+                k0 |= p << 40;
+            } else {
+                entry_kmer(ent, e, pre, rec0 + x, x, k0, k1);
+            }
+        };
         uint64_t b0, b1;
-        entry_kmer(ent, e, pre, rec0 + i, i, b0, b1);
+        kmer(i, b0, b1);
         int l = 0;
         if (i > 0) {
             uint64_t a0, a1;
-            entry_kmer(ent, e, pre, rec0 + i - 1, i - 1, a0, a1);
+            kmer(i - 1, a0, a1);
             if (a0 > b0 || (a0 == b0 && a1 > b1)) { bad++; if ((uint64_t) i < first) first = (uint64_t) i; }
             l = lcp_bases(a0, a1, b0, b1);
         }
