@@ -233,16 +233,17 @@ def human_pair(scale, cache=None):
 
 def kernel_lines(info, nseed, per, traffic_profile):
     """The step's two longest launches, each with its algorithmic bytes per
-    launch.  Tile sort: every record read once (16 B); its GIX entry (e B),
-    LCP/strand byte and 24-bit prefix (4 B) written once.  Merge emit: every
-    GIX entry read once (e B), every seed (s B) and the two buck vectors
-    written once.  Returns (dominant, other)."""
+    launch.  Tile sort: every record read once (16 B); its GIX entry (e B) and
+    LCP/strand byte written once (the prefix table gets group-end marks, a few
+    per tile, not counted).  Merge emit: every GIX entry read once (e B),
+    every seed (s B) and the two buck vectors written once.  Returns
+    (dominant, other)."""
     def line(name, nbytes, ms):
         gbs = nbytes / (ms * 1e-3) / 1e9
         traffic, src = measured_traffic(name, traffic_profile)
         return dict(kernel=name, achieved=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
                     traffic_source=src, bytes_per_launch=int(nbytes), avg_launch_ms=round(ms, 4))
-    sort_k = line("k_tile_sort", info["nrec"] * (16 + info["e"] + 1 + 4), per["tile_sort_ms"])
+    sort_k = line("k_tile_sort", info["nrec"] * (16 + info["e"] + 1), per["tile_sort_ms"])
     emit_k = line("k_merge_emit", info["nrec"] * info["e"] + nseed * info["s"] + 16 * info["ncontig"],
                   per["merge_emit_ms"])
     return (emit_k, sort_k) if per["merge_emit_ms"] > per["tile_sort_ms"] else (sort_k, emit_k)
