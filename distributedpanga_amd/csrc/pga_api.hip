@@ -94,7 +94,7 @@ struct pga_ctx {
     std::vector<int64_t> nseeds, cseeds;
 
     // device buffers
-    DBuf bps, ctg, bctg, blkcnt, blkoff, bsum, pblk, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
+    DBuf bps, ctg, bctg, bsum, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax, cbmax;
     bool pre_full = true;             // pre holds every record's prefix (else the tile ends' only)
@@ -444,7 +444,7 @@ void pga_destroy(pga_ctx *c)
     if (!c) return;
     (void) hipSetDevice(c->o.device);
     if (c->st) (void) hipStreamSynchronize(c->st);
-    DBuf *all[] = {&c->bps, &c->ctg, &c->bctg, &c->blkcnt, &c->blkoff, &c->bsum, &c->pblk, &c->pr, &c->hist1,
+    DBuf *all[] = {&c->bps, &c->ctg, &c->bctg, &c->bsum, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->cbmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
@@ -569,11 +569,6 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
     }
     TRY(ensure(c, c->bctg, bctg.size() * sizeof(int32_t)));
     HIPCHK(c, hipMemcpyAsync(c->bctg.p, bctg.data(), bctg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
-    std::vector<int64_t> pblk(npairs + 1);
-    for (int p = 0; p < npairs; p++) pblk[p] = c->hp[p].blk0;
-    pblk[npairs] = c->nblk;
-    TRY(ensure(c, c->pblk, pblk.size() * sizeof(int64_t)));
-    HIPCHK(c, hipMemcpyAsync(c->pblk.p, pblk.data(), pblk.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->st));
     // extraction ranges: each pair's blocks cut into runs of about nblk / (one
     // resident wave of count workgroups), a multiple of the emit's blocks per
     // round; every range lies inside one pair
@@ -721,26 +716,30 @@ int pga_build_index(pga_ctx *c)
     const int CAP = cap_of(c);
 
     TRY(record(c, 0));
-    // ---- extraction: count, scan, pair record ranges
+    // ---- extraction: count; per range and first digit, the range's offset in
+    // the digit's bucket; per pair, the digit totals and the record range
     const int64_t nblk = c->nblk;
-    TRY(ensure(c, c->blkcnt, (size_t) std::max<int64_t>(nblk, 1) * sizeof(uint32_t)));
-    TRY(ensure(c, c->blkoff, (size_t) (nblk + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->sel, (size_t) std::max<int64_t>(nblk, 1) * 256));    // 2 bits per position
+    TRY(ensure(c, c->hist1, (size_t) npairs * 256 * sizeof(uint64_t)));
     if (nblk > 0) {
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
                            dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
                            (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
-                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
+                           c->nrange, P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
                            (uint32_t) c->shard_lo, (uint32_t) c->shard_hi);
         LAUNCHCHK(c);
-        TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
-    } else {
-        TRY(fill_u64(c, c->blkoff.p, 1, 0));
     }
-    hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
-                       (const int64_t *) P<int64_t>(c->blkoff), (const int64_t *) P<int64_t>(c->pblk), npairs,
-                       P<int64_t>(c->pr));
+    if (c->nrange > 0) {
+        hipLaunchKernelGGL(k_xrange_scan, dim3(npairs, 16), dim3(256), 0, c->st, (const int *) P<int>(c->prange),
+                           (const uint32_t *) P<uint32_t>(c->rangehist), P<int64_t>(c->rangeoff),
+                           P<unsigned long long>(c->hist1));
+        LAUNCHCHK(c);
+    } else {
+        TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
+    }
+    hipLaunchKernelGGL(k_pair_totals, dim3(1), dim3(256), 0, c->st,
+                       (const unsigned long long *) P<unsigned long long>(c->hist1), npairs, P<int64_t>(c->pr));
     LAUNCHCHK(c);
     std::vector<int64_t> pr(npairs + 1);
     HIPCHK(c, hipMemcpyAsync(pr.data(), c->pr.p, pr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
@@ -804,14 +803,6 @@ int pga_build_index(pga_ctx *c)
     // every record straight into its level-1 bucket in record buffer 1
     int *dbad = P<int>(c->counters) + 8;
     HIPCHK(c, hipMemsetAsync(dbad, 0, 4 * sizeof(int), c->st));     // [8] digit mismatch, [10..11] tile sort paths
-    if (c->nrange > 0) {
-        hipLaunchKernelGGL(k_xrange_scan, dim3(npairs, 16), dim3(256), 0, c->st, (const int *) P<int>(c->prange),
-                           (const uint32_t *) P<uint32_t>(c->rangehist), P<int64_t>(c->rangeoff),
-                           P<unsigned long long>(c->hist1));
-        LAUNCHCHK(c);
-    } else {
-        TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
-    }
     hipLaunchKernelGGL(k_xrange_base, dim3(npairs), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr),
                        (const unsigned long long *) P<unsigned long long>(c->hist1), P<int64_t>(c->bstart));
     LAUNCHCHK(c);
@@ -1414,8 +1405,6 @@ int graph_prepare(pga_ctx *c)
     TRY(graph_sizes(c, R_ub, ent_ub, nmb_ub));
     const size_t r1 = (size_t) std::max<int64_t>(R_ub, 1);
     const int64_t nblk = c->nblk;
-    TRY(ensure(c, c->blkcnt, (size_t) std::max<int64_t>(nblk, 1) * sizeof(uint32_t)));
-    TRY(ensure(c, c->blkoff, (size_t) (nblk + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->pr, (size_t) (npairs + 1) * sizeof(int64_t)));
     TRY(ensure(c, c->sel, (size_t) std::max<int64_t>(nblk, 1) * 256));
     TRY(ensure(c, c->rec[0], r1 * sizeof(Rec)));
@@ -1501,20 +1490,10 @@ int graph_enqueue(pga_ctx *c)
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
                            dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
                            (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
-                           c->nrange, P<uint32_t>(c->blkcnt), P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
+                           c->nrange, P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
                            (uint32_t) c->shard_lo, (uint32_t) c->shard_hi);
         LAUNCHCHK(c);
-        TRY(scan_u32(c, P<uint32_t>(c->blkcnt), nblk, P<int64_t>(c->blkoff)));
-    } else {
-        TRY(fill_u64(c, c->blkoff.p, 1, 0));
     }
-    hipLaunchKernelGGL(k_pair_ranges, dim3((npairs + 256) / 256), dim3(256), 0, c->st,
-                       (const int64_t *) P<int64_t>(c->blkoff), (const int64_t *) P<int64_t>(c->pblk), npairs,
-                       P<int64_t>(c->pr));
-    hipLaunchKernelGGL(k_graph_layout, dim3(1), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr), npairs, MT,
-                       CAP, P<PairDev>(c->dpairs), P<int64_t>(c->mblk), P<Seg>(c->segs[0]), P<Tile>(c->tiles), gs);
-    LAUNCHCHK(c);
-    TRY(fill_u64(c, P<int>(c->counters) + 10, 1, 0));                       // tile sort path counters
     if (c->nrange > 0) {
         hipLaunchKernelGGL(k_xrange_scan, dim3(npairs, 16), dim3(256), 0, c->st, (const int *) P<int>(c->prange),
                            (const uint32_t *) P<uint32_t>(c->rangehist), P<int64_t>(c->rangeoff),
@@ -1523,6 +1502,12 @@ int graph_enqueue(pga_ctx *c)
     } else {
         TRY(fill_u64(c, c->hist1.p, (int64_t) npairs * 256, 0));
     }
+    hipLaunchKernelGGL(k_pair_totals, dim3(1), dim3(256), 0, c->st,
+                       (const unsigned long long *) P<unsigned long long>(c->hist1), npairs, P<int64_t>(c->pr));
+    hipLaunchKernelGGL(k_graph_layout, dim3(1), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr), npairs, MT,
+                       CAP, P<PairDev>(c->dpairs), P<int64_t>(c->mblk), P<Seg>(c->segs[0]), P<Tile>(c->tiles), gs);
+    LAUNCHCHK(c);
+    TRY(fill_u64(c, P<int>(c->counters) + 10, 1, 0));                       // tile sort path counters
     hipLaunchKernelGGL(k_xrange_base, dim3(npairs), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr),
                        (const unsigned long long *) P<unsigned long long>(c->hist1), P<int64_t>(c->bstart));
     LAUNCHCHK(c);

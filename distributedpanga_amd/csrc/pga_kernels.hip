@@ -305,13 +305,12 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
                                                       const ContigDev *__restrict__ ctg,
                                                       const int32_t *__restrict__ bctg, int nctg,
                                                       int64_t F, const XRange *__restrict__ ranges, int nrange,
-                                                      uint32_t *__restrict__ blkcnt, uint8_t *__restrict__ sel,
+                                                      uint8_t *__restrict__ sel,
                                                       uint32_t *__restrict__ rangehist, uint32_t bk_lo, uint32_t bk_hi)
 {
     const bool shard = bk_lo > 0 || bk_hi < 1024;
     static_assert(XT == 256, "one thread per digit");
     __shared__ uint8_t tl[256], tm[256];
-    __shared__ uint64_t red[XT / 64 + 1];
     __shared__ uint32_t hist[256];
     load_tables(tl, tm);
     for (int r = blockIdx.x; r < nrange; r += gridDim.x) {
@@ -326,7 +325,7 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
     int c = cnext;
     if (LB + 1 < xr.lb1) cnext = bctg[LB + 1];
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
-    uint32_t cnt = 0, bits = 0;
+    uint32_t bits = 0;
     ContigDev c0 = ctg[c];
     while (c + 1 < nctg) {
         const ContigDev nx = ctg[c + 1];
@@ -355,7 +354,6 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
                 fwd = fwd && bf >= bk_lo && bf < bk_hi;
                 rev = rev && br >= bk_lo && br < bk_hi;
             }
-            cnt += (uint32_t) fwd + (uint32_t) rev;
             bits |= ((fwd ? 1u : 0u) | (rev ? 2u : 0u)) << (2 * u);
             if (fwd) atomicAdd(&hist[digit_fwd(W0 >> (2 * u))], 1u);
             if (rev) atomicAdd(&hist[digit_rev(W0 >> (2 * u))], 1u);
@@ -375,15 +373,12 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
                 x.fwd = x.fwd && bf >= bk_lo && bf < bk_hi;
                 x.rev = x.rev && br >= bk_lo && br < bk_hi;
             }
-            cnt += (uint32_t) x.fwd + (uint32_t) x.rev;
             bits |= ((x.fwd ? 1u : 0u) | (x.rev ? 2u : 0u)) << (2 * u);
             if (x.fwd) atomicAdd(&hist[x.dg & 0xFFu], 1u);
             if (x.rev) atomicAdd(&hist[x.dg >> 8], 1u);
         }
     }
     sel[fb / XU + threadIdx.x] = (uint8_t) bits;       // the emit pass reads the selection back
-    uint64_t tot = block_sum_u64<XT>(cnt, red);
-    if (threadIdx.x == 0) blkcnt[LB] = (uint32_t) tot;
     }
     __syncthreads();
     rangehist[(size_t) r * 256 + threadIdx.x] = hist[threadIdx.x];
@@ -691,12 +686,32 @@ __global__ void k_fill_u64_dev(unsigned long long *__restrict__ p, const int *__
         p[i] = v;
 }
 
-// Record ranges of the pairs from the extraction block offsets.
-__global__ void k_pair_ranges(const int64_t *__restrict__ blkoff, const int64_t *__restrict__ pblk,
+// Entries of a scanned array at the given indexes: out[p] = ofs[idx[p]], p <= npairs.
+__global__ void k_pair_ranges(const int64_t *__restrict__ ofs, const int64_t *__restrict__ idx,
                               int npairs, int64_t *__restrict__ out)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p <= npairs) out[p] = blkoff[pblk[p]];
+    if (p <= npairs) out[p] = ofs[idx[p]];
+}
+
+// Record ranges of the pairs (one block): pr[p+1] - pr[p] = pair p's records,
+// the sum of its first-digit totals.
+__global__ __launch_bounds__(256) void k_pair_totals(const unsigned long long *__restrict__ hist1, int npairs,
+                                                     int64_t *__restrict__ pr)
+{
+    __shared__ int64_t sh[256 / 64 + 1];
+    int64_t run = 0;
+    for (int p0 = 0; p0 < npairs; p0 += 256) {
+        const int p = p0 + (int) threadIdx.x;
+        int64_t n = 0;
+        if (p < npairs)
+            for (int d = 0; d < 256; d++) n += (int64_t) hist1[(size_t) p * 256 + d];
+        int64_t tot;
+        const int64_t x = block_excl_scan<256, int64_t>(n, sh, &tot);
+        if (p < npairs) pr[p] = run + x;
+        run += tot;
+    }
+    if (threadIdx.x == 0) pr[npairs] = run;
 }
 
 // ---------------------------------------------------------------- radix partition
