@@ -826,7 +826,7 @@ int pga_build_index(pga_ctx *c)
     for (int p = 0; p < npairs; p++) {
         const PairDev &d = c->pairs[p];
         if (d.nrec > CAP) seg0.push_back(Seg{d.rec0, d.nrec, p, 1, 1, 0});
-        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 1, 1});
+        else if (d.nrec > 0) tile0.push_back(Tile{d.rec0, (int32_t) d.nrec, p, 1, 1, drange_all(8)});
     }
     int levels = 0;
     // the LCP/strand bytes are written by the tile sort: until then their
@@ -1285,7 +1285,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         std::vector<Seg> seg0;
         std::vector<Tile> tile0;
         if (n > CAP) seg0.push_back(Seg{0, n, pair, 1});
-        else tile0.push_back(Tile{0, (int32_t) n, pair, 0, 1});
+        else tile0.push_back(Tile{0, (int32_t) n, pair, 0, 1, drange_all(8)});
         const int64_t seg_cap = n / CAP + 16;
         const int64_t tile_cap = 34 * (n / CAP) + 1024;
         TRY(ensure(c, c->segs[0], (size_t) seg_cap * sizeof(Seg)));

@@ -59,7 +59,10 @@ struct Tile {                    // a bucket run small enough for one workgroup 
     int64_t start;
     int32_t count, pair, buf;
     int32_t level;               // radix digit (1..16) its buckets differ in; all share the digits above
+    int32_t drange;              // its smallest and largest digit at that level: lo | hi << 16
 };
+
+constexpr int32_t drange_all(int bits) { return ((1 << bits) - 1) << 16; }   // every digit of a level
 
 struct XRange {                  // a run of extraction blocks inside one pair, one workgroup's share
     int64_t lb0, lb1;            // logical blocks [lb0, lb1)

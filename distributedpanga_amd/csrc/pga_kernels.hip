@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     __shared__ int64_t sh[256 / 64 + 1];
     __shared__ int64_t cnt[ND_MAX], off[ND_MAX];
     __shared__ uint16_t tfirst[ND_MAX], tlast[ND_MAX], lnext[ND_MAX];
-    __shared__ int nt, nn, tbase, nbase;
+    __shared__ int nt, nn, tbase, nbase, one_digit;
     const int nd = 1 << digit_bits(level, w2);
     const int pd = nd / 256;                        // digits per thread, contiguous
     if (dn) nseg = *dn;
@@ -867,6 +867,9 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
         sum += c;
         nz += c != 0;
     }
+    if (nz == 1)                                    // the digit, should it be the segment's only one
+        for (int q = 0; q < pd; q++)
+            if (cnt[threadIdx.x * pd + q]) one_digit = threadIdx.x * pd + q;
     const int holders = __syncthreads_count(nz != 0);
     if (holders == 1 && __syncthreads_count(nz > 1) == 0) {
         // a single bucket: no scatter; the child keeps the buffer and the range
@@ -877,7 +880,8 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
                 if (q < next_cap) next[q] = Seg{sg.start, sg.count, sg.pair, level + 1, sg.buf, 0};
             } else {
                 const int t = atomicAdd(ntiles, 1);
-                if (t < tile_cap) tiles[t] = Tile{sg.start, (int32_t) sg.count, sg.pair, sg.buf, level};
+                if (t < tile_cap) tiles[t] = Tile{sg.start, (int32_t) sg.count, sg.pair, sg.buf, level,
+                                                  one_digit | one_digit << 16};
             }
             atomicAdd(kept, (unsigned long long) sg.count);
         }
@@ -938,7 +942,8 @@ __global__ __launch_bounds__(256) void k_seg_plan(Seg *__restrict__ segs,
     // sized from a bound that a correct count never exceeds)
     for (int i = threadIdx.x; i < nt && tbase + i < tile_cap; i += 256) {
         const int f = tfirst[i], l = tlast[i];
-        tiles[tbase + i] = Tile{sg.start + off[f], (int32_t) (off[l] + cnt[l] - off[f]), sg.pair, dstbuf, level};
+        tiles[tbase + i] = Tile{sg.start + off[f], (int32_t) (off[l] + cnt[l] - off[f]), sg.pair, dstbuf, level,
+                                f | l << 16};
     }
     for (int i = threadIdx.x; i < nn && nbase + i < next_cap; i += 256) {
         const int d = lnext[i];
@@ -1143,7 +1148,6 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     constexpr int NB = CAP;
     uint32_t *bh = reinterpret_cast<uint32_t *>(smem_raw + (size_t) CAP * sizeof(Rec));
     auto bin_start = [&](uint32_t b) -> uint32_t { return (bh[b >> 1] >> ((b & 1u) << 4)) & 0xFFFFu; };
-    __shared__ uint32_t dmn[T / 64], dmx[T / 64];
     __shared__ uint32_t scan_sh[T / 64 + 1];
     __shared__ int fallback;
     // crowded ranges (bins, then sub-bins, of more than FB_BIN records): start
@@ -1153,32 +1157,13 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
     __shared__ unsigned long long cx[MAXC][2];
 
     // smallest and largest digit at the tile's level (its records share the
-    // digits above it)
+    // digits above it), as the plan listed them
     const int L = tl.level;
     const int dp = digit_pos(L, w2), dbits = digit_bits(L, w2);
-    uint32_t dlo = ~0u, dhi = 0u;
-#pragma unroll
-    for (int k = 0; k < E; k++) {
-        if (k * T + tid < n) {
-            const uint32_t d = (uint32_t) (key_bits_from(r[k].w0, r[k].w1, dp) >> (64 - dbits));
-            dlo = min(dlo, d);
-            dhi = max(dhi, d);
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        dlo = min(dlo, (uint32_t) __shfl_xor((int) dlo, o, 64));
-        dhi = max(dhi, (uint32_t) __shfl_xor((int) dhi, o, 64));
-    }
-    if ((tid & 63) == 0) { dmn[tid >> 6] = dlo; dmx[tid >> 6] = dhi; }
+    const uint32_t dmin = (uint32_t) tl.drange & 0xFFFFu, dmax = (uint32_t) tl.drange >> 16;
     for (int c = tid; c < NB / 2; c += T) bh[c] = 0;
     if (tid == 0) { fallback = 0; ncrow[0] = 0; }
     __syncthreads();
-    uint32_t dmin = dmn[0], dmax = dmx[0];
-    for (int w = 1; w < T / 64; w++) {
-        dmin = min(dmin, dmn[w]);
-        dmax = max(dmax, dmx[w]);
-    }
     // Bin of a record: the top BB bits of (its digit at the tile's level - the
     // smallest such digit, then the key bits right below that digit).  A tile
     // is a run of buckets of one parent, so buckets map to disjoint bin ranges
@@ -2171,7 +2156,7 @@ __global__ __launch_bounds__(256) void k_graph_layout(const int64_t *__restrict_
             mb += (d.nrec + mt - 1) / mt;
             pairs[p] = d;
             if (d.nrec > cap) seg0[nseg++] = Seg{d.rec0, d.nrec, p, 1, 1, 0};
-            else if (d.nrec > 0) tile0[nt++] = Tile{d.rec0, (int32_t) d.nrec, p, 1, 1};
+            else if (d.nrec > 0) tile0[nt++] = Tile{d.rec0, (int32_t) d.nrec, p, 1, 1, drange_all(8)};
         }
         mblk[npairs] = mb;
         st->R = pr[npairs];
