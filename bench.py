@@ -275,7 +275,7 @@ PLAIN_WORKLOAD = ("plain single pair: 100 Mbp A (500 lognormal contigs, GC 0.41,
 def plain_leg(args):
     """The 100 Mbp pair (autotuned, cached per shape): a side leg beside the
     human-scale headline, with the diagonal seed sort of its two streams."""
-    opts = dict(tile_records=4096, chunk_records=8192, merge_tile=512, digit2_bits=10)
+    opts = dict(tile_records=4096, chunk_records=8192, merge_tile=1024, digit2_bits=10)
     t = time.perf_counter()
     g = synth.workload(2, 1.0)
     log(f"[bench] 100 Mbp pair generated: {g.ncontig} contigs, {g.seqtot} bases ({time.perf_counter() - t:.1f}s)")
@@ -319,7 +319,7 @@ def batch_leg(args):
     gdbs = synth.batch(args.batch_pairs, scale=1.0)
     tot = sum(x.seqtot for x in gdbs)
     log(f"[bench] batch workload generated: {len(gdbs)} pairs, {tot} bases ({time.perf_counter() - t:.1f}s)")
-    base = dict(tile_records=4096, chunk_records=8192, merge_tile=512, digit2_bits=10)
+    base = dict(tile_records=4096, chunk_records=8192, merge_tile=1024, digit2_bits=10)
     bopts = dict(base)
     bctx = pga.Context(**bopts)
     bctx.load_batch(gdbs)
@@ -382,7 +382,7 @@ def main():
                     help="scale of the human-scale pair (1.0 = 3.1 Gbp A + B)")
     ap.add_argument("--tile", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=8192)
-    ap.add_argument("--merge-tile", type=int, default=512)
+    ap.add_argument("--merge-tile", type=int, default=1024)
     ap.add_argument("--digit2", type=int, default=8, help="bits of the partition's second digit (8..10)")
     ap.add_argument("--no-plain", action="store_true", help="skip the 100 Mbp pair leg")
     ap.add_argument("--no-batch", action="store_true")

@@ -381,7 +381,7 @@ void pga_default_opts(pga_opts *o)
     o->freq = 10;
     o->tile_records = 4096;
     o->chunk_records = 8192;
-    o->merge_tile = 512;
+    o->merge_tile = 1024;
     o->build_cum = 1;
     o->digit2_bits = 10;
 }

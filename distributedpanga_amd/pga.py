@@ -91,7 +91,7 @@ class PgaError(RuntimeError):
 class Context:
     """One device context: one HIP stream and a pool of device buffers."""
 
-    def __init__(self, device=0, freq=10, tile_records=4096, chunk_records=8192, merge_tile=512,
+    def __init__(self, device=0, freq=10, tile_records=4096, chunk_records=8192, merge_tile=1024,
                  build_cum=True, digit2_bits=None):
         self._L = _setup(hip_lib())
         o = Opts()
