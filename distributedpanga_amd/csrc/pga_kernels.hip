@@ -1083,6 +1083,29 @@ __device__ __forceinline__ uint32_t comes_before(const Rec &y, const Rec &x, boo
     return (uint32_t) b1;
 }
 
+// rank + 1 if (y, i < me) comes before x (the borrow of y - x - (i < me) over
+// the four 32-bit words), else rank: five VALU instructions chained through
+// VCC.  Written out because the compiler turns the borrow chain of
+// comes_before into 64-bit compares combined in scalar registers.
+__device__ __forceinline__ uint32_t rank_step(uint32_t rank, const Rec &y, const Rec &x, uint32_t i, uint32_t me)
+{
+    uint32_t t0, t1, t2, t3;
+    asm("v_cmp_lt_u32 vcc, %[i], %[me]\n\t"
+        "v_subb_co_u32 %[t0], vcc, %[y1l], %[x1l], vcc\n\t"
+        "v_subb_co_u32 %[t1], vcc, %[y1h], %[x1h], vcc\n\t"
+        "v_subb_co_u32 %[t2], vcc, %[y0l], %[x0l], vcc\n\t"
+        "v_subb_co_u32 %[t3], vcc, %[y0h], %[x0h], vcc\n\t"
+        "v_addc_co_u32 %[r], vcc, 0, %[r], vcc"
+        : [r] "+v"(rank), [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+        : [i] "v"(i), [me] "v"(me),
+          [y1l] "v"((uint32_t) y.w1), [y1h] "v"((uint32_t) (y.w1 >> 32)),
+          [y0l] "v"((uint32_t) y.w0), [y0h] "v"((uint32_t) (y.w0 >> 32)),
+          [x1l] "v"((uint32_t) x.w1), [x1h] "v"((uint32_t) (x.w1 >> 32)),
+          [x0l] "v"((uint32_t) x.w0), [x0h] "v"((uint32_t) (x.w0 >> 32))
+        : "vcc");
+    return rank;
+}
+
 // Leading equal bases of two keys (40 if the k-mers are equal), with no
 // branches: the first differing bit in w0, else in the 16 k-mer bits of w1
 // (bit 15 set below them stands for "none", giving 32 + 8).
@@ -1181,12 +1204,18 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         return (uint32_t) ((key_bits_from(w0, w1, dp) - dsub) >> bsh);
     };
     uint32_t rk[E];
+    // all the atomics issued before the first result is taken apart: which
+    // half of its counter word a record's bin is goes to bit k of odd
+    uint32_t odd = 0;
 #pragma unroll
     for (int k = 0; k < E; k++)
         if (k * T + tid < n) {
-            const uint32_t b = bin_of(r[k].w0, r[k].w1), sh = (b & 1u) << 4;
-            rk[k] = (atomicAdd(&bh[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
+            const uint32_t b = bin_of(r[k].w0, r[k].w1);
+            odd |= (b & 1u) << k;
+            rk[k] = atomicAdd(&bh[b >> 1], 1u << ((b & 1u) << 4));
         }
+#pragma unroll
+    for (int k = 0; k < E; k++) rk[k] = (rk[k] >> (((odd >> k) & 1u) << 4)) & 0xFFFFu;
     __syncthreads();
     // Bin counts -> bin starts (16-bit, two to a word).  A bin of more than
     // FB_BIN records (repeated k-mers) is listed as a crowded range through
@@ -1241,9 +1270,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             uint32_t rank = 0, i = st;
             for (; i + 2 <= en; i += 2) {
                 const Rec y0 = sk[i], y1 = sk[i + 1];
-                rank += comes_before(y0, r[k], i < me) + comes_before(y1, r[k], i + 1 < me);
+                rank = rank_step(rank_step(rank, y0, r[k], i, me), y1, r[k], i + 1, me);
             }
-            if (i < en) rank += comes_before(sk[i], r[k], i < me);
+            if (i < en) rank = rank_step(rank, sk[i], r[k], i, me);
             rk[k] = st + rank;                          // the record's place in the tile
         }
         __syncthreads();
@@ -1354,7 +1383,7 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                 uint32_t rank = 0;
                 for (uint32_t x = st; x < en; x++) {
                     const Rec y = sk[x];
-                    rank += comes_before(y, r[k], x < me);
+                    rank = rank_step(rank, y, r[k], x, me);
                 }
                 rk[k] = st + rank;
             }
