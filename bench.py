@@ -390,6 +390,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-seed-sort", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the human-scale pair only (profiling)")
+    ap.add_argument("--only-plain", action="store_true", help="the 100 Mbp pair leg only (profiling)")
+    ap.add_argument("--only-serve", action="store_true", help="the serving leg only (profiling)")
     ap.add_argument("--gdb-cache", default=None,
                     help="a .npz file holding the generated human-scale pair (read if present, else written)")
     ap.add_argument("--cpu-bases", type=int, default=500_000_000)
@@ -418,6 +420,9 @@ def main():
         sys.exit(2)
     if args.only_headline:
         args.no_plain = args.no_batch = args.no_cpu = args.no_serve = args.no_seed_sort = True
+    if args.only_plain or args.only_serve:          # one side leg alone, printed as its JSON object
+        print(json.dumps(plain_leg(args) if args.only_plain else serving_leg(args)), flush=True)
+        return
 
     # ---- headline: the human-scale pair, input resident in HBM
     opts = dict(tile_records=args.tile, chunk_records=args.chunk, merge_tile=args.merge_tile,
