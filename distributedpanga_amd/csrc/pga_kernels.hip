@@ -435,10 +435,17 @@ __global__ __launch_bounds__(256) void k_xrange_base(const int64_t *__restrict__
 // each thread keeps its (at most 2*XU) records in registers and takes a rank
 // in its digit (an LDS counter); a scan of the round's digit counts lays the
 // records out in digit order in an LDS stage, and the stage goes out as one
-// contiguous run per digit at the digit's next free slots.  A record past its
+// contiguous run per digit at the digit's next free slots.  A round's run of a
+// digit is short (about six records), so its last HOLD-aligned segment is
+// usually incomplete: thread d keeps digit d's records past the run's last
+// whole segment in LDS and puts them in front of the digit's records in the
+// next round's stage, so that the bucket is written in whole aligned segments
+// (the range's last round writes everything; partial lines written at
+// different times reach HBM as separate partial writes).  A record past its
 // digit's run (the two passes disagreeing on a digit) is not written and sets
 // *bad.
-constexpr int EMIT_QB = 2;           // blocks per emit round (512 threads, 64 KB stage, two workgroups a CU)
+constexpr int EMIT_QB = 2;           // blocks per emit round (512 threads, 48 KB stage, two workgroups a CU)
+constexpr int EMIT_HOLD = 4;         // records per aligned bucket segment (64 B)
 
 template <int QB>
 __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__restrict__ bps64,
@@ -453,13 +460,19 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
 {
     constexpr int NT = 256 * QB;
     constexpr int NS = 2 * XU;                         // record slots per thread
+    constexpr int HOLD = EMIT_HOLD;
     static_assert(XT == 256 && NT >= 256, "one thread per digit");
     static_assert(emit_clear_covers(NT), "the counter clearing must reach every digit");
-    __shared__ Rec stage[NS * NT];
-    __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; its slots this round
+    static_assert((HOLD & (HOLD - 1)) == 0 && HOLD >= 2 && HOLD <= 8, "segment of 2-8 records");
+    constexpr int SCAP = 3 * 1024;                     // stage records (a round holds about 1600)
+    __shared__ Rec stage[SCAP];
+    __shared__ Rec held[256][HOLD - 1];                 // a digit's records past its last whole segment
+    __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; its slot this round
     __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
-    __shared__ uint32_t stot;
+    __shared__ uint32_t hcnt[256], wcnt[256];          // held records; records written this round
+    __shared__ uint32_t stot, spill;
     const int tid = threadIdx.x, q = tid >> 8, t = tid & 255;
+    uint32_t hn = 0;                                   // threads below 256: digit t's held records
     // Inputs of a round, in two stages so that no load waits on one issued
     // in the same round: two rounds ahead, the selection bits and the block's
     // first contig (fetch1); one round ahead, from those, the contig of the
@@ -497,7 +510,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         }
     };
     int par = 0;                                       // which digit counters this round uses
-    if (tid < 256) cnt[0][tid] = cnt[1][tid] = 0;
+    if (tid < 256) cnt[0][tid] = cnt[1][tid] = hcnt[tid] = 0;
     for (int r = blockIdx.x; r < nrange; r += gridDim.x) {
     const XRange xr = ranges[r];
     __syncthreads();                                   // the previous range is done with the runs
@@ -510,6 +523,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     fetch2();
     fetch1(xr.lb0 + QB + q, xr.lb1);
     for (int64_t lb = xr.lb0; lb < xr.lb1; lb += QB) {
+    const bool last = lb + QB >= xr.lb1;
     Rec rr[NS];
     uint32_t rk[NS];
     uint32_t m = 0;                                    // slots holding a record
@@ -556,13 +570,18 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         fetch1(lb + 2 * QB + q, xr.lb1);
     }
     __syncthreads();
-    // the round's digit runs (one wave: four digits a lane): offsets in the
-    // stage, slots in the buckets; the other counters are cleared for the
-    // next round
+    // the round's digit runs (one wave: four digits a lane): each digit's
+    // held records, then its new ones, in stage order.  A round whose records
+    // overflow the stage writes every record straight to its slot instead
+    // (spill; low-complexity sequence only).  The other counters are cleared
+    // for the next round.
     if (tid < 64) {
         uint32_t cd[4], sum = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) { cd[k] = cnt[par][4 * tid + k]; sum += cd[k]; }
+        for (int k = 0; k < 4; k++) {
+            cd[k] = cnt[par][4 * tid + k] + hcnt[4 * tid + k];
+            sum += cd[k];
+        }
         uint32_t x = sum;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -572,28 +591,77 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         uint32_t run = x - sum;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int d = 4 * tid + k;
-            loff[d] = run;
-            gpos[d] = base[d] + done[d];
-            done[d] += cd[k];
+            loff[4 * tid + k] = run;
             run += cd[k];
         }
-        if (tid == 63) stot = x;
+        if (tid == 63) { stot = x; spill = x > (uint32_t) SCAP; }
     } else {
         for (int d = tid - 64; d < 256; d += NT - 64) cnt[par ^ 1][d] = 0;   // emit_clear_covers(NT)
     }
     __syncthreads();
+    if (spill) {
+        if (tid < 256) {
+            const int64_t g = base[t] + done[t];
+            gpos[t] = g;
+            done[t] += cnt[par][t];
+            for (uint32_t k = 0; k < hn; k++) {
+                const int64_t dst = g - hn + k;
+                if (dst >= base[t] && dst < base[t] + lim[t]) out[dst] = held[t][k];
+                else *bad = 1;
+            }
+            hcnt[t] = hn = 0;
+        }
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NS; k++)
-        if ((m >> k) & 1u) stage[loff[(uint32_t) (rr[k].w0 >> 56)] + rk[k]] = rr[k];
-    __syncthreads();
-    const int tot = (int) stot;
-    for (int i = tid; i < tot; i += NT) {
-        const Rec x = stage[i];
-        const uint32_t d = (uint32_t) (x.w0 >> 56);
-        const int64_t dst = gpos[d] + (i - (int64_t) loff[d]);
-        if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = x;
-        else *bad = 1;
+        for (int k = 0; k < NS; k++)
+            if ((m >> k) & 1u) {
+                const uint32_t d = (uint32_t) (rr[k].w0 >> 56);
+                const int64_t dst = gpos[d] + rk[k];
+                if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = rr[k];
+                else *bad = 1;
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+            if ((m >> k) & 1u) {
+                const uint32_t d = (uint32_t) (rr[k].w0 >> 56);
+                stage[loff[d] + hcnt[d] + rk[k]] = rr[k];
+            }
+        uint32_t reg = 0;
+        if (tid < 256) {
+            // digit t: its held records go in front; the run's first slot and
+            // how much of it goes out now (whole segments, or all of it in the
+            // range's last round)
+            const uint32_t lo = loff[t];
+            for (uint32_t k = 0; k < hn; k++) stage[lo + k] = held[t][k];
+            const uint32_t cn = cnt[par][t];
+            const int64_t g = base[t] + done[t] - hn;
+            reg = hn + cn;
+            const int64_t e = g + reg;
+            const int64_t w = last ? e : (e & ~(int64_t) (HOLD - 1));
+            gpos[t] = g;
+            wcnt[t] = w > g ? (uint32_t) (w - g) : 0u;
+            done[t] += cn;
+        }
+        __syncthreads();
+        const int tot = (int) stot;
+        for (int i = tid; i < tot; i += NT) {
+            const Rec x = stage[i];
+            const uint32_t d = (uint32_t) (x.w0 >> 56);
+            const uint32_t o = (uint32_t) i - loff[d];
+            if (o < wcnt[d]) {
+                const int64_t dst = gpos[d] + o;
+                if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = x;
+                else *bad = 1;
+            }
+        }
+        if (tid < 256) {                               // the run's incomplete last segment stays
+            const uint32_t w = wcnt[t];
+            hn = reg - w;
+            const uint32_t h0 = loff[t] + w;
+            for (uint32_t k = 0; k < hn; k++) held[t][k] = stage[h0 + k];
+            hcnt[t] = hn;
+        }
     }
     par ^= 1;
     }
