@@ -94,7 +94,7 @@ struct pga_ctx {
     std::vector<int64_t> nseeds, cseeds;
 
     // device buffers
-    DBuf bps, ctg, bctg, bsum, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
+    DBuf bps, ctg, blk, bsum, pr, hist1, rec[2], bufs, entries, ls, pre, cum,
         counters, segs[2], seghist, cursor, chk0, tiles, dpairs, maxpre, mcntN, mcntC, mofsN, mofsC,
         mblk, pofs, seedN, seedC, nbuck, cbuck, sel, plist, gmax, cbmax;
     bool pre_full = true;             // pre holds every record's prefix (else the tile ends' only)
@@ -444,7 +444,7 @@ void pga_destroy(pga_ctx *c)
     if (!c) return;
     (void) hipSetDevice(c->o.device);
     if (c->st) (void) hipStreamSynchronize(c->st);
-    DBuf *all[] = {&c->bps, &c->ctg, &c->bctg, &c->bsum, &c->pr, &c->hist1,
+    DBuf *all[] = {&c->bps, &c->ctg, &c->blk, &c->bsum, &c->pr, &c->hist1,
                    &c->rec[0], &c->rec[1], &c->bufs, &c->entries, &c->ls, &c->pre, &c->cum,
                    &c->counters, &c->segs[0], &c->segs[1], &c->seghist, &c->cursor, &c->chk0, &c->tiles,
                    &c->dpairs, &c->maxpre, &c->mcntN, &c->mcntC, &c->mofsN, &c->mofsC, &c->mblk, &c->pofs, &c->vout, &c->minfo, &c->sel, &c->plist, &c->gmax, &c->cbmax, &c->ssk[0], &c->ssk[1], &c->ssbufs, &c->ssout,
@@ -561,14 +561,26 @@ static int load_batch_impl(pga_ctx *c, int npairs, const uint8_t *const *bps, co
     TRY(ensure(c, c->ctg, tab.size() * sizeof(ContigDev)));
     HIPCHK(c, hipMemcpyAsync(c->ctg.p, tab.data(), tab.size() * sizeof(ContigDev), hipMemcpyHostToDevice, c->st));
     // contig of every extraction block's first position: the last contig whose
-    // first candidate position is <= it (what a binary search over pos0 gives)
-    std::vector<int32_t> bctg((size_t) std::max<int64_t>(c->nblk, 1), 0);
+    // first candidate position is <= it (what a binary search over pos0 gives),
+    // and where the block lies in it
+    std::vector<BlkDev> blk((size_t) std::max<int64_t>(c->nblk, 1), BlkDev{});
     for (int64_t b = 0, i = 0; b < c->nblk; b++) {
-        while (i + 1 < nctg && tab[i + 1].pos0 <= b * XBLOCK) i++;
-        bctg[b] = (int32_t) i;
+        const int64_t fb = b * XBLOCK;
+        while (i + 1 < nctg && tab[i + 1].pos0 <= fb) i++;
+        BlkDev &d = blk[b];
+        d.c = (int32_t) i;
+        if (nctg == 0 || fb < tab[i].pos0) continue;
+        const ContigDev &cc = tab[i];
+        const int64_t lo = std::max<int64_t>(0, cc.pos0 + 28 - fb);
+        const int64_t hi = std::min<int64_t>(XBLOCK, cc.pos0 + cc.ncand - fb);
+        if (hi <= lo) continue;
+        d.bbit = cc.bitoff + 2 * (fb - cc.pos0);
+        d.jblk = (uint32_t) (fb - cc.pos0);
+        d.local = (uint32_t) cc.local;
+        d.frange = (uint32_t) lo | (uint32_t) hi << 16;
     }
-    TRY(ensure(c, c->bctg, bctg.size() * sizeof(int32_t)));
-    HIPCHK(c, hipMemcpyAsync(c->bctg.p, bctg.data(), bctg.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+    TRY(ensure(c, c->blk, blk.size() * sizeof(BlkDev)));
+    HIPCHK(c, hipMemcpyAsync(c->blk.p, blk.data(), blk.size() * sizeof(BlkDev), hipMemcpyHostToDevice, c->st));
     // extraction ranges: each pair's blocks cut into runs of about nblk / (one
     // resident wave of count workgroups), a multiple of the emit's blocks per
     // round; every range lies inside one pair
@@ -725,7 +737,7 @@ int pga_build_index(pga_ctx *c)
     if (nblk > 0) {
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
                            dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
-                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
+                           (const BlkDev *) P<BlkDev>(c->blk), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
                            c->nrange, P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
                            (uint32_t) c->shard_lo, (uint32_t) c->shard_hi);
         LAUNCHCHK(c);
@@ -810,7 +822,7 @@ int pga_build_index(pga_ctx *c)
         hipLaunchKernelGGL(k_extract_emit<EMIT_QB>,
                            dim3(grid_for(c, (const void *) k_extract_emit<EMIT_QB>, 256 * EMIT_QB, 0, c->nrange)),
                            dim3(256 * EMIT_QB), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
-                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
+                           (const BlkDev *) P<BlkDev>(c->blk), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
                            (const XRange *) P<XRange>(c->xrange), c->nrange,
                            (const uint32_t *) P<uint32_t>(c->rangehist), (const int64_t *) P<int64_t>(c->rangeoff),
                            (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), dbad);
@@ -1489,7 +1501,7 @@ int graph_enqueue(pga_ctx *c)
     if (nblk > 0) {
         hipLaunchKernelGGL(k_extract_count, dim3(grid_for(c, (const void *) k_extract_count, 256, 0, c->nrange)),
                            dim3(256), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
-                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
+                           (const BlkDev *) P<BlkDev>(c->blk), c->nctg, c->F, (const XRange *) P<XRange>(c->xrange),
                            c->nrange, P<uint8_t>(c->sel), P<uint32_t>(c->rangehist),
                            (uint32_t) c->shard_lo, (uint32_t) c->shard_hi);
         LAUNCHCHK(c);
@@ -1515,7 +1527,7 @@ int graph_enqueue(pga_ctx *c)
         hipLaunchKernelGGL(k_extract_emit<EMIT_QB>,
                            dim3(grid_for(c, (const void *) k_extract_emit<EMIT_QB>, 256 * EMIT_QB, 0, c->nrange)),
                            dim3(256 * EMIT_QB), 0, c->st, P<uint64_t>(c->bps), P<ContigDev>(c->ctg),
-                           (const int32_t *) P<int32_t>(c->bctg), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
+                           (const BlkDev *) P<BlkDev>(c->blk), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
                            (const XRange *) P<XRange>(c->xrange), c->nrange,
                            (const uint32_t *) P<uint32_t>(c->rangehist), (const int64_t *) P<int64_t>(c->rangeoff),
                            (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), &gs->bad);

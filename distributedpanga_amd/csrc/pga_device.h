@@ -35,6 +35,19 @@ struct ContigDev {
     int32_t pad;
 };
 
+// An extraction block's (XT*XU positions) first contig, as the extraction
+// kernels need it: the contig (the last whose first candidate position is at
+// or before the block's), the .bps bit and contig offset of the block's first
+// position, the contig's local id, and the block-relative positions [lo, hi)
+// that lie inside the contig at offset 28 or more (zero when none do).
+struct BlkDev {
+    int64_t bbit;
+    uint32_t jblk;
+    uint32_t local;
+    uint32_t frange;             // lo | hi << 16
+    int32_t c;
+};
+
 struct PairDev {
     int64_t rec0, nrec;          // the pair's records in the global record space
     int64_t ent_off;             // byte offset of its GIX entries

@@ -297,13 +297,13 @@ __device__ __forceinline__ void record_keys(const uint64_t *__restrict__ bps64, 
 // Per range of logical blocks (a workgroup's share, inside one pair): the
 // selection bits of every position (forward / reverse-complement record), the
 // record count of every block, and the range's count of each first radix
-// digit.  bctg[LB] is the contig of block LB's first position; every thread
+// digit.  blk[LB].c is the contig of block LB's first position; every thread
 // steps forward from it (a thread of XU positions crosses few contig starts).
 // Only records whose 5-base bucket lies in [bk_lo, bk_hi) are selected (a
 // shard of the index; [0, 1024) keeps all).
 __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict__ bps64,
                                                       const ContigDev *__restrict__ ctg,
-                                                      const int32_t *__restrict__ bctg, int nctg,
+                                                      const BlkDev *__restrict__ blk, int nctg,
                                                       int64_t F, const XRange *__restrict__ ranges, int nrange,
                                                       uint8_t *__restrict__ sel,
                                                       uint32_t *__restrict__ rangehist, uint32_t bk_lo, uint32_t bk_hi)
@@ -319,11 +319,11 @@ __global__ __launch_bounds__(XT) void k_extract_count(const uint64_t *__restrict
     const XRange xr = ranges[r];
     // the next block's first contig is loaded a block ahead, and a contig
     // together with the next one's start
-    int cnext = xr.lb0 < xr.lb1 ? bctg[xr.lb0] : 0;
+    int cnext = xr.lb0 < xr.lb1 ? blk[xr.lb0].c : 0;
     for (int64_t LB = xr.lb0; LB < xr.lb1; LB++) {
     const int64_t fb = LB * XT * XU;
     int c = cnext;
-    if (LB + 1 < xr.lb1) cnext = bctg[LB + 1];
+    if (LB + 1 < xr.lb1) cnext = blk[LB + 1].c;
     const int64_t f0 = fb + (int64_t) threadIdx.x * XU;
     uint32_t bits = 0;
     ContigDev c0 = ctg[c];
@@ -450,7 +450,7 @@ constexpr int EMIT_HOLD = 4;         // records per aligned bucket segment (64 B
 template <int QB>
 __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__restrict__ bps64,
                                                            const ContigDev *__restrict__ ctg,
-                                                           const int32_t *__restrict__ bctg, int nctg,
+                                                           const BlkDev *__restrict__ blk, int nctg,
                                                            int jbits, const uint8_t *__restrict__ sel,
                                                            const XRange *__restrict__ ranges, int nrange,
                                                            const uint32_t *__restrict__ rangehist,
@@ -473,41 +473,45 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     __shared__ uint32_t stot, spill;
     const int tid = threadIdx.x, q = tid >> 8, t = tid & 255;
     uint32_t hn = 0;                                   // threads below 256: digit t's held records
-    // Inputs of a round, in two stages so that no load waits on one issued
-    // in the same round: two rounds ahead, the selection bits and the block's
-    // first contig (fetch1); one round ahead, from those, the contig of the
-    // thread's first position and, when its XU positions lie in that contig
-    // past base 28, the forward and reverse-complement 128-bit spans (fetch2).
-    uint32_t bits = 0, bits_n = 0;
-    int c = 0, c_n = 0;
-    bool fast = false;
-    int64_t f0 = 0, lb_n = 0;
-    ContigDev c0{};
-    uint64_t A0 = 0, A1 = 0, B0 = 0, B1 = 0;
-    auto fetch1 = [&](int64_t LB, int64_t lb1) {
-        const bool in = LB < lb1;
+    // Inputs of a round.  Two rounds ahead: the selection bits and the
+    // block's first contig (fetch_s).  At the start of the round before, from
+    // those: whether the thread's XU positions lie in that contig past base
+    // 28, and where their forward and reverse-complement 128-bit spans start
+    // (plan_w).  At the end of the round before, after its stores: the words
+    // under the spans (fetch_w).  Every load is issued unconditionally (a
+    // clamped block, a harmless address) and after the round's stores, so the
+    // one wait at the start of a round covers them all and no load waits where
+    // it is issued.
+    uint32_t bits_n = 0;
+    int64_t lb_n = 0;
+    BlkDev bk_n{};
+    uint32_t ninfo = 0, njl = 0, nloc = 0;             // the next round's: bits | fast << 8 | shift << 9
+    int64_t nwb = 0;                                   // and the first of its four words
+    uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    auto fetch_s = [&](int64_t LB, int64_t lb1) {
+        const int64_t L = LB < lb1 ? LB : lb1 - 1;     // lb1 > lb0: in range
         lb_n = LB;
-        bits_n = in ? sel[LB * XT + t] : 0u;
-        c_n = in ? bctg[LB] : 0;
+        bits_n = sel[L * XT + t];
+        bk_n = blk[L];
     };
-    auto fetch2 = [&]() {
-        bits = bits_n;
-        c = c_n;
-        fast = false;
-        if (!bits) return;
-        f0 = lb_n * XT * XU + (int64_t) t * XU;
-        c0 = ctg[c];
-        while (c + 1 < nctg) {
-            const ContigDev nx = ctg[c + 1];
-            if (nx.pos0 > f0) break;
-            c++;
-            c0 = nx;
-        }
-        fast = f0 >= c0.pos0 && f0 + XU <= c0.pos0 + c0.ncand && f0 - c0.pos0 >= 28;
-        if (fast) {
-            window128(bps64, c0.bitoff + 2 * (f0 - c0.pos0), A0, A1);
-            window128(bps64, c0.bitoff + 2 * (f0 - c0.pos0 - 28), B0, B1);
-        }
+    auto plan_w = [&](int64_t lb1) {
+        const uint32_t b = lb_n < lb1 ? bits_n : 0u;
+        const uint32_t p = (uint32_t) t * XU;
+        const bool f = b && p >= (bk_n.frange & 0xFFFFu) && p + XU <= (bk_n.frange >> 16);
+        // the reverse-complement span starts 28 bases (56 bits) before the
+        // forward one; the four words from the former's hold both
+        const int64_t bb = f ? bk_n.bbit + 2 * (int64_t) p - 56 : 0;
+        nwb = bb >> 6;
+        ninfo = b | (f ? 1u << 8 : 0u) | ((uint32_t) (bb & 63) << 9);
+        njl = bk_n.jblk + p;
+        nloc = bk_n.local;
+    };
+    auto fetch_w = [&]() {
+        const uint64_t *pw = bps64 + nwb;
+        w0 = pw[0];
+        w1 = pw[1];
+        w2 = pw[2];
+        w3 = pw[3];
     };
     int par = 0;                                       // which digit counters this round uses
     if (tid < 256) cnt[0][tid] = cnt[1][tid] = hcnt[tid] = 0;
@@ -519,55 +523,78 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         lim[tid] = rangehist[(size_t) r * 256 + tid];
         done[tid] = 0;
     }
-    fetch1(xr.lb0 + q, xr.lb1);
-    fetch2();
-    fetch1(xr.lb0 + QB + q, xr.lb1);
+    if (xr.lb0 >= xr.lb1) continue;                    // (ranges are never empty)
+    fetch_s(xr.lb0 + q, xr.lb1);
+    plan_w(xr.lb1);
+    fetch_w();
+    fetch_s(xr.lb0 + QB + q, xr.lb1);
     for (int64_t lb = xr.lb0; lb < xr.lb1; lb += QB) {
     const bool last = lb + QB >= xr.lb1;
-    Rec rr[NS];
-    uint32_t rk[NS];
-    uint32_t m = 0;                                    // slots holding a record
-    auto put = [&](int slot, uint64_t w0, uint64_t w1) {
-        rr[slot].w0 = w0;
-        rr[slot].w1 = w1;
-        rk[slot] = atomicAdd(&cnt[par][(uint32_t) (w0 >> 56)], 1u);
-        m |= 1u << slot;
+    const uint32_t bits = ninfo & 0xFFu, sb = ninfo >> 9, jl = njl, loc = nloc;
+    const bool fast = (ninfo >> 8) & 1u;
+    if (!last) plan_w(xr.lb1);                         // the next round's spans
+    // The thread's records are made twice: here only their first digits, for
+    // their ranks; after the layout in full, into the stage (16-byte records
+    // held in registers across the layout would take 32 VGPRs).
+    auto funnel = [](uint64_t x, uint64_t y, uint32_t s) { return s ? (x >> s) | (y << (64 - s)) : x; };
+    // every record of the thread, by slot (forward 2u, reverse complement 2u+1)
+    auto records = [&](auto &&sink) {
+        if (fast) {
+            // the spans from their words: reverse complement at bit sb of the
+            // first word, forward 56 bits on
+            const uint64_t B0 = funnel(w0, w1, sb), B1 = funnel(w1, w2, sb);
+            const bool one = sb < 8;                   // the forward span starts in the first word too
+            const uint32_t sa = (sb + 56u) & 63u;
+            const uint64_t A0 = funnel(one ? w0 : w1, one ? w1 : w2, sa);
+            const uint64_t A1 = funnel(one ? w1 : w2, one ? w2 : w3, sa);
+#pragma unroll
+            for (int u = 0; u < XU; u++) {
+                const uint64_t pay = ((uint64_t) loc << (jbits + 1)) | ((uint64_t) (jl + u) << 1);
+                const int sh = 2 * u;
+                if ((bits >> (2 * u)) & 1u) {
+                    const uint64_t W0 = sh ? (A0 >> sh) | (A1 << (64 - sh)) : A0;
+                    const uint32_t W1 = (uint32_t) (A1 >> sh) & 0xFFFFu;
+                    sink(2 * u, field_rev64(W0), ((uint64_t) field_rev16(W1) << 48) | pay);
+                }
+                if ((bits >> (2 * u + 1)) & 1u) {
+                    const uint64_t V0 = sh ? (B0 >> sh) | (B1 << (64 - sh)) : B0;
+                    const uint32_t V1 = (uint32_t) (B1 >> sh) & 0xFFFFu;
+                    sink(2 * u + 1, ~(((uint64_t) V1 << 48) | (V0 >> 16)),
+                         ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull);
+                }
+            }
+        } else if (bits) {
+            // a block crossing a contig start, or positions near a contig's ends
+            const int64_t f0 = (lb + q) * XT * XU + (int64_t) t * XU;
+            int c = blk[lb + q].c;
+#pragma unroll
+            for (int u = 0; u < XU; u++) {
+                const int64_t f = f0 + u;
+                if (!((bits >> (2 * u)) & 3u)) continue;
+                while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
+                const ContigDev cc = ctg[c];
+                XEmit e;
+                e.fwd = (bits >> (2 * u)) & 1u;
+                e.rev = (bits >> (2 * u + 1)) & 1u;
+                record_keys(bps64, cc, f - cc.pos0, jbits, e);
+                if (e.fwd) sink(2 * u, e.f0, e.f1);
+                if (e.rev) sink(2 * u + 1, e.r0, e.r1);
+            }
+        }
     };
-    if (bits && fast) {
-        const int64_t j0 = f0 - c0.pos0;
+    uint32_t rk[NS];                                   // each record's rank in its digit this round
+    if (fast) {
+        // first digits from the forward span's low bits alone
+        const bool one = sb < 8;
+        const uint64_t A0 = funnel(one ? w0 : w1, one ? w1 : w2, (sb + 56u) & 63u);
 #pragma unroll
         for (int u = 0; u < XU; u++) {
-            const uint64_t pay = ((uint64_t) c0.local << (jbits + 1)) | ((uint64_t) (j0 + u) << 1);
-            const int sh = 2 * u;
-            if ((bits >> (2 * u)) & 1u) {
-                const uint64_t W0 = sh ? (A0 >> sh) | (A1 << (64 - sh)) : A0;
-                const uint32_t W1 = (uint32_t) (A1 >> sh) & 0xFFFFu;
-                put(2 * u, field_rev64(W0), ((uint64_t) field_rev16(W1) << 48) | pay);
-            }
-            if ((bits >> (2 * u + 1)) & 1u) {
-                const uint64_t V0 = sh ? (B0 >> sh) | (B1 << (64 - sh)) : B0;
-                const uint32_t V1 = (uint32_t) (B1 >> sh) & 0xFFFFu;
-                put(2 * u + 1, ~(((uint64_t) V1 << 48) | (V0 >> 16)), ((uint64_t) ((~V0) & 0xFFFFu) << 48) | pay | 1ull);
-            }
+            const uint64_t W = A0 >> (2 * u);
+            if ((bits >> (2 * u)) & 1u) rk[2 * u] = atomicAdd(&cnt[par][digit_fwd(W)], 1u);
+            if ((bits >> (2 * u + 1)) & 1u) rk[2 * u + 1] = atomicAdd(&cnt[par][digit_rev(W)], 1u);
         }
     } else if (bits) {
-#pragma unroll
-        for (int u = 0; u < XU; u++) {
-            const int64_t f = f0 + u;
-            if (!((bits >> (2 * u)) & 3u)) continue;
-            while (c + 1 < nctg && ctg[c + 1].pos0 <= f) c++;
-            const ContigDev cc = ctg[c];
-            XEmit e;
-            e.fwd = (bits >> (2 * u)) & 1u;
-            e.rev = (bits >> (2 * u + 1)) & 1u;
-            record_keys(bps64, cc, f - cc.pos0, jbits, e);
-            if (e.fwd) put(2 * u, e.f0, e.f1);
-            if (e.rev) put(2 * u + 1, e.r0, e.r1);
-        }
-    }
-    if (lb + QB < xr.lb1) {                            // the next rounds' inputs, in flight meanwhile
-        fetch2();
-        fetch1(lb + 2 * QB + q, xr.lb1);
+        records([&](int k, uint64_t a, uint64_t) { rk[k] = atomicAdd(&cnt[par][(uint32_t) (a >> 56)], 1u); });
     }
     __syncthreads();
     // the round's digit runs (one wave: four digits a lane): each digit's
@@ -612,21 +639,17 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             hcnt[t] = hn = 0;
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < NS; k++)
-            if ((m >> k) & 1u) {
-                const uint32_t d = (uint32_t) (rr[k].w0 >> 56);
-                const int64_t dst = gpos[d] + rk[k];
-                if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = rr[k];
-                else *bad = 1;
-            }
+        records([&](int k, uint64_t a, uint64_t b) {
+            const uint32_t d = (uint32_t) (a >> 56);
+            const int64_t dst = gpos[d] + rk[k];
+            if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = Rec{a, b};
+            else *bad = 1;
+        });
     } else {
-#pragma unroll
-        for (int k = 0; k < NS; k++)
-            if ((m >> k) & 1u) {
-                const uint32_t d = (uint32_t) (rr[k].w0 >> 56);
-                stage[loff[d] + hcnt[d] + rk[k]] = rr[k];
-            }
+        records([&](int k, uint64_t a, uint64_t b) {
+            const uint32_t d = (uint32_t) (a >> 56);
+            stage[loff[d] + hcnt[d] + rk[k]] = Rec{a, b};
+        });
         uint32_t reg = 0;
         if (tid < 256) {
             // digit t: its held records go in front; the run's first slot and
@@ -663,12 +686,16 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             hcnt[t] = hn;
         }
     }
+    if (!last) {                                       // the next rounds' inputs, after this round's stores
+        fetch_w();
+        fetch_s(lb + 2 * QB + q, xr.lb1);
+    }
     par ^= 1;
     }
     }
 }
 
-template __global__ void k_extract_emit<EMIT_QB>(const uint64_t *, const ContigDev *, const int32_t *, int, int,
+template __global__ void k_extract_emit<EMIT_QB>(const uint64_t *, const ContigDev *, const BlkDev *, int, int,
                                                  const uint8_t *, const XRange *, int, const uint32_t *,
                                                  const int64_t *, const int64_t *, Rec *, int *);
 
