@@ -467,9 +467,9 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     constexpr int SCAP = 3 * 1024;                     // stage records (a round holds about 1600)
     __shared__ Rec stage[SCAP];
     __shared__ Rec held[256][HOLD - 1];                 // a digit's records past its last whole segment
-    __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; its slot this round
+    __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; this round's slot (less the stage index)
     __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
-    __shared__ uint32_t hcnt[256], wcnt[256];          // held records; records written this round
+    __shared__ uint32_t hcnt[256], wcnt[256], wout[256];   // held records; records out this round (in bounds)
     __shared__ uint32_t stot, spill;
     const int tid = threadIdx.x, q = tid >> 8, t = tid & 255;
     uint32_t hn = 0;                                   // threads below 256: digit t's held records
@@ -477,22 +477,29 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     // block's first contig (fetch_s).  At the start of the round before, from
     // those: whether the thread's XU positions lie in that contig past base
     // 28, and where their forward and reverse-complement 128-bit spans start
-    // (plan_w).  At the end of the round before, after its stores: the words
+    // (plan_w).  In the round before, once its own words are used: the words
     // under the spans (fetch_w).  Every load is issued unconditionally (a
-    // clamped block, a harmless address) and after the round's stores, so the
-    // one wait at the start of a round covers them all and no load waits where
-    // it is issued.
+    // clamped block, a harmless address), so no load waits where it is issued;
+    // they are in flight while the round before writes its records.
     uint32_t bits_n = 0;
     int64_t lb_n = 0;
     BlkDev bk_n{};
     uint32_t ninfo = 0, njl = 0, nloc = 0;             // the next round's: bits | fast << 8 | shift << 9
     int64_t nwb = 0;                                   // and the first of its four words
     uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    // the addresses of the loads in flight, kept until the loads are used:
+    // the compiler waits for an outstanding load before it reuses the
+    // registers of its address
+    const uint8_t *as = sel;
+    const BlkDev *ab = blk;
+    const uint64_t *aw = bps64;
     auto fetch_s = [&](int64_t LB, int64_t lb1) {
         const int64_t L = LB < lb1 ? LB : lb1 - 1;     // lb1 > lb0: in range
         lb_n = LB;
-        bits_n = sel[L * XT + t];
-        bk_n = blk[L];
+        as = sel + L * XT + t;
+        ab = blk + L;
+        bits_n = *as;
+        bk_n = *ab;
     };
     auto plan_w = [&](int64_t lb1) {
         const uint32_t b = lb_n < lb1 ? bits_n : 0u;
@@ -507,11 +514,11 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         nloc = bk_n.local;
     };
     auto fetch_w = [&]() {
-        const uint64_t *pw = bps64 + nwb;
-        w0 = pw[0];
-        w1 = pw[1];
-        w2 = pw[2];
-        w3 = pw[3];
+        aw = bps64 + nwb;
+        w0 = aw[0];
+        w1 = aw[1];
+        w2 = aw[2];
+        w3 = aw[3];
     };
     int par = 0;                                       // which digit counters this round uses
     if (tid < 256) cnt[0][tid] = cnt[1][tid] = hcnt[tid] = 0;
@@ -533,6 +540,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     const uint32_t bits = ninfo & 0xFFu, sb = ninfo >> 9, jl = njl, loc = nloc;
     const bool fast = (ninfo >> 8) & 1u;
     if (!last) plan_w(xr.lb1);                         // the next round's spans
+    asm volatile("" ::"v"(as), "v"(ab), "v"(aw));      // (ALU only: the addresses' registers stay live)
     // The thread's records are made twice: here only their first digits, for
     // their ranks; after the layout in full, into the stage (16-byte records
     // held in registers across the layout would take 32 VGPRs).
@@ -580,6 +588,17 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
                 if (e.fwd) sink(2 * u, e.f0, e.f1);
                 if (e.rev) sink(2 * u + 1, e.r0, e.r1);
             }
+            // nothing of this path stays in flight: otherwise the compiler,
+            // which tracks outstanding loads per register and not per path,
+            // makes the copy-out wait for the prefetch
+            __builtin_amdgcn_s_waitcnt(0x0F70);        // vmcnt(0)
+        }
+    };
+    // the next rounds' inputs, once this round's words are used
+    auto prefetch = [&]() {
+        if (!last) {
+            fetch_w();
+            fetch_s(lb + 2 * QB + q, xr.lb1);
         }
     };
     uint32_t rk[NS];                                   // each record's rank in its digit this round
@@ -626,6 +645,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         for (int d = tid - 64; d < 256; d += NT - 64) cnt[par ^ 1][d] = 0;   // emit_clear_covers(NT)
     }
     __syncthreads();
+    uint32_t reg = 0;
     if (spill) {
         if (tid < 256) {
             const int64_t g = base[t] + done[t];
@@ -650,7 +670,6 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             const uint32_t d = (uint32_t) (a >> 56);
             stage[loff[d] + hcnt[d] + rk[k]] = Rec{a, b};
         });
-        uint32_t reg = 0;
         if (tid < 256) {
             // digit t: its held records go in front; the run's first slot and
             // how much of it goes out now (whole segments, or all of it in the
@@ -662,33 +681,38 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             reg = hn + cn;
             const int64_t e = g + reg;
             const int64_t w = last ? e : (e & ~(int64_t) (HOLD - 1));
+            const uint32_t wc = w > g ? (uint32_t) (w - g) : 0u;
+            // the run must lie in the digit's slots (the two passes agreeing
+            // on its digit counts); what does not is not written
+            const int64_t room = base[t] + lim[t] - g;
+            if ((int64_t) wc > room) *bad = 1;
+            wcnt[t] = wc;
+            wout[t] = (int64_t) wc > room ? (uint32_t) max<int64_t>(room, 0) : wc;
             gpos[t] = g;
-            wcnt[t] = w > g ? (uint32_t) (w - g) : 0u;
             done[t] += cn;
         }
-        __syncthreads();
-        const int tot = (int) stot;
-        for (int i = tid; i < tot; i += NT) {
+    }
+    prefetch();                                        // in flight during the copy-out
+    __syncthreads();
+    const int tot = spill ? 0 : (int) stot;
+    // unrolled over the stage's capacity: as a loop, the compiler would wait
+    // for the prefetch before entering it
+#pragma unroll
+    for (int k = 0; k < (SCAP + NT - 1) / NT; k++) {
+        const int i = tid + k * NT;
+        if (i < tot) {
             const Rec x = stage[i];
             const uint32_t d = (uint32_t) (x.w0 >> 56);
             const uint32_t o = (uint32_t) i - loff[d];
-            if (o < wcnt[d]) {
-                const int64_t dst = gpos[d] + o;
-                if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = x;
-                else *bad = 1;
-            }
-        }
-        if (tid < 256) {                               // the run's incomplete last segment stays
-            const uint32_t w = wcnt[t];
-            hn = reg - w;
-            const uint32_t h0 = loff[t] + w;
-            for (uint32_t k = 0; k < hn; k++) held[t][k] = stage[h0 + k];
-            hcnt[t] = hn;
+            if (o < wout[d]) out[gpos[d] + o] = x;
         }
     }
-    if (!last) {                                       // the next rounds' inputs, after this round's stores
-        fetch_w();
-        fetch_s(lb + 2 * QB + q, xr.lb1);
+    if (tot && tid < 256) {                            // the run's incomplete last segment stays
+        const uint32_t w = wcnt[t];
+        hn = reg - w;
+        const uint32_t h0 = loff[t] + w;
+        for (uint32_t k = 0; k < hn; k++) held[t][k] = stage[h0 + k];
+        hcnt[t] = hn;
     }
     par ^= 1;
     }
