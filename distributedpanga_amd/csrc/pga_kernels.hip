@@ -432,8 +432,8 @@ __global__ __launch_bounds__(256) void k_xrange_base(const int64_t *__restrict__
 // Emit every selected record straight into its first-level bucket.  Range r's
 // records of digit d fill the run of rangehist[r][d] slots at bstart[pair][d]
 // + rangeoff[r][d].  A workgroup of 256*QB threads takes QB blocks per round:
-// each thread keeps its (at most 2*XU) records in registers and takes a rank
-// in its digit (an LDS counter); a scan of the round's digit counts lays the
+// each thread makes its (at most 2*XU) records' first digits and takes a rank
+// in each digit (an LDS counter); a scan of the round's digit counts lays the
 // records out in digit order in an LDS stage, and the stage goes out as one
 // contiguous run per digit at the digit's next free slots.  A round's run of a
 // digit is short (about six records), so its last HOLD-aligned segment is
@@ -445,7 +445,8 @@ __global__ __launch_bounds__(256) void k_xrange_base(const int64_t *__restrict__
 // digit's run (the two passes disagreeing on a digit) is not written and sets
 // *bad.
 constexpr int EMIT_QB = 2;           // blocks per emit round (512 threads, 48 KB stage, two workgroups a CU)
-constexpr int EMIT_HOLD = 4;         // records per aligned bucket segment (64 B)
+constexpr int EMIT_HOLD = 2;         // records per aligned bucket segment (32 B: measured faster than
+                                     // 64 B, whose longer carries cost more than the writes they save)
 
 template <int QB>
 __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__restrict__ bps64,
@@ -463,10 +464,10 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
     constexpr int HOLD = EMIT_HOLD;
     static_assert(XT == 256 && NT >= 256, "one thread per digit");
     static_assert(emit_clear_covers(NT), "the counter clearing must reach every digit");
-    static_assert((HOLD & (HOLD - 1)) == 0 && HOLD >= 2 && HOLD <= 8, "segment of 2-8 records");
+    static_assert((HOLD & (HOLD - 1)) == 0 && HOLD >= 1 && HOLD <= 8, "segment of 1-8 records");
     constexpr int SCAP = 3 * 1024;                     // stage records (a round holds about 1600)
     __shared__ Rec stage[SCAP];
-    __shared__ Rec held[256][HOLD - 1];                 // a digit's records past its last whole segment
+    __shared__ Rec held[256][HOLD > 1 ? HOLD - 1 : 1];  // a digit's records past its last whole segment
     __shared__ int64_t base[256], gpos[256];           // a digit's run in the range; this round's slot (less the stage index)
     __shared__ uint32_t cnt[2][256], loff[256], lim[256], done[256];
     __shared__ uint32_t hcnt[256], wcnt[256], wout[256];   // held records; records out this round (in bounds)
