@@ -622,10 +622,11 @@ namespace {
 // The tiles' `buf` index refers to {buf0, buf1}.  dig (may be null): a byte
 // per record position, free during the partition; each scatter from the
 // second level on writes the moved records' next digits there, and the next
-// level counts those segments from it.
+// level counts those segments from it.  dig1: the level-1 records' second
+// digits are there already (level1_in_place only).
 int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, const std::vector<Tile> &tile0,
               const unsigned long long *hist1, bool level1_in_place, int64_t seg_cap, int64_t tile_cap,
-              uint8_t *dig, int *levels_out)
+              uint8_t *dig, bool dig1, int *levels_out)
 {
     const int CAP = cap_of(c);
     const int per = chunk_per(c);
@@ -648,7 +649,10 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         LAUNCHCHK(c);
         const bool counted = level == 1 && hist1;
         const bool in_place = level == 1 && level1_in_place;
-        const bool dig_next = dig && !in_place && level >= 2;        // the next level's digits are 8-bit
+        // the next level's digits are in dig: written by the level's scatter
+        // (8-bit digits from the third level on), or by the extraction for
+        // level 2 (dig1)
+        const bool dig_next = dig && (in_place ? dig1 : level >= 2);
         const int64_t nd = level == 2 ? (int64_t) 1 << c->o.digit2_bits : 256;   // digit rows of this level
         TRY(ensure(c, c->seghist, (size_t) nseg * nd * sizeof(uint64_t)));
         TRY(ensure(c, c->cursor, (size_t) nseg * nd * sizeof(uint64_t)));
@@ -818,6 +822,10 @@ int pga_build_index(pga_ctx *c)
     hipLaunchKernelGGL(k_xrange_base, dim3(npairs), dim3(256), 0, c->st, (const int64_t *) P<int64_t>(c->pr),
                        (const unsigned long long *) P<unsigned long long>(c->hist1), P<int64_t>(c->bstart));
     LAUNCHCHK(c);
+    // with an 8-bit second digit, the extraction also writes each record's
+    // second digit to the LCP/strand buffer (free until the tile sort), and
+    // the second level counts those bytes instead of the records
+    const bool dig2 = c->o.digit2_bits == 8;
     if (c->nrange > 0 && R > 0) {
         hipLaunchKernelGGL(k_extract_emit<EMIT_QB>,
                            dim3(grid_for(c, (const void *) k_extract_emit<EMIT_QB>, 256 * EMIT_QB, 0, c->nrange)),
@@ -825,7 +833,8 @@ int pga_build_index(pga_ctx *c)
                            (const BlkDev *) P<BlkDev>(c->blk), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
                            (const XRange *) P<XRange>(c->xrange), c->nrange,
                            (const uint32_t *) P<uint32_t>(c->rangehist), (const int64_t *) P<int64_t>(c->rangeoff),
-                           (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), dbad);
+                           (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]),
+                           dig2 ? P<uint8_t>(c->ls) : nullptr, dbad);
         LAUNCHCHK(c);
     }
     c->xbad = 0;
@@ -844,7 +853,7 @@ int pga_build_index(pga_ctx *c)
     // the LCP/strand bytes are written by the tile sort: until then their
     // buffer holds the partition's digit bytes
     TRY(partition(c, P<Rec>(c->rec[0]), P<Rec>(c->rec[1]), seg0, tile0, P<unsigned long long>(c->hist1), true,
-                  seg_cap, tile_cap, P<uint8_t>(c->ls), &levels));
+                  seg_cap, tile_cap, P<uint8_t>(c->ls), dig2, &levels));
     if (c->xbad) return fail(c, PGA_EINVAL, "internal: extraction passes disagree on a record's first digit");
     c->tm.partition_levels = levels;
     c->tm.partition_launches = c->nscatter;
@@ -1311,7 +1320,7 @@ int pga_seed_sort(pga_ctx *c, int pair, int comp, int64_t *nels, int *swide_out)
         const int64_t nt = c->ntiles;
         int levels = 0;
         const int rc = partition(c, P<Rec>(c->ssk[0]), P<Rec>(c->ssk[1]), seg0, tile0,
-                                 P<unsigned long long>(c->hist1), false, seg_cap, tile_cap, nullptr, &levels);
+                                 P<unsigned long long>(c->hist1), false, seg_cap, tile_cap, nullptr, false, &levels);
         const int64_t sort_tiles = c->ntiles;
         c->nscatter = ns;
         c->scatter_bytes = sb;
@@ -1530,7 +1539,8 @@ int graph_enqueue(pga_ctx *c)
                            (const BlkDev *) P<BlkDev>(c->blk), c->nctg, c->jbits, (const uint8_t *) P<uint8_t>(c->sel),
                            (const XRange *) P<XRange>(c->xrange), c->nrange,
                            (const uint32_t *) P<uint32_t>(c->rangehist), (const int64_t *) P<int64_t>(c->rangeoff),
-                           (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), &gs->bad);
+                           (const int64_t *) P<int64_t>(c->bstart), P<Rec>(c->rec[1]), (uint8_t *) nullptr,
+                           &gs->bad);
         LAUNCHCHK(c);
     }
     // ---- MSD partition, GRAPH_LEVELS levels, counts on the device

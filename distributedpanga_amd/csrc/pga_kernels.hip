@@ -449,7 +449,7 @@ constexpr int EMIT_HOLD = 2;         // records per aligned bucket segment (32 B
                                      // 64 B, whose longer carries cost more than the writes they save)
 
 template <int QB>
-__global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__restrict__ bps64,
+__global__ __launch_bounds__(256 * QB, 4) void k_extract_emit(const uint64_t *__restrict__ bps64,
                                                            const ContigDev *__restrict__ ctg,
                                                            const BlkDev *__restrict__ blk, int nctg,
                                                            int jbits, const uint8_t *__restrict__ sel,
@@ -457,7 +457,8 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
                                                            const uint32_t *__restrict__ rangehist,
                                                            const int64_t *__restrict__ rangeoff,
                                                            const int64_t *__restrict__ bstart,
-                                                           Rec *__restrict__ out, int *__restrict__ bad)
+                                                           Rec *__restrict__ out, uint8_t *__restrict__ dig,
+                                                           int *__restrict__ bad)
 {
     constexpr int NT = 256 * QB;
     constexpr int NS = 2 * XU;                         // record slots per thread
@@ -654,8 +655,12 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             done[t] += cnt[par][t];
             for (uint32_t k = 0; k < hn; k++) {
                 const int64_t dst = g - hn + k;
-                if (dst >= base[t] && dst < base[t] + lim[t]) out[dst] = held[t][k];
-                else *bad = 1;
+                if (dst >= base[t] && dst < base[t] + lim[t]) {
+                    out[dst] = held[t][k];
+                    if (dig) dig[dst] = (uint8_t) (held[t][k].w0 >> 48);
+                } else {
+                    *bad = 1;
+                }
             }
             hcnt[t] = hn = 0;
         }
@@ -663,8 +668,12 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
         records([&](int k, uint64_t a, uint64_t b) {
             const uint32_t d = (uint32_t) (a >> 56);
             const int64_t dst = gpos[d] + rk[k];
-            if (dst >= base[d] && dst < base[d] + lim[d]) out[dst] = Rec{a, b};
-            else *bad = 1;
+            if (dst >= base[d] && dst < base[d] + lim[d]) {
+                out[dst] = Rec{a, b};
+                if (dig) dig[dst] = (uint8_t) (a >> 48);
+            } else {
+                *bad = 1;
+            }
         });
     } else {
         records([&](int k, uint64_t a, uint64_t b) {
@@ -705,10 +714,14 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
             const Rec x = stage[i];
             const uint32_t d = (uint32_t) (x.w0 >> 56);
             const uint32_t o = (uint32_t) i - loff[d];
-            if (o < wout[d]) out[gpos[d] + o] = x;
+            if (o < wout[d]) {
+                out[gpos[d] + o] = x;
+                if (dig) dig[gpos[d] + o] = (uint8_t) (x.w0 >> 48);
+            }
         }
     }
-    if (tot && tid < 256) {                            // the run's incomplete last segment stays
+    if (tot && tid < 256) {
+        // the run's incomplete last segment stays
         const uint32_t w = wcnt[t];
         hn = reg - w;
         const uint32_t h0 = loff[t] + w;
@@ -722,7 +735,7 @@ __global__ __launch_bounds__(256 * QB) void k_extract_emit(const uint64_t *__res
 
 template __global__ void k_extract_emit<EMIT_QB>(const uint64_t *, const ContigDev *, const BlkDev *, int, int,
                                                  const uint8_t *, const XRange *, int, const uint32_t *,
-                                                 const int64_t *, const int64_t *, Rec *, int *);
+                                                 const int64_t *, const int64_t *, Rec *, uint8_t *, int *);
 
 // ---------------------------------------------------------------- scans
 

@@ -66,6 +66,26 @@ def test_second_digit_widths(tile, chunk, digit2):
     assert got["timings"]["partition_levels"] >= 2
 
 
+@pytest.mark.parametrize("tile,chunk", [(2048, 2048), (2048, 8192), (4096, 8192)])
+def test_second_digit_bytes_from_extraction(tile, chunk):
+    """An 8-bit second digit: the extraction also writes every record's
+    second digit (whole 32-byte blocks a bucket run, single bytes at a run's
+    ends), and the second level counts those bytes instead of the records."""
+    got, _ = _compare(synth.workload(1, 1.0), tile_records=tile, chunk_records=chunk, digit2_bits=8)
+    assert got["timings"]["partition_levels"] >= 2
+
+
+def test_second_digit_bytes_low_complexity():
+    """The same where rounds of the extraction overflow its stage (poly-A and
+    short tandem repeats): those rounds write every record and its digit
+    byte straight to its slot."""
+    contigs = [np.zeros(30000, np.uint8), np.tile(np.array([0, 1], np.uint8), 12000),
+               np.tile(np.array([2, 2, 3], np.uint8), 5000)]
+    rng = np.random.default_rng(5)
+    contigs += [rng.integers(0, 4, 20000).astype(np.uint8) for _ in range(3)]
+    _compare(synth.from_codes(contigs), tile_records=2048, digit2_bits=8)
+
+
 @pytest.mark.parametrize("chunk,mt", [(2048, 512), (8192, 1024), (4096, 256)])
 def test_chunk_and_merge_tiles(chunk, mt):
     _compare(synth.workload(1, 0.2), chunk_records=chunk, merge_tile=mt)
