@@ -925,10 +925,29 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
     const int CH = PT * PER;
     const int64_t nlog = chk0[nseg];
     const int nd = 1 << digit_bits(level, w2);
-    for (int64_t LB = blockIdx.x; LB < nlog; LB += gridDim.x) {
-        for (int d = threadIdx.x; d < nd; d += PT) hist[d] = 0;
+    // a contiguous run of chunks a block, counted into one LDS histogram that
+    // goes out (global atomics) only when the segment changes: a chunk of
+    // digit bytes is 8 KB, too little to pay for 2^bits global atomics
+    const int64_t per = (nlog + gridDim.x - 1) / gridDim.x;
+    const int64_t c0 = (int64_t) blockIdx.x * per, c1 = c0 + per < nlog ? c0 + per : nlog;
+    for (int d = threadIdx.x; d < nd; d += PT) hist[d] = 0;
+    __syncthreads();
+    auto flush = [&](int s) {
         __syncthreads();
-        const int s = seg_of_chunk(chk0, nseg, LB);
+        for (int d = threadIdx.x; d < nd; d += PT) {
+            const uint32_t h = hist[d];
+            hist[d] = 0;
+            if (h) atomicAdd(&seghist[(size_t) s * nd + d], (unsigned long long) h);
+        }
+        __syncthreads();
+    };
+    int cur = -1;
+    for (int64_t LB = c0; LB < c1; LB++) {
+        const int s = cur >= 0 && LB < chk0[cur + 1] ? cur : seg_of_chunk(chk0, nseg, LB);
+        if (s != cur) {
+            if (cur >= 0) flush(cur);
+            cur = s;
+        }
         const Seg sg = segs[s];
         const int64_t off = (LB - chk0[s]) * (int64_t) CH;
         const int n = (int) min((int64_t) CH, sg.count - off);
@@ -950,13 +969,8 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
                 }
             }
         }
-        __syncthreads();
-        for (int d = threadIdx.x; d < nd; d += PT) {
-            const uint32_t h = hist[d];
-            if (h) atomicAdd(&seghist[(size_t) s * nd + d], (unsigned long long) h);
-        }
-        __syncthreads();
     }
+    if (cur >= 0) flush(cur);
 }
 
 // Per segment: digit offsets (cursors) and the children.  Children of at most
