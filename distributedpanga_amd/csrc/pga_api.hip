@@ -642,6 +642,9 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
     if (nseg)
         HIPCHK(c, hipMemcpyAsync(c->segs[0].p, seg0.data(), seg0.size() * sizeof(Seg), hipMemcpyHostToDevice, c->st));
     int cur = 0, level = 1;
+    // the previous level's scatter wrote digit bytes, and the records it left in place
+    bool prev_dig = false;
+    int64_t prev_kept = 0;
     while (nseg > 0) {
         if (level > MAX_LEVELS) return fail(c, PGA_EINVAL, "radix partition did not converge");
         hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, c->st, (const Seg *) P<Seg>(c->segs[cur]),
@@ -689,6 +692,11 @@ int partition(pga_ctx *c, Rec *buf0, Rec *buf1, const std::vector<Seg> &seg0, co
         memcpy(&kept, &hcount[2], sizeof(kept));
         if (!in_place)
             c->scatter_bytes += 2 * (int64_t) sizeof(Rec) * (ck[1] - (int64_t) kept);   // moved records, read + write
+        // the previous scatter's digit bytes: one for each record of this
+        // level's segments that it moved (the others it left in place)
+        if (prev_dig) c->scatter_bytes += ck[1] - prev_kept;
+        prev_dig = dig_next && !in_place;
+        prev_kept = (int64_t) kept;
         nseg = hcount[0];
         cur ^= 1;
         level++;

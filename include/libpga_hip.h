@@ -87,7 +87,8 @@ typedef struct {
     int    partition_launches;
     int64_t tiles;
     /* the radix partition scatter (dominant HBM kernel): summed kernel time
-       and bytes moved (records read + written) */
+       and bytes moved (records read + written, and the next level's digit
+       byte written beside each record bound for a next-level segment) */
     double scatter_ms;
     int64_t scatter_bytes;
     int    scatter_launches;
