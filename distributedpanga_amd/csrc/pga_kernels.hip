@@ -1413,8 +1413,13 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
             const uint32_t st = bin_start(b);
             const uint32_t en = b + 1 < (uint32_t) NB ? bin_start(b + 1) : (uint32_t) n;
             if (en - st > (uint32_t) FB_BIN) { rk[k] = me; continue; }   // crowded: sorted below
-            // two reads in flight a step
+            // four reads in flight a step over a long bin, then two
             uint32_t rank = 0, i = st;
+            for (; i + 4 <= en; i += 4) {
+                const Rec y0 = sk[i], y1 = sk[i + 1], y2 = sk[i + 2], y3 = sk[i + 3];
+                rank = rank_step(rank_step(rank, y0, r[k], i, me), y1, r[k], i + 1, me);
+                rank = rank_step(rank_step(rank, y2, r[k], i + 2, me), y3, r[k], i + 3, me);
+            }
             for (; i + 2 <= en; i += 2) {
                 const Rec y0 = sk[i], y1 = sk[i + 1];
                 rank = rank_step(rank_step(rank, y0, r[k], i, me), y1, r[k], i + 1, me);
@@ -1526,12 +1531,16 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                 const uint32_t st = bin_start(sb) + base;
                 const uint32_t en = (sb + 1 < (uint32_t) csb[nc] ? bin_start(sb + 1) : (uint32_t) M) + base;
                 if (en - st > (uint32_t) FB_BIN) { rk[k] = ~0u; continue; }   // the next round's
+                // four reads in flight a step: a round's few lanes each walk a
+                // sub-bin of up to FB_BIN records, so the walk is LDS latency
                 const uint32_t me = rk[k];
-                uint32_t rank = 0;
-                for (uint32_t x = st; x < en; x++) {
-                    const Rec y = sk[x];
-                    rank = rank_step(rank, y, r[k], x, me);
+                uint32_t rank = 0, x = st;
+                for (; x + 4 <= en; x += 4) {
+                    const Rec y0 = sk[x], y1 = sk[x + 1], y2 = sk[x + 2], y3 = sk[x + 3];
+                    rank = rank_step(rank_step(rank, y0, r[k], x, me), y1, r[k], x + 1, me);
+                    rank = rank_step(rank_step(rank, y2, r[k], x + 2, me), y3, r[k], x + 3, me);
                 }
+                for (; x < en; x++) rank = rank_step(rank, sk[x], r[k], x, me);
                 rk[k] = st + rank;
             }
             __syncthreads();
