@@ -1435,8 +1435,9 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
         // Crowded ranges, sorted in place round by round: each range's records
         // share the key bits above the first bit where any two of them differ
         // (the OR of their XORs with the range's first record), so their
-        // sub-bin is the next q = floor(log2 m) bits from there (<= m sub-bins
-        // a range, <= NB in all).  Sub-bins of at most FB_BIN records are
+        // sub-bin is the next q bits from there: q = floor(log2 m) (<= m
+        // sub-bins a range, <= NB in all), or all of the bin bits for a round's
+        // only range.  Sub-bins of at most FB_BIN records are
         // ranked as above; larger ones are the next round's ranges.  Ranges of
         // identical keys (the seed sort's duplicates) are sorted already.
         int cur = 0;
@@ -1477,7 +1478,8 @@ __global__ __launch_bounds__(T, 4) void k_tile_sort(Rec *const *__restrict__ buf
                     const unsigned long long x0 = cx[j][0], x1 = cx[j][1];
                     const int p = x0 ? __builtin_clzll(x0) : x1 ? 64 + __builtin_clzll(x1) : 128;
                     const int lg = 31 - __builtin_clz((uint32_t) crm[cur][j]);
-                    const int q = p >= 128 ? 0 : min(min(lg, 12), 128 - p);
+                    // a lone range (the usual case) takes all NB sub-bins
+                    const int q = p >= 128 ? 0 : min(min(nc == 1 ? BB : lg, 12), 128 - p);
                     cp[j] = p;
                     cq[j] = q;
                     csb[j] = acc;
