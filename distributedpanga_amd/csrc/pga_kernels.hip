@@ -441,9 +441,10 @@ __global__ __launch_bounds__(256) void k_xrange_base(const int64_t *__restrict__
 // whole segment in LDS and puts them in front of the digit's records in the
 // next round's stage, so that the bucket is written in whole aligned segments
 // (the range's last round writes everything; partial lines written at
-// different times reach HBM as separate partial writes).  A record past its
-// digit's run (the two passes disagreeing on a digit) is not written and sets
-// *bad.
+// different times reach HBM as separate partial writes).  With dig (an 8-bit
+// second digit), each record's second digit goes into dig at the record's
+// slot, for the second partition level's counts.  A record past its digit's
+// run (the two passes disagreeing on a digit) is not written and sets *bad.
 constexpr int EMIT_QB = 2;           // blocks per emit round (512 threads, 48 KB stage, two workgroups a CU)
 constexpr int EMIT_HOLD = 2;         // records per aligned bucket segment (32 B: measured faster than
                                      // 64 B, whose longer carries cost more than the writes they save)
