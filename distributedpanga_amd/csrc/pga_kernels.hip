@@ -1212,7 +1212,7 @@ template __global__ void k_seg_hist<8>(const Rec *, const Rec *, const Seg *, in
 template __global__ void k_seg_hist<32>(const Rec *, const Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, const uint8_t *, const int *);
 template __global__ void k_seg_scatter<256, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
 template __global__ void k_seg_scatter<256, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
-template __global__ void k_seg_scatter<512, 16>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
+template __global__ void k_seg_scatter<1024, 8>(Rec *, Rec *, const Seg *, int, const int64_t *, int, int, unsigned long long *, uint8_t *, const int *);
 
 // ---------------------------------------------------------------- tile sort
 
