@@ -309,9 +309,22 @@ def plain_leg(args):
                 tiles=last.get("tiles"), crowded_tiles=last.get("crowded_tiles"),
                 fallback_tiles=last.get("fallback_tiles"),
                 roofline={k: roof[k] for k in ("kernel", "achieved", "frac", "radix_scatter", "path")},
-                seed_sort=dict(streams=ss, records=sum(v["records"] for v in ss.values()),
-                               ms=round(sum(v["ms"] for v in ss.values()), 3),
-                               note="reimport + per-panel sort of both seed streams, records left on the device"))
+                seed_sort=seed_sort_line(ss, sinfo["s"]))
+
+
+def seed_sort_line(ss, s):
+    """The seed sort's figures: both streams' records and time, and a roofline
+    on the sort's own input and output (each seed read once, s bytes, and its
+    sorted record written once, s bytes); the passes over 16-byte keys in
+    between are the sort's cost, not counted as useful bytes."""
+    n = sum(v["records"] for v in ss.values())
+    ms = sum(v["ms"] for v in ss.values())
+    alg = 2 * s * n
+    ach = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return dict(streams=ss, records=n, ms=round(ms, 3),
+                roofline=dict(bound="hbm", algorithmic_bytes=alg, achieved=round(ach, 1), peak=HBM_PEAK_GBS,
+                              unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4)),
+                note="reimport + per-panel sort of both seed streams, records left on the device")
 
 
 def batch_leg(args):
