@@ -952,7 +952,26 @@ __global__ __launch_bounds__(PT) void k_seg_hist(const Rec *__restrict__ buf0, c
         const Seg sg = segs[s];
         const int64_t off = (LB - chk0[s]) * (int64_t) CH;
         const int n = (int) min((int64_t) CH, sg.count - off);
-        if (dig && (sg.skip & 2)) {
+        if (dig && (sg.skip & 2) && PER % 16 == 0) {
+            // a thread's PER consecutive bytes, sixteen at a time (unaligned
+            // 16-byte loads; a run cut by the chunk's end byte by byte)
+            constexpr int V = PER >= 16 ? PER / 16 : 1;
+            const uint8_t *q = dig + sg.start + off + (int64_t) threadIdx.x * PER;
+            const int m = n - (int) threadIdx.x * PER;           // bytes of this thread's run
+            if (m >= PER) {
+                uint4 v[V];
+#pragma unroll
+                for (int k = 0; k < V; k++) __builtin_memcpy(&v[k], q + 16 * k, 16);
+#pragma unroll
+                for (int k = 0; k < V; k++) {
+                    const uint32_t w[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                    for (int b = 0; b < 16; b++) atomicAdd(&hist[(w[b >> 2] >> (8 * (b & 3))) & 0xFFu], 1u);
+                }
+            } else {
+                for (int k = 0; k < m; k++) atomicAdd(&hist[q[k]], 1u);
+            }
+        } else if (dig && (sg.skip & 2)) {
             const uint8_t *q = dig + sg.start + off;
 #pragma unroll 4
             for (int k = 0; k < PER; k++) {
