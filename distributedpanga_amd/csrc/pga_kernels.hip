@@ -2059,11 +2059,15 @@ __global__ __launch_bounds__(MTH) void k_merge_count(const PairDev *__restrict__
         const PairDev &pd = pairs[merge_block_pair(pairs, npairs, lb)];
         lo = pd.rec0 + (lb - pd.mblk0) * (int64_t) TA;
         end = pd.rec0 + pd.nrec;
+        // window bytes [kmin, kmax) lie inside the pair (the tile's first
+        // anchor does, so kmin <= H < kmax)
+        const int64_t w0 = lo - H;
+        const int kmin = (int) (pd.rec0 > w0 ? pd.rec0 - w0 : 0);
+        const int kmax = (int) (end - w0 < TA + 2 * H ? end - w0 : TA + 2 * H);
 #pragma unroll
         for (int j = 0; j < WI; j++) {
             const int k = (int) threadIdx.x + j * MTH;
-            const int64_t g = lo - H + k;
-            x[j] = k < TA + 2 * H && g >= pd.rec0 && g < end ? ls[g] : 0;
+            x[j] = k >= kmin && k < kmax ? ls[w0 + k] : 0;
         }
     };
     int64_t nlo = 0, nend = 0;
