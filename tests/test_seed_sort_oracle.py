@@ -121,3 +121,45 @@ def test_reimport_inverts_back_to_the_seeds(comp):
         w = sorted(zip(ipos[icont == c].tolist(), jpos[icont == c].tolist()))
         assert sorted(zip((i2[sl] // 2).tolist(), (j2[sl] // 2).tolist())) == w
         off += counts[c]
+
+
+@pytest.mark.parametrize("comp", [0, 1])
+@pytest.mark.parametrize("maxctg,P,Cb,ncontig", [(100, 1, 1, 3), (40000, 2, 2, 300), (3_000_000, 3, 2, 9)])
+def test_reimport_inverts_to_the_seed_positions(comp, maxctg, P, Cb, ncontig):
+    """Undo each record: rebuild diag from band and its low six bits, read anti,
+    solve the two linear equations of the strand for (ipos, jpos) and compare
+    with the seed that made the record.  For the forward strand
+    i - j = diag - M and i + j = anti; for the reverse one i + j = 2M - diag
+    and i - j = M - anti (M the longest contig; src/pga-mpi-merge.c:415-421,
+    src/fastga_pipeline.c:1434-1446).  A swapped strand branch, a band shifted
+    by one or a mis-sized DBYTE field gives back other positions."""
+    rng = np.random.default_rng(maxctg + comp)
+    n = 2000
+    lcp = rng.integers(12, 256, n)
+    ipos = rng.integers(0, maxctg, n)
+    jpos = rng.integers(0, maxctg, n)
+    icont = rng.integers(0, ncontig, n)
+    jcont = rng.integers(0, min(ncontig, 1 << (8 * Cb - 1)), n)
+    jstr = rng.integers(0, 2, n)
+    raw = []
+    for k in range(n):
+        raw += seed(int(lcp[k]), int(ipos[k]), int(icont[k]), int(jpos[k]), int(jcont[k]), int(jstr[k]), P, Cb)
+    recs, sw = oracle.seed_sort(np.array(raw, np.uint8), P, Cb, ncontig, maxctg, comp, sort=False)
+    d = oracle.dbyte(maxctg)
+    assert sw == 2 + 2 * d + Cb and recs.shape == (n, sw)
+    le = lambda cols: sum(recs[:, c].astype(np.int64) << (8 * t) for t, c in enumerate(cols))
+    anti = le(range(2, 2 + d))
+    band = le(range(2 + d, 2 + 2 * d))
+    diag = (band << 6) + recs[:, 1].astype(np.int64)
+    assert (recs[:, 1] < 64).all()
+    M = maxctg
+    if comp:
+        s, t = 2 * M - diag, M - anti          # s = i + j, t = i - j
+    else:
+        s, t = anti, diag - M
+    assert ((s + t) % 2 == 0).all()
+    i, j = (s + t) // 2, (s - t) // 2
+    order = np.argsort(icont, kind="stable")   # panels in contig order, seeds in input order
+    assert (i == ipos[order]).all() and (j == jpos[order]).all()
+    assert (recs[:, 0] == lcp[order]).all()
+    assert (le(range(2 + 2 * d, sw)) == jcont[order]).all()     # strand bit dropped
