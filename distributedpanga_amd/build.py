@@ -48,7 +48,10 @@ def build_hip():
 def build_host():
     out = os.path.join(PKG, "libpga_host.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, s) for s in HOST_SRCS if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in HOST_SRCS]
+    missing = [s for s in srcs if not os.path.exists(s)]
+    if missing:
+        raise FileNotFoundError("host library sources missing: " + ", ".join(missing))
     _run(["gcc", "-shared"] + CC_FLAGS + ["-o", tmp] + srcs + ["-lm"])
     os.replace(tmp, out)
     return out
@@ -57,7 +60,7 @@ def build_host():
 def build_driver():
     src = os.path.join(CSRC, "pga_drive.c")
     if not os.path.exists(src):
-        return None
+        raise FileNotFoundError("C driver source missing: " + src)
     out = os.path.join(PKG, "pga_drive")
     tmp = out + ".tmp"
     _run(["gcc"] + CC_FLAGS + ["-o", tmp, src, "-L" + PKG, "-lpga_hip", "-lpga_host", "-lm",
